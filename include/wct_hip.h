@@ -1,0 +1,159 @@
+/* C ABI of the MI355X (gfx950) implementation of Wire-Cell's 2D
+ * signal-processing deconvolution (the hot path of OmnibusSigProc).
+ *
+ * Everything here is plain C: device pointers, sizes, doubles, an int
+ * status and a thread-local error string.  No C++ or runtime types
+ * cross this boundary; "stream" is an opaque hipStream_t (NULL = the
+ * default stream).  Status: 0 on success, a negative WCT_ERR_* code
+ * otherwise, with wct_last_error() describing it.
+ *
+ * Reference interfaces each group replaces are cited per declaration
+ * (paths relative to the Wire-Cell Toolkit source tree).
+ */
+#ifndef WCT_HIP_H
+#define WCT_HIP_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WCT_OK 0
+#define WCT_ERR_ARG (-1)
+#define WCT_ERR_HIP (-2)
+#define WCT_ERR_UNSUPPORTED (-3)
+#define WCT_ERR_NOMEM (-4)
+#define WCT_ERR_STATE (-5)
+
+/* Message of the last failing call on this thread ("" if none).
+ * Replaces the reference's exceptions (util/inc/WireCellUtil/Exceptions.h:43),
+ * which the C++ host classes re-raise from it. */
+const char* wct_last_error(void);
+int wct_abi_version(void);
+
+/* ---- device and memory plumbing (no reference counterpart: the
+ *      reference is host-only; these let a C/C++ host drive the
+ *      device without runtime headers) ---- */
+int wct_device_count(int* n);
+int wct_set_device(int device);
+int wct_malloc(void** dptr, size_t bytes);
+int wct_free(void* dptr);
+int wct_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream);
+int wct_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
+int wct_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
+int wct_memset(void* dptr, int value, size_t bytes, void* stream);
+int wct_stream_create(void** stream);
+int wct_stream_destroy(void* stream);
+int wct_stream_sync(void* stream);
+
+/* ---- IDFT drop-in: iface/inc/WireCellIface/IDFT.h:80-127 ----
+ * Device pointers to interleaved single-precision complex arrays
+ * (layout of std::complex<float>), row-major.  in == out is allowed.
+ * Forward transforms are unnormalised, inverse ones scale by 1/n of the
+ * transformed extent (IDFT.h:40-66, aux/src/FftwDFT.cxx:113-116,177-182,
+ * 212-216). */
+/* IDFT::fwd1d / inv1d (IDFT.h:93-97; FftwDFT.cxx:87-117) */
+int wct_dft_fwd1d(const float* in, float* out, int size, void* stream);
+int wct_dft_inv1d(const float* in, float* out, int size, void* stream);
+/* IDFT::fwd1b / inv1b (IDFT.h:101-107; FftwDFT.cxx:149-183):
+ * axis = 1 transforms each row, axis = 0 each column. */
+int wct_dft_fwd1b(const float* in, float* out, int nrows, int ncols, int axis, void* stream);
+int wct_dft_inv1b(const float* in, float* out, int nrows, int ncols, int axis, void* stream);
+/* IDFT::fwd2d / inv2d (IDFT.h:111-116; FftwDFT.cxx:186-217).  FftwDFT
+ * plans fftwf_plan_dft_2d(ncols, nrows, ...): the buffer is transformed
+ * as a row-major (ncols x nrows) array; these follow that. */
+int wct_dft_fwd2d(const float* in, float* out, int nrows, int ncols, void* stream);
+int wct_dft_inv2d(const float* in, float* out, int nrows, int ncols, void* stream);
+/* IDFT::transpose (IDFT.h:120-125; iface/src/IDFT.cxx:53-95):
+ * out (ncols x nrows) = in (nrows x ncols)^T, in == out allowed. */
+int wct_dft_transpose_real(const float* in, float* out, int nrows, int ncols, void* stream);
+int wct_dft_transpose_complex(const float* in, float* out, int nrows, int ncols, void* stream);
+/* Double-precision batched transforms (same conventions as 1b), used
+ * to build response spectra in FP64. */
+int wct_dft_fwd1b_f64(const double* in, double* out, int nrows, int ncols, int axis, void* stream);
+int wct_dft_inv1b_f64(const double* in, double* out, int nrows, int ncols, int axis, void* stream);
+/* Largest supported transform length and whether n is supported
+ * (all prime factors in {2,3,5,7,11,13}). */
+int wct_dft_supported(int n);
+
+/* ---- response spectra, FP64 on the device ----
+ * OmnibusSigProc::init_overall_response
+ * (sigproc/src/OmnibusSigProc.cxx:707-865) after
+ * Response::wire_region_average: each region-averaged field-response row
+ * (nin samples at fine_period, zero-padded to nfine) is convolved with
+ * the electronics samples elec[nfine] scaled by -inter_gain*adc_mv and
+ * by fine_period, shifted by fine_time_offset and redigitised onto nt
+ * coarse ticks (including the reference's interpolation weights).
+ * Host arrays in and out. */
+int wct_overall_response(const double* fr_avg, int nr, int nin, const double* elec, int nfine,
+                         double fine_period, double coarse_period, double inter_gain,
+                         double adc_mv, double fine_time_offset, int nt, double* resp_out,
+                         void* stream);
+
+/* Per-plane deconvolution kernel, the fused form of decon_2D_init and a
+ * decon_2D_*ROI time filter (OmnibusSigProc.cxx:944-1117):
+ *   K(m,k) = Fw[m] Ft[k] exp(-2 pi i (m ws/nw + k ts/nt)) / (R(m,k) nw nt)
+ * with R = FFT_w(FFT_t(resp rows padded to nw)) built in FP64 on the
+ * device, K = 0 where R == 0 (the NaN/Inf fix-up of :1010-1017), ts = 0
+ * unless time_shift > 0 (:1040-1048), for k <= nt/2 (the c2r mirror of
+ * aux/src/DftTools.cxx:133-139 only reads those bins).  Bins k >= kc are
+ * dropped when max_m |K(m,k)| <= cut_tol * max |K| (cut_tol = 0 keeps
+ * all).  The kernel is stored in FP32 on the device.
+ * resp, wire_filter (nw) and time_filter (nt) are host arrays. */
+typedef struct wct_decon_kernel_s* wct_decon_kernel;
+int wct_decon_kernel_create(wct_decon_kernel* out, const double* resp, int nr, int nw, int nt,
+                            const double* wire_filter, const double* time_filter,
+                            int wire_shift, int time_shift, double cut_tol, void* stream);
+int wct_decon_kernel_info(wct_decon_kernel k, int* nw, int* nt, int* kc);
+/* Copies K (kc x nw complex, k-major) to a host buffer of 2*kc*nw floats. */
+int wct_decon_kernel_download(wct_decon_kernel k, float* host_out);
+int wct_decon_kernel_destroy(wct_decon_kernel k);
+
+/* One plane of one frame.  Replaces, per plane, load_data + decon_2D_init
+ * + decon_2D_tightROI (+ decon_2D_tighterROI) + restore_baseline +
+ * ROI_formation::find_ROI_by_decon_itself's RMS and threshold
+ * (OmnibusSigProc.cxx:304-352,944-1160; ROI_formation.cxx:364-531).
+ *   adc   : device, nw rows of nt floats, row stride ld_adc
+ *   out   : device, nw x ld_out floats: the tight decon after
+ *           restore_baseline (save_mode 1 also zeroes negatives and
+ *           [lo,hi) bad ranges as save_data does, :383-418)
+ *   mask  : device, nw x ld_mask bytes or NULL; bit 0 = tight > th,
+ *           bit 1 = tighter > th (th = th_factor * rms + 1)
+ *   rms   : device, nw floats or NULL (ROI_formation's per-wire rms)
+ *   bad_* : device CSR of per-wire bad tick ranges or NULL:
+ *           bad_ptr[nw+1], ranges [bad_lo, bad_hi) are zeroed on input
+ *           (load_data) and [bad_lo, bad_hi] are excluded from the RMS and
+ *           the mask (find_ROI_by_decon_itself)
+ *   lf_tight / lf_tighter : device, kc floats or NULL: extra per-bin
+ *           time factors for the tight and the tighter output. */
+typedef struct {
+    wct_decon_kernel kernel;
+    const float* adc;
+    int ld_adc;
+    float* out;
+    int ld_out;
+    unsigned char* mask;
+    int ld_mask;
+    float* rms;
+    const int* bad_ptr;
+    const int* bad_lo;
+    const int* bad_hi;
+    const float* lf_tight;
+    const float* lf_tighter;
+    float th_factor;
+    int restore_baseline;
+    int save_mode;
+} wct_decon_job;
+
+/* Device scratch needed to run the jobs (spectrum buffers). */
+size_t wct_decon_workspace_size(const wct_decon_job* jobs, int njobs);
+/* Runs all jobs: one launch per stage over every plane of the batch.
+ * jobs is a host array; workspace is device memory of at least
+ * wct_decon_workspace_size bytes. */
+int wct_decon_run(const wct_decon_job* jobs, int njobs, void* workspace, size_t ws_bytes,
+                  void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

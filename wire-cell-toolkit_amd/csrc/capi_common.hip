@@ -1,0 +1,246 @@
+// C ABI plumbing: error string, device/memory/stream helpers and the
+// per-device FFT plan cache.
+#include <cstdarg>
+#include <cstring>
+
+#include "common.hpp"
+
+namespace wct {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+static bool small_primes_only(int n)
+{
+    if (n < 1) return false;
+    const int primes[] = {2, 3, 5, 7, 11, 13};
+    for (int p : primes)
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+bool fft_supported_length(int n) { return small_primes_only(n); }
+
+// Radix order: 8s, then a 4 or 2 for the rest of the power of two, then
+// 5, 3, 7, 11, 13.  Larger radices mean fewer passes through LDS.
+static void factorize(int n, FftPlan& pl)
+{
+    int m = n, np = 0;
+    while (m % 8 == 0) { pl.rad[np++] = 8; m /= 8; }
+    if (m % 4 == 0) { pl.rad[np++] = 4; m /= 4; }
+    if (m % 2 == 0) { pl.rad[np++] = 2; m /= 2; }
+    const int odd[] = {5, 3, 7, 11, 13};
+    for (int p : odd)
+        while (m % p == 0) { pl.rad[np++] = p; m /= p; }
+    pl.npass = np;
+    pl.n = n;
+}
+
+template <typename T>
+static void fill_twiddles(const FftPlan& pl, std::vector<T>& tab)
+{
+    // pass twiddles w^{r k} for k < Ns, r in [1, R), w = exp(-2 pi i/(Ns R))
+    int Ns = 1;
+    for (int p = 0; p < pl.npass; ++p) {
+        const int R = pl.rad[p];
+        const long M = (long)Ns * R;
+        for (int k = 0; k < Ns; ++k)
+            for (int r = 1; r < R; ++r) {
+                const long q = ((long)r * k) % M;
+                const double ang = -2.0 * M_PI * (double)q / (double)M;
+                tab.push_back((T)cos(ang));
+                tab.push_back((T)sin(ang));
+            }
+        Ns *= R;
+    }
+    // generic small-DFT tables exp(-2 pi i q / R), R = 7, 11, 13
+    const int gr[] = {7, 11, 13};
+    for (int R : gr)
+        for (int q = 0; q < R; ++q) {
+            const double ang = -2.0 * M_PI * q / R;
+            tab.push_back((T)cos(ang));
+            tab.push_back((T)sin(ang));
+        }
+}
+
+using PlanKey = std::tuple<int, int, bool>;   // device, n, double
+static std::mutex g_plan_mutex;
+static std::map<PlanKey, DevPlan>* g_plans = new std::map<PlanKey, DevPlan>();
+using LineKey = std::tuple<int, int, int, bool>;
+static std::map<LineKey, void*>* g_lines = new std::map<LineKey, void*>();
+
+int get_plan(int n, bool is_double, const DevPlan** out)
+{
+    WCT_REQUIRE(fft_supported_length(n), WCT_ERR_UNSUPPORTED,
+                "FFT length %d has a prime factor above 13", n);
+    int dev = 0;
+    WCT_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_plan_mutex);
+    auto key = PlanKey(dev, n, is_double);
+    auto it = g_plans->find(key);
+    if (it != g_plans->end()) {
+        *out = &it->second;
+        return WCT_OK;
+    }
+    DevPlan dp;
+    memset(&dp.plan, 0, sizeof(dp.plan));
+    factorize(n, dp.plan);
+    int Ns = 1;
+    size_t off = 0;
+    for (int p = 0; p < dp.plan.npass; ++p) {
+        dp.plan.twoff[p] = (int)off;
+        off += (size_t)Ns * (dp.plan.rad[p] - 1);
+        Ns *= dp.plan.rad[p];
+    }
+    dp.plan.dftoff = (int)off;
+    if (is_double) {
+        std::vector<double> tab;
+        fill_twiddles<double>(dp.plan, tab);
+        dp.tw_count = tab.size() / 2;
+        WCT_HIP_CHECK(hipMalloc(&dp.tw, tab.size() * sizeof(double)));
+        WCT_HIP_CHECK(hipMemcpy(dp.tw, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tab;
+        fill_twiddles<float>(dp.plan, tab);
+        dp.tw_count = tab.size() / 2;
+        WCT_HIP_CHECK(hipMalloc(&dp.tw, tab.size() * sizeof(float)));
+        WCT_HIP_CHECK(hipMemcpy(dp.tw, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    auto res = g_plans->emplace(key, dp);
+    *out = &res.first->second;
+    return WCT_OK;
+}
+
+int get_twiddle_line(int N, int count, bool is_double, const void** out)
+{
+    int dev = 0;
+    WCT_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_plan_mutex);
+    auto key = LineKey(dev, N, count, is_double);
+    auto it = g_lines->find(key);
+    if (it != g_lines->end()) {
+        *out = it->second;
+        return WCT_OK;
+    }
+    void* d = nullptr;
+    if (is_double) {
+        std::vector<double> tab(2 * (size_t)count);
+        for (int k = 0; k < count; ++k) {
+            const double ang = -2.0 * M_PI * (double)(k % N) / N;
+            tab[2 * k] = cos(ang);
+            tab[2 * k + 1] = sin(ang);
+        }
+        WCT_HIP_CHECK(hipMalloc(&d, tab.size() * sizeof(double)));
+        WCT_HIP_CHECK(hipMemcpy(d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tab(2 * (size_t)count);
+        for (int k = 0; k < count; ++k) {
+            const double ang = -2.0 * M_PI * (double)(k % N) / N;
+            tab[2 * k] = (float)cos(ang);
+            tab[2 * k + 1] = (float)sin(ang);
+        }
+        WCT_HIP_CHECK(hipMalloc(&d, tab.size() * sizeof(float)));
+        WCT_HIP_CHECK(hipMemcpy(d, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    (*g_lines)[key] = d;
+    *out = d;
+    return WCT_OK;
+}
+
+}  // namespace wct
+
+using namespace wct;
+
+extern "C" {
+
+const char* wct_last_error(void) { return g_err; }
+int wct_abi_version(void) { return 1; }
+
+int wct_device_count(int* n)
+{
+    WCT_REQUIRE(n, WCT_ERR_ARG, "null pointer");
+    WCT_HIP_CHECK(hipGetDeviceCount(n));
+    return WCT_OK;
+}
+
+int wct_set_device(int device)
+{
+    WCT_HIP_CHECK(hipSetDevice(device));
+    return WCT_OK;
+}
+
+int wct_malloc(void** dptr, size_t bytes)
+{
+    WCT_REQUIRE(dptr, WCT_ERR_ARG, "null pointer");
+    hipError_t e = hipMalloc(dptr, bytes ? bytes : 1);
+    if (e == hipErrorOutOfMemory) {
+        set_error("hipMalloc(%zu) out of memory", bytes);
+        return WCT_ERR_NOMEM;
+    }
+    WCT_HIP_CHECK(e);
+    return WCT_OK;
+}
+
+int wct_free(void* dptr)
+{
+    if (dptr) WCT_HIP_CHECK(hipFree(dptr));
+    return WCT_OK;
+}
+
+int wct_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream)
+{
+    WCT_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+    WCT_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
+    return WCT_OK;
+}
+
+int wct_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream)
+{
+    WCT_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+    WCT_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
+    return WCT_OK;
+}
+
+int wct_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream)
+{
+    WCT_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
+    return WCT_OK;
+}
+
+int wct_memset(void* dptr, int value, size_t bytes, void* stream)
+{
+    WCT_HIP_CHECK(hipMemsetAsync(dptr, value, bytes, as_stream(stream)));
+    return WCT_OK;
+}
+
+int wct_stream_create(void** stream)
+{
+    WCT_REQUIRE(stream, WCT_ERR_ARG, "null pointer");
+    hipStream_t s;
+    WCT_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (void*)s;
+    return WCT_OK;
+}
+
+int wct_stream_destroy(void* stream)
+{
+    if (stream) WCT_HIP_CHECK(hipStreamDestroy(as_stream(stream)));
+    return WCT_OK;
+}
+
+int wct_stream_sync(void* stream)
+{
+    WCT_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
+    return WCT_OK;
+}
+
+int wct_dft_supported(int n) { return fft_supported_length(n) ? 1 : 0; }
+
+}  // extern "C"

@@ -1,0 +1,71 @@
+// Shared host-side plumbing of the HIP library: status codes, the
+// thread-local error string of the C ABI, and the per-device cache of
+// FFT plans (radix schedule + twiddle tables in device memory).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/wct_hip.h"
+#include "fft_lds.hpp"
+
+namespace wct {
+
+void set_error(const char* fmt, ...);
+
+struct Status {
+    int code;
+};
+
+#define WCT_HIP_CHECK(expr)                                                               \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            ::wct::set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #expr,           \
+                             hipGetErrorString(e_));                                      \
+            return WCT_ERR_HIP;                                                           \
+        }                                                                                 \
+    } while (0)
+
+#define WCT_REQUIRE(cond, code, ...)                                                      \
+    do {                                                                                  \
+        if (!(cond)) {                                                                    \
+            ::wct::set_error(__VA_ARGS__);                                                \
+            return (code);                                                                \
+        }                                                                                 \
+    } while (0)
+
+// Largest prime factor allowed by the LDS kernels.
+bool fft_supported_length(int n);
+
+// A device-resident plan for one length and precision.
+struct DevPlan {
+    FftPlan plan;
+    void* tw = nullptr;      // T2 table: pass twiddles then generic tables
+    size_t tw_count = 0;
+};
+
+// Returns a cached plan (created on first use) for length n on the
+// current device.  is_double selects double2 twiddles.
+int get_plan(int n, bool is_double, const DevPlan** out);
+
+// Twiddles exp(-2 pi i k / N) for k in [0, count), as T2, cached.
+int get_twiddle_line(int N, int count, bool is_double, const void** out);
+
+// Batched transforms used across translation units (dft.hip).
+template <typename T>
+int launch_rows(const void* in, void* out, int n, int nrows, long dist, int sign, T scale,
+                hipStream_t s);
+template <typename T>
+int launch_cols_any(const void* in, void* out, int nrows, int ncols, int sign, T scale,
+                    hipStream_t s);
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace wct

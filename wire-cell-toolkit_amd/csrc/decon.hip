@@ -1,0 +1,720 @@
+// Fused 2D deconvolution of OmnibusSigProc (sigproc/src/OmnibusSigProc.cxx:
+// 944-1160) plus the tight-ROI RMS/threshold of ROI_formation
+// (sigproc/src/ROI_formation.cxx:364-531), batched over planes.
+//
+// The reference runs, per plane, 7 full complex transforms in
+// decon_2D_init and 3 more per time-filter variant, with element-wise
+// passes in between.  All of them are linear and their composition is
+//   Y = roll_w,t( c2r_t( IFFT_w( FFT_w( r2c_t(D) ) * Fw/R ) * Ft ) )
+// so the same output comes from three stages with one spectrum buffer:
+//   K1  row pass:    real FFT of every wire row (r2c, half spectrum),
+//                    keeping the kc bins the time filter passes;
+//   K2  wire pass:   for tiles of TK bins, FFT along wires in LDS,
+//                    multiply by K = Fw Ft phase / (R nw nt), inverse FFT;
+//   K3  row pass:    inverse real FFT, restore_baseline, cal_RMS,
+//                    threshold mask, outputs.
+// K is built once per plane type in FP64 (wct_decon_kernel_create).
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "rowstats.hpp"
+
+struct wct_decon_kernel_s {
+    int device;
+    int nw, nt, kc;
+    float2* K;          // kc x nw, k-major
+};
+
+namespace wct {
+
+constexpr int kNT = 256;
+
+// Device-side description of one plane job of a batch.
+struct DJob {
+    const float* adc;
+    float* out;
+    unsigned char* mask;
+    float* rms;
+    const int* bad_ptr;
+    const int* bad_lo;
+    const int* bad_hi;
+    const float* lf1;
+    const float* lf2;
+    const float2* K;
+    float2* S;
+    const float2* twM;
+    const float2* wN;
+    const float2* twW;
+    long row0;
+    long tile0;
+    int ld_adc, ld_out, ld_mask;
+    int nw, nt, M, kc, ldk;
+    float th_factor;
+    int restore, save_mode;
+    FftPlan planM;
+    FftPlan planW;
+};
+
+__device__ __forceinline__ int find_job_rows(const DJob* jobs, int njobs, long r)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].row0 <= r) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int find_job_tiles(const DJob* jobs, int njobs, long t)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile0 <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool in_bad(const DJob& J, int w, int t, bool inclusive)
+{
+    if (!J.bad_ptr) return false;
+    const int q0 = J.bad_ptr[w], q1 = J.bad_ptr[w + 1];
+    for (int q = q0; q < q1; ++q) {
+        const int lo = J.bad_lo[q], hi = J.bad_hi[q];
+        if (t >= lo && (inclusive ? t <= hi : t < hi)) return true;
+    }
+    return false;
+}
+
+// XCD-aware block order: consecutive logical tiles land on the same XCD
+// (blocks are dispatched round-robin over the 8 XCDs), so tiles sharing
+// cache lines share an L2.
+__device__ __forceinline__ long xcd_remap(long b, long nblocks)
+{
+    constexpr int kXcd = 8;
+    const long per = nblocks / kXcd;
+    if (b >= per * kXcd) return b;
+    return (b % kXcd) * per + b / kXcd;
+}
+
+// ---- K1: real FFT along ticks --------------------------------------
+template <int NT, int MMAX>
+__global__ __launch_bounds__(NT) void k_row_r2c(const DJob* __restrict__ jobs, int njobs)
+{
+    __shared__ float2 buf[MMAX];
+    const long r = blockIdx.x;
+    const DJob& J = jobs[find_job_rows(jobs, njobs, r)];
+    const int w = (int)(r - J.row0);
+    const int M = J.M;
+    const float* src = J.adc + (long)w * J.ld_adc;
+    const bool has_bad = J.bad_ptr && J.bad_ptr[w + 1] > J.bad_ptr[w];
+    for (int i = threadIdx.x; i < M; i += NT) {
+        float2 z = *reinterpret_cast<const float2*>(src + 2 * i);
+        if (has_bad) {   // load_data zeroes [lo, hi) of bad channels (:337-346)
+            if (in_bad(J, w, 2 * i, false)) z.x = 0.f;
+            if (in_bad(J, w, 2 * i + 1, false)) z.y = 0.f;
+        }
+        buf[i] = z;
+    }
+    __syncthreads();
+    fft_lds<float2, NT, MMAX>(buf, J.planM, J.twM, -1, threadIdx.x);
+    float2* dst = J.S + (long)w * J.ldk;
+    for (int k = threadIdx.x; k < J.ldk; k += NT) {
+        float2 X;
+        X.x = 0.f;
+        X.y = 0.f;
+        if (k < J.kc) {
+            const float2 Zk = buf[k == M ? 0 : k];
+            const float2 Zm = buf[k == 0 ? 0 : M - k];
+            // X = (Zk + conj Zm)/2 - i W^k (Zk - conj Zm)/2
+            const float sr = Zk.x + Zm.x, si = Zk.y - Zm.y;
+            const float dr = Zk.x - Zm.x, di = Zk.y + Zm.y;
+            const float2 W = J.wN[k];
+            const float wr = W.x * dr - W.y * di, wi = W.x * di + W.y * dr;
+            X.x = 0.5f * (sr + wi);
+            X.y = 0.5f * (si - wr);
+        }
+        dst[k] = X;
+    }
+}
+
+// ---- K2: wire FFT, multiply by K, inverse wire FFT ------------------
+template <int NT, int TK, int NWMAX>
+__global__ __launch_bounds__(NT) void k_wire_pass(const DJob* __restrict__ jobs, int njobs,
+                                                  long ntiles)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* buf = reinterpret_cast<float2*>(smem_raw);
+    const long t = xcd_remap(blockIdx.x, ntiles);
+    const DJob& J = jobs[find_job_tiles(jobs, njobs, t)];
+    const int k0 = (int)(t - J.tile0) * TK;
+    const int nw = J.nw, ld = nw + 1;
+    float2* S = J.S;
+    for (int idx = threadIdx.x; idx < nw * TK; idx += NT) {
+        const int cc = idx % TK, w = idx / TK;
+        buf[cc * ld + w] = S[(long)w * J.ldk + k0 + cc];
+    }
+    __syncthreads();
+    constexpr int SL = NT / TK;
+    const int slice = threadIdx.x / SL, st = threadIdx.x % SL;
+    float2* col = buf + slice * ld;
+    fft_lds<float2, SL, NWMAX>(col, J.planW, J.twW, -1, st);
+    const int k = k0 + slice;
+    const float2* Kk = J.K + (long)k * nw;
+    for (int m = st; m < nw; m += SL) {
+        float2 v = col[m];
+        float2 o;
+        o.x = 0.f;
+        o.y = 0.f;
+        if (k < J.kc) o = cmul(v, Kk[m]);
+        col[m] = o;
+    }
+    __syncthreads();
+    fft_lds<float2, SL, NWMAX>(col, J.planW, J.twW, +1, st);
+    for (int idx = threadIdx.x; idx < nw * TK; idx += NT) {
+        const int cc = idx % TK, w = idx / TK;
+        S[(long)w * J.ldk + k0 + cc] = buf[cc * ld + w];
+    }
+}
+
+// Builds Z'[k] = (Y[k] + conj Y[M-k]) + i conj(W^k) (Y[k] - conj Y[M-k])
+// for the inverse real FFT; DC and Nyquist use their real parts only
+// (the c2r mirror, util/inc/WireCellUtil/Spectrum.h:56-89).
+__device__ __forceinline__ float2 c2r_pre(const DJob& J, const float2* Y, const float* g, int k)
+{
+    const int M = J.M;
+    float2 a, b;
+    a.x = a.y = b.x = b.y = 0.f;
+    if (k < J.kc) {
+        a = Y[k];
+        if (g) { a.x *= g[k]; a.y *= g[k]; }
+        if (k == 0) a.y = 0.f;
+    }
+    const int km = M - k;   // in (0, M]
+    if (km < J.kc) {
+        b = Y[km];
+        if (g) { b.x *= g[km]; b.y *= g[km]; }
+        if (km == M) b.y = 0.f;
+    }
+    // conj(b)
+    const float sr = a.x + b.x, si = a.y - b.y;
+    const float dr = a.x - b.x, di = a.y + b.y;
+    const float2 W = J.wN[k];
+    // i * conj(W) * d = i (W.x - i W.y)(dr + i di) = i[(W.x dr + W.y di) + i(W.x di - W.y dr)]
+    const float pr = W.x * dr + W.y * di, pi = W.x * di - W.y * dr;
+    float2 z;
+    z.x = sr - pi;
+    z.y = si + pr;
+    return z;
+}
+
+// ---- K3: inverse real FFT, baseline, RMS, threshold, outputs ---------
+template <int NT, int MMAX>
+__global__ __launch_bounds__(NT) void k_row_c2r_roi(const DJob* __restrict__ jobs, int njobs)
+{
+    constexpr int VH = MMAX / NT;          // complex slots per thread
+    constexpr int VPT = 2 * VH;            // real samples per thread
+    __shared__ float2 buf[MMAX];
+    // the statistics scratch reuses the FFT buffer once the row is in registers
+    static_assert(sizeof(RowScratch) <= sizeof(float2) * MMAX, "scratch does not fit");
+    RowScratch& sh = *reinterpret_cast<RowScratch*>(buf);
+    const long r = blockIdx.x;
+    const DJob& J = jobs[find_job_rows(jobs, njobs, r)];
+    const int w = (int)(r - J.row0);
+    const int M = J.M, nt = J.nt;
+    const float2* Y = J.S + (long)w * J.ldk;
+    const bool has_bad = J.bad_ptr && J.bad_ptr[w + 1] > J.bad_ptr[w];
+
+    for (int k = threadIdx.x; k < M; k += NT) buf[k] = c2r_pre(J, Y, J.lf1, k);
+    __syncthreads();
+    fft_lds<float2, NT, MMAX>(buf, J.planM, J.twM, +1, threadIdx.x);
+
+    float val[VPT];
+    uint64_t valid = 0, pm = 0;
+#pragma unroll
+    for (int i = 0; i < VH; ++i) {
+        const int n = threadIdx.x + i * NT;
+        float2 z;
+        z.x = z.y = 0.f;
+        if (n < M) {
+            z = buf[n];
+            valid |= 3ull << (2 * i);
+        }
+        val[2 * i] = z.x;
+        val[2 * i + 1] = z.y;
+    }
+    pm = valid;
+    if (has_bad) {
+#pragma unroll
+        for (int i = 0; i < VPT; ++i) {
+            const int pos = 2 * (threadIdx.x + (i / 2) * NT) + (i & 1);
+            if (((valid >> i) & 1ull) && in_bad(J, w, pos, true)) pm &= ~(1ull << i);
+        }
+    }
+    if (J.restore) row_restore_baseline<NT, VPT>(val, valid, &sh);
+    const float rms = row_cal_rms<NT, VPT>(val, pm, &sh);
+    const double th = (double)J.th_factor * (double)rms + 1.0;
+
+    unsigned char mbits[VPT];
+#pragma unroll
+    for (int i = 0; i < VPT; ++i)
+        mbits[i] = (((pm >> i) & 1ull) && (double)val[i] > th) ? 1 : 0;
+
+    if (J.lf2) {   // the "tighter" variant for the mask (decon_2D_tighterROI, :1120-1160)
+        __syncthreads();
+        for (int k = threadIdx.x; k < M; k += NT) buf[k] = c2r_pre(J, Y, J.lf2, k);
+        __syncthreads();
+        fft_lds<float2, NT, MMAX>(buf, J.planM, J.twM, +1, threadIdx.x);
+        float v2[VPT];
+#pragma unroll
+        for (int i = 0; i < VH; ++i) {
+            const int n = threadIdx.x + i * NT;
+            float2 z;
+            z.x = z.y = 0.f;
+            if (n < M) z = buf[n];
+            v2[2 * i] = z.x;
+            v2[2 * i + 1] = z.y;
+        }
+        if (J.restore) row_restore_baseline<NT, VPT>(v2, valid, &sh);
+#pragma unroll
+        for (int i = 0; i < VPT; ++i)
+            if (((pm >> i) & 1ull) && (double)v2[i] > th) mbits[i] |= 2;
+    }
+
+    float* o = J.out + (long)w * J.ld_out;
+    unsigned char* mk = J.mask ? J.mask + (long)w * J.ld_mask : nullptr;
+#pragma unroll
+    for (int i = 0; i < VH; ++i) {
+        const int n = threadIdx.x + i * NT;
+        if (n < M) {
+            float a = val[2 * i], b = val[2 * i + 1];
+            if (J.save_mode) {   // save_data (:383-418): negatives and [lo, hi) bad ranges to 0
+                a = a > 0.f ? a : 0.f;
+                b = b > 0.f ? b : 0.f;
+                if (has_bad) {
+                    if (in_bad(J, w, 2 * n, false)) a = 0.f;
+                    if (in_bad(J, w, 2 * n + 1, false)) b = 0.f;
+                }
+            }
+            float2 ab;
+            ab.x = a;
+            ab.y = b;
+            *reinterpret_cast<float2*>(o + 2 * n) = ab;
+            if (mk) {
+                const unsigned short mm = (unsigned short)mbits[2 * i] | ((unsigned short)mbits[2 * i + 1] << 8);
+                *reinterpret_cast<unsigned short*>(mk + 2 * n) = mm;
+            }
+        }
+    }
+    if (J.rms && threadIdx.x == 0) J.rms[w] = rms;
+    (void)nt;
+}
+
+// ---- FP64 kernel build ------------------------------------------------
+
+// K(m,k) for k <= nt/2, stored k-major in double with |K| per entry.
+__global__ void k_build_K(const double2* __restrict__ R, int nw, int nt, int kh,
+                          const double* __restrict__ Fw, const double* __restrict__ Ft,
+                          int wshift, int tshift, double2* __restrict__ Kd,
+                          double* __restrict__ Kabs)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)kh * nw) return;
+    const int k = (int)(i / nw), m = (int)(i % nw);
+    const double2 r = R[(long)m * nt + k];
+    const double den = r.x * r.x + r.y * r.y;
+    double2 out;
+    out.x = out.y = 0.0;
+    if (den != 0.0) {
+        const long qm = ((long)m * wshift) % nw, qk = ((long)k * tshift) % nt;
+        const double ang = -2.0 * M_PI * ((double)qm / nw + (double)qk / nt);
+        double sn, cs;
+        sincos(ang, &sn, &cs);
+        const double g = Fw[m] * Ft[k] / (den * (double)nw * (double)nt);
+        // phase / R = phase * conj(R) / |R|^2
+        const double pr = cs * r.x + sn * r.y;   // (cs + i sn)(r.x - i r.y)
+        const double pi = sn * r.x - cs * r.y;
+        out.x = pr * g;
+        out.y = pi * g;
+    }
+    Kd[i] = out;
+    Kabs[i] = sqrt(out.x * out.x + out.y * out.y);
+}
+
+__global__ void k_colmax(const double* __restrict__ Kabs, int nw, int kh, double* __restrict__ cmax)
+{
+    __shared__ double red[kNT];
+    const int k = blockIdx.x;
+    double m = 0;
+    for (int i = threadIdx.x; i < nw; i += kNT) m = fmax(m, Kabs[(long)k * nw + i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kNT / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cmax[k] = red[0];
+}
+
+__global__ void k_to_f32(const double2* __restrict__ in, float2* __restrict__ out, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        float2 v;
+        v.x = (float)in[i].x;
+        v.y = (float)in[i].y;
+        out[i] = v;
+    }
+}
+
+__global__ void k_real_to_cplx(const double* __restrict__ in, double2* __restrict__ out, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        double2 v;
+        v.x = in[i];
+        v.y = 0.0;
+        out[i] = v;
+    }
+}
+
+// Response convolution of init_overall_response (:793-813): each row's
+// spectrum times the scaled electronics spectrum times fine_period, then
+// the Hermitian mirror (bins above n/2 from the conjugate of those below,
+// DC and Nyquist real) ahead of the inverse.
+__global__ void k_resp_mul_mirror(double2* __restrict__ rows, const double2* __restrict__ elec,
+                                  int nr, int n, double fine_period)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)nr * n) return;
+    const int r = (int)(i / n), k = (int)(i % n);
+    double2* row = rows + (long)r * n;
+    // only the lower half is computed here; the mirror fills the rest
+    if (k > n / 2) return;
+    double2 c = row[k], e = elec[k], p;
+    p.x = (c.x * e.x - c.y * e.y) * fine_period;
+    p.y = (c.x * e.y + c.y * e.x) * fine_period;
+    if (k == 0 || (n % 2 == 0 && k == n / 2)) p.y = 0.0;
+    row[k] = p;
+    if (k > 0 && n - k != k) {
+        double2 q;
+        q.x = p.x;
+        q.y = -p.y;
+        row[n - k] = q;
+    }
+}
+
+static int pick_tk(int nw) { return nw + 1 <= 1024 ? 8 : (nw + 1 <= 2048 ? 4 : 2); }
+
+template <typename F>
+static int dispatch_M(int Mmax, F&& f)
+{
+    if (Mmax <= 2048) return f(std::integral_constant<int, 2048>());
+    if (Mmax <= 4096) return f(std::integral_constant<int, 4096>());
+    if (Mmax <= 8192) return f(std::integral_constant<int, 8192>());
+    set_error("time extent %d above the supported 16384 ticks", 2 * Mmax);
+    return WCT_ERR_UNSUPPORTED;
+}
+
+}  // namespace wct
+
+using namespace wct;
+
+extern "C" {
+
+int wct_overall_response(const double* fr_avg, int nr, int nin, const double* elec, int nfine,
+                         double fine_period, double coarse_period, double inter_gain,
+                         double adc_mv, double fine_time_offset, int nt, double* resp_out,
+                         void* stream)
+{
+    WCT_REQUIRE(fr_avg && elec && resp_out && nr > 0 && nin > 0 && nfine >= nin && nt > 0 &&
+                    fine_period > 0 && coarse_period > 0,
+                WCT_ERR_ARG, "overall_response: bad arguments");
+    WCT_REQUIRE(fft_supported_length(nfine), WCT_ERR_UNSUPPORTED,
+                "fine length %d has a prime factor above 13", nfine);
+    hipStream_t s = as_stream(stream);
+    // host staging: rows zero-padded to nfine, electronics scaled by
+    // -inter_gain*adc_mv (Waveform::scale at :768)
+    std::vector<double> h((size_t)(nr + 1) * nfine * 2, 0.0);
+    for (int r = 0; r < nr; ++r)
+        for (int i = 0; i < nin; ++i) h[2 * ((size_t)r * nfine + i)] = fr_avg[(size_t)r * nin + i];
+    const double esc = inter_gain * adc_mv * (-1);
+    for (int i = 0; i < nfine; ++i) h[2 * ((size_t)nr * nfine + i)] = elec[i] * esc;
+    double2* d = nullptr;
+    const size_t bytes = h.size() * sizeof(double);
+    WCT_HIP_CHECK(hipMallocAsync((void**)&d, bytes, s));
+    WCT_HIP_CHECK(hipMemcpyAsync(d, h.data(), bytes, hipMemcpyHostToDevice, s));
+    int rc = launch_rows<double>(d, d, nfine, nr + 1, nfine, -1, 1.0, s);
+    if (rc) return rc;
+    const long tot = (long)nr * nfine;
+    hipLaunchKernelGGL(k_resp_mul_mirror, dim3((tot + 255) / 256), dim3(256), 0, s, d,
+                       d + (size_t)nr * nfine, nr, nfine, fine_period);
+    WCT_HIP_CHECK(hipGetLastError());
+    rc = launch_rows<double>(d, d, nfine, nr, nfine, +1, 1.0 / nfine, s);
+    if (rc) return rc;
+    WCT_HIP_CHECK(hipMemcpyAsync(h.data(), d, (size_t)nr * nfine * 2 * sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+    WCT_HIP_CHECK(hipFreeAsync(d, s));
+    WCT_HIP_CHECK(hipStreamSynchronize(s));
+    // fine shift (:816-831) and redigitise (:834-859) on the host: a
+    // short serial interpolation per row, kept in the reference's form
+    // (float tick grids, swapped interpolation weights).
+    std::vector<double> arr((size_t)nfine);
+    const int fts = (int)(fine_time_offset / fine_period);
+    for (int r = 0; r < nr; ++r) {
+        for (int i = 0; i < nfine; ++i) {
+            const int dst = fts > 0 ? (i + fts) % nfine : i;
+            arr[dst] = h[2 * ((size_t)r * nfine + i)];
+        }
+        size_t fcount = 1;
+        for (int i = 0; i < nt; ++i) {
+            const double ctime = (float)(i * coarse_period);
+            if (fcount < (size_t)nfine) {
+                while (ctime > (double)(float)(fcount * fine_period)) {
+                    fcount++;
+                    if (fcount >= (size_t)nfine) break;
+                }
+            }
+            double v = 0;
+            if (fcount < (size_t)nfine) {
+                const double f0 = (float)((fcount - 1) * fine_period);
+                const double f1 = (float)(fcount * fine_period);
+                v = (ctime - f0) / fine_period * arr[fcount - 1] + (f1 - ctime) / fine_period * arr[fcount];
+            }
+            resp_out[(size_t)r * nt + i] = v;
+        }
+    }
+    return WCT_OK;
+}
+
+int wct_decon_kernel_create(wct_decon_kernel* out, const double* resp, int nr, int nw, int nt,
+                            const double* wire_filter, const double* time_filter, int wire_shift,
+                            int time_shift, double cut_tol, void* stream)
+{
+    WCT_REQUIRE(out && resp && wire_filter && time_filter && nr > 0 && nw > 0 && nt > 1,
+                WCT_ERR_ARG, "kernel_create: bad arguments");
+    WCT_REQUIRE(nt % 2 == 0, WCT_ERR_UNSUPPORTED, "odd tick count %d not supported", nt);
+    WCT_REQUIRE(fft_supported_length(nw) && fft_supported_length(nt / 2), WCT_ERR_UNSUPPORTED,
+                "plane %d x %d: extents need prime factors <= 13", nw, nt);
+    WCT_REQUIRE(nw <= 2048, WCT_ERR_UNSUPPORTED, "more than 2048 wires in a plane");
+    *out = nullptr;
+    hipStream_t s = as_stream(stream);
+    const int kh = nt / 2 + 1;
+    const size_t N = (size_t)nw * nt;
+    double2* R = nullptr;
+    double* tmp = nullptr;
+    WCT_HIP_CHECK(hipMallocAsync((void**)&R, N * sizeof(double2), s));
+    WCT_HIP_CHECK(hipMemsetAsync(R, 0, N * sizeof(double2), s));
+    const int nrr = std::min(nr, nw);
+    WCT_HIP_CHECK(hipMallocAsync((void**)&tmp, (size_t)nrr * nt * sizeof(double), s));
+    WCT_HIP_CHECK(hipMemcpyAsync(tmp, resp, (size_t)nrr * nt * sizeof(double), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_real_to_cplx, dim3(((long)nrr * nt + 255) / 256), dim3(256), 0, s, tmp, R,
+                       (long)nrr * nt);
+    WCT_HIP_CHECK(hipGetLastError());
+    // R = FFT_w(FFT_t(resp)) in FP64 (:988-998)
+    int rc = launch_rows<double>(R, R, nt, nrr, nt, -1, 1.0, s);
+    if (rc) return rc;
+    rc = launch_cols_any<double>(R, R, nw, nt, -1, 1.0, s);
+    if (rc) return rc;
+    double *Fw = nullptr, *Ft = nullptr, *Kabs = nullptr, *cmax = nullptr;
+    double2* Kd = nullptr;
+    WCT_HIP_CHECK(hipMallocAsync((void**)&Fw, nw * sizeof(double), s));
+    WCT_HIP_CHECK(hipMallocAsync((void**)&Ft, nt * sizeof(double), s));
+    WCT_HIP_CHECK(hipMallocAsync((void**)&Kd, (size_t)kh * nw * sizeof(double2), s));
+    WCT_HIP_CHECK(hipMallocAsync((void**)&Kabs, (size_t)kh * nw * sizeof(double), s));
+    WCT_HIP_CHECK(hipMallocAsync((void**)&cmax, kh * sizeof(double), s));
+    WCT_HIP_CHECK(hipMemcpyAsync(Fw, wire_filter, nw * sizeof(double), hipMemcpyHostToDevice, s));
+    WCT_HIP_CHECK(hipMemcpyAsync(Ft, time_filter, nt * sizeof(double), hipMemcpyHostToDevice, s));
+    const long nk = (long)kh * nw;
+    hipLaunchKernelGGL(k_build_K, dim3((nk + 255) / 256), dim3(256), 0, s, R, nw, nt, kh, Fw, Ft,
+                       wire_shift, time_shift > 0 ? time_shift : 0, Kd, Kabs);
+    WCT_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_colmax, dim3(kh), dim3(kNT), 0, s, Kabs, nw, kh, cmax);
+    WCT_HIP_CHECK(hipGetLastError());
+    std::vector<double> hc(kh);
+    WCT_HIP_CHECK(hipMemcpyAsync(hc.data(), cmax, kh * sizeof(double), hipMemcpyDeviceToHost, s));
+    WCT_HIP_CHECK(hipStreamSynchronize(s));
+    double gmax = 0;
+    for (double v : hc) gmax = std::max(gmax, v);
+    int kc = kh;
+    if (cut_tol > 0) {
+        kc = 1;
+        for (int k = kh - 1; k >= 0; --k)
+            if (hc[k] > cut_tol * gmax) { kc = k + 1; break; }
+    }
+    auto* kk = new wct_decon_kernel_s();
+    kk->nw = nw;
+    kk->nt = nt;
+    kk->kc = kc;
+    WCT_HIP_CHECK(hipGetDevice(&kk->device));
+    hipError_t e = hipMalloc((void**)&kk->K, (size_t)kc * nw * sizeof(float2));
+    if (e != hipSuccess) {
+        delete kk;
+        set_error("kernel_create: hipMalloc failed: %s", hipGetErrorString(e));
+        return WCT_ERR_NOMEM;
+    }
+    const long nkc = (long)kc * nw;
+    hipLaunchKernelGGL(k_to_f32, dim3((nkc + 255) / 256), dim3(256), 0, s, Kd, kk->K, nkc);
+    WCT_HIP_CHECK(hipGetLastError());
+    WCT_HIP_CHECK(hipFreeAsync(R, s));
+    WCT_HIP_CHECK(hipFreeAsync(tmp, s));
+    WCT_HIP_CHECK(hipFreeAsync(Fw, s));
+    WCT_HIP_CHECK(hipFreeAsync(Ft, s));
+    WCT_HIP_CHECK(hipFreeAsync(Kd, s));
+    WCT_HIP_CHECK(hipFreeAsync(Kabs, s));
+    WCT_HIP_CHECK(hipFreeAsync(cmax, s));
+    WCT_HIP_CHECK(hipStreamSynchronize(s));
+    *out = kk;
+    return WCT_OK;
+}
+
+int wct_decon_kernel_info(wct_decon_kernel k, int* nw, int* nt, int* kc)
+{
+    WCT_REQUIRE(k, WCT_ERR_ARG, "null kernel");
+    if (nw) *nw = k->nw;
+    if (nt) *nt = k->nt;
+    if (kc) *kc = k->kc;
+    return WCT_OK;
+}
+
+int wct_decon_kernel_download(wct_decon_kernel k, float* host_out)
+{
+    WCT_REQUIRE(k && host_out, WCT_ERR_ARG, "kernel_download: bad arguments");
+    WCT_HIP_CHECK(hipMemcpy(host_out, k->K, (size_t)k->kc * k->nw * sizeof(float2), hipMemcpyDeviceToHost));
+    return WCT_OK;
+}
+
+int wct_decon_kernel_destroy(wct_decon_kernel k)
+{
+    if (!k) return WCT_OK;
+    if (k->K) WCT_HIP_CHECK(hipFree(k->K));
+    delete k;
+    return WCT_OK;
+}
+
+static int job_ldk(const wct_decon_job& j)
+{
+    const int tk = pick_tk(j.kernel->nw);
+    return (j.kernel->kc + tk - 1) / tk * tk;
+}
+
+size_t wct_decon_workspace_size(const wct_decon_job* jobs, int njobs)
+{
+    if (!jobs || njobs <= 0) return 0;
+    size_t tot = 0;
+    for (int i = 0; i < njobs; ++i) {
+        if (!jobs[i].kernel) return 0;
+        tot += (size_t)jobs[i].kernel->nw * job_ldk(jobs[i]) * sizeof(float2);
+        tot = (tot + 255) & ~(size_t)255;
+    }
+    return tot + 256;
+}
+
+int wct_decon_run(const wct_decon_job* jobs, int njobs, void* workspace, size_t ws_bytes,
+                  void* stream)
+{
+    WCT_REQUIRE(jobs && njobs > 0, WCT_ERR_ARG, "decon_run: no jobs");
+    const size_t need = wct_decon_workspace_size(jobs, njobs);
+    WCT_REQUIRE(need > 0, WCT_ERR_ARG, "decon_run: a job has no kernel");
+    WCT_REQUIRE(workspace && ws_bytes >= need, WCT_ERR_ARG,
+                "decon_run: workspace of %zu bytes, %zu needed", ws_bytes, need);
+    hipStream_t s = as_stream(stream);
+    std::vector<DJob> h(njobs);
+    size_t off = 0;
+    long rows = 0, tiles = 0;
+    int Mmax = 0, tk = pick_tk(jobs[0].kernel->nw), nwmax = 0;
+    for (int i = 0; i < njobs; ++i) {
+        const wct_decon_job& j = jobs[i];
+        const wct_decon_kernel K = j.kernel;
+        WCT_REQUIRE(pick_tk(K->nw) == tk, WCT_ERR_UNSUPPORTED,
+                    "decon_run: planes of one batch need the same wire tile (%d vs %d wires)",
+                    jobs[0].kernel->nw, K->nw);
+        WCT_REQUIRE(j.adc && j.out && j.ld_adc >= K->nt && j.ld_out >= K->nt && j.ld_adc % 2 == 0 &&
+                        j.ld_out % 2 == 0 && (!j.mask || (j.ld_mask >= K->nt && j.ld_mask % 2 == 0)),
+                    WCT_ERR_ARG, "decon_run: job %d has bad pointers or strides", i);
+        WCT_REQUIRE((reinterpret_cast<uintptr_t>(j.adc) & 7) == 0 &&
+                        (reinterpret_cast<uintptr_t>(j.out) & 7) == 0 &&
+                        (!j.mask || (reinterpret_cast<uintptr_t>(j.mask) & 1) == 0),
+                    WCT_ERR_ARG, "decon_run: job %d buffers are misaligned", i);
+        DJob& d = h[i];
+        memset(&d, 0, sizeof(d));
+        d.adc = j.adc;
+        d.out = j.out;
+        d.mask = j.mask;
+        d.rms = j.rms;
+        d.bad_ptr = j.bad_ptr;
+        d.bad_lo = j.bad_lo;
+        d.bad_hi = j.bad_hi;
+        d.lf1 = j.lf_tight;
+        d.lf2 = j.lf_tighter;
+        d.K = K->K;
+        d.ld_adc = j.ld_adc;
+        d.ld_out = j.ld_out;
+        d.ld_mask = j.ld_mask;
+        d.nw = K->nw;
+        d.nt = K->nt;
+        d.M = K->nt / 2;
+        d.kc = K->kc;
+        d.ldk = job_ldk(j);
+        d.th_factor = j.th_factor;
+        d.restore = j.restore_baseline;
+        d.save_mode = j.save_mode;
+        d.S = reinterpret_cast<float2*>(static_cast<char*>(workspace) + off);
+        off += (size_t)d.nw * d.ldk * sizeof(float2);
+        off = (off + 255) & ~(size_t)255;
+        d.row0 = rows;
+        d.tile0 = tiles;
+        rows += d.nw;
+        tiles += d.ldk / tk;
+        const DevPlan* pM = nullptr;
+        const DevPlan* pW = nullptr;
+        int rc = get_plan(d.M, false, &pM);
+        if (rc) return rc;
+        rc = get_plan(d.nw, false, &pW);
+        if (rc) return rc;
+        d.planM = pM->plan;
+        d.twM = (const float2*)pM->tw;
+        d.planW = pW->plan;
+        d.twW = (const float2*)pW->tw;
+        const void* line = nullptr;
+        rc = get_twiddle_line(d.nt, d.M + 1, false, &line);
+        if (rc) return rc;
+        d.wN = (const float2*)line;
+        Mmax = std::max(Mmax, d.M);
+        nwmax = std::max(nwmax, d.nw);
+    }
+    DJob* dj = nullptr;
+    WCT_HIP_CHECK(hipMallocAsync((void**)&dj, njobs * sizeof(DJob), s));
+    WCT_HIP_CHECK(hipMemcpyAsync(dj, h.data(), njobs * sizeof(DJob), hipMemcpyHostToDevice, s));
+    int rc = dispatch_M(Mmax, [&](auto MM) -> int {
+        constexpr int MMAX = decltype(MM)::value;
+        hipLaunchKernelGGL((k_row_r2c<kNT, MMAX>), dim3(rows), dim3(kNT), 0, s, dj, njobs);
+        return WCT_OK;
+    });
+    if (rc) return rc;
+    WCT_HIP_CHECK(hipGetLastError());
+    const size_t lds2 = (size_t)tk * (nwmax + 1) * sizeof(float2);
+    if (tk == 8) {
+        hipLaunchKernelGGL((k_wire_pass<kNT, 8, 1024>), dim3(tiles), dim3(kNT), lds2, s, dj, njobs, tiles);
+    } else if (tk == 4) {
+        hipLaunchKernelGGL((k_wire_pass<kNT, 4, 2048>), dim3(tiles), dim3(kNT), lds2, s, dj, njobs, tiles);
+    } else {
+        set_error("decon_run: %d wires not supported", nwmax);
+        return WCT_ERR_UNSUPPORTED;
+    }
+    WCT_HIP_CHECK(hipGetLastError());
+    rc = dispatch_M(Mmax, [&](auto MM) -> int {
+        constexpr int MMAX = decltype(MM)::value;
+        hipLaunchKernelGGL((k_row_c2r_roi<kNT, MMAX>), dim3(rows), dim3(kNT), 0, s, dj, njobs);
+        return WCT_OK;
+    });
+    if (rc) return rc;
+    WCT_HIP_CHECK(hipGetLastError());
+    WCT_HIP_CHECK(hipFreeAsync(dj, s));
+    return WCT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,355 @@
+// IDFT drop-in kernels: batched complex FFTs along the contiguous axis
+// (rows) and along the strided axis (columns, through an LDS tile), and
+// LDS-tiled transposes.  These implement the IDFT contract of
+// iface/inc/WireCellIface/IDFT.h:9-127 with FftwDFT's conventions
+// (aux/src/FftwDFT.cxx:88-283).
+#include "common.hpp"
+
+namespace wct {
+
+constexpr int kBlock = 256;
+
+// ---- rows: each slice of SLICE threads transforms one row in LDS ----
+template <typename T, int SLICE, int NMAX>
+__global__ __launch_bounds__(kBlock) void k_fft_rows(const typename C2<T>::type* __restrict__ in,
+                                                     typename C2<T>::type* __restrict__ out,
+                                                     int nrows, long dist, FftPlan pl,
+                                                     const typename C2<T>::type* __restrict__ tw,
+                                                     int sign, T scale)
+{
+    using T2 = typename C2<T>::type;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T2* smem = reinterpret_cast<T2*>(smem_raw);
+    constexpr int RPB = kBlock / SLICE;
+    const int slice = threadIdx.x / SLICE;
+    const int tid = threadIdx.x % SLICE;
+    const long row = (long)blockIdx.x * RPB + slice;
+    const int n = pl.n;
+    T2* buf = smem + (size_t)slice * n;
+    const bool active = row < nrows;
+    if (active) {
+        const T2* src = in + row * dist;
+        for (int i = tid; i < n; i += SLICE) buf[i] = src[i];
+    }
+    __syncthreads();
+    fft_lds<T2, SLICE, NMAX>(buf, pl, tw, sign, tid);
+    if (active) {
+        T2* dst = out + row * dist;
+        for (int i = tid; i < n; i += SLICE) {
+            T2 v = buf[i];
+            v.x *= scale;
+            v.y *= scale;
+            dst[i] = v;
+        }
+    }
+}
+
+// ---- columns: a tile of TC columns, each transformed by its own slice ----
+template <typename T, int TC, int NMAX>
+__global__ __launch_bounds__(kBlock) void k_fft_cols(const typename C2<T>::type* __restrict__ in,
+                                                     typename C2<T>::type* __restrict__ out,
+                                                     int nrows, int ncols, long bstride, FftPlan pl,
+                                                     const typename C2<T>::type* __restrict__ tw,
+                                                     int sign, T scale)
+{
+    using T2 = typename C2<T>::type;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T2* smem = reinterpret_cast<T2*>(smem_raw);
+    constexpr int SLICE = kBlock / TC;
+    in += blockIdx.y * bstride;
+    out += blockIdx.y * bstride;
+    const int ld = nrows + 1;   // odd stride in 8-byte words: no bank conflicts on the transposed store
+    const long c0 = (long)blockIdx.x * TC;
+    // coalesced load: TC consecutive columns of each row
+    for (int idx = threadIdx.x; idx < nrows * TC; idx += kBlock) {
+        const int cc = idx % TC, r = idx / TC;
+        T2 v;
+        v.x = 0;
+        v.y = 0;
+        if (c0 + cc < ncols) v = in[(long)r * ncols + c0 + cc];
+        smem[cc * ld + r] = v;
+    }
+    __syncthreads();
+    const int slice = threadIdx.x / SLICE;
+    const int tid = threadIdx.x % SLICE;
+    fft_lds<T2, SLICE, NMAX>(smem + slice * ld, pl, tw, sign, tid);
+    for (int idx = threadIdx.x; idx < nrows * TC; idx += kBlock) {
+        const int cc = idx % TC, r = idx / TC;
+        if (c0 + cc < ncols) {
+            T2 v = smem[cc * ld + r];
+            v.x *= scale;
+            v.y *= scale;
+            out[(long)r * ncols + c0 + cc] = v;
+        }
+    }
+}
+
+// ---- transpose through a 32x32 LDS tile ----
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_transpose(const V* __restrict__ in, V* __restrict__ out,
+                                                      int nrows, int ncols)
+{
+    __shared__ V tile[32][33];
+    in += (long)blockIdx.z * nrows * ncols;
+    out += (long)blockIdx.z * nrows * ncols;
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;   // 32 x 8
+    for (int j = ty; j < 32; j += 8) {
+        const int r = by + j, c = bx + tx;
+        if (r < nrows && c < ncols) tile[j][tx] = in[(long)r * ncols + c];
+    }
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) {
+        const int r = bx + j, c = by + tx;   // out is ncols x nrows
+        if (r < ncols && c < nrows) out[(long)r * nrows + c] = tile[tx][j];
+    }
+}
+
+// ---- host dispatch ---------------------------------------------------
+
+template <typename T>
+static int launch_rows_lds(const void* in, void* out, int n, int nrows, long dist, int sign, T scale,
+                           hipStream_t s)
+{
+    using T2 = typename C2<T>::type;
+    const DevPlan* dp = nullptr;
+    int rc = get_plan(n, sizeof(T) == 8, &dp);
+    if (rc) return rc;
+    const T2* tw = (const T2*)dp->tw;
+    const T2* src = (const T2*)in;
+    T2* dst = (T2*)out;
+    constexpr bool dbl = sizeof(T) == 8;
+#define WCT_ROWS(SL, NM)                                                                         \
+    do {                                                                                         \
+        const int rpb = kBlock / SL;                                                             \
+        dim3 grid((nrows + rpb - 1) / rpb);                                                      \
+        size_t lds = (size_t)rpb * n * sizeof(T2);                                               \
+        hipLaunchKernelGGL((k_fft_rows<T, SL, NM>), grid, dim3(kBlock), lds, s, src, dst, nrows, \
+                           dist, dp->plan, tw, sign, scale);                                     \
+    } while (0)
+    if (n <= 64) WCT_ROWS(16, 64);
+    else if (n <= 256) WCT_ROWS(64, 256);
+    else if (n <= 1024) WCT_ROWS(128, 1024);
+    else if (n <= 4096) WCT_ROWS(256, 4096);
+    else if (!dbl && n <= 8192) WCT_ROWS(256, 8192);
+    else {
+        set_error("transform length %d is above the LDS limit", n);
+        return WCT_ERR_UNSUPPORTED;
+    }
+#undef WCT_ROWS
+    WCT_HIP_CHECK(hipGetLastError());
+    return WCT_OK;
+}
+
+template <typename T>
+static int launch_cols(const void* in, void* out, int nrows, int ncols, int sign, T scale,
+                       hipStream_t s, int nbatch = 1, long bstride = 0)
+{
+    using T2 = typename C2<T>::type;
+    const DevPlan* dp = nullptr;
+    int rc = get_plan(nrows, sizeof(T) == 8, &dp);
+    if (rc) return rc;
+    const T2* tw = (const T2*)dp->tw;
+    const T2* src = (const T2*)in;
+    T2* dst = (T2*)out;
+#define WCT_COLS(TC, NM)                                                                         \
+    do {                                                                                         \
+        dim3 grid((ncols + TC - 1) / TC, nbatch);                                                \
+        size_t lds = (size_t)TC * (nrows + 1) * sizeof(T2);                                      \
+        hipLaunchKernelGGL((k_fft_cols<T, TC, NM>), grid, dim3(kBlock), lds, s, src, dst, nrows, \
+                           ncols, bstride, dp->plan, tw, sign, scale);                           \
+    } while (0)
+    // TC columns per tile, keeping the tile within 64 KB of LDS
+    constexpr bool dbl = sizeof(T) == 8;
+    if (nrows <= 256) WCT_COLS(8, 256);
+    else if (!dbl && nrows <= 1024) WCT_COLS(4, 1024);
+    else if (!dbl && nrows <= 2048) WCT_COLS(2, 2048);
+    else if (!dbl && nrows <= 4096) WCT_COLS(1, 4096);
+    else if (dbl && nrows <= 1024) WCT_COLS(2, 1024);
+    else if (dbl && nrows <= 2048) WCT_COLS(1, 2048);
+    else {
+        set_error("column transform length %d is above the LDS limit", nrows);
+        return WCT_ERR_UNSUPPORTED;
+    }
+#undef WCT_COLS
+    WCT_HIP_CHECK(hipGetLastError());
+    return WCT_OK;
+}
+
+template <typename V>
+static int launch_transpose(const V* in, V* out, int nrows, int ncols, hipStream_t s)
+{
+    const size_t bytes = (size_t)nrows * ncols * sizeof(V);
+    V* dst = out;
+    V* tmp = nullptr;
+    if (in == out) {
+        WCT_HIP_CHECK(hipMallocAsync((void**)&tmp, bytes ? bytes : 1, s));
+        dst = tmp;
+    }
+    dim3 grid((ncols + 31) / 32, (nrows + 31) / 32);
+    hipLaunchKernelGGL((k_transpose<V>), grid, dim3(kBlock), 0, s, in, dst, nrows, ncols);
+    WCT_HIP_CHECK(hipGetLastError());
+    if (tmp) {
+        WCT_HIP_CHECK(hipMemcpyAsync(out, tmp, bytes, hipMemcpyDeviceToDevice, s));
+        WCT_HIP_CHECK(hipFreeAsync(tmp, s));
+    }
+    return WCT_OK;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fourstep_twiddle(typename C2<T>::type* x, int n1, int n2,
+                                                             int n, long total, int sign)
+{
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= total) return;
+    const long inner = i % ((long)n1 * n2);
+    const int k1 = (int)(inner / n2), c = (int)(inner % n2);
+    const long q = ((long)k1 * c) % n;
+    double sn, cs;
+    sincos(sign * 2.0 * M_PI * (double)q / (double)n, &sn, &cs);
+    auto v = x[i];
+    const T wr = (T)cs, wi = (T)sn;
+    const T xr = v.x * wr - v.y * wi, xi = v.x * wi + v.y * wr;
+    v.x = xr;
+    v.y = xi;
+    x[i] = v;
+}
+
+// Largest row length the single-pass LDS kernel takes.
+template <typename T> constexpr int lds_row_limit() { return sizeof(T) == 8 ? 4096 : 8192; }
+
+// Batched row transforms of any supported length: one LDS pass when the
+// row fits, otherwise the four-step split n = n1 * n2 (column
+// transforms of length n1, twiddles, row transforms of length n2, then a
+// transpose of each n1 x n2 block).  Rows are contiguous (dist == n).
+template <typename T>
+int launch_rows(const void* in, void* out, int n, int nrows, long dist, int sign, T scale,
+                hipStream_t s)
+{
+    using T2 = typename C2<T>::type;
+    if (n <= lds_row_limit<T>()) return launch_rows_lds<T>(in, out, n, nrows, dist, sign, scale, s);
+    WCT_REQUIRE(dist == n, WCT_ERR_UNSUPPORTED, "strided rows above the LDS limit");
+    WCT_REQUIRE(fft_supported_length(n), WCT_ERR_UNSUPPORTED,
+                "FFT length %d has a prime factor above 13", n);
+    // n2: largest divisor within the row limit with n1 = n / n2 small enough for a column tile
+    int n2 = 0;
+    for (int d = lds_row_limit<T>() / 2; d >= 2; --d)
+        if (n % d == 0 && n / d <= 2048) { n2 = d; break; }
+    WCT_REQUIRE(n2 > 0, WCT_ERR_UNSUPPORTED, "no four-step split for length %d", n);
+    const int n1 = n / n2;
+    const long total = (long)nrows * n;
+    T2* o = (T2*)out;
+    int rc = launch_cols<T>(in, out, n1, n2, sign, T(1), s, nrows, (long)n);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_fourstep_twiddle<T>), dim3((total + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                       s, o, n1, n2, n, total, sign);
+    WCT_HIP_CHECK(hipGetLastError());
+    rc = launch_rows_lds<T>(out, out, n2, nrows * n1, n2, sign, scale, s);
+    if (rc) return rc;
+    T2* tmp = nullptr;
+    WCT_HIP_CHECK(hipMallocAsync((void**)&tmp, total * sizeof(T2), s));
+    dim3 grid((n2 + 31) / 32, (n1 + 31) / 32, nrows);
+    hipLaunchKernelGGL((k_transpose<T2>), grid, dim3(kBlock), 0, s, (const T2*)o, tmp, n1, n2);
+    WCT_HIP_CHECK(hipGetLastError());
+    WCT_HIP_CHECK(hipMemcpyAsync(o, tmp, total * sizeof(T2), hipMemcpyDeviceToDevice, s));
+    WCT_HIP_CHECK(hipFreeAsync(tmp, s));
+    return WCT_OK;
+}
+template int launch_rows<float>(const void*, void*, int, int, long, int, float, hipStream_t);
+template int launch_rows<double>(const void*, void*, int, int, long, int, double, hipStream_t);
+
+template <typename T>
+int launch_cols_any(const void* in, void* out, int nrows, int ncols, int sign, T scale, hipStream_t s)
+{
+    return launch_cols<T>(in, out, nrows, ncols, sign, scale, s);
+}
+template int launch_cols_any<float>(const void*, void*, int, int, int, float, hipStream_t);
+template int launch_cols_any<double>(const void*, void*, int, int, int, double, hipStream_t);
+
+template <typename T>
+static int dft_1b(const T* in, T* out, int nrows, int ncols, int axis, int sign, hipStream_t s)
+{
+    WCT_REQUIRE(in && out && nrows > 0 && ncols > 0 && (axis == 0 || axis == 1), WCT_ERR_ARG,
+                "dft 1b: bad arguments (nrows=%d ncols=%d axis=%d)", nrows, ncols, axis);
+    if (axis == 1) {
+        const T scale = sign > 0 ? T(1) / T(ncols) : T(1);
+        return launch_rows<T>(in, out, ncols, nrows, ncols, sign, scale, s);
+    }
+    const T scale = sign > 0 ? T(1) / T(nrows) : T(1);
+    return launch_cols<T>(in, out, nrows, ncols, sign, scale, s);
+}
+
+}  // namespace wct
+
+using namespace wct;
+
+extern "C" {
+
+int wct_dft_fwd1d(const float* in, float* out, int size, void* stream)
+{
+    WCT_REQUIRE(in && out && size > 0, WCT_ERR_ARG, "fwd1d: bad arguments");
+    return launch_rows<float>(in, out, size, 1, size, -1, 1.0f, as_stream(stream));
+}
+
+int wct_dft_inv1d(const float* in, float* out, int size, void* stream)
+{
+    WCT_REQUIRE(in && out && size > 0, WCT_ERR_ARG, "inv1d: bad arguments");
+    return launch_rows<float>(in, out, size, 1, size, +1, 1.0f / (float)size, as_stream(stream));
+}
+
+int wct_dft_fwd1b(const float* in, float* out, int nrows, int ncols, int axis, void* stream)
+{
+    return dft_1b<float>(in, out, nrows, ncols, axis, -1, as_stream(stream));
+}
+
+int wct_dft_inv1b(const float* in, float* out, int nrows, int ncols, int axis, void* stream)
+{
+    return dft_1b<float>(in, out, nrows, ncols, axis, +1, as_stream(stream));
+}
+
+int wct_dft_fwd1b_f64(const double* in, double* out, int nrows, int ncols, int axis, void* stream)
+{
+    return dft_1b<double>(in, out, nrows, ncols, axis, -1, as_stream(stream));
+}
+
+int wct_dft_inv1b_f64(const double* in, double* out, int nrows, int ncols, int axis, void* stream)
+{
+    return dft_1b<double>(in, out, nrows, ncols, axis, +1, as_stream(stream));
+}
+
+// FftwDFT plans fftwf_plan_dft_2d(ncols, nrows): the buffer is an
+// (R = ncols) x (C = nrows) row-major array for the transform.
+int wct_dft_fwd2d(const float* in, float* out, int nrows, int ncols, void* stream)
+{
+    WCT_REQUIRE(in && out && nrows > 0 && ncols > 0, WCT_ERR_ARG, "fwd2d: bad arguments");
+    const int R = ncols, C = nrows;
+    hipStream_t s = as_stream(stream);
+    int rc = launch_rows<float>(in, out, C, R, C, -1, 1.0f, s);
+    if (rc) return rc;
+    return launch_cols<float>(out, out, R, C, -1, 1.0f, s);
+}
+
+int wct_dft_inv2d(const float* in, float* out, int nrows, int ncols, void* stream)
+{
+    WCT_REQUIRE(in && out && nrows > 0 && ncols > 0, WCT_ERR_ARG, "inv2d: bad arguments");
+    const int R = ncols, C = nrows;
+    hipStream_t s = as_stream(stream);
+    int rc = launch_rows<float>(in, out, C, R, C, +1, 1.0f / (float)C, s);
+    if (rc) return rc;
+    return launch_cols<float>(out, out, R, C, +1, 1.0f / (float)R, s);
+}
+
+int wct_dft_transpose_real(const float* in, float* out, int nrows, int ncols, void* stream)
+{
+    WCT_REQUIRE(in && out && nrows > 0 && ncols > 0, WCT_ERR_ARG, "transpose: bad arguments");
+    return launch_transpose<float>(in, out, nrows, ncols, as_stream(stream));
+}
+
+int wct_dft_transpose_complex(const float* in, float* out, int nrows, int ncols, void* stream)
+{
+    WCT_REQUIRE(in && out && nrows > 0 && ncols > 0, WCT_ERR_ARG, "transpose: bad arguments");
+    return launch_transpose<float2>((const float2*)in, (float2*)out, nrows, ncols,
+                                    as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,251 @@
+// LDS-resident mixed-radix Stockham FFT for gfx950, shared by the IDFT
+// kernels and the fused deconvolution kernels.
+//
+// One transform of length n lives in LDS (T2 = float2 or double2).  A
+// group of NT threads (a whole workgroup or a slice of it) walks the
+// passes; each pass loads every butterfly's inputs into registers,
+// synchronises, then twiddles, runs the radix-R DFT and stores in
+// autosorted (Stockham) order, so no bit reversal is ever needed.
+//
+// Pass p with radix R and span Ns (product of the earlier radices):
+//   for j < n/R:  k = j mod Ns
+//     v[r]  = x[j + r n/R] * w^{r k},  w = exp(sign 2 pi i / (Ns R))
+//     v     = DFT_R(v)
+//     y[(j-k) R + k + r Ns] = v[r]
+// Twiddles come from a per-length table built on the host in double and
+// rounded to T once (forward sign; conjugated for the inverse).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace wct {
+
+constexpr int kMaxPasses = 24;
+
+// Radix schedule of one length.  Passed by value as a kernel argument.
+struct FftPlan {
+    int n;
+    int npass;
+    int rad[kMaxPasses];
+    int twoff[kMaxPasses];   // offset (in complex entries) of each pass's twiddles
+    int dftoff;              // offset of the generic small-DFT tables (7, 11, 13)
+};
+
+template <typename T> struct C2;
+template <> struct C2<float> { using type = float2; };
+template <> struct C2<double> { using type = double2; };
+
+template <typename T2>
+__device__ __forceinline__ T2 cmul(T2 a, T2 b)
+{
+    T2 r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+template <typename T2>
+__device__ __forceinline__ T2 cadd(T2 a, T2 b) { T2 r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename T2>
+__device__ __forceinline__ T2 csub(T2 a, T2 b) { T2 r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+// i * sign * a
+template <typename T2>
+__device__ __forceinline__ T2 cisgn(T2 a, int sign)
+{
+    T2 r;
+    if (sign > 0) { r.x = -a.y; r.y = a.x; }
+    else { r.x = a.y; r.y = -a.x; }
+    return r;
+}
+
+// ---- radix-R DFT kernels on registers ---------------------------------
+
+template <typename T2, int R> struct Dft;
+
+template <typename T2> struct Dft<T2, 2> {
+    static __device__ __forceinline__ void run(T2* v, int, const T2*)
+    {
+        T2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <typename T2> struct Dft<T2, 3> {
+    static __device__ __forceinline__ void run(T2* v, int sign, const T2*)
+    {
+        using T = decltype(v[0].x);
+        const T h = T(0.86602540378443864676);
+        T2 s = cadd(v[1], v[2]);
+        T2 d = csub(v[1], v[2]);
+        T2 m;
+        m.x = v[0].x - T(0.5) * s.x;
+        m.y = v[0].y - T(0.5) * s.y;
+        T2 rot = cisgn(d, sign);      // i*sign*d
+        v[0] = cadd(v[0], s);
+        v[1].x = m.x + h * rot.x; v[1].y = m.y + h * rot.y;
+        v[2].x = m.x - h * rot.x; v[2].y = m.y - h * rot.y;
+    }
+};
+
+template <typename T2> struct Dft<T2, 4> {
+    static __device__ __forceinline__ void run(T2* v, int sign, const T2*)
+    {
+        T2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        T2 t2 = cadd(v[1], v[3]), t3 = cisgn(csub(v[1], v[3]), sign);
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    }
+};
+
+template <typename T2> struct Dft<T2, 5> {
+    static __device__ __forceinline__ void run(T2* v, int sign, const T2*)
+    {
+        using T = decltype(v[0].x);
+        const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
+        const T s1 = T(0.95105651629515357212), s2 = T(0.58778525229247312917);
+        T2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+        T2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+        T2 a1, a2, b1, b2;
+        a1.x = v[0].x + c1 * t1.x + c2 * t2.x; a1.y = v[0].y + c1 * t1.y + c2 * t2.y;
+        a2.x = v[0].x + c2 * t1.x + c1 * t2.x; a2.y = v[0].y + c2 * t1.y + c1 * t2.y;
+        b1.x = s1 * t3.x + s2 * t4.x; b1.y = s1 * t3.y + s2 * t4.y;
+        b2.x = s2 * t3.x - s1 * t4.x; b2.y = s2 * t3.y - s1 * t4.y;
+        v[0] = cadd(v[0], cadd(t1, t2));
+        T2 r1 = cisgn(b1, sign), r2 = cisgn(b2, sign);
+        v[1] = cadd(a1, r1);
+        v[4] = csub(a1, r1);
+        v[2] = cadd(a2, r2);
+        v[3] = csub(a2, r2);
+    }
+};
+
+template <typename T2> struct Dft<T2, 8> {
+    static __device__ __forceinline__ void run(T2* v, int sign, const T2*)
+    {
+        using T = decltype(v[0].x);
+        const T r2 = T(0.70710678118654752440);
+        T2 e[4] = {v[0], v[2], v[4], v[6]};
+        T2 o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<T2, 4>::run(e, sign, nullptr);
+        Dft<T2, 4>::run(o, sign, nullptr);
+        // w8^1 = (1 + i sign)/sqrt2, w8^2 = i sign, w8^3 = (-1 + i sign)/sqrt2
+        T2 o1, o2, o3;
+        {
+            T2 is = cisgn(o[1], sign);
+            o1.x = r2 * (o[1].x + is.x); o1.y = r2 * (o[1].y + is.y);
+        }
+        o2 = cisgn(o[2], sign);
+        {
+            T2 is = cisgn(o[3], sign);
+            o3.x = r2 * (is.x - o[3].x); o3.y = r2 * (is.y - o[3].y);
+        }
+        v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);   v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);   v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);   v[7] = csub(e[3], o3);
+    }
+};
+
+// Generic odd prime radix (7, 11, 13) from a small table of
+// exp(-2 pi i q / R), q < R (forward sign).
+template <typename T2, int R> struct DftGeneric {
+    static __device__ __forceinline__ void run(T2* v, int sign, const T2* tab)
+    {
+        T2 out[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            T2 acc = v[0];
+#pragma unroll
+            for (int j = 1; j < R; ++j) {
+                T2 w = tab[(j * k) % R];
+                if (sign > 0) w.y = -w.y;
+                T2 p = cmul(v[j], w);
+                acc = cadd(acc, p);
+            }
+            out[k] = acc;
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) v[k] = out[k];
+    }
+};
+template <typename T2> struct Dft<T2, 7> : DftGeneric<T2, 7> {};
+template <typename T2> struct Dft<T2, 11> : DftGeneric<T2, 11> {};
+template <typename T2> struct Dft<T2, 13> : DftGeneric<T2, 13> {};
+
+// Offset of radix R's small table inside the plan's generic-table block.
+__host__ __device__ constexpr int generic_tab_offset(int R)
+{
+    return R == 7 ? 0 : (R == 11 ? 7 : (R == 13 ? 18 : 0));
+}
+constexpr int kGenericTabSize = 7 + 11 + 13;
+
+// ---- one Stockham pass ------------------------------------------------
+// buf: LDS sequence of length n.  The group has NT threads, this thread
+// is tid.  MAXB bounds the butterflies one thread owns in this pass.
+
+template <typename T2, int R, int NT, int MAXB>
+__device__ __forceinline__ void stockham_pass(T2* buf, int n, int Ns, const T2* __restrict__ tw,
+                                              const T2* __restrict__ gtab, int sign, int tid)
+{
+    const int m = n / R;
+    T2 v[MAXB][R];
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int j = tid + b * NT;
+        if (j < m) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * m];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int j = tid + b * NT;
+        if (j < m) {
+            const int k = j % Ns;
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    T2 w = tw[k * (R - 1) + (r - 1)];
+                    if (sign > 0) w.y = -w.y;
+                    v[b][r] = cmul(v[b][r], w);
+                }
+            }
+            Dft<T2, R>::run(v[b], sign, gtab + generic_tab_offset(R));
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Full transform of the LDS sequence buf (length plan.n <= NMAX).
+// Every thread of the workgroup must call this (it contains barriers);
+// threads with tid >= NT (when the workgroup is split into slices) pass
+// their own slice index through buf/tid.
+template <typename T2, int NT, int NMAX>
+__device__ void fft_lds(T2* buf, const FftPlan& pl, const T2* __restrict__ tw, int sign, int tid)
+{
+    const T2* gtab = tw + pl.dftoff;
+    int Ns = 1;
+    for (int p = 0; p < pl.npass; ++p) {
+        const int R = pl.rad[p];
+        const T2* t = tw + pl.twoff[p];
+        switch (R) {
+        case 2: stockham_pass<T2, 2, NT, (NMAX / 2 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
+        case 3: stockham_pass<T2, 3, NT, (NMAX / 3 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
+        case 4: stockham_pass<T2, 4, NT, (NMAX / 4 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
+        case 5: stockham_pass<T2, 5, NT, (NMAX / 5 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
+        case 8: stockham_pass<T2, 8, NT, (NMAX / 8 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
+        case 7: stockham_pass<T2, 7, NT, (NMAX / 7 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
+        case 11: stockham_pass<T2, 11, NT, (NMAX / 11 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
+        case 13: stockham_pass<T2, 13, NT, (NMAX / 13 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
+        default: break;
+        }
+        Ns *= R;
+    }
+}
+
+}  // namespace wct

@@ -1,0 +1,91 @@
+"""ctypes declarations of include/wct_hip.h."""
+import ctypes as C
+import os
+
+_PKG = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+LIB_PATH = os.path.join(_PKG, "libwct_hip.so")
+_LIB = None
+
+
+class WctError(RuntimeError):
+    pass
+
+
+class DeconJob(C.Structure):
+    _fields_ = [
+        ("kernel", C.c_void_p),
+        ("adc", C.c_void_p), ("ld_adc", C.c_int),
+        ("out", C.c_void_p), ("ld_out", C.c_int),
+        ("mask", C.c_void_p), ("ld_mask", C.c_int),
+        ("rms", C.c_void_p),
+        ("bad_ptr", C.c_void_p), ("bad_lo", C.c_void_p), ("bad_hi", C.c_void_p),
+        ("lf_tight", C.c_void_p), ("lf_tighter", C.c_void_p),
+        ("th_factor", C.c_float),
+        ("restore_baseline", C.c_int),
+        ("save_mode", C.c_int),
+    ]
+
+
+# name -> (restype, argtypes)
+_V, _I, _D, _F, _Z = C.c_void_p, C.c_int, C.c_double, C.c_float, C.c_size_t
+_SIGS = {
+    "wct_last_error": (C.c_char_p, []),
+    "wct_abi_version": (_I, []),
+    "wct_device_count": (_I, [C.POINTER(C.c_int)]),
+    "wct_set_device": (_I, [_I]),
+    "wct_malloc": (_I, [C.POINTER(C.c_void_p), _Z]),
+    "wct_free": (_I, [_V]),
+    "wct_memcpy_htod": (_I, [_V, _V, _Z, _V]),
+    "wct_memcpy_dtoh": (_I, [_V, _V, _Z, _V]),
+    "wct_memcpy_dtod": (_I, [_V, _V, _Z, _V]),
+    "wct_memset": (_I, [_V, _I, _Z, _V]),
+    "wct_stream_create": (_I, [C.POINTER(C.c_void_p)]),
+    "wct_stream_destroy": (_I, [_V]),
+    "wct_stream_sync": (_I, [_V]),
+    "wct_dft_fwd1d": (_I, [_V, _V, _I, _V]),
+    "wct_dft_inv1d": (_I, [_V, _V, _I, _V]),
+    "wct_dft_fwd1b": (_I, [_V, _V, _I, _I, _I, _V]),
+    "wct_dft_inv1b": (_I, [_V, _V, _I, _I, _I, _V]),
+    "wct_dft_fwd2d": (_I, [_V, _V, _I, _I, _V]),
+    "wct_dft_inv2d": (_I, [_V, _V, _I, _I, _V]),
+    "wct_dft_transpose_real": (_I, [_V, _V, _I, _I, _V]),
+    "wct_dft_transpose_complex": (_I, [_V, _V, _I, _I, _V]),
+    "wct_dft_fwd1b_f64": (_I, [_V, _V, _I, _I, _I, _V]),
+    "wct_dft_inv1b_f64": (_I, [_V, _V, _I, _I, _I, _V]),
+    "wct_dft_supported": (_I, [_I]),
+    "wct_overall_response": (_I, [_V, _I, _I, _V, _I, _D, _D, _D, _D, _D, _I, _V, _V]),
+    "wct_decon_kernel_create": (_I, [C.POINTER(C.c_void_p), _V, _I, _I, _I, _V, _V, _I, _I, _D, _V]),
+    "wct_decon_kernel_info": (_I, [_V, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "wct_decon_kernel_download": (_I, [_V, _V]),
+    "wct_decon_kernel_destroy": (_I, [_V]),
+    "wct_decon_workspace_size": (_Z, [C.POINTER(DeconJob), _I]),
+    "wct_decon_run": (_I, [C.POINTER(DeconJob), _I, _V, _Z, _V]),
+}
+
+# Every symbol include/wct_hip.h declares (checked by the CPU tests).
+EXPORTED = sorted(_SIGS)
+MISSING = []
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise WctError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                MISSING.append(name)
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().wct_last_error().decode(errors="replace")
+        raise WctError(f"wct status {rc}: {msg}")
+    return rc

@@ -1,0 +1,112 @@
+"""Host-side driver of the fused deconvolution through the C ABI.
+
+PlaneKernel builds one plane type's FP64 response spectrum and its
+regularised inverse on the device (wct_overall_response +
+wct_decon_kernel_create); Batch runs any number of plane jobs with one
+launch per stage (wct_decon_run).  Buffers are torch tensors on the
+current device, used as plain device memory.
+"""
+import ctypes as C
+
+import numpy as np
+
+from .capi import DeconJob, check, lib
+from . import workload as W
+
+
+def _stream():
+    import torch
+    return torch.cuda.current_stream().cuda_stream
+
+
+def overall_response(spec, nt=None, stream=None):
+    """OmnibusSigProc::init_overall_response on the device in FP64."""
+    nt = spec.nt if nt is None else nt
+    rows = np.ascontiguousarray(spec.fr_rows, np.float64)
+    nr, nin = rows.shape
+    elec = np.ascontiguousarray(spec.elec, np.float64)
+    out = np.zeros((nr, nt))
+    check(lib().wct_overall_response(
+        rows.ctypes.data, nr, nin, elec.ctypes.data, elec.size, W.FINE_PERIOD, W.TICK,
+        W.INTER_GAIN, W.ADC_MV, W.FTOFFSET, nt, out.ctypes.data,
+        _stream() if stream is None else stream))
+    return out
+
+
+class PlaneKernel:
+    """Device-resident K(m, k) for one plane type (see include/wct_hip.h)."""
+
+    def __init__(self, spec, cut_tol=1e-7, time_filter=None, resp=None, stream=None):
+        self.spec = spec
+        self.resp = overall_response(spec) if resp is None else np.ascontiguousarray(resp, np.float64)
+        tf = spec.time_filter if time_filter is None else time_filter
+        tf = np.ascontiguousarray(tf, np.float64)
+        wf = np.ascontiguousarray(spec.wire_filter, np.float64)
+        h = C.c_void_p()
+        check(lib().wct_decon_kernel_create(
+            C.byref(h), self.resp.ctypes.data, self.resp.shape[0], spec.nw, spec.nt,
+            wf.ctypes.data, tf.ctypes.data, spec.wire_shift, spec.time_shift, cut_tol,
+            _stream() if stream is None else stream))
+        self.handle = h
+        nw, nt, kc = C.c_int(), C.c_int(), C.c_int()
+        check(lib().wct_decon_kernel_info(h, C.byref(nw), C.byref(nt), C.byref(kc)))
+        self.nw, self.nt, self.kc = nw.value, nt.value, kc.value
+
+    def download(self):
+        buf = np.zeros(2 * self.kc * self.nw, np.float32)
+        check(lib().wct_decon_kernel_download(self.handle, buf.ctypes.data))
+        return buf.view(np.complex64).reshape(self.kc, self.nw)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().wct_decon_kernel_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _ptr(t):
+    return None if t is None else t.data_ptr()
+
+
+def make_job(kernel, adc, out, mask=None, rms=None, th_factor=W.TH_FACTOR, restore=1,
+             save_mode=0, lf_tight=None, lf_tighter=None, bad=None):
+    """One wct_decon_job.  adc/out/mask/rms are torch tensors (2D for the
+    frames); bad is (ptr, lo, hi) int32 device tensors or None."""
+    j = DeconJob()
+    j.kernel = kernel.handle
+    j.adc = adc.data_ptr()
+    j.ld_adc = adc.stride(0)
+    j.out = out.data_ptr()
+    j.ld_out = out.stride(0)
+    j.mask = _ptr(mask)
+    j.ld_mask = mask.stride(0) if mask is not None else 0
+    j.rms = _ptr(rms)
+    if bad is not None:
+        j.bad_ptr, j.bad_lo, j.bad_hi = (_ptr(b) for b in bad)
+    j.lf_tight = _ptr(lf_tight)
+    j.lf_tighter = _ptr(lf_tighter)
+    j.th_factor = th_factor
+    j.restore_baseline = restore
+    j.save_mode = save_mode
+    return j
+
+
+class Batch:
+    """A fixed list of plane jobs and its workspace; run() launches the
+    three stages once over all of them."""
+
+    def __init__(self, jobs, device=None):
+        import torch
+        self.jobs = (DeconJob * len(jobs))(*jobs)
+        self.n = len(jobs)
+        self.ws_bytes = lib().wct_decon_workspace_size(self.jobs, self.n)
+        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=device or "cuda")
+
+    def run(self, stream=None):
+        check(lib().wct_decon_run(self.jobs, self.n, self.ws.data_ptr(), self.ws_bytes,
+                                  _stream() if stream is None else stream))

@@ -1,0 +1,75 @@
+"""ctypes declarations of include/wct_host.h (libWireCellHip.so)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .capi import WctError, _PKG
+
+HOST_LIB_PATH = os.path.join(_PKG, "libWireCellHip.so")
+_LIB = None
+_f64 = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i32 = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_SIGS = {
+    "wcth_coldelec_samples": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, _f64]),
+    "wcth_hf_filter": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _f64]),
+    "wcth_lf_filter": (C.c_int, [C.c_int, C.c_double, C.c_double, _f64]),
+    "wcth_fft_best_length": (C.c_long, [C.c_long, C.c_int]),
+    "wcth_wire_region_average": (C.c_int, [C.c_int, _f64, _f64, C.c_int, C.c_double, C.c_int,
+                                           _i32, _f64]),
+}
+EXPORTED = sorted(_SIGS)
+MISSING = []
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(HOST_LIB_PATH):
+            raise WctError(f"host library not built: {HOST_LIB_PATH}")
+        L = C.CDLL(HOST_LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                MISSING.append(name)
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def coldelec_samples(n, tmin, tmax, gain, shaping, postgain):
+    out = np.zeros(n)
+    if lib().wcth_coldelec_samples(n, tmin, tmax, gain, shaping, postgain, out):
+        raise WctError("coldelec_samples: bad arguments")
+    return out
+
+
+def hf_filter(n, max_freq, sigma, power, flag):
+    out = np.zeros(n)
+    lib().wcth_hf_filter(n, max_freq, sigma, power, int(bool(flag)), out)
+    return out
+
+
+def lf_filter(n, max_freq, tau):
+    out = np.zeros(n)
+    lib().wcth_lf_filter(n, max_freq, tau, out)
+    return out
+
+
+def fft_best_length(n, keep_odd_even=False):
+    return int(lib().wcth_fft_best_length(n, int(keep_odd_even)))
+
+
+def wire_region_average(pitchpos, current, pitch, max_regions=256):
+    pp = np.ascontiguousarray(pitchpos, np.float64)
+    cur = np.ascontiguousarray(current, np.float64)
+    npaths, nsamp = cur.shape
+    regs = np.zeros(max_regions, np.int32)
+    avg = np.zeros((max_regions, nsamp))
+    n = lib().wcth_wire_region_average(npaths, pp, cur, nsamp, pitch, max_regions, regs, avg)
+    if n < 0:
+        raise WctError("wire_region_average failed")
+    return regs[:n].copy(), avg[:n].copy()

@@ -1,0 +1,115 @@
+"""Synthetic planes of the benchmark workloads (WCT units: ns, mm, e+).
+
+Field responses are Garfield-style path sets at pitch positions
+-D..D pitches (the full symmetric set: Response::wire_region_average,
+util/src/Response.cxx:79-196, only sizes its averages when a position
+repeats).  Collection: a unipolar Gaussian current, sigma = 3 ticks,
+neighbour d scaled by 1/(1+d^2), d <= 5.  Induction: the time
+derivative of that Gaussian (positive lobe of unit area), d <= 10.
+Electronics: cold electronics, 14 mV/fC, 2 us; ADC_mV = 4096/2000 mV;
+postgain 1.  Decon filters: Wiener tight sigma 0.1 MHz with the PDSP
+Wiener_tight_W power 3.35324 (cfg/pgrapher/experiment/pdsp/
+sp-filters.jsonnet:95-98), induction adds ROI_tight_lf (0.014 MHz) and
+the mask's ROI_tighter_lf (0.06 MHz); wire filter sigma 0.75, power 2.
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import host
+
+US = 1000.0
+MHZ = 1e-3
+MV = 1e-9
+FC = 1.0e-15 / 1.602176487e-19
+
+TICK = 0.5 * US
+FINE_PERIOD = 0.1 * US
+FR_SAMPLES = 1000
+PITCH = 4.0
+FR_PEAK = 30.0 * US
+ORIGIN = 48.0                 # mm
+SPEED = 1.6 / US              # mm/ns  -> origin/speed = 30 us
+CTOFFSET = 0.0
+FTOFFSET = 0.0
+GAIN = 14.0 * MV / FC
+SHAPING = 2.0 * US
+ADC_MV = 4096 / (2000.0 * MV)
+INTER_GAIN = 1.0
+SIGMA_T = 3 * TICK
+
+WIENER = dict(max_freq=1 * MHZ, sigma=0.1 * MHZ, power=3.35324, flag=True)
+WIRE = dict(max_freq=1.0, sigma=0.75, power=2, flag=False)
+LF_TIGHT = dict(max_freq=1 * MHZ, tau=0.014 * MHZ)
+LF_TIGHTER = dict(max_freq=1 * MHZ, tau=0.06 * MHZ)
+TH_FACTOR = 3.0
+NOISE_ADC = 1.5
+Q_PER_TICK = 5000.0
+
+
+def fr_paths(kind):
+    """(pitchpos[], current[npaths, FR_SAMPLES]) for 'col' or 'ind'."""
+    dmax = 5 if kind == "col" else 10
+    t = np.arange(FR_SAMPLES) * FINE_PERIOD
+    g = np.exp(-0.5 * ((t - FR_PEAK) / SIGMA_T) ** 2) / (SIGMA_T * math.sqrt(2 * math.pi))
+    if kind == "ind":
+        g = SIGMA_T * math.sqrt(2 * math.pi) * (-(t - FR_PEAK) / SIGMA_T ** 2) * g
+    pos, cur = [], []
+    for d in range(-dmax, dmax + 1):
+        pos.append(d * PITCH)
+        cur.append(-g / (1.0 + d * d))
+    return np.array(pos), np.array(cur)
+
+
+@dataclass
+class PlaneSpec:
+    kind: str            # "col" or "ind"
+    nw: int
+    nt: int
+    fr_rows: np.ndarray = field(repr=False, default=None)   # region-averaged rows
+    elec: np.ndarray = field(repr=False, default=None)
+    nfine: int = 0
+    wire_shift: int = 0
+    time_shift: int = 0
+    wire_filter: np.ndarray = field(repr=False, default=None)
+    time_filter: np.ndarray = field(repr=False, default=None)   # Wiener (x tight LF if folded)
+    lf_tight: np.ndarray = field(repr=False, default=None)
+    lf_tighter: np.ndarray = field(repr=False, default=None)
+
+
+def plane_spec(kind, nw, nt, fold_lf=True):
+    """All host-side inputs of one plane's decon (OmnibusSigProc.cxx:707-865)."""
+    pos, cur = fr_paths(kind)
+    regions, rows = host.wire_region_average(pos, cur, PITCH)
+    nfine = host.fft_best_length(FR_SAMPLES)
+    elec = host.coldelec_samples(nfine, 0.0, nfine * FINE_PERIOD, GAIN, SHAPING, 1.0)
+    sp = PlaneSpec(kind=kind, nw=nw, nt=nt, fr_rows=rows, elec=elec, nfine=nfine)
+    sp.wire_shift = (len(regions) - 1) // 2
+    sp.time_shift = int((CTOFFSET + ORIGIN / SPEED) / TICK)
+    sp.wire_filter = host.hf_filter(nw, **WIRE)
+    wiener = host.hf_filter(nt, **WIENER)
+    if kind == "ind":
+        lt = host.lf_filter(nt, **LF_TIGHT)
+        sp.lf_tight = lt
+        sp.lf_tighter = host.lf_filter(nt, **LF_TIGHTER)
+        sp.time_filter = wiener * lt if fold_lf else wiener
+    else:
+        sp.time_filter = wiener
+    return sp
+
+
+# Plane lists of the workloads.
+APA_PLANES = (("ind", 800), ("ind", 800), ("col", 960))
+
+
+def config_planes(name):
+    if name == "col480":          # one collection plane, 480 x 6000
+        return [("col", 480, 6000)]
+    if name == "apa":             # one APA, U/V/W 800/800/960 x 6000
+        return [(k, nw, 6000) for k, nw in APA_PLANES]
+    if name == "batch24":         # 8 APAs x 3 planes, Nt cycling 6000/8000/10000
+        return [(k, nw, (6000, 8000, 10000)[a % 3]) for a in range(8) for k, nw in APA_PLANES]
+    if name == "apa150":          # 150 APAs x 2560 channels x 6000 ticks
+        return [(k, nw, 6000) for a in range(150) for k, nw in APA_PLANES]
+    raise KeyError(name)
