@@ -104,6 +104,14 @@ typedef struct wct_decon_kernel_s* wct_decon_kernel;
 int wct_decon_kernel_create(wct_decon_kernel* out, const double* resp, int nr, int nw, int nt,
                             const double* wire_filter, const double* time_filter,
                             int wire_shift, int time_shift, double cut_tol, void* stream);
+/* The forward response convolution with the same layout, for synthetic
+ * frames and the adjoint's forward model: the job output is
+ *   D = IFFT2( FFT2(q) R exp(+2 pi i (m ws/nw + k ts/nt)) )
+ * i.e. the frame whose decon with unit filters returns q (the field x
+ * electronics convolution of gen/src/ImpactTransform.cxx:100-308 on the
+ * region-averaged response). */
+int wct_forward_kernel_create(wct_decon_kernel* out, const double* resp, int nr, int nw, int nt,
+                              int wire_shift, int time_shift, void* stream);
 int wct_decon_kernel_info(wct_decon_kernel k, int* nw, int* nt, int* kc);
 /* Copies K (kc x nw complex, k-major) to a host buffer of 2*kc*nw floats. */
 int wct_decon_kernel_download(wct_decon_kernel k, float* host_out);
@@ -147,7 +155,18 @@ typedef struct {
 
 /* Device scratch needed to run the jobs (spectrum buffers). */
 size_t wct_decon_workspace_size(const wct_decon_job* jobs, int njobs);
-/* Runs all jobs: one launch per stage over every plane of the batch.
+/* A batch of jobs bound to a workspace: the job table is validated and
+ * uploaded once (synchronously); wct_decon_plan_run then only enqueues
+ * one launch per stage over every plane of the batch (no allocation, no
+ * copy), so it can be replayed and captured in a hipGraph.  The plan
+ * keeps the device pointers of the jobs; runs that share a workspace
+ * must be ordered on one stream. */
+typedef struct wct_decon_plan_s* wct_decon_plan;
+int wct_decon_plan_create(wct_decon_plan* out, const wct_decon_job* jobs, int njobs,
+                          void* workspace, size_t ws_bytes);
+int wct_decon_plan_run(wct_decon_plan plan, void* stream);
+int wct_decon_plan_destroy(wct_decon_plan plan);
+/* One-shot convenience: create a plan, run it, wait, destroy it.
  * jobs is a host array; workspace is device memory of at least
  * wct_decon_workspace_size bytes. */
 int wct_decon_run(const wct_decon_job* jobs, int njobs, void* workspace, size_t ws_bytes,

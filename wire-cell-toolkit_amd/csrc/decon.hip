@@ -28,6 +28,15 @@ struct wct_decon_kernel_s {
     float2* K;          // kc x nw, k-major
 };
 
+namespace wct { struct DJob; }
+
+struct wct_decon_plan_s {
+    wct::DJob* jobs = nullptr;   // device job table
+    int njobs = 0;
+    long rows = 0, tiles = 0;
+    int Mmax = 0, nwmax = 0, tk = 8;
+};
+
 namespace wct {
 
 constexpr int kNT = 256;
@@ -247,7 +256,8 @@ __global__ __launch_bounds__(NT) void k_row_c2r_roi(const DJob* __restrict__ job
         val[2 * i + 1] = z.y;
     }
     pm = valid;
-    if (has_bad) {
+    const bool stats = J.restore || J.mask || J.rms;
+    if (stats && has_bad) {
 #pragma unroll
         for (int i = 0; i < VPT; ++i) {
             const int pos = 2 * (threadIdx.x + (i / 2) * NT) + (i & 1);
@@ -255,7 +265,7 @@ __global__ __launch_bounds__(NT) void k_row_c2r_roi(const DJob* __restrict__ job
         }
     }
     if (J.restore) row_restore_baseline<NT, VPT>(val, valid, &sh);
-    const float rms = row_cal_rms<NT, VPT>(val, pm, &sh);
+    const float rms = stats ? row_cal_rms<NT, VPT>(val, pm, &sh) : 0.0f;
     const double th = (double)J.th_factor * (double)rms + 1.0;
 
     unsigned char mbits[VPT];
@@ -340,6 +350,27 @@ __global__ void k_build_K(const double2* __restrict__ R, int nw, int nt, int kh,
         out.x = pr * g;
         out.y = pi * g;
     }
+    Kd[i] = out;
+    Kabs[i] = sqrt(out.x * out.x + out.y * out.y);
+}
+
+// Forward response spectrum for the synthetic-frame generator and the
+// adjoint's forward model: K = R exp(+2 pi i (m ws/nw + k ts/nt)) / (nw nt).
+__global__ void k_build_fwd(const double2* __restrict__ R, int nw, int nt, int kh, int wshift,
+                            int tshift, double2* __restrict__ Kd, double* __restrict__ Kabs)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)kh * nw) return;
+    const int k = (int)(i / nw), m = (int)(i % nw);
+    const double2 r = R[(long)m * nt + k];
+    const long qm = ((long)m * wshift) % nw, qk = ((long)k * tshift) % nt;
+    const double ang = 2.0 * M_PI * ((double)qm / nw + (double)qk / nt);
+    double sn, cs;
+    sincos(ang, &sn, &cs);
+    const double g = 1.0 / ((double)nw * (double)nt);
+    double2 out;
+    out.x = (r.x * cs - r.y * sn) * g;
+    out.y = (r.x * sn + r.y * cs) * g;
     Kd[i] = out;
     Kabs[i] = sqrt(out.x * out.x + out.y * out.y);
 }
@@ -490,11 +521,12 @@ int wct_overall_response(const double* fr_avg, int nr, int nin, const double* el
     return WCT_OK;
 }
 
-int wct_decon_kernel_create(wct_decon_kernel* out, const double* resp, int nr, int nw, int nt,
-                            const double* wire_filter, const double* time_filter, int wire_shift,
-                            int time_shift, double cut_tol, void* stream)
+static int kernel_create(wct_decon_kernel* out, const double* resp, int nr, int nw, int nt,
+                         const double* wire_filter, const double* time_filter, int wire_shift,
+                         int time_shift, double cut_tol, bool forward, void* stream)
 {
-    WCT_REQUIRE(out && resp && wire_filter && time_filter && nr > 0 && nw > 0 && nt > 1,
+    WCT_REQUIRE(out && resp && nr > 0 && nw > 0 && nt > 1 &&
+                    (forward || (wire_filter && time_filter)),
                 WCT_ERR_ARG, "kernel_create: bad arguments");
     WCT_REQUIRE(nt % 2 == 0, WCT_ERR_UNSUPPORTED, "odd tick count %d not supported", nt);
     WCT_REQUIRE(fft_supported_length(nw) && fft_supported_length(nt / 2), WCT_ERR_UNSUPPORTED,
@@ -526,11 +558,16 @@ int wct_decon_kernel_create(wct_decon_kernel* out, const double* resp, int nr, i
     WCT_HIP_CHECK(hipMallocAsync((void**)&Kd, (size_t)kh * nw * sizeof(double2), s));
     WCT_HIP_CHECK(hipMallocAsync((void**)&Kabs, (size_t)kh * nw * sizeof(double), s));
     WCT_HIP_CHECK(hipMallocAsync((void**)&cmax, kh * sizeof(double), s));
-    WCT_HIP_CHECK(hipMemcpyAsync(Fw, wire_filter, nw * sizeof(double), hipMemcpyHostToDevice, s));
-    WCT_HIP_CHECK(hipMemcpyAsync(Ft, time_filter, nt * sizeof(double), hipMemcpyHostToDevice, s));
     const long nk = (long)kh * nw;
-    hipLaunchKernelGGL(k_build_K, dim3((nk + 255) / 256), dim3(256), 0, s, R, nw, nt, kh, Fw, Ft,
-                       wire_shift, time_shift > 0 ? time_shift : 0, Kd, Kabs);
+    if (forward) {
+        hipLaunchKernelGGL(k_build_fwd, dim3((nk + 255) / 256), dim3(256), 0, s, R, nw, nt, kh,
+                           wire_shift, time_shift > 0 ? time_shift : 0, Kd, Kabs);
+    } else {
+        WCT_HIP_CHECK(hipMemcpyAsync(Fw, wire_filter, nw * sizeof(double), hipMemcpyHostToDevice, s));
+        WCT_HIP_CHECK(hipMemcpyAsync(Ft, time_filter, nt * sizeof(double), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_build_K, dim3((nk + 255) / 256), dim3(256), 0, s, R, nw, nt, kh, Fw, Ft,
+                           wire_shift, time_shift > 0 ? time_shift : 0, Kd, Kabs);
+    }
     WCT_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_colmax, dim3(kh), dim3(kNT), 0, s, Kabs, nw, kh, cmax);
     WCT_HIP_CHECK(hipGetLastError());
@@ -569,6 +606,21 @@ int wct_decon_kernel_create(wct_decon_kernel* out, const double* resp, int nr, i
     WCT_HIP_CHECK(hipStreamSynchronize(s));
     *out = kk;
     return WCT_OK;
+}
+
+int wct_decon_kernel_create(wct_decon_kernel* out, const double* resp, int nr, int nw, int nt,
+                            const double* wire_filter, const double* time_filter, int wire_shift,
+                            int time_shift, double cut_tol, void* stream)
+{
+    return kernel_create(out, resp, nr, nw, nt, wire_filter, time_filter, wire_shift, time_shift,
+                         cut_tol, false, stream);
+}
+
+int wct_forward_kernel_create(wct_decon_kernel* out, const double* resp, int nr, int nw, int nt,
+                              int wire_shift, int time_shift, void* stream)
+{
+    return kernel_create(out, resp, nr, nw, nt, nullptr, nullptr, wire_shift, time_shift, 0.0,
+                         true, stream);
 }
 
 int wct_decon_kernel_info(wct_decon_kernel k, int* nw, int* nt, int* kc)
@@ -613,15 +665,15 @@ size_t wct_decon_workspace_size(const wct_decon_job* jobs, int njobs)
     return tot + 256;
 }
 
-int wct_decon_run(const wct_decon_job* jobs, int njobs, void* workspace, size_t ws_bytes,
-                  void* stream)
+int wct_decon_plan_create(wct_decon_plan* out, const wct_decon_job* jobs, int njobs,
+                          void* workspace, size_t ws_bytes)
 {
-    WCT_REQUIRE(jobs && njobs > 0, WCT_ERR_ARG, "decon_run: no jobs");
+    WCT_REQUIRE(out && jobs && njobs > 0, WCT_ERR_ARG, "decon_plan: no jobs");
+    *out = nullptr;
     const size_t need = wct_decon_workspace_size(jobs, njobs);
-    WCT_REQUIRE(need > 0, WCT_ERR_ARG, "decon_run: a job has no kernel");
+    WCT_REQUIRE(need > 0, WCT_ERR_ARG, "decon_plan: a job has no kernel");
     WCT_REQUIRE(workspace && ws_bytes >= need, WCT_ERR_ARG,
-                "decon_run: workspace of %zu bytes, %zu needed", ws_bytes, need);
-    hipStream_t s = as_stream(stream);
+                "decon_plan: workspace of %zu bytes, %zu needed", ws_bytes, need);
     std::vector<DJob> h(njobs);
     size_t off = 0;
     long rows = 0, tiles = 0;
@@ -630,15 +682,15 @@ int wct_decon_run(const wct_decon_job* jobs, int njobs, void* workspace, size_t 
         const wct_decon_job& j = jobs[i];
         const wct_decon_kernel K = j.kernel;
         WCT_REQUIRE(pick_tk(K->nw) == tk, WCT_ERR_UNSUPPORTED,
-                    "decon_run: planes of one batch need the same wire tile (%d vs %d wires)",
+                    "decon_plan: planes of one batch need the same wire tile (%d vs %d wires)",
                     jobs[0].kernel->nw, K->nw);
         WCT_REQUIRE(j.adc && j.out && j.ld_adc >= K->nt && j.ld_out >= K->nt && j.ld_adc % 2 == 0 &&
                         j.ld_out % 2 == 0 && (!j.mask || (j.ld_mask >= K->nt && j.ld_mask % 2 == 0)),
-                    WCT_ERR_ARG, "decon_run: job %d has bad pointers or strides", i);
+                    WCT_ERR_ARG, "decon_plan: job %d has bad pointers or strides", i);
         WCT_REQUIRE((reinterpret_cast<uintptr_t>(j.adc) & 7) == 0 &&
                         (reinterpret_cast<uintptr_t>(j.out) & 7) == 0 &&
                         (!j.mask || (reinterpret_cast<uintptr_t>(j.mask) & 1) == 0),
-                    WCT_ERR_ARG, "decon_run: job %d buffers are misaligned", i);
+                    WCT_ERR_ARG, "decon_plan: job %d buffers are misaligned", i);
         DJob& d = h[i];
         memset(&d, 0, sizeof(d));
         d.adc = j.adc;
@@ -686,35 +738,84 @@ int wct_decon_run(const wct_decon_job* jobs, int njobs, void* workspace, size_t 
         Mmax = std::max(Mmax, d.M);
         nwmax = std::max(nwmax, d.nw);
     }
-    DJob* dj = nullptr;
-    WCT_HIP_CHECK(hipMallocAsync((void**)&dj, njobs * sizeof(DJob), s));
-    WCT_HIP_CHECK(hipMemcpyAsync(dj, h.data(), njobs * sizeof(DJob), hipMemcpyHostToDevice, s));
-    int rc = dispatch_M(Mmax, [&](auto MM) -> int {
+    WCT_REQUIRE(Mmax <= 8192, WCT_ERR_UNSUPPORTED, "decon_plan: %d ticks above 16384", 2 * Mmax);
+    auto* p = new wct_decon_plan_s();
+    p->njobs = njobs;
+    p->rows = rows;
+    p->tiles = tiles;
+    p->Mmax = Mmax;
+    p->nwmax = nwmax;
+    p->tk = tk;
+    hipError_t e = hipMalloc((void**)&p->jobs, njobs * sizeof(DJob));
+    if (e == hipSuccess) e = hipMemcpy(p->jobs, h.data(), njobs * sizeof(DJob), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (p->jobs) (void)hipFree(p->jobs);
+        delete p;
+        set_error("decon_plan: job table upload failed: %s", hipGetErrorString(e));
+        return WCT_ERR_HIP;
+    }
+    *out = p;
+    return WCT_OK;
+}
+
+int wct_decon_plan_run(wct_decon_plan p, void* stream)
+{
+    WCT_REQUIRE(p && p->jobs, WCT_ERR_ARG, "decon_plan_run: null plan");
+    hipStream_t s = as_stream(stream);
+    DJob* dj = p->jobs;
+    const int njobs = p->njobs;
+    const long rows = p->rows, tiles = p->tiles;
+    int rc = dispatch_M(p->Mmax, [&](auto MM) -> int {
         constexpr int MMAX = decltype(MM)::value;
         hipLaunchKernelGGL((k_row_r2c<kNT, MMAX>), dim3(rows), dim3(kNT), 0, s, dj, njobs);
         return WCT_OK;
     });
     if (rc) return rc;
     WCT_HIP_CHECK(hipGetLastError());
-    const size_t lds2 = (size_t)tk * (nwmax + 1) * sizeof(float2);
-    if (tk == 8) {
+    const size_t lds2 = (size_t)p->tk * (p->nwmax + 1) * sizeof(float2);
+    if (p->tk == 8) {
         hipLaunchKernelGGL((k_wire_pass<kNT, 8, 1024>), dim3(tiles), dim3(kNT), lds2, s, dj, njobs, tiles);
-    } else if (tk == 4) {
+    } else if (p->tk == 4) {
         hipLaunchKernelGGL((k_wire_pass<kNT, 4, 2048>), dim3(tiles), dim3(kNT), lds2, s, dj, njobs, tiles);
     } else {
-        set_error("decon_run: %d wires not supported", nwmax);
+        set_error("decon_plan_run: %d wires not supported", p->nwmax);
         return WCT_ERR_UNSUPPORTED;
     }
     WCT_HIP_CHECK(hipGetLastError());
-    rc = dispatch_M(Mmax, [&](auto MM) -> int {
+    rc = dispatch_M(p->Mmax, [&](auto MM) -> int {
         constexpr int MMAX = decltype(MM)::value;
         hipLaunchKernelGGL((k_row_c2r_roi<kNT, MMAX>), dim3(rows), dim3(kNT), 0, s, dj, njobs);
         return WCT_OK;
     });
     if (rc) return rc;
     WCT_HIP_CHECK(hipGetLastError());
-    WCT_HIP_CHECK(hipFreeAsync(dj, s));
     return WCT_OK;
+}
+
+int wct_decon_plan_destroy(wct_decon_plan p)
+{
+    if (!p) return WCT_OK;
+    if (p->jobs) WCT_HIP_CHECK(hipFree(p->jobs));
+    delete p;
+    return WCT_OK;
+}
+
+int wct_decon_run(const wct_decon_job* jobs, int njobs, void* workspace, size_t ws_bytes,
+                  void* stream)
+{
+    wct_decon_plan p = nullptr;
+    int rc = wct_decon_plan_create(&p, jobs, njobs, workspace, ws_bytes);
+    if (rc) return rc;
+    rc = wct_decon_plan_run(p, stream);
+    // the job table must outlive the launches: wait before freeing it
+    hipError_t e = hipStreamSynchronize(as_stream(stream));
+    int rc2 = wct_decon_plan_destroy(p);
+    if (rc) return rc;
+    if (e != hipSuccess) {
+        set_error("decon_run: %s", hipGetErrorString(e));
+        return WCT_ERR_HIP;
+    }
+    return rc2;
 }
 
 }  // extern "C"

@@ -55,11 +55,15 @@ _SIGS = {
     "wct_dft_supported": (_I, [_I]),
     "wct_overall_response": (_I, [_V, _I, _I, _V, _I, _D, _D, _D, _D, _D, _I, _V, _V]),
     "wct_decon_kernel_create": (_I, [C.POINTER(C.c_void_p), _V, _I, _I, _I, _V, _V, _I, _I, _D, _V]),
+    "wct_forward_kernel_create": (_I, [C.POINTER(C.c_void_p), _V, _I, _I, _I, _I, _I, _V]),
     "wct_decon_kernel_info": (_I, [_V, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "wct_decon_kernel_download": (_I, [_V, _V]),
     "wct_decon_kernel_destroy": (_I, [_V]),
     "wct_decon_workspace_size": (_Z, [C.POINTER(DeconJob), _I]),
     "wct_decon_run": (_I, [C.POINTER(DeconJob), _I, _V, _Z, _V]),
+    "wct_decon_plan_create": (_I, [C.POINTER(C.c_void_p), C.POINTER(DeconJob), _I, _V, _Z]),
+    "wct_decon_plan_run": (_I, [_V, _V]),
+    "wct_decon_plan_destroy": (_I, [_V]),
 }
 
 # Every symbol include/wct_hip.h declares (checked by the CPU tests).

@@ -100,13 +100,58 @@ class Batch:
     """A fixed list of plane jobs and its workspace; run() launches the
     three stages once over all of them."""
 
-    def __init__(self, jobs, device=None):
+    def __init__(self, jobs, device=None, workspace=None):
         import torch
         self.jobs = (DeconJob * len(jobs))(*jobs)
         self.n = len(jobs)
         self.ws_bytes = lib().wct_decon_workspace_size(self.jobs, self.n)
-        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=device or "cuda")
+        if workspace is None:
+            self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=device or "cuda")
+        else:   # a shared workspace (runs on one stream reuse it in turn)
+            if workspace.numel() < self.ws_bytes:
+                raise ValueError("shared workspace too small")
+            self.ws = workspace
+
+        h = C.c_void_p()
+        check(lib().wct_decon_plan_create(C.byref(h), self.jobs, self.n, self.ws.data_ptr(),
+                                          self.ws.numel()))
+        self.plan = h
 
     def run(self, stream=None):
-        check(lib().wct_decon_run(self.jobs, self.n, self.ws.data_ptr(), self.ws_bytes,
-                                  _stream() if stream is None else stream))
+        check(lib().wct_decon_plan_run(self.plan, _stream() if stream is None else stream))
+
+    def close(self):
+        if getattr(self, "plan", None):
+            import torch
+            torch.cuda.synchronize()
+            lib().wct_decon_plan_destroy(self.plan)
+            self.plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ForwardKernel(PlaneKernel):
+    """The forward response convolution (wct_forward_kernel_create): a job
+    run with it turns a charge raster into the ADC frame whose decon with
+    unit filters returns the raster."""
+
+    def __init__(self, spec, resp=None, stream=None):
+        self.spec = spec
+        self.resp = overall_response(spec) if resp is None else np.ascontiguousarray(resp, np.float64)
+        h = C.c_void_p()
+        check(lib().wct_forward_kernel_create(
+            C.byref(h), self.resp.ctypes.data, self.resp.shape[0], spec.nw, spec.nt,
+            spec.wire_shift, spec.time_shift, _stream() if stream is None else stream))
+        self.handle = h
+        nw, nt, kc = C.c_int(), C.c_int(), C.c_int()
+        check(lib().wct_decon_kernel_info(h, C.byref(nw), C.byref(nt), C.byref(kc)))
+        self.nw, self.nt, self.kc = nw.value, nt.value, kc.value
+
+
+def convolve(fwd_kernel, q, out):
+    """out = forward response convolution of q (torch tensors on device)."""
+    Batch([make_job(fwd_kernel, q, out, restore=0)]).run()

@@ -113,3 +113,24 @@ def config_planes(name):
     if name == "apa150":          # 150 APAs x 2560 channels x 6000 ticks
         return [(k, nw, 6000) for a in range(150) for k, nw in APA_PLANES]
     raise KeyError(name)
+
+
+def tracks_raster(rng, ntracks, nw, nt, q_per_tick=Q_PER_TICK):
+    """Straight tracks with uniform endpoints in [0,nw) x [0,nt), each
+    depositing q_per_tick electrons per tick along its length (float32)."""
+    q = np.zeros((nw, nt), np.float32)
+    for _ in range(ntracks):
+        w0, w1 = rng.uniform(0, nw, 2)
+        t0, t1 = rng.uniform(0, nt, 2)
+        dw, dt = abs(w1 - w0), abs(t1 - t0)
+        nstep = int(math.ceil(max(dw, dt))) + 1
+        f = np.linspace(0.0, 1.0, nstep) if nstep > 1 else np.zeros(1)
+        w = np.clip(np.floor(w0 + f * (w1 - w0)).astype(int), 0, nw - 1)
+        t = np.clip(np.floor(t0 + f * (t1 - t0)).astype(int), 0, nt - 1)
+        np.add.at(q, (w, t), q_per_tick * (dt + 1.0) / nstep)
+    return q
+
+
+def tracks_per_plane(nw, per_480=20):
+    """Track count at the density of 20 tracks per 480 wires."""
+    return max(1, int(round(per_480 * nw / 480.0)))
