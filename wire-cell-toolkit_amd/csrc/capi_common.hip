@@ -28,6 +28,20 @@ static bool small_primes_only(int n)
 
 bool fft_supported_length(int n) { return small_primes_only(n); }
 
+int lds_limit_bytes()
+{
+    static int lim = -1;
+    if (lim < 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+            lim = v;
+        else
+            lim = 65536;
+    }
+    return lim;
+}
+
 // Radix order: 8s, then a 4 or 2 for the rest of the power of two, then
 // 5, 3, 7, 11, 13.  Larger radices mean fewer passes through LDS.
 static void factorize(int n, FftPlan& pl)
@@ -100,6 +114,7 @@ int get_plan(int n, bool is_double, const DevPlan** out)
         Ns *= dp.plan.rad[p];
     }
     dp.plan.dftoff = (int)off;
+    dp.plan.twcount = (int)off + kGenericTabSize;
     if (is_double) {
         std::vector<double> tab;
         fill_twiddles<double>(dp.plan, tab);

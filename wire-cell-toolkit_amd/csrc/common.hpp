@@ -44,6 +44,24 @@ struct Status {
 // Largest prime factor allowed by the LDS kernels.
 bool fft_supported_length(int n);
 
+// Device limit of LDS bytes per workgroup (160 KB on gfx950).
+int lds_limit_bytes();
+
+// Raises a kernel's dynamic-LDS ceiling when a launch needs more than the
+// default 64 KB.
+template <typename K>
+inline int allow_lds(K kernel, size_t bytes)
+{
+    if (bytes <= 65536) return WCT_OK;
+    if (bytes > (size_t)lds_limit_bytes()) {
+        set_error("launch needs %zu bytes of LDS, the device allows %d", bytes, lds_limit_bytes());
+        return WCT_ERR_UNSUPPORTED;
+    }
+    WCT_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return WCT_OK;
+}
+
 // A device-resident plan for one length and precision.
 struct DevPlan {
     FftPlan plan;

@@ -35,6 +35,9 @@ struct wct_decon_plan_s {
     int njobs = 0;
     long rows = 0, tiles = 0;
     int Mmax = 0, nwmax = 0, tk = 8;
+    int twM = 0, twW = 0;        // largest twiddle tables (complex entries)
+    int ncu = 0;                 // compute units of the device
+    bool smooth5 = true;         // every length is 2^a 3^b 5^c
 };
 
 namespace wct {
@@ -109,218 +112,314 @@ __device__ __forceinline__ long xcd_remap(long b, long nblocks)
     return (b % kXcd) * per + b / kXcd;
 }
 
-// ---- K1: real FFT along ticks --------------------------------------
-template <int NT, int MMAX>
-__global__ __launch_bounds__(NT) void k_row_r2c(const DJob* __restrict__ jobs, int njobs)
+// LDS layout of the row kernels: the row (padded_len(Mmax) complex), the
+// twiddle table of the current length, then (K3) the statistics scratch.
+// K1 stages the twiddle table in LDS after the row; K3 reads it from the
+// (L1-resident) global table and instead needs room for the statistics
+// scratch and one row of sorted keys, which alias the row buffer once the
+// row is in registers.
+struct RowLds {
+    int buf_len;      // complex entries of the row buffer
+    int tw_len;       // complex entries of the twiddle region
+    size_t bytes() const { return (size_t)(buf_len + tw_len) * sizeof(float2); }
+};
+
+inline RowLds row_lds(int Mmax, int twmax, bool stats)
 {
-    __shared__ float2 buf[MMAX];
-    const long r = blockIdx.x;
-    const DJob& J = jobs[find_job_rows(jobs, njobs, r)];
-    const int w = (int)(r - J.row0);
-    const int M = J.M;
-    const float* src = J.adc + (long)w * J.ld_adc;
-    const bool has_bad = J.bad_ptr && J.bad_ptr[w + 1] > J.bad_ptr[w];
-    for (int i = threadIdx.x; i < M; i += NT) {
-        float2 z = *reinterpret_cast<const float2*>(src + 2 * i);
-        if (has_bad) {   // load_data zeroes [lo, hi) of bad channels (:337-346)
-            if (in_bad(J, w, 2 * i, false)) z.x = 0.f;
-            if (in_bad(J, w, 2 * i + 1, false)) z.y = 0.f;
-        }
-        buf[i] = z;
+    RowLds l;
+    if (stats) {
+        const size_t scratch = sizeof(RowScratch) + (size_t)2 * Mmax * sizeof(uint32_t);
+        l.buf_len = std::max(padded_len(Mmax), (int)((scratch + sizeof(float2) - 1) / sizeof(float2)));
+        l.tw_len = 0;
+    } else {
+        l.buf_len = padded_len(Mmax);
+        l.tw_len = twmax;
     }
-    __syncthreads();
-    fft_lds<float2, NT, MMAX>(buf, J.planM, J.twM, -1, threadIdx.x);
-    float2* dst = J.S + (long)w * J.ldk;
-    for (int k = threadIdx.x; k < J.ldk; k += NT) {
-        float2 X;
-        X.x = 0.f;
-        X.y = 0.f;
-        if (k < J.kc) {
-            const float2 Zk = buf[k == M ? 0 : k];
-            const float2 Zm = buf[k == 0 ? 0 : M - k];
-            // X = (Zk + conj Zm)/2 - i W^k (Zk - conj Zm)/2
-            const float sr = Zk.x + Zm.x, si = Zk.y - Zm.y;
-            const float dr = Zk.x - Zm.x, di = Zk.y + Zm.y;
-            const float2 W = J.wN[k];
-            const float wr = W.x * dr - W.y * di, wi = W.x * di + W.y * dr;
-            X.x = 0.5f * (sr + wi);
-            X.y = 0.5f * (si - wr);
+    return l;
+}
+
+// ---- K1: real FFT along ticks --------------------------------------
+// Persistent: each block walks rows blockIdx.x, +gridDim.x, ... and
+// restages twiddles only when the row's length changes.
+template <int NT, int VH, bool S5>
+__global__ __launch_bounds__(NT) void k_row_r2c(const DJob* __restrict__ jobs, int njobs,
+                                                long nrows, RowLds lay)
+{
+    constexpr int MMAX = NT * VH;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* buf = reinterpret_cast<float2*>(smem_raw);
+    float2* twl = buf + lay.buf_len;
+    int curM = -1;
+    // the next row's samples are fetched into registers while the current
+    // row is transformed, so the loads of a row are all in flight at once
+    float2 z[VH];
+    auto fetch = [&](long rr) {
+        const DJob& Jn = jobs[find_job_rows(jobs, njobs, rr)];
+        const int wn = (int)(rr - Jn.row0);
+        const float2* src = reinterpret_cast<const float2*>(Jn.adc + (long)wn * Jn.ld_adc);
+#pragma unroll
+        for (int i = 0; i < VH; ++i) {
+            const int n = threadIdx.x + i * NT;
+            if (n < Jn.M) z[i] = src[n];
         }
-        dst[k] = X;
+    };
+    if (blockIdx.x < nrows) fetch(blockIdx.x);
+    for (long r = blockIdx.x; r < nrows; r += gridDim.x) {
+        const DJob& J = jobs[find_job_rows(jobs, njobs, r)];
+        const int w = (int)(r - J.row0);
+        const int M = J.M;
+        if (M != curM) {
+            stage_twiddles(twl, J.twM, J.planM.twcount, threadIdx.x, NT);
+            curM = M;
+        }
+        const bool has_bad = J.bad_ptr && J.bad_ptr[w + 1] > J.bad_ptr[w];
+#pragma unroll
+        for (int i = 0; i < VH; ++i) {
+            const int n = threadIdx.x + i * NT;
+            if (n < M) {
+                float2 v = z[i];
+                if (has_bad) {   // load_data zeroes [lo, hi) of bad channels (:337-346)
+                    if (in_bad(J, w, 2 * n, false)) v.x = 0.f;
+                    if (in_bad(J, w, 2 * n + 1, false)) v.y = 0.f;
+                }
+                buf[pidx(n)] = v;
+            }
+        }
+        if (r + gridDim.x < nrows) fetch(r + gridDim.x);
+        __syncthreads();
+        fft_lds<float2, NT, MMAX, S5>(buf, J.planM, twl, -1, threadIdx.x);
+        float2* dst = J.S + (long)w * J.ldk;
+        for (int k = threadIdx.x; k < J.ldk; k += NT) {
+            float2 X;
+            X.x = 0.f;
+            X.y = 0.f;
+            if (k < J.kc) {
+                const float2 Zk = buf[pidx(k == M ? 0 : k)];
+                const float2 Zm = buf[pidx(k == 0 ? 0 : M - k)];
+                // X = (Zk + conj Zm)/2 - i W^k (Zk - conj Zm)/2
+                const float sr = Zk.x + Zm.x, si = Zk.y - Zm.y;
+                const float dr = Zk.x - Zm.x, di = Zk.y + Zm.y;
+                const float2 W = J.wN[k];
+                const float wr = W.x * dr - W.y * di, wi = W.x * di + W.y * dr;
+                X.x = 0.5f * (sr + wi);
+                X.y = 0.5f * (si - wr);
+            }
+            dst[k] = X;
+        }
+        __syncthreads();
     }
 }
 
 // ---- K2: wire FFT, multiply by K, inverse wire FFT ------------------
-template <int NT, int TK, int NWMAX>
+template <int NT, int TK, int NWMAX, bool S5>
 __global__ __launch_bounds__(NT) void k_wire_pass(const DJob* __restrict__ jobs, int njobs,
-                                                  long ntiles)
+                                                  long ntiles, int ldc)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float2* buf = reinterpret_cast<float2*>(smem_raw);
+    float2* twl = buf + (size_t)TK * ldc;
     const long t = xcd_remap(blockIdx.x, ntiles);
     const DJob& J = jobs[find_job_tiles(jobs, njobs, t)];
     const int k0 = (int)(t - J.tile0) * TK;
-    const int nw = J.nw, ld = nw + 1;
+    const int nw = J.nw;
     float2* S = J.S;
-    for (int idx = threadIdx.x; idx < nw * TK; idx += NT) {
-        const int cc = idx % TK, w = idx / TK;
-        buf[cc * ld + w] = S[(long)w * J.ldk + k0 + cc];
+    stage_twiddles(twl, J.twW, J.planW.twcount, threadIdx.x, NT);
+    constexpr int NLD = (NWMAX * TK + NT - 1) / NT;   // elements per thread, all loads in flight
+    {
+        float2 t[NLD];
+#pragma unroll
+        for (int q = 0; q < NLD; ++q) {
+            const int idx = threadIdx.x + q * NT;
+            if (idx < nw * TK) t[q] = S[(long)(idx / TK) * J.ldk + k0 + idx % TK];
+        }
+#pragma unroll
+        for (int q = 0; q < NLD; ++q) {
+            const int idx = threadIdx.x + q * NT;
+            if (idx < nw * TK) buf[(idx % TK) * ldc + pidx(idx / TK)] = t[q];
+        }
     }
     __syncthreads();
     constexpr int SL = NT / TK;
     const int slice = threadIdx.x / SL, st = threadIdx.x % SL;
-    float2* col = buf + slice * ld;
-    fft_lds<float2, SL, NWMAX>(col, J.planW, J.twW, -1, st);
+    float2* col = buf + slice * ldc;
+    fft_lds<float2, SL, NWMAX, S5>(col, J.planW, twl, -1, st);
     const int k = k0 + slice;
     const float2* Kk = J.K + (long)k * nw;
     for (int m = st; m < nw; m += SL) {
-        float2 v = col[m];
         float2 o;
         o.x = 0.f;
         o.y = 0.f;
-        if (k < J.kc) o = cmul(v, Kk[m]);
-        col[m] = o;
+        if (k < J.kc) o = cmul(col[pidx(m)], Kk[m]);
+        col[pidx(m)] = o;
     }
     __syncthreads();
-    fft_lds<float2, SL, NWMAX>(col, J.planW, J.twW, +1, st);
+    fft_lds<float2, SL, NWMAX, S5>(col, J.planW, twl, +1, st);
     for (int idx = threadIdx.x; idx < nw * TK; idx += NT) {
         const int cc = idx % TK, w = idx / TK;
-        S[(long)w * J.ldk + k0 + cc] = buf[cc * ld + w];
+        S[(long)w * J.ldk + k0 + cc] = buf[cc * ldc + pidx(w)];
     }
 }
 
-// Builds Z'[k] = (Y[k] + conj Y[M-k]) + i conj(W^k) (Y[k] - conj Y[M-k])
-// for the inverse real FFT; DC and Nyquist use their real parts only
-// (the c2r mirror, util/inc/WireCellUtil/Spectrum.h:56-89).
-__device__ __forceinline__ float2 c2r_pre(const DJob& J, const float2* Y, const float* g, int k)
+// All threads load their Y[k], Y[M-k] pairs first (every load in
+// flight), then build Z' in LDS.
+template <int NT, int VH>
+__device__ __forceinline__ void c2r_stage(const DJob& J, const float2* __restrict__ Y,
+                                          const float* __restrict__ g, float2* buf)
 {
-    const int M = J.M;
-    float2 a, b;
-    a.x = a.y = b.x = b.y = 0.f;
-    if (k < J.kc) {
-        a = Y[k];
-        if (g) { a.x *= g[k]; a.y *= g[k]; }
-        if (k == 0) a.y = 0.f;
+    const int M = J.M, kc = J.kc;
+    float2 a[VH], b[VH];
+    float ga[VH], gb[VH];
+#pragma unroll
+    for (int i = 0; i < VH; ++i) {
+        const int k = threadIdx.x + i * NT;
+        const int km = M - k;
+        a[i].x = a[i].y = b[i].x = b[i].y = 0.f;
+        ga[i] = gb[i] = 1.f;
+        if (k < M) {
+            if (k < kc) { a[i] = Y[k]; if (g) ga[i] = g[k]; }
+            if (km < kc) { b[i] = Y[km]; if (g) gb[i] = g[km]; }
+        }
     }
-    const int km = M - k;   // in (0, M]
-    if (km < J.kc) {
-        b = Y[km];
-        if (g) { b.x *= g[km]; b.y *= g[km]; }
-        if (km == M) b.y = 0.f;
+#pragma unroll
+    for (int i = 0; i < VH; ++i) {
+        const int k = threadIdx.x + i * NT;
+        if (k < M) {
+            float2 av = a[i], bv = b[i];
+            av.x *= ga[i]; av.y *= ga[i];
+            bv.x *= gb[i]; bv.y *= gb[i];
+            if (k == 0) av.y = 0.f;
+            if (M - k == M) bv.y = 0.f;
+            const float sr = av.x + bv.x, si = av.y - bv.y;
+            const float dr = av.x - bv.x, di = av.y + bv.y;
+            const float2 W = J.wN[k];
+            const float pr = W.x * dr + W.y * di, pi = W.x * di - W.y * dr;
+            float2 zz;
+            zz.x = sr - pi;
+            zz.y = si + pr;
+            buf[pidx(k)] = zz;
+        }
     }
-    // conj(b)
-    const float sr = a.x + b.x, si = a.y - b.y;
-    const float dr = a.x - b.x, di = a.y + b.y;
-    const float2 W = J.wN[k];
-    // i * conj(W) * d = i (W.x - i W.y)(dr + i di) = i[(W.x dr + W.y di) + i(W.x di - W.y dr)]
-    const float pr = W.x * dr + W.y * di, pi = W.x * di - W.y * dr;
-    float2 z;
-    z.x = sr - pi;
-    z.y = si + pr;
-    return z;
 }
 
 // ---- K3: inverse real FFT, baseline, RMS, threshold, outputs ---------
-template <int NT, int MMAX>
-__global__ __launch_bounds__(NT) void k_row_c2r_roi(const DJob* __restrict__ jobs, int njobs)
+template <int NT, int VH, bool S5>
+__global__ __launch_bounds__(NT, 2) void k_row_c2r_roi(const DJob* __restrict__ jobs, int njobs,
+                                                    long nrows, RowLds lay)
 {
-    constexpr int VH = MMAX / NT;          // complex slots per thread
+    constexpr int MMAX = NT * VH;
     constexpr int VPT = 2 * VH;            // real samples per thread
-    __shared__ float2 buf[MMAX];
-    // the statistics scratch reuses the FFT buffer once the row is in registers
-    static_assert(sizeof(RowScratch) <= sizeof(float2) * MMAX, "scratch does not fit");
-    RowScratch& sh = *reinterpret_cast<RowScratch*>(buf);
-    const long r = blockIdx.x;
-    const DJob& J = jobs[find_job_rows(jobs, njobs, r)];
-    const int w = (int)(r - J.row0);
-    const int M = J.M, nt = J.nt;
-    const float2* Y = J.S + (long)w * J.ldk;
-    const bool has_bad = J.bad_ptr && J.bad_ptr[w + 1] > J.bad_ptr[w];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* buf = reinterpret_cast<float2*>(smem_raw);
+    RowScratch* sh = reinterpret_cast<RowScratch*>(buf);   // reused once the row is in registers
+    for (long r = blockIdx.x; r < nrows; r += gridDim.x) {
+        const DJob& J = jobs[find_job_rows(jobs, njobs, r)];
+        const int w = (int)(r - J.row0);
+        const int M = J.M;
+        const float2* twl = J.twM;
+        const float2* Y = J.S + (long)w * J.ldk;
+        const bool has_bad = J.bad_ptr && J.bad_ptr[w + 1] > J.bad_ptr[w];
 
-    for (int k = threadIdx.x; k < M; k += NT) buf[k] = c2r_pre(J, Y, J.lf1, k);
-    __syncthreads();
-    fft_lds<float2, NT, MMAX>(buf, J.planM, J.twM, +1, threadIdx.x);
-
-    float val[VPT];
-    uint64_t valid = 0, pm = 0;
-#pragma unroll
-    for (int i = 0; i < VH; ++i) {
-        const int n = threadIdx.x + i * NT;
-        float2 z;
-        z.x = z.y = 0.f;
-        if (n < M) {
-            z = buf[n];
-            valid |= 3ull << (2 * i);
-        }
-        val[2 * i] = z.x;
-        val[2 * i + 1] = z.y;
-    }
-    pm = valid;
-    const bool stats = J.restore || J.mask || J.rms;
-    if (stats && has_bad) {
-#pragma unroll
-        for (int i = 0; i < VPT; ++i) {
-            const int pos = 2 * (threadIdx.x + (i / 2) * NT) + (i & 1);
-            if (((valid >> i) & 1ull) && in_bad(J, w, pos, true)) pm &= ~(1ull << i);
-        }
-    }
-    if (J.restore) row_restore_baseline<NT, VPT>(val, valid, &sh);
-    const float rms = stats ? row_cal_rms<NT, VPT>(val, pm, &sh) : 0.0f;
-    const double th = (double)J.th_factor * (double)rms + 1.0;
-
-    unsigned char mbits[VPT];
-#pragma unroll
-    for (int i = 0; i < VPT; ++i)
-        mbits[i] = (((pm >> i) & 1ull) && (double)val[i] > th) ? 1 : 0;
-
-    if (J.lf2) {   // the "tighter" variant for the mask (decon_2D_tighterROI, :1120-1160)
+        c2r_stage<NT, VH>(J, Y, J.lf1, buf);
         __syncthreads();
-        for (int k = threadIdx.x; k < M; k += NT) buf[k] = c2r_pre(J, Y, J.lf2, k);
-        __syncthreads();
-        fft_lds<float2, NT, MMAX>(buf, J.planM, J.twM, +1, threadIdx.x);
-        float v2[VPT];
+        fft_lds<float2, NT, MMAX, S5>(buf, J.planM, twl, +1, threadIdx.x);
+
+        float val[VPT];
+        uint64_t valid = 0;
 #pragma unroll
         for (int i = 0; i < VH; ++i) {
             const int n = threadIdx.x + i * NT;
             float2 z;
             z.x = z.y = 0.f;
-            if (n < M) z = buf[n];
-            v2[2 * i] = z.x;
-            v2[2 * i + 1] = z.y;
+            if (n < M) {
+                z = buf[pidx(n)];
+                valid |= 3ull << (2 * i);
+            }
+            val[2 * i] = z.x;
+            val[2 * i + 1] = z.y;
         }
-        if (J.restore) row_restore_baseline<NT, VPT>(v2, valid, &sh);
+        uint64_t pm = valid;
+        const bool stats = J.restore || J.mask || J.rms;
+        if (stats && has_bad) {
+#pragma unroll
+            for (int i = 0; i < VPT; ++i) {
+                const int pos = 2 * (threadIdx.x + (i / 2) * NT) + (i & 1);
+                if (((valid >> i) & 1ull) && in_bad(J, w, pos, true)) pm &= ~(1ull << i);
+            }
+        }
+        float rms = 0.0f;
+        if (stats) {
+            // the fast path needs every valid sample in the RMS set
+            const bool row_bad = has_bad && (__syncthreads_or(pm != valid) != 0);
+            if (!row_bad) {
+                rms = row_stats_fast<NT, VPT>(val, valid, J.restore != 0, true, sh);
+            } else {
+                if (J.restore) row_restore_baseline<NT, VPT>(val, valid, sh);
+                rms = row_cal_rms<NT, VPT>(val, pm, sh);
+            }
+        }
+        const double th = (double)J.th_factor * (double)rms + 1.0;
+
+        uint64_t m1 = 0, m2 = 0;   // bit i: sample i above threshold (tight, tighter)
 #pragma unroll
         for (int i = 0; i < VPT; ++i)
-            if (((pm >> i) & 1ull) && (double)v2[i] > th) mbits[i] |= 2;
-    }
+            if (((pm >> i) & 1ull) && (double)val[i] > th) m1 |= 1ull << i;
 
-    float* o = J.out + (long)w * J.ld_out;
-    unsigned char* mk = J.mask ? J.mask + (long)w * J.ld_mask : nullptr;
+        float* o = J.out + (long)w * J.ld_out;
 #pragma unroll
-    for (int i = 0; i < VH; ++i) {
-        const int n = threadIdx.x + i * NT;
-        if (n < M) {
-            float a = val[2 * i], b = val[2 * i + 1];
-            if (J.save_mode) {   // save_data (:383-418): negatives and [lo, hi) bad ranges to 0
-                a = a > 0.f ? a : 0.f;
-                b = b > 0.f ? b : 0.f;
-                if (has_bad) {
-                    if (in_bad(J, w, 2 * n, false)) a = 0.f;
-                    if (in_bad(J, w, 2 * n + 1, false)) b = 0.f;
+        for (int i = 0; i < VH; ++i) {
+            const int n = threadIdx.x + i * NT;
+            if (n < M) {
+                float a = val[2 * i], b = val[2 * i + 1];
+                if (J.save_mode) {   // save_data (:383-418): negatives and [lo, hi) bad ranges to 0
+                    a = a > 0.f ? a : 0.f;
+                    b = b > 0.f ? b : 0.f;
+                    if (has_bad) {
+                        if (in_bad(J, w, 2 * n, false)) a = 0.f;
+                        if (in_bad(J, w, 2 * n + 1, false)) b = 0.f;
+                    }
                 }
-            }
-            float2 ab;
-            ab.x = a;
-            ab.y = b;
-            *reinterpret_cast<float2*>(o + 2 * n) = ab;
-            if (mk) {
-                const unsigned short mm = (unsigned short)mbits[2 * i] | ((unsigned short)mbits[2 * i + 1] << 8);
-                *reinterpret_cast<unsigned short*>(mk + 2 * n) = mm;
+                float2 ab;
+                ab.x = a;
+                ab.y = b;
+                *reinterpret_cast<float2*>(o + 2 * n) = ab;
             }
         }
+
+        if (J.lf2) {   // the "tighter" variant for the mask (decon_2D_tighterROI, :1120-1160)
+            __syncthreads();
+            c2r_stage<NT, VH>(J, Y, J.lf2, buf);
+            __syncthreads();
+            fft_lds<float2, NT, MMAX, S5>(buf, J.planM, twl, +1, threadIdx.x);
+#pragma unroll
+            for (int i = 0; i < VH; ++i) {
+                const int n = threadIdx.x + i * NT;
+                float2 z;
+                z.x = z.y = 0.f;
+                if (n < M) z = buf[pidx(n)];
+                val[2 * i] = z.x;
+                val[2 * i + 1] = z.y;
+            }
+            if (J.restore) row_stats_fast<NT, VPT>(val, valid, true, false, sh);
+#pragma unroll
+            for (int i = 0; i < VPT; ++i)
+                if (((pm >> i) & 1ull) && (double)val[i] > th) m2 |= 1ull << i;
+        }
+
+        if (J.mask) {
+            unsigned char* mk = J.mask + (long)w * J.ld_mask;
+#pragma unroll
+            for (int i = 0; i < VH; ++i) {
+                const int n = threadIdx.x + i * NT;
+                if (n < M) {
+                    const unsigned a0 = (unsigned)((m1 >> (2 * i)) & 1ull) | ((unsigned)((m2 >> (2 * i)) & 1ull) << 1);
+                    const unsigned a1 = (unsigned)((m1 >> (2 * i + 1)) & 1ull) | ((unsigned)((m2 >> (2 * i + 1)) & 1ull) << 1);
+                    *reinterpret_cast<unsigned short*>(mk + 2 * n) = (unsigned short)(a0 | (a1 << 8));
+                }
+            }
+        }
+        if (J.rms && threadIdx.x == 0) J.rms[w] = rms;
+        __syncthreads();
     }
-    if (J.rms && threadIdx.x == 0) J.rms[w] = rms;
-    (void)nt;
 }
 
 // ---- FP64 kernel build ------------------------------------------------
@@ -438,16 +537,101 @@ __global__ void k_resp_mul_mirror(double2* __restrict__ rows, const double2* __r
     }
 }
 
+// Persistent grid: as many blocks as the device keeps resident at once
+// (a larger grid leaves a tail of blocks that start only as others end).
+template <typename K>
+static int resident_grid(K kernel, int nt, size_t lds, int ncu, long work, int* grid)
+{
+    int per_cu = 0;
+    WCT_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, nt, lds));
+    if (per_cu < 1) per_cu = 1;
+    *grid = (int)std::max<long>(1, std::min<long>(work, (long)per_cu * ncu));
+    return WCT_OK;
+}
+
+static bool is_smooth5(int n)
+{
+    for (int q : {2, 3, 5})
+        while (n % q == 0) n /= q;
+    return n == 1;
+}
+
 static int pick_tk(int nw) { return nw + 1 <= 1024 ? 8 : (nw + 1 <= 2048 ? 4 : 2); }
 
+// Complex slots per thread of the row kernels for rows of up to Mmax
+// complex samples (M = nt/2), at kNT threads per row.
 template <typename F>
-static int dispatch_M(int Mmax, F&& f)
+static int dispatch_vh(int Mmax, F&& f)
 {
-    if (Mmax <= 2048) return f(std::integral_constant<int, 2048>());
-    if (Mmax <= 4096) return f(std::integral_constant<int, 4096>());
-    if (Mmax <= 8192) return f(std::integral_constant<int, 8192>());
+    const int vh = (Mmax + kNT - 1) / kNT;
+    if (vh <= 4) return f(std::integral_constant<int, 4>());
+    if (vh <= 8) return f(std::integral_constant<int, 8>());
+    if (vh <= 12) return f(std::integral_constant<int, 12>());
+    if (vh <= 16) return f(std::integral_constant<int, 16>());
+    if (vh <= 20) return f(std::integral_constant<int, 20>());
+    if (vh <= 24) return f(std::integral_constant<int, 24>());
+    if (vh <= 32) return f(std::integral_constant<int, 32>());
     set_error("time extent %d above the supported 16384 ticks", 2 * Mmax);
     return WCT_ERR_UNSUPPORTED;
+}
+
+}  // namespace wct
+
+namespace wct {
+
+template <bool S5>
+int plan_run(wct_decon_plan p, hipStream_t s)
+{
+    DJob* dj = p->jobs;
+    const int njobs = p->njobs;
+    const long rows = p->rows, tiles = p->tiles;
+    const RowLds lay = row_lds(p->Mmax, p->twM, false);
+    const RowLds lay3 = row_lds(p->Mmax, p->twM, true);
+    int rc = dispatch_vh(p->Mmax, [&](auto V) -> int {
+        constexpr int VH = decltype(V)::value;
+        const size_t lds = lay.bytes();
+        int r_ = allow_lds(k_row_r2c<kNT, VH, S5>, lds);
+        if (r_) return r_;
+        int grid = 0;
+        r_ = resident_grid(k_row_r2c<kNT, VH, S5>, kNT, lds, p->ncu, rows, &grid);
+        if (r_) return r_;
+        hipLaunchKernelGGL((k_row_r2c<kNT, VH, S5>), dim3(grid), dim3(kNT), lds, s, dj, njobs, rows, lay);
+        return WCT_OK;
+    });
+    if (rc) return rc;
+    WCT_HIP_CHECK(hipGetLastError());
+    const int ldc = padded_len(p->nwmax) | 1;
+    const size_t lds2 = ((size_t)p->tk * ldc + p->twW) * sizeof(float2);
+    if (p->tk == 8) {
+        rc = allow_lds(k_wire_pass<kNT, 8, 1024, S5>, lds2);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_wire_pass<kNT, 8, 1024, S5>), dim3(tiles), dim3(kNT), lds2, s, dj, njobs,
+                           tiles, ldc);
+    } else if (p->tk == 4) {
+        rc = allow_lds(k_wire_pass<kNT, 4, 2048, S5>, lds2);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_wire_pass<kNT, 4, 2048, S5>), dim3(tiles), dim3(kNT), lds2, s, dj, njobs,
+                           tiles, ldc);
+    } else {
+        set_error("decon_plan_run: %d wires not supported", p->nwmax);
+        return WCT_ERR_UNSUPPORTED;
+    }
+    WCT_HIP_CHECK(hipGetLastError());
+    rc = dispatch_vh(p->Mmax, [&](auto V) -> int {
+        constexpr int VH = decltype(V)::value;
+        const size_t lds = lay3.bytes();
+        int r_ = allow_lds(k_row_c2r_roi<kNT, VH, S5>, lds);
+        if (r_) return r_;
+        int grid = 0;
+        r_ = resident_grid(k_row_c2r_roi<kNT, VH, S5>, kNT, lds, p->ncu, rows, &grid);
+        if (r_) return r_;
+        hipLaunchKernelGGL((k_row_c2r_roi<kNT, VH, S5>), dim3(grid), dim3(kNT), lds, s, dj, njobs,
+                           rows, lay3);
+        return WCT_OK;
+    });
+    if (rc) return rc;
+    WCT_HIP_CHECK(hipGetLastError());
+    return WCT_OK;
 }
 
 }  // namespace wct
@@ -677,7 +861,7 @@ int wct_decon_plan_create(wct_decon_plan* out, const wct_decon_job* jobs, int nj
     std::vector<DJob> h(njobs);
     size_t off = 0;
     long rows = 0, tiles = 0;
-    int Mmax = 0, tk = pick_tk(jobs[0].kernel->nw), nwmax = 0;
+    int Mmax = 0, tk = pick_tk(jobs[0].kernel->nw), nwmax = 0, twM = 0, twW = 0;
     for (int i = 0; i < njobs; ++i) {
         const wct_decon_job& j = jobs[i];
         const wct_decon_kernel K = j.kernel;
@@ -737,9 +921,21 @@ int wct_decon_plan_create(wct_decon_plan* out, const wct_decon_job* jobs, int nj
         d.wN = (const float2*)line;
         Mmax = std::max(Mmax, d.M);
         nwmax = std::max(nwmax, d.nw);
+        twM = std::max(twM, d.planM.twcount);
+        twW = std::max(twW, d.planW.twcount);
     }
     WCT_REQUIRE(Mmax <= 8192, WCT_ERR_UNSUPPORTED, "decon_plan: %d ticks above 16384", 2 * Mmax);
     auto* p = new wct_decon_plan_s();
+    p->smooth5 = true;
+    for (int i = 0; i < njobs; ++i)
+        if (!is_smooth5(h[i].M) || !is_smooth5(h[i].nw)) p->smooth5 = false;
+    p->twM = twM;
+    p->twW = twW;
+    {
+        int dev = 0;
+        WCT_HIP_CHECK(hipGetDevice(&dev));
+        WCT_HIP_CHECK(hipDeviceGetAttribute(&p->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
     p->njobs = njobs;
     p->rows = rows;
     p->tiles = tiles;
@@ -761,35 +957,7 @@ int wct_decon_plan_create(wct_decon_plan* out, const wct_decon_job* jobs, int nj
 int wct_decon_plan_run(wct_decon_plan p, void* stream)
 {
     WCT_REQUIRE(p && p->jobs, WCT_ERR_ARG, "decon_plan_run: null plan");
-    hipStream_t s = as_stream(stream);
-    DJob* dj = p->jobs;
-    const int njobs = p->njobs;
-    const long rows = p->rows, tiles = p->tiles;
-    int rc = dispatch_M(p->Mmax, [&](auto MM) -> int {
-        constexpr int MMAX = decltype(MM)::value;
-        hipLaunchKernelGGL((k_row_r2c<kNT, MMAX>), dim3(rows), dim3(kNT), 0, s, dj, njobs);
-        return WCT_OK;
-    });
-    if (rc) return rc;
-    WCT_HIP_CHECK(hipGetLastError());
-    const size_t lds2 = (size_t)p->tk * (p->nwmax + 1) * sizeof(float2);
-    if (p->tk == 8) {
-        hipLaunchKernelGGL((k_wire_pass<kNT, 8, 1024>), dim3(tiles), dim3(kNT), lds2, s, dj, njobs, tiles);
-    } else if (p->tk == 4) {
-        hipLaunchKernelGGL((k_wire_pass<kNT, 4, 2048>), dim3(tiles), dim3(kNT), lds2, s, dj, njobs, tiles);
-    } else {
-        set_error("decon_plan_run: %d wires not supported", p->nwmax);
-        return WCT_ERR_UNSUPPORTED;
-    }
-    WCT_HIP_CHECK(hipGetLastError());
-    rc = dispatch_M(p->Mmax, [&](auto MM) -> int {
-        constexpr int MMAX = decltype(MM)::value;
-        hipLaunchKernelGGL((k_row_c2r_roi<kNT, MMAX>), dim3(rows), dim3(kNT), 0, s, dj, njobs);
-        return WCT_OK;
-    });
-    if (rc) return rc;
-    WCT_HIP_CHECK(hipGetLastError());
-    return WCT_OK;
+    return p->smooth5 ? plan_run<true>(p, as_stream(stream)) : plan_run<false>(p, as_stream(stream));
 }
 
 int wct_decon_plan_destroy(wct_decon_plan p)

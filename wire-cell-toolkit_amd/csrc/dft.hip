@@ -10,6 +10,7 @@ namespace wct {
 constexpr int kBlock = 256;
 
 // ---- rows: each slice of SLICE threads transforms one row in LDS ----
+// LDS: rows of padded_len(n) entries, then the plan's twiddle table.
 template <typename T, int SLICE, int NMAX>
 __global__ __launch_bounds__(kBlock) void k_fft_rows(const typename C2<T>::type* __restrict__ in,
                                                      typename C2<T>::type* __restrict__ out,
@@ -25,18 +26,21 @@ __global__ __launch_bounds__(kBlock) void k_fft_rows(const typename C2<T>::type*
     const int tid = threadIdx.x % SLICE;
     const long row = (long)blockIdx.x * RPB + slice;
     const int n = pl.n;
-    T2* buf = smem + (size_t)slice * n;
+    const int ld = padded_len(n);
+    T2* buf = smem + (size_t)slice * ld;
+    T2* twl = smem + (size_t)RPB * ld;
+    stage_twiddles(twl, tw, pl.twcount, threadIdx.x, kBlock);
     const bool active = row < nrows;
     if (active) {
         const T2* src = in + row * dist;
-        for (int i = tid; i < n; i += SLICE) buf[i] = src[i];
+        for (int i = tid; i < n; i += SLICE) buf[pidx(i)] = src[i];
     }
     __syncthreads();
-    fft_lds<T2, SLICE, NMAX>(buf, pl, tw, sign, tid);
+    fft_lds<T2, SLICE, NMAX>(buf, pl, twl, sign, tid);
     if (active) {
         T2* dst = out + row * dist;
         for (int i = tid; i < n; i += SLICE) {
-            T2 v = buf[i];
+            T2 v = buf[pidx(i)];
             v.x *= scale;
             v.y *= scale;
             dst[i] = v;
@@ -58,7 +62,9 @@ __global__ __launch_bounds__(kBlock) void k_fft_cols(const typename C2<T>::type*
     constexpr int SLICE = kBlock / TC;
     in += blockIdx.y * bstride;
     out += blockIdx.y * bstride;
-    const int ld = nrows + 1;   // odd stride in 8-byte words: no bank conflicts on the transposed store
+    const int ld = padded_len(nrows) | 1;   // odd stride: the transposed store spreads over banks
+    T2* twl = smem + (size_t)TC * ld;
+    stage_twiddles(twl, tw, pl.twcount, threadIdx.x, kBlock);
     const long c0 = (long)blockIdx.x * TC;
     // coalesced load: TC consecutive columns of each row
     for (int idx = threadIdx.x; idx < nrows * TC; idx += kBlock) {
@@ -67,16 +73,16 @@ __global__ __launch_bounds__(kBlock) void k_fft_cols(const typename C2<T>::type*
         v.x = 0;
         v.y = 0;
         if (c0 + cc < ncols) v = in[(long)r * ncols + c0 + cc];
-        smem[cc * ld + r] = v;
+        smem[cc * ld + pidx(r)] = v;
     }
     __syncthreads();
     const int slice = threadIdx.x / SLICE;
     const int tid = threadIdx.x % SLICE;
-    fft_lds<T2, SLICE, NMAX>(smem + slice * ld, pl, tw, sign, tid);
+    fft_lds<T2, SLICE, NMAX>(smem + slice * ld, pl, twl, sign, tid);
     for (int idx = threadIdx.x; idx < nrows * TC; idx += kBlock) {
         const int cc = idx % TC, r = idx / TC;
         if (c0 + cc < ncols) {
-            T2 v = smem[cc * ld + r];
+            T2 v = smem[cc * ld + pidx(r)];
             v.x *= scale;
             v.y *= scale;
             out[(long)r * ncols + c0 + cc] = v;
@@ -123,7 +129,9 @@ static int launch_rows_lds(const void* in, void* out, int n, int nrows, long dis
     do {                                                                                         \
         const int rpb = kBlock / SL;                                                             \
         dim3 grid((nrows + rpb - 1) / rpb);                                                      \
-        size_t lds = (size_t)rpb * n * sizeof(T2);                                               \
+        size_t lds = ((size_t)rpb * padded_len(n) + dp->plan.twcount) * sizeof(T2);              \
+        int rc_ = allow_lds(k_fft_rows<T, SL, NM>, lds);                                         \
+        if (rc_) return rc_;                                                                     \
         hipLaunchKernelGGL((k_fft_rows<T, SL, NM>), grid, dim3(kBlock), lds, s, src, dst, nrows, \
                            dist, dp->plan, tw, sign, scale);                                     \
     } while (0)
@@ -155,7 +163,9 @@ static int launch_cols(const void* in, void* out, int nrows, int ncols, int sign
 #define WCT_COLS(TC, NM)                                                                         \
     do {                                                                                         \
         dim3 grid((ncols + TC - 1) / TC, nbatch);                                                \
-        size_t lds = (size_t)TC * (nrows + 1) * sizeof(T2);                                      \
+        size_t lds = ((size_t)TC * (padded_len(nrows) | 1) + dp->plan.twcount) * sizeof(T2);     \
+        int rc_ = allow_lds(k_fft_cols<T, TC, NM>, lds);                                         \
+        if (rc_) return rc_;                                                                     \
         hipLaunchKernelGGL((k_fft_cols<T, TC, NM>), grid, dim3(kBlock), lds, s, src, dst, nrows, \
                            ncols, bstride, dp->plan, tw, sign, scale);                           \
     } while (0)

@@ -28,7 +28,14 @@ struct FftPlan {
     int rad[kMaxPasses];
     int twoff[kMaxPasses];   // offset (in complex entries) of each pass's twiddles
     int dftoff;              // offset of the generic small-DFT tables (7, 11, 13)
+    int twcount;             // entries in the whole table (passes + generic tables)
 };
+
+// LDS index with one pad slot every 16 complex entries: the strided
+// stores of the early passes (stride R entries) then spread over the
+// banks instead of piling onto a few.
+__host__ __device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
+__host__ __device__ __forceinline__ int padded_len(int n) { return n + (n >> 4) + 1; }
 
 template <typename T> struct C2;
 template <> struct C2<float> { using type = float2; };
@@ -195,15 +202,17 @@ __device__ __forceinline__ void stockham_pass(T2* buf, int n, int Ns, const T2* 
         const int j = tid + b * NT;
         if (j < m) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * m];
+            for (int r = 0; r < R; ++r) v[b][r] = buf[pidx(j + r * m)];
         }
     }
     __syncthreads();
+    // k = j mod Ns, stepped from the first butterfly (one division per pass)
+    int k = tid % Ns;
+    const int kstep = NT % Ns;
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
         const int j = tid + b * NT;
         if (j < m) {
-            const int k = j % Ns;
             if (Ns > 1) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
@@ -215,18 +224,25 @@ __device__ __forceinline__ void stockham_pass(T2* buf, int n, int Ns, const T2* 
             Dft<T2, R>::run(v[b], sign, gtab + generic_tab_offset(R));
             const int base = (j - k) * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[b][r];
+            for (int r = 0; r < R; ++r) buf[pidx(base + r * Ns)] = v[b][r];
         }
+        k += kstep;
+        if (k >= Ns) k -= Ns;
     }
     __syncthreads();
 }
 
-// Full transform of the LDS sequence buf (length plan.n <= NMAX).
-// Every thread of the workgroup must call this (it contains barriers);
-// threads with tid >= NT (when the workgroup is split into slices) pass
-// their own slice index through buf/tid.
-template <typename T2, int NT, int NMAX>
-__device__ void fft_lds(T2* buf, const FftPlan& pl, const T2* __restrict__ tw, int sign, int tid)
+// SMOOTH5 instantiates only the radix 2/3/4/5/8 passes (lengths whose
+// prime factors are 2, 3 and 5); the generic 7/11/13 passes carry large
+// constant tables that would inflate the register allocation of every
+// kernel that includes them.
+// Full transform of the LDS sequence buf (length plan.n <= NMAX, stored
+// at padded indices pidx(i)).  tw is the plan's twiddle table, normally
+// staged in LDS by the caller (stage_twiddles).  Every thread of the
+// workgroup must call this (it contains barriers); when the workgroup is
+// split into slices each slice passes its own buf and slice-local tid.
+template <typename T2, int NT, int NMAX, bool SMOOTH5 = false>
+__device__ __forceinline__ void fft_lds(T2* buf, const FftPlan& pl, const T2* __restrict__ tw, int sign, int tid)
 {
     const T2* gtab = tw + pl.dftoff;
     int Ns = 1;
@@ -239,13 +255,27 @@ __device__ void fft_lds(T2* buf, const FftPlan& pl, const T2* __restrict__ tw, i
         case 4: stockham_pass<T2, 4, NT, (NMAX / 4 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
         case 5: stockham_pass<T2, 5, NT, (NMAX / 5 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
         case 8: stockham_pass<T2, 8, NT, (NMAX / 8 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
-        case 7: stockham_pass<T2, 7, NT, (NMAX / 7 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
-        case 11: stockham_pass<T2, 11, NT, (NMAX / 11 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
-        case 13: stockham_pass<T2, 13, NT, (NMAX / 13 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
+        case 7:
+            if (!SMOOTH5) stockham_pass<T2, 7, NT, (NMAX / 7 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid);
+            break;
+        case 11:
+            if (!SMOOTH5) stockham_pass<T2, 11, NT, (NMAX / 11 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid);
+            break;
+        case 13:
+            if (!SMOOTH5) stockham_pass<T2, 13, NT, (NMAX / 13 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid);
+            break;
         default: break;
         }
         Ns *= R;
     }
+}
+
+// Copies a plan's twiddle table to LDS (all threads of the block).
+template <typename T2>
+__device__ __forceinline__ void stage_twiddles(T2* dst, const T2* __restrict__ src, int count, int tid,
+                                               int nthreads)
+{
+    for (int i = tid; i < count; i += nthreads) dst[i] = src[i];
 }
 
 }  // namespace wct

@@ -3,7 +3,7 @@ import ctypes as C
 import os
 
 _PKG = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
-LIB_PATH = os.path.join(_PKG, "libwct_hip.so")
+LIB_PATH = os.environ.get("WCT_HIP_LIBRARY") or os.path.join(_PKG, "libwct_hip.so")
 _LIB = None
 
 

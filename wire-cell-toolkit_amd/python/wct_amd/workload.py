@@ -110,6 +110,8 @@ def config_planes(name):
         return [(k, nw, 6000) for k, nw in APA_PLANES]
     if name == "batch24":         # 8 APAs x 3 planes, Nt cycling 6000/8000/10000
         return [(k, nw, (6000, 8000, 10000)[a % 3]) for a in range(8) for k, nw in APA_PLANES]
+    if name.startswith("apa") and name[3:].isdigit():   # N APAs of U/V/W 800/800/960 x 6000
+        return [(k, nw, 6000) for a in range(int(name[3:])) for k, nw in APA_PLANES]
     if name == "apa150":          # 150 APAs x 2560 channels x 6000 ticks
         return [(k, nw, 6000) for a in range(150) for k, nw in APA_PLANES]
     raise KeyError(name)
