@@ -42,6 +42,14 @@ int wct_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream);
 int wct_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
 int wct_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
 int wct_memset(void* dptr, int value, size_t bytes, void* stream);
+/* Page-locked host memory for the staging buffers of the C++ host
+ * components (hipHostMalloc; freed with wct_host_free). */
+int wct_host_malloc(void** hptr, size_t bytes);
+int wct_host_free(void* hptr);
+/* Asynchronous copies on a stream (the wct_memcpy_htod/dtoh pair above
+ * waits for the copy). */
+int wct_memcpy_htod_async(void* dst, const void* src, size_t bytes, void* stream);
+int wct_memcpy_dtoh_async(void* dst, const void* src, size_t bytes, void* stream);
 int wct_stream_create(void** stream);
 int wct_stream_destroy(void* stream);
 int wct_stream_sync(void* stream);

@@ -29,6 +29,54 @@ long wcth_fft_best_length(long n, int keep_odd_even);
 int wcth_wire_region_average(int npaths, const double* pitchpos, const double* current, int nsamp,
                              double pitch, int max_regions, int* regions_out, double* avg_out);
 
+/* ---- the component layer (C++ classes registered by type name) ---- */
+const char* wcth_last_error(void);
+/* Configure components from a JSON list of {"type", "name", "data"}
+ * objects, in order, each with default_configuration() updated by
+ * "data" (apps/src/Main.cxx:330-350). */
+int wcth_configure(const char* json);
+/* Drop all component instances. */
+int wcth_reset(void);
+/* Newline-separated registered type names; returns the string length. */
+int wcth_known_types(char* buf, size_t cap);
+
+/* Calls an IDFT component ("type:name") through its virtual interface
+ * on host arrays (interleaved complex float, row-major).  1d ops use
+ * nrows as the size. */
+enum {
+    WCTH_FWD1D = 0, WCTH_INV1D, WCTH_FWD1B, WCTH_INV1B, WCTH_FWD2D, WCTH_INV2D,
+    WCTH_TRANSPOSE_REAL, WCTH_TRANSPOSE_COMPLEX
+};
+int wcth_dft(const char* tn, int op, const float* in, float* out, int nrows, int ncols, int axis);
+
+/* IFilterWaveform::filter_waveform(n) of component tn into out[n]. */
+int wcth_filter_waveform(const char* tn, int n, float* out);
+
+/* Frames: build an input frame, run an IFrameFilter ("type:name") on it
+ * (in = NULL is end-of-stream; *out = NULL then), read the result. */
+void* wcth_frame_new(int ident, double time, double tick);
+void wcth_frame_free(void* frame);
+int wcth_frame_add_trace(void* frame, int channel, int tbin, int n, const float* charge);
+int wcth_frame_add_mask(void* frame, const char* name, int channel, int first, int second);
+int wcth_frame_filter(const char* tn, void* in, void** out);
+int wcth_frame_info(void* frame, int* ident, double* time, double* tick, int* ntraces);
+int wcth_frame_trace(void* frame, int i, int* channel, int* tbin, int* n, const float** charge);
+/* which = 0: frame tags, 1: trace tags.  i < 0 returns the count. */
+int wcth_frame_tags(void* frame, int which, int i, const char** tag);
+int wcth_frame_tagged(void* frame, const char* tag, const size_t** idx, int* n);
+int wcth_frame_summary(void* frame, const char* tag, const double** val, int* n);
+/* Ranges of one channel-mask-map entry; returns the total count (only
+ * the first cap are written). */
+int wcth_frame_masks(void* frame, const char* name, int* channels, int* firsts, int* seconds, int cap);
+
+/* ROI_formation::find_ROI_by_decon_itself's run scan, extend_ROI_self
+ * and create_ROI_connect_info (sigproc/src/ROI_formation.cxx:167-360,
+ * 458-531) over a threshold mask (nonzero = above threshold), nw rows
+ * of nt bytes at row stride ld.  roi_count[nw] and up to cap inclusive
+ * (first, last) pairs; returns the pair count, -2 if cap was exceeded. */
+int wcth_roi_lists(const unsigned char* mask, int nw, int nt, long ld, int pad, float asy,
+                   int* roi_count, int* roi_pairs, int cap);
+
 #ifdef __cplusplus
 }
 #endif

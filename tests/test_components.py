@@ -1,0 +1,177 @@
+"""The C++ component layer on the CPU: configuration, filters, the ROI
+list builder against the oracle and brute-force cases, frame handles,
+and the frame filter's end-of-stream / empty-frame / configuration error
+behaviour (OmnibusSigProc.cxx:54-217, :1382-1400)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+@pytest.fixture(autouse=True)
+def _fresh():
+    from wct_amd import components as K
+    K.reset()
+    yield
+    K.reset()
+
+
+def test_filters_match_reference_formula():
+    from wct_amd import components as K
+    from wct_amd import workload as W
+    K.configure([
+        {"type": "HfFilter", "name": "Wiener_tight_W", "data": W.WIENER},
+        {"type": "HfFilter", "name": "Wire_col", "data": W.WIRE},
+        {"type": "LfFilter", "name": "ROI_tight_lf", "data": W.LF_TIGHT},
+    ])
+    n = 6000
+    f = np.arange(n) * 2 * W.MHZ / n
+    f = np.where(f > W.MHZ, f - 2 * W.MHZ, f)
+    hf = np.exp(-0.5 * (np.abs(f) / W.WIENER["sigma"]) ** W.WIENER["power"])
+    hf[0] = 0.0   # flag: zero at DC
+    np.testing.assert_allclose(K.filter_waveform("HfFilter:Wiener_tight_W", n), hf.astype(np.float32),
+                               rtol=1e-6, atol=1e-30)
+    lf = 1 - np.exp(-(np.abs(f) / W.LF_TIGHT["tau"]) ** 2)
+    np.testing.assert_allclose(K.filter_waveform("LfFilter:ROI_tight_lf", n), lf.astype(np.float32),
+                               rtol=1e-6, atol=1e-7)
+    nw = 480
+    g = np.arange(nw) * 2.0 / nw
+    g = np.where(g > 1.0, g - 2.0, g)
+    np.testing.assert_allclose(K.filter_waveform("HfFilter:Wire_col", nw),
+                               np.exp(-0.5 * (np.abs(g) / 0.75) ** 2).astype(np.float32), rtol=1e-6)
+
+
+def test_defaults_follow_reference_headers():
+    from wct_amd import components as K
+    # HfFilter() = (1 MHz, 3 MHz, 2, true): at f = 0.5 MHz exp(-0.5 (1/6)^2)
+    n = 8
+    v = K.filter_waveform("HfFilter:dflt", n)
+    assert v[0] == 0.0
+    assert v[2] == pytest.approx(np.exp(-0.5 * (0.5 / 3.0) ** 2), rel=1e-6)
+    # LfFilter() tau = 0.02 MHz
+    v = K.filter_waveform("LfFilter:dflt", n)
+    assert v[1] == pytest.approx(1 - np.exp(-(0.25 / 0.02) ** 2), rel=1e-6)
+
+
+def _random_mask(rng, nw, nt, density, runlen):
+    m = np.zeros((nw, nt), np.uint8)
+    for w in range(nw):
+        k = rng.poisson(density * nt / runlen)
+        for _ in range(k):
+            a = rng.integers(0, nt)
+            m[w, a:a + rng.integers(1, 2 * runlen)] = rng.integers(1, 4)
+    return m
+
+
+@pytest.mark.parametrize("seed,nw,nt,density,runlen,pad,asy", [
+    (1, 40, 300, 0.05, 6, 5, 0.1), (2, 64, 1000, 0.15, 12, 5, 0.1), (3, 30, 200, 0.4, 3, 2, 0.3),
+    (4, 80, 500, 0.02, 20, 5, 0.1), (5, 16, 64, 0.9, 8, 0, 0.5)])
+def test_roi_lists_match_oracle(seed, nw, nt, density, runlen, pad, asy):
+    from wct_amd import components as K
+    rng = np.random.default_rng(seed)
+    m = _random_mask(rng, nw, nt, density, runlen)
+    assert K.roi_lists(m, pad, asy) == O.roi_lists(m, pad, asy)
+
+
+def test_roi_lists_edge_cases():
+    from wct_amd import components as K
+    nt = 50
+    m = np.zeros((4, nt), np.uint8)
+    m[0, nt - 1] = 1            # a run cannot start on the last tick
+    m[1, :] = 1                 # whole row
+    m[2, 0] = 1                 # clamped at 0 after padding
+    m[3, 10:13] = 1
+    m[3, 14] = 1                # a separate run: 13 is below threshold
+    got = K.roi_lists(m, pad=5, asy=0.1)
+    assert got == O.roi_lists(m, pad=5, asy=0.1)
+    assert got[0] == []
+    assert got[1] == [(0, nt - 1)]
+    assert got[2][0] == (0, 5)
+    assert K.roi_lists(np.zeros((0, 10), np.uint8)) == []
+    assert K.roi_lists(np.zeros((3, 1), np.uint8)) == [[], [], []]
+
+
+def test_roi_connect_by_brute_force():
+    """Wire 1 gains the midpoint ROI of wires 0 and 2 (similar lengths),
+    and not when lengths differ or it already overlaps."""
+    from wct_amd import components as K
+    nt = 200
+    m = np.zeros((3, nt), np.uint8)
+    m[0, 50:61] = 1
+    m[2, 54:65] = 1
+    rois = K.roi_lists(m, pad=0, asy=0.1)
+    assert rois[1] == [(52, 62)]
+    m[2, 54:100] = 1
+    assert K.roi_lists(m, pad=0, asy=0.1)[1] == []
+    m[2, 54:65] = 1
+    m[2, 65:] = 0
+    m[1, 55:58] = 1
+    assert K.roi_lists(m, pad=0, asy=0.1)[1] == [(55, 57)]
+
+
+def test_frame_handles_round_trip():
+    from wct_amd import components as K
+    f = K.Frame(7, 2.5, 500.0)
+    f.add_trace(11, [1.0, -2.0, 3.0], tbin=4)
+    f.add_trace(12, np.arange(5))
+    f.add_mask("bad", 11, 0, 3)
+    f.add_mask("bad", 11, 10, 20)
+    assert f.info() == dict(ident=7, time=2.5, tick=500.0, ntraces=2)
+    ch, tb, q = f.trace(0)
+    assert (ch, tb) == (11, 4) and list(q) == [1.0, -2.0, 3.0]
+    assert f.masks("bad") == {11: [(0, 3), (10, 20)]}
+    assert f.masks("none") == {}
+
+
+def _osp_config(extra=None):
+    data = {"anode": "AnodeChannels:a0"}
+    data.update(extra or {})
+    return [{"type": "AnodeChannels", "name": "a0", "data": {"plane_ranges": [[0, 8], [100, 8], [200, 10]]}},
+            {"type": "HipDFT", "name": ""},
+            {"type": "HipOmnibusSigProc", "name": "osp", "data": data}]
+
+
+def test_sigproc_eos_and_empty_frame():
+    from wct_amd import components as K
+    K.configure(_osp_config())
+    assert K.frame_filter("HipOmnibusSigProc:osp", None) is None
+    out = K.frame_filter("HipOmnibusSigProc:osp", K.Frame(3, 1.5, 500.0))
+    assert out.info() == dict(ident=3, time=1.5, tick=500.0, ntraces=0)
+    assert out.frame_tags() == []
+
+
+@pytest.mark.parametrize("extra,msg", [({"per_chan_resp": "PerChannelResponse"}, "per_chan_resp"),
+                                       ({"fft_flag": 1}, "fft_flag"),
+                                       ({"process_planes": [0, 3]}, "process_planes"),
+                                       ({"anode": "AnodeChannels:missing_planes"}, "plane")])
+def test_sigproc_configuration_errors(extra, msg):
+    from wct_amd import components as K
+    from wct_amd.capi import WctError
+    with pytest.raises(WctError, match=msg):
+        K.configure(_osp_config(extra))
+
+
+def test_unknown_component_type():
+    from wct_amd import components as K
+    from wct_amd.capi import WctError
+    with pytest.raises(WctError, match="unknown component type"):
+        K.configure([{"type": "NoSuchThing", "name": "x"}])
+
+
+def test_configuration_parse_error():
+    from wct_amd.host import lib
+    rc = lib().wcth_configure(b'[{"type": "HfFilter", "name": "x", "data": {"sigma": }}]')
+    assert rc != 0
+    assert "JSON parse error" in lib().wcth_last_error().decode()
+
+
+def test_sigproc_full_configuration_parses_field_response(tmp_path):
+    """The three-plane configuration of the device tests, including a
+    field-response file whose currents reach subnormal values."""
+    from wct_amd import components as K
+    from test_gpu_components import _sigproc_config
+    cfg = _sigproc_config(tmp_path)
+    text = open(tmp_path / "fr.json").read()
+    assert "e-310" in text or "e-309" in text
+    K.configure(cfg)
+    assert K.frame_filter("HipOmnibusSigProc:osp", None) is None

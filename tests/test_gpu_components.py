@@ -1,0 +1,277 @@
+"""The C++ components on the device, called through their interfaces.
+
+HipDFT (IDFT): the aux/test/test_idft.cxx cases (impulse, flat, zero,
+round trips, 1b both axes, 2d, transposes; tolerance 1e-8 absolute on
+those exact cases) plus random data against the FP64 oracle DFT, and
+calls from several threads at once.
+
+HipOmnibusSigProc (IFrameFilter): a three-plane anode frame pushed
+through the component must give exactly what the C ABI gives on the
+same data (dense wiener traces, per-wire RMS summary), save_data's
+post-processing (negatives and bad ranges zeroed; debug mode keeps
+negatives; sparse runs), and tight ROI lists equal to the oracle's
+run/extend/connect rules applied to the device mask.
+"""
+import json
+import os
+import threading
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def K(dev):
+    from wct_amd import components
+    components.reset()
+    yield components
+    components.reset()
+
+
+def _crand(rng, *shape):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+
+
+def test_hipdft_reference_cases(K):
+    K.configure([{"type": "HipDFT", "name": ""}])
+    for n in (1, 2, 8, 60, 6000):
+        imp = np.zeros(n, np.complex64)
+        imp[0] = 1
+        np.testing.assert_allclose(K.dft("HipDFT", "fwd1d", imp), np.ones(n), atol=1e-8)
+        flat = np.ones(n, np.complex64)
+        back = K.dft("HipDFT", "inv1d", K.dft("HipDFT", "fwd1d", imp))
+        np.testing.assert_allclose(back, imp, atol=1e-7)
+        np.testing.assert_allclose(K.dft("HipDFT", "fwd1d", flat)[0], n, rtol=1e-7)
+        assert not np.any(K.dft("HipDFT", "fwd1d", np.zeros(n, np.complex64)))
+    for shape in [(4, 6), (16, 1), (1, 16), (64, 96)]:
+        imp = np.zeros(shape, np.complex64)
+        imp[0, 0] = 1
+        for axis in (0, 1):
+            np.testing.assert_allclose(K.dft("HipDFT", "fwd1b", imp, axis), np.fft.fft(imp, axis=axis), atol=1e-8)
+        np.testing.assert_allclose(K.dft("HipDFT", "fwd2d", imp), np.ones(shape), atol=1e-8)
+        np.testing.assert_allclose(K.dft("HipDFT", "inv2d", np.ones(shape, np.complex64)), imp, atol=1e-7)
+
+
+@pytest.mark.parametrize("shape", [(48, 600), (960, 64), (77, 143)])
+def test_hipdft_random_against_oracle(K, shape):
+    K.configure([{"type": "HipDFT", "name": "gpu0", "data": {"device": 0}}])
+    rng = np.random.default_rng(shape[0])
+    x = _crand(rng, *shape)
+    for axis in (0, 1):
+        ref = np.fft.fft(x.astype(np.complex128), axis=axis)
+        got = K.dft("HipDFT:gpu0", "fwd1b", x, axis)
+        assert np.max(np.abs(got - ref)) <= 2e-6 * np.max(np.abs(ref))
+        back = K.dft("HipDFT:gpu0", "inv1b", got, axis)
+        assert np.max(np.abs(back - x)) <= 2e-6 * np.max(np.abs(x))
+    ref2 = O.fwd2d(x, np.float64)
+    got2 = K.dft("HipDFT:gpu0", "fwd2d", x)
+    assert np.max(np.abs(got2 - ref2)) <= 2e-6 * np.max(np.abs(ref2))
+    xr = rng.standard_normal(shape).astype(np.float32)
+    np.testing.assert_array_equal(K.dft("HipDFT:gpu0", "transpose", xr), xr.T)
+    np.testing.assert_array_equal(K.dft("HipDFT:gpu0", "transpose", x), x.T)
+
+
+def test_hipdft_concurrent_threads(K):
+    K.configure([{"type": "HipDFT", "name": ""}])
+    rng = np.random.default_rng(5)
+    xs = [_crand(rng, 8, 1000 + 100 * i) for i in range(6)]
+    errs = [None] * len(xs)
+
+    def work(i):
+        try:
+            for _ in range(5):
+                got = K.dft("HipDFT", "fwd1b", xs[i], 1)
+                ref = np.fft.fft(xs[i].astype(np.complex128), axis=1)
+                assert np.max(np.abs(got - ref)) <= 2e-6 * np.max(np.abs(ref))
+        except Exception as e:    # reported below
+            errs[i] = e
+
+    th = [threading.Thread(target=work, args=(i,)) for i in range(len(xs))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert errs == [None] * len(xs)
+
+
+# ---- HipOmnibusSigProc -------------------------------------------------
+
+NW = (48, 48, 56)
+NT = 1200
+CH0 = (1000, 2000, 3000)
+
+
+def _sigproc_config(tmpdir, extra=None):
+    from wct_amd import components as K
+    from wct_amd import workload as W
+    planes = []
+    for ip, kind in enumerate(("ind", "ind", "col")):
+        pos, cur = W.fr_paths(kind)
+        planes.append((ip, 0.0, W.PITCH, pos, cur))
+    frj = K.field_response_json(planes, W.ORIGIN, 0.0, W.FINE_PERIOD, W.SPEED)
+    path = os.path.join(str(tmpdir), "fr.json")
+    with open(path, "w") as f:
+        json.dump(frj, f)
+    data = {"anode": "AnodeChannels:apa0", "field_response": "FieldResponse:synth",
+            "elecresponse": "ColdElecResponse:ce", "ctoffset": W.CTOFFSET, "ftoffset": W.FTOFFSET,
+            "postgain": W.INTER_GAIN, "ADC_mV": W.ADC_MV, "troi_ind_th_factor": 3.0,
+            "troi_col_th_factor": 5.0}
+    data.update(extra or {})
+    wire = dict(W.WIRE)
+    return [
+        {"type": "AnodeChannels", "name": "apa0",
+         "data": {"ident": 0, "plane_ranges": [[CH0[i], NW[i]] for i in range(3)]}},
+        {"type": "FieldResponse", "name": "synth", "data": {"filename": path}},
+        {"type": "ColdElecResponse", "name": "ce",
+         "data": {"gain": W.GAIN, "shaping": W.SHAPING, "postgain": 1.0}},
+        {"type": "HfFilter", "name": "Wiener_tight_U", "data": W.WIENER},
+        {"type": "HfFilter", "name": "Wiener_tight_V", "data": W.WIENER},
+        {"type": "HfFilter", "name": "Wiener_tight_W", "data": W.WIENER},
+        {"type": "HfFilter", "name": "Wire_ind", "data": wire},
+        {"type": "HfFilter", "name": "Wire_col", "data": wire},
+        {"type": "LfFilter", "name": "ROI_tight_lf", "data": W.LF_TIGHT},
+        {"type": "LfFilter", "name": "ROI_tighter_lf", "data": W.LF_TIGHTER},
+        {"type": "HipDFT", "name": ""},
+        {"type": "HipOmnibusSigProc", "name": "osp", "data": data},
+    ]
+
+
+def _adc_planes(seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for ip in range(3):
+        a = rng.normal(0.0, 1.5, (NW[ip], NT)).astype(np.float32)
+        for _ in range(4):   # a few bipolar/unipolar pulses
+            w, t = rng.integers(2, NW[ip] - 2), rng.integers(100, NT - 100)
+            a[w - 1:w + 2, t:t + 20] += (40.0 if ip == 2 else 25.0) * np.hanning(20)
+        out.append(a)
+    return out
+
+
+def _expected(adc, bad, th):
+    """The same planes through the C ABI (workload's spec, padded to
+    plane 0's 21 response regions as the reference sizes them)."""
+    import torch
+    from wct_amd import decon as D
+    from wct_amd import workload as W
+    dev = torch.device("cuda:0")
+    kinds = ("ind", "ind", "col")
+    specs, kernels, jobs, keep = [], [], [], []
+    nfine_rows = len(W.fr_paths("ind")[0])
+    for ip, kind in enumerate(kinds):
+        sp = W.plane_spec(kind, NW[ip], NT, fold_lf=False)
+        rows = np.zeros((nfine_rows, sp.fr_rows.shape[1]))
+        rows[:sp.fr_rows.shape[0]] = sp.fr_rows
+        sp.fr_rows = rows
+        sp.wire_shift = (nfine_rows - 1) // 2
+        pk = D.PlaneKernel(sp)
+        a = torch.from_numpy(adc[ip]).to(dev)
+        y = torch.empty_like(a)
+        m = torch.empty(a.shape, dtype=torch.uint8, device=dev)
+        r = torch.empty(NW[ip], dtype=torch.float32, device=dev)
+        ptr, lo, hi = [0], [], []
+        for w in range(NW[ip]):
+            for (b0, b1) in bad.get((ip, w), []):
+                lo.append(b0)
+                hi.append(b1)
+            ptr.append(len(lo))
+        badt = tuple(torch.tensor(v if v else [0], dtype=torch.int32, device=dev) for v in (ptr, lo, hi))
+        lt = lth = None
+        if kind == "ind":
+            lt = torch.from_numpy(sp.lf_tight[:pk.kc].astype(np.float32)).to(dev)
+            lth = torch.from_numpy(sp.lf_tighter[:pk.kc].astype(np.float32)).to(dev)
+        jobs.append(D.make_job(pk, a, y, mask=m, rms=r, th_factor=th[ip], restore=1, save_mode=1,
+                               lf_tight=lt, lf_tighter=lth, bad=badt))
+        keep += [pk, a, y, m, r, badt, lt, lth]
+        specs.append((y, m, r))
+    b = D.Batch(jobs, device=dev)
+    b.run()
+    torch.cuda.synchronize()
+    res = [(y.cpu().numpy(), m.cpu().numpy(), r.cpu().numpy()) for (y, m, r) in specs]
+    b.close()
+    return res
+
+
+def _frame(K, adc, bad):
+    f = K.Frame(17, 3.0, 500.0)
+    for ip in range(3):
+        for w in range(NW[ip]):
+            f.add_trace(CH0[ip] + w, adc[ip][w])
+    for (ip, w), ranges in bad.items():
+        for (b0, b1) in ranges:
+            f.add_mask("bad", CH0[ip] + w, b0, b1)
+    f.add_mask("bad", 99999, 0, 10)   # a channel of another anode is ignored
+    return f
+
+
+def test_sigproc_matches_c_abi(K, tmp_path):
+    print("sigproc vs C ABI, 3 planes", NW, "x", NT, flush=True)
+    K.configure(_sigproc_config(tmp_path))
+    adc = _adc_planes(3)
+    bad = {(0, 5): [(100, 300)], (2, 7): [(0, NT)], (2, 8): [(500, 510), (900, 950)]}
+    out = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
+    exp = _expected(adc, bad, (3.0, 3.0, 5.0))
+    info = out.info()
+    assert info["ident"] == 17 and info["time"] == 3.0 and info["tick"] == 500.0
+    assert out.frame_tags() == ["sigproc"]
+    assert out.trace_tags() == ["wiener"]
+    tr = out.traces()
+    assert len(tr) == sum(NW)
+    idx = out.tagged_traces("wiener")
+    assert list(idx) == list(range(sum(NW)))
+    summ = out.trace_summary("wiener")
+    k = 0
+    for ip in range(3):
+        y, m, r = exp[ip]
+        for w in range(NW[ip]):
+            ch, tb, q = tr[k]
+            assert ch == CH0[ip] + w and tb == 0
+            np.testing.assert_array_equal(q, y[w])
+            assert summ[k] == r[w]
+            k += 1
+        assert np.all(y >= 0)
+    # bad ranges zeroed, input "bad" map passed through
+    assert not np.any(tr[5][2][100:300])
+    assert not np.any(tr[NW[0] + NW[1] + 7][2])
+    assert out.masks("bad")[CH0[0] + 5] == [(100, 300)]
+    # tight ROIs: the oracle's rules over the device mask
+    rois = out.masks("tight_roi")
+    for ip in range(3):
+        y, m, r = exp[ip]
+        ref = O.roi_lists(m, pad=5, asy=0.1)
+        for w in range(NW[ip]):
+            assert rois.get(CH0[ip] + w, []) == [(a, b + 1) for (a, b) in ref[w]]
+    assert any(rois.get(CH0[2] + w) for w in range(NW[2]))
+    # a second frame of the same shape reuses the spectra and gives the same
+    out2 = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
+    np.testing.assert_array_equal(out2.trace(3)[2], tr[3][2])
+
+
+def test_sigproc_debug_and_sparse(K, tmp_path):
+    print("sigproc debug + sparse", flush=True)
+    K.configure(_sigproc_config(tmp_path, {"use_roi_debug_mode": True, "process_planes": [2]}))
+    adc = _adc_planes(4)
+    bad = {(2, 3): [(10, 20)]}
+    out = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
+    assert sorted(out.trace_tags()) == ["tight_lf", "wiener"]
+    tr = out.traces()
+    assert len(tr) == NW[2] and tr[0][0] == CH0[2]
+    assert any(np.any(q < 0) for _, _, q in tr)         # debug keeps negatives
+    assert not np.any(tr[3][2][10:20])
+    dense = {ch: q for ch, _, q in tr}
+
+    K.reset()
+    K.configure(_sigproc_config(tmp_path, {"sparse": True, "process_planes": [2]}))
+    out = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
+    rebuilt = {}
+    summ = out.trace_summary("wiener")
+    assert len(summ) == out.info()["ntraces"]
+    for ch, tb, q in out.traces():
+        assert q.size > 0 and np.all(q > 0)
+        rebuilt.setdefault(ch, np.zeros(NT, np.float32))[tb:tb + q.size] = q
+    for ch, q in dense.items():
+        np.testing.assert_array_equal(rebuilt.get(ch, np.zeros(NT, np.float32)), np.maximum(q, 0))

@@ -235,6 +235,36 @@ int wct_memset(void* dptr, int value, size_t bytes, void* stream)
     return WCT_OK;
 }
 
+int wct_host_malloc(void** hptr, size_t bytes)
+{
+    WCT_REQUIRE(hptr, WCT_ERR_ARG, "null pointer");
+    hipError_t e = hipHostMalloc(hptr, bytes ? bytes : 1, hipHostMallocDefault);
+    if (e == hipErrorOutOfMemory) {
+        set_error("hipHostMalloc(%zu) out of memory", bytes);
+        return WCT_ERR_NOMEM;
+    }
+    WCT_HIP_CHECK(e);
+    return WCT_OK;
+}
+
+int wct_host_free(void* hptr)
+{
+    if (hptr) WCT_HIP_CHECK(hipHostFree(hptr));
+    return WCT_OK;
+}
+
+int wct_memcpy_htod_async(void* dst, const void* src, size_t bytes, void* stream)
+{
+    WCT_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+    return WCT_OK;
+}
+
+int wct_memcpy_dtoh_async(void* dst, const void* src, size_t bytes, void* stream)
+{
+    WCT_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+    return WCT_OK;
+}
+
 int wct_stream_create(void** stream)
 {
     WCT_REQUIRE(stream, WCT_ERR_ARG, "null pointer");

@@ -1,0 +1,566 @@
+// HipOmnibusSigProc: the IFrameFilter of OmnibusSigProc
+// (sigproc/src/OmnibusSigProc.cxx) with its per-plane deconvolution and
+// tight-ROI stage on the gfx950 kernels of libwct_hip.
+//
+// Per frame (operator(), :1382-1702): EOS and empty frames are forwarded
+// as in the reference; channel masks are re-indexed per plane (:1402-1416);
+// every processed plane is loaded (load_data, :304-352), deconvolved
+// (decon_2D_init, :944-1051), filtered with the tight time filter
+// (decon_2D_tightROI, :1077-1117, and decon_2D_tighterROI for the
+// induction planes, :1120-1160), baseline-restored (:867-906), and
+// thresholded (ROI_formation::find_ROI_by_decon_itself with cal_RMS,
+// sigproc/src/ROI_formation.cxx:364-531).  All planes of a frame run as
+// one batched launch per kernel stage.  The output follows save_data
+// (:371-472): traces tagged "wiener" carry the tight-stage decon
+// (negatives and bad ranges zeroed unless use_roi_debug_mode) with the
+// per-wire RMS as trace summary; use_roi_debug_mode adds the same traces
+// under "tight_lf"; the tight ROIs (extended and connected) are added to
+// the output channel-mask map under "tight_roi_mask" as half-open tick
+// ranges.  The later ROI refinement stages (loose ROIs, break/shrink/
+// extend, hits and charge decons under "wiener"/"gauss") are not part of
+// this component.
+//
+// The response spectra are built once per (nticks, tick) on the device
+// in FP64 and reused for later frames of the same shape (the reference
+// rebuilds them per frame, :1429; the inputs do not change).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <unordered_map>
+
+#include "../../include/wct_hip.h"
+#include "Components.h"
+#include "Factory.h"
+#include "Frame.h"
+#include "Interface.h"
+#include "Response.h"
+#include "RoiFormation.h"
+
+namespace WireCell {
+namespace Hip {
+
+struct SigProcError : public std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+static void hcheck(int rc, const char* what)
+{
+    if (rc != WCT_OK) throw SigProcError(std::string("HipOmnibusSigProc ") + what + ": " + wct_last_error());
+}
+
+// Device allocation owned by a component (freed on destruction).
+struct DevBuf {
+    void* p{nullptr};
+    size_t bytes{0};
+    void ensure(size_t n)
+    {
+        if (n <= bytes && p) return;
+        release();
+        hcheck(wct_malloc(&p, n), "device allocation");
+        bytes = n;
+    }
+    void release()
+    {
+        if (p) wct_free(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+struct HostBuf {
+    void* p{nullptr};
+    size_t bytes{0};
+    void ensure(size_t n)
+    {
+        if (n <= bytes && p) return;
+        release();
+        hcheck(wct_host_malloc(&p, n), "pinned allocation");
+        bytes = n;
+    }
+    void release()
+    {
+        if (p) wct_host_free(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
+  public:
+    ~HipOmnibusSigProc() override { release(); }
+
+    Configuration default_configuration() const override
+    {
+        Configuration cfg;
+        cfg["anode"] = m_anode_tn;
+        cfg["dft"] = m_dft_tn;
+        cfg["field_response"] = m_field_response;
+        cfg["elecresponse"] = m_elecresponse_tn;
+        cfg["per_chan_resp"] = m_per_chan_resp;
+        cfg["ftoffset"] = m_fine_time_offset;
+        cfg["ctoffset"] = m_coarse_time_offset;
+        cfg["fft_flag"] = 0;
+        cfg["postgain"] = m_inter_gain;
+        cfg["ADC_mV"] = m_ADC_mV;
+        cfg["troi_ind_th_factor"] = (double)m_th_factor_ind;
+        cfg["troi_col_th_factor"] = (double)m_th_factor_col;
+        cfg["troi_pad"] = m_pad;
+        cfg["troi_asy"] = (double)m_asy;
+        Configuration pp = Configuration::makeArray();
+        for (int p : m_process_planes) pp.append(p);
+        cfg["process_planes"] = pp;
+        cfg["wiener_tag"] = m_wiener_tag;
+        cfg["gauss_tag"] = m_gauss_tag;
+        cfg["frame_tag"] = m_frame_tag;
+        cfg["tight_lf_tag"] = m_tight_lf_tag;
+        cfg["use_roi_debug_mode"] = m_use_roi_debug_mode;
+        cfg["use_roi_refinement"] = m_use_roi_refinement;
+        cfg["sparse"] = m_sparse;
+        // Keys of this component only.
+        cfg["tight_roi_mask"] = m_tight_roi_mask;
+        cfg["spectral_cut"] = m_spectral_cut;
+        cfg["device"] = m_device;
+        return cfg;
+    }
+
+    void configure(const Configuration& config) override
+    {
+        m_anode_tn = get(config, "anode", m_anode_tn);
+        m_dft_tn = get(config, "dft", m_dft_tn);
+        m_field_response = get(config, "field_response", m_field_response);
+        m_elecresponse_tn = get(config, "elecresponse", m_elecresponse_tn);
+        m_per_chan_resp = get(config, "per_chan_resp", m_per_chan_resp);
+        if (!m_per_chan_resp.empty())
+            throw ConfigError("HipOmnibusSigProc: per-channel response correction (per_chan_resp = \"" +
+                              m_per_chan_resp + "\") is not supported; set it to \"\"");
+        if (get(config, "fft_flag", 0) != 0)
+            throw ConfigError("HipOmnibusSigProc: fft_flag != 0 (padded transforms) is not supported");
+        m_fine_time_offset = get(config, "ftoffset", m_fine_time_offset);
+        m_coarse_time_offset = get(config, "ctoffset", m_coarse_time_offset);
+        m_inter_gain = get(config, "postgain", m_inter_gain);
+        m_ADC_mV = get(config, "ADC_mV", m_ADC_mV);
+        m_th_factor_ind = get(config, "troi_ind_th_factor", m_th_factor_ind);
+        m_th_factor_col = get(config, "troi_col_th_factor", m_th_factor_col);
+        m_pad = get(config, "troi_pad", m_pad);
+        m_asy = get(config, "troi_asy", m_asy);
+        if (config["process_planes"].isArray()) {
+            m_process_planes.clear();
+            for (const auto& p : config["process_planes"]) {
+                const int ip = p.asInt();
+                if (ip < 0 || ip > 2) throw ConfigError("HipOmnibusSigProc: process_planes entries must be 0, 1 or 2");
+                m_process_planes.push_back(ip);
+            }
+        }
+        m_wiener_tag = get(config, "wiener_tag", m_wiener_tag);
+        m_gauss_tag = get(config, "gauss_tag", m_gauss_tag);
+        m_frame_tag = get(config, "frame_tag", m_frame_tag);
+        m_tight_lf_tag = get(config, "tight_lf_tag", m_tight_lf_tag);
+        m_use_roi_debug_mode = get(config, "use_roi_debug_mode", m_use_roi_debug_mode);
+        m_use_roi_refinement = get(config, "use_roi_refinement", m_use_roi_refinement);
+        m_sparse = get(config, "sparse", m_sparse);
+        m_tight_roi_mask = get(config, "tight_roi_mask", m_tight_roi_mask);
+        m_spectral_cut = get(config, "spectral_cut", m_spectral_cut);
+        m_device = get(config, "device", m_device);
+
+        // The dft component must exist, as in the reference (:62-63).
+        Factory::find_tn<IDFT>(m_dft_tn);
+
+        // OSP channel map (:171-216): WCT channel -> (plane, wire).
+        auto anode = Factory::find_tn<IAnodeChannels>(m_anode_tn);
+        m_chmap.clear();
+        for (int ip = 0; ip < 3; ++ip) {
+            m_channels[ip] = anode->plane_channels(ip);
+            for (size_t w = 0; w < m_channels[ip].size(); ++w) m_chmap[m_channels[ip][w]] = {ip, (int)w};
+        }
+        if (m_chmap.empty())
+            throw ConfigError("HipOmnibusSigProc: anode " + m_anode_tn + " has no channels in any plane");
+        release();
+    }
+
+    bool operator()(const input_pointer& in, output_pointer& out) override
+    {
+        out = nullptr;
+        if (!in) return true;   // EOS
+        auto traces = in->traces();
+        if (!traces || traces->empty()) {
+            out = std::make_shared<Aux::SimpleFrame>(in->ident(), in->time(),
+                                                     std::make_shared<ITrace::vector>(), in->tick());
+            return true;
+        }
+        hcheck(wct_set_device(m_device), "set device");
+
+        // Frame extent as init_overall_response takes it (:709-722).
+        int tbinmin = traces->front()->tbin(), tbinmax = tbinmin;
+        for (const auto& t : *traces) {
+            tbinmin = std::min(tbinmin, t->tbin());
+            tbinmax = std::max(tbinmax, t->tbin() + (int)t->charge().size());
+        }
+        const int nticks = tbinmax - tbinmin;
+        if (nticks <= 0) throw SigProcError("HipOmnibusSigProc: frame without samples");
+        if (!wct_dft_supported(nticks))
+            throw SigProcError("HipOmnibusSigProc: nticks " + std::to_string(nticks) +
+                               " has a prime factor above 13");
+        prepare(nticks, in->tick());
+
+        // Bad ranges per plane, indexed by wire (:1402-1416).
+        std::vector<std::vector<Waveform::BinRange>> bad[3];
+        for (int ip = 0; ip < 3; ++ip) bad[ip].assign(m_channels[ip].size(), {});
+        const auto inmasks = in->masks();
+        auto badit = inmasks.find("bad");
+        if (badit != inmasks.end()) {
+            for (const auto& cm : badit->second) {
+                auto it = m_chmap.find(cm.first);
+                if (it == m_chmap.end()) continue;   // another anode's channel
+                auto& dst = bad[it->second.first][it->second.second];
+                dst.insert(dst.end(), cm.second.begin(), cm.second.end());
+            }
+        }
+
+        // load_data: dense (wire, tick) frames in pinned memory.
+        float* hadc = (float*)m_h_adc.p;
+        for (int ip : m_process_planes) {
+            Plane& pl = m_planes[ip];
+            std::memset(hadc + pl.off, 0, sizeof(float) * (size_t)pl.nw * nticks);
+        }
+        for (const auto& t : *traces) {
+            auto it = m_chmap.find(t->channel());
+            if (it == m_chmap.end()) continue;
+            Plane& pl = m_planes[it->second.first];
+            if (!pl.active) continue;
+            const auto& q = t->charge();
+            const int tb = t->tbin();
+            const int n = std::min((int)q.size(), nticks);
+            float* row = hadc + pl.off + (size_t)it->second.second * nticks;
+            for (int i = 0; i < n; ++i) {
+                const int k = tb + i;
+                if (k >= 0 && k < nticks) row[k] = q[i];
+            }
+        }
+
+        // Bad ranges as CSR for the device: [lo, hi) zeroed on input,
+        // [lo, hi] left out of the RMS and the mask.
+        size_t nbad_tot = 0;
+        for (int ip : m_process_planes)
+            for (auto& v : bad[ip]) nbad_tot += v.size();
+        const size_t csr_ints = [&] {
+            size_t n = 0;
+            for (int ip : m_process_planes) n += m_planes[ip].nw + 1;
+            return n + 2 * nbad_tot;
+        }();
+        std::vector<int> csr(csr_ints);
+        size_t cpos = 0;
+        std::vector<size_t> ptr_off(3), lo_off(3), hi_off(3);
+        for (int ip : m_process_planes) {
+            const int nw = m_planes[ip].nw;
+            size_t nb = 0;
+            for (auto& v : bad[ip]) nb += v.size();
+            ptr_off[ip] = cpos;
+            lo_off[ip] = cpos + nw + 1;
+            hi_off[ip] = lo_off[ip] + nb;
+            int k = 0;
+            for (int w = 0; w < nw; ++w) {
+                csr[ptr_off[ip] + w] = k;
+                for (auto& r : bad[ip][w]) {
+                    csr[lo_off[ip] + k] = std::max(0, r.first);
+                    csr[hi_off[ip] + k] = std::min(nticks, r.second);
+                    ++k;
+                }
+            }
+            csr[ptr_off[ip] + nw] = k;
+            cpos = hi_off[ip] + nb;
+        }
+        const bool plan_stale = csr.size() * sizeof(int) > m_d_bad.bytes || !m_plan;
+        m_d_bad.ensure(std::max<size_t>(csr.size(), 1) * sizeof(int));
+
+        hcheck(wct_memcpy_htod_async(m_d_adc.p, m_h_adc.p, m_adc_floats * sizeof(float), m_stream), "upload");
+        if (!csr.empty())
+            hcheck(wct_memcpy_htod_async(m_d_bad.p, csr.data(), csr.size() * sizeof(int), m_stream), "upload");
+        if (plan_stale || m_bad_layout != std::vector<size_t>{ptr_off[0], ptr_off[1], ptr_off[2], lo_off[0],
+                                                              lo_off[1], lo_off[2], hi_off[0], hi_off[1],
+                                                              hi_off[2]}) {
+            m_bad_layout = {ptr_off[0], ptr_off[1], ptr_off[2], lo_off[0], lo_off[1], lo_off[2],
+                            hi_off[0], hi_off[1], hi_off[2]};
+            hcheck(wct_stream_sync(m_stream), "sync");
+            make_plan(ptr_off, lo_off, hi_off);
+        }
+        hcheck(wct_decon_plan_run(m_plan, m_stream), "decon");
+        hcheck(wct_memcpy_dtoh_async(m_h_out.p, m_d_out.p, m_adc_floats * sizeof(float), m_stream), "download");
+        hcheck(wct_memcpy_dtoh_async(m_h_mask.p, m_d_mask.p, m_adc_floats, m_stream), "download");
+        hcheck(wct_memcpy_dtoh_async(m_h_rms.p, m_d_rms.p, m_rms_floats * sizeof(float), m_stream), "download");
+        hcheck(wct_stream_sync(m_stream), "sync");
+
+        // save_data (:371-472) per processed plane, in plane order.
+        auto itraces = std::make_shared<ITrace::vector>();
+        IFrame::trace_list_t wiener_traces;
+        IFrame::trace_summary_t thresholds;
+        Waveform::ChannelMaskMap outmasks = inmasks;
+        const float* hout = (const float*)m_h_out.p;
+        const float* hrms = (const float*)m_h_rms.p;
+        const uint8_t* hmask = (const uint8_t*)m_h_mask.p;
+        for (int ip = 0; ip < 3; ++ip) {
+            const Plane& pl = m_planes[ip];
+            if (!pl.active) continue;
+            for (int w = 0; w < pl.nw; ++w) {
+                const float* row = hout + pl.off + (size_t)w * nticks;
+                ITrace::ChargeSequence charge(row, row + nticks);
+                if (m_use_roi_debug_mode) {
+                    // the device left negatives in; bad ranges are zeroed
+                    // in both modes
+                    for (auto& r : bad[ip][w])
+                        for (int k = std::max(0, r.first); k < std::min(nticks, r.second); ++k) charge[k] = 0;
+                }
+                const double thresh = hrms[pl.rms_off + w];
+                const int ident = m_channels[ip][w];
+                if (m_sparse) {
+                    int i = 0;
+                    while (i < nticks) {
+                        while (i < nticks && !(charge[i] > 0)) ++i;
+                        if (i >= nticks) break;
+                        int e = i;
+                        while (e < nticks && charge[e] != 0) ++e;
+                        wiener_traces.push_back(itraces->size());
+                        itraces->push_back(std::make_shared<Aux::SimpleTrace>(
+                            ident, i, ITrace::ChargeSequence(charge.begin() + i, charge.begin() + e)));
+                        thresholds.push_back(thresh);
+                        i = e;
+                    }
+                } else {
+                    wiener_traces.push_back(itraces->size());
+                    itraces->push_back(std::make_shared<Aux::SimpleTrace>(ident, 0, std::move(charge)));
+                    thresholds.push_back(thresh);
+                }
+            }
+            if (!m_tight_roi_mask.empty()) {
+                auto rois = tight_rois(hmask + pl.off, pl.nw, nticks, nticks, m_pad, m_asy);
+                auto& cm = outmasks[m_tight_roi_mask];
+                for (int w = 0; w < pl.nw; ++w) {
+                    if (rois[w].empty()) continue;
+                    auto& dst = cm[m_channels[ip][w]];
+                    for (auto& r : rois[w]) dst.emplace_back(r.first, r.second + 1);
+                }
+            }
+        }
+
+        auto sframe = std::make_shared<Aux::SimpleFrame>(in->ident(), in->time(), itraces, in->tick(), outmasks);
+        sframe->tag_frame(m_frame_tag);
+        if (!m_wiener_tag.empty()) sframe->tag_traces(m_wiener_tag, wiener_traces, thresholds);
+        if (m_use_roi_debug_mode && !m_tight_lf_tag.empty()) sframe->tag_traces(m_tight_lf_tag, wiener_traces);
+        out = sframe;
+        return true;
+    }
+
+  private:
+    struct Plane {
+        bool active{false};
+        int nw{0};
+        size_t off{0};       // first sample of this plane in the frame buffers
+        size_t rms_off{0};
+        wct_decon_kernel kernel{nullptr};
+        DevBuf lf_tight, lf_tighter;
+        float th_factor{3};
+    };
+
+    void release()
+    {
+        if (m_plan) wct_decon_plan_destroy(m_plan);
+        m_plan = nullptr;
+        for (auto& pl : m_planes) {
+            if (pl.kernel) wct_decon_kernel_destroy(pl.kernel);
+            pl.kernel = nullptr;
+            pl.lf_tight.release();
+            pl.lf_tighter.release();
+            pl.active = false;
+        }
+        for (DevBuf* b : {&m_d_adc, &m_d_out, &m_d_mask, &m_d_rms, &m_d_bad, &m_d_ws}) b->release();
+        for (HostBuf* b : {&m_h_adc, &m_h_out, &m_h_mask, &m_h_rms}) b->release();
+        if (m_stream) wct_stream_destroy(m_stream);
+        m_stream = nullptr;
+        m_built_nticks = -1;
+        m_bad_layout.clear();
+    }
+
+    // Response spectra and buffers for frames of nticks at this tick
+    // (init_overall_response, :707-865).
+    void prepare(int nticks, double period)
+    {
+        if (nticks == m_built_nticks && period == m_built_period) return;
+        release();
+        hcheck(wct_set_device(m_device), "set device");
+        hcheck(wct_stream_create(&m_stream), "stream");
+
+        auto ifr = Factory::find_tn<IFieldResponse>(m_field_response);
+        const FieldResponse& fr = ifr->field_response();
+        if (fr.planes.size() < 3) throw SigProcError("HipOmnibusSigProc: field response with fewer than 3 planes");
+        std::vector<RegionAverage> avg(3);
+        for (int ip = 0; ip < 3; ++ip) avg[ip] = wire_region_average(fr.planes[ip]);
+        if (avg[0].rows.empty()) throw SigProcError("HipOmnibusSigProc: empty region-averaged field response");
+        // The reference sizes every plane's response by plane 0 (:760-761).
+        const int fine_nwires = (int)avg[0].rows.size();
+        const int nin = (int)avg[0].rows[0].size();
+        const int nfine = (int)fft_best_length((size_t)nin);
+        const Binning tbins(nfine, 0, nfine * fr.period);
+        std::vector<double> elec;
+        auto iwf = Factory::find_tn<IWaveform>(m_elecresponse_tn);
+        if (auto ce = std::dynamic_pointer_cast<ColdElecResponse>(iwf))
+            elec = ce->samples_f64(tbins);
+        else {
+            auto f = iwf->waveform_samples(tbins);
+            elec.assign(f.begin(), f.end());
+        }
+        const double intrinsic = fr.origin / fr.speed;
+        const int time_shift = (int)((m_coarse_time_offset + intrinsic) / period);
+        const int wire_shift = (fine_nwires - 1) / 2;
+
+        static const char* wiener_names[3] = {"Wiener_tight_U", "Wiener_tight_V", "Wiener_tight_W"};
+        size_t off = 0, rms_off = 0;
+        for (int ip : m_process_planes) {
+            Plane& pl = m_planes[ip];
+            if (pl.active) continue;   // listed twice
+            pl.nw = (int)m_channels[ip].size();
+            if (pl.nw == 0) continue;
+            const auto& rows = avg[ip].rows;
+            if ((int)rows.size() > fine_nwires)
+                throw SigProcError("HipOmnibusSigProc: plane " + std::to_string(ip) +
+                                   " has more response regions than plane 0");
+            std::vector<double> fr_rows((size_t)fine_nwires * nin, 0.0);
+            for (size_t r = 0; r < rows.size(); ++r) {
+                if ((int)rows[r].size() != nin) throw SigProcError("HipOmnibusSigProc: ragged field response paths");
+                std::copy(rows[r].begin(), rows[r].end(), fr_rows.begin() + r * nin);
+            }
+            std::vector<double> resp((size_t)fine_nwires * nticks);
+            hcheck(wct_overall_response(fr_rows.data(), fine_nwires, nin, elec.data(), nfine, fr.period, period,
+                                        m_inter_gain, m_ADC_mV, m_fine_time_offset, nticks, resp.data(),
+                                        m_stream),
+                   "response");
+            auto wire_f = filter_f64("HfFilter", ip < 2 ? "Wire_ind" : "Wire_col", pl.nw);
+            auto time_f = filter_f64("HfFilter", wiener_names[ip], nticks);
+            hcheck(wct_decon_kernel_create(&pl.kernel, resp.data(), fine_nwires, pl.nw, nticks, wire_f.data(),
+                                           time_f.data(), wire_shift, time_shift, m_spectral_cut, m_stream),
+                   "kernel");
+            int knw = 0, knt = 0, kc = 0;
+            hcheck(wct_decon_kernel_info(pl.kernel, &knw, &knt, &kc), "kernel");
+            if (ip < 2) {
+                // tight and tighter outputs of the induction planes differ
+                // by their low-frequency filter (:1083-1097, :1126-1140)
+                auto lt = Factory::find<IFilterWaveform>("LfFilter", "ROI_tight_lf")->filter_waveform(nticks);
+                auto ltt = Factory::find<IFilterWaveform>("LfFilter", "ROI_tighter_lf")->filter_waveform(nticks);
+                pl.lf_tight.ensure(sizeof(float) * kc);
+                pl.lf_tighter.ensure(sizeof(float) * kc);
+                hcheck(wct_memcpy_htod(pl.lf_tight.p, lt.data(), sizeof(float) * kc, m_stream), "upload");
+                hcheck(wct_memcpy_htod(pl.lf_tighter.p, ltt.data(), sizeof(float) * kc, m_stream), "upload");
+            }
+            pl.th_factor = ip < 2 ? m_th_factor_ind : m_th_factor_col;
+            pl.off = off;
+            pl.rms_off = rms_off;
+            off += (size_t)pl.nw * nticks;
+            rms_off += pl.nw;
+            pl.active = true;
+        }
+        m_adc_floats = off;
+        m_rms_floats = rms_off;
+        m_nticks = nticks;
+        m_d_adc.ensure(sizeof(float) * std::max<size_t>(off, 1));
+        m_d_out.ensure(sizeof(float) * std::max<size_t>(off, 1));
+        m_d_mask.ensure(std::max<size_t>(off, 1));
+        m_d_rms.ensure(sizeof(float) * std::max<size_t>(rms_off, 1));
+        m_h_adc.ensure(sizeof(float) * std::max<size_t>(off, 1));
+        m_h_out.ensure(sizeof(float) * std::max<size_t>(off, 1));
+        m_h_mask.ensure(std::max<size_t>(off, 1));
+        m_h_rms.ensure(sizeof(float) * std::max<size_t>(rms_off, 1));
+        m_built_nticks = nticks;
+        m_built_period = period;
+    }
+
+    std::vector<double> filter_f64(const char* type, const char* name, int n) const
+    {
+        auto f = Factory::find<IFilterWaveform>(type, name);
+        if (auto hf = std::dynamic_pointer_cast<HfFilter>(f)) return hf->filter_waveform_f64(n);
+        auto v = f->filter_waveform(n);
+        return std::vector<double>(v.begin(), v.end());
+    }
+
+    void make_plan(const std::vector<size_t>& ptr_off, const std::vector<size_t>& lo_off,
+                   const std::vector<size_t>& hi_off)
+    {
+        if (m_plan) wct_decon_plan_destroy(m_plan);
+        m_plan = nullptr;
+        std::vector<wct_decon_job> jobs;
+        const int* dbad = (const int*)m_d_bad.p;
+        for (int ip = 0; ip < 3; ++ip) {
+            const Plane& pl = m_planes[ip];
+            if (!pl.active) continue;
+            wct_decon_job j;
+            std::memset(&j, 0, sizeof(j));
+            j.kernel = pl.kernel;
+            j.adc = (const float*)m_d_adc.p + pl.off;
+            j.ld_adc = m_nticks;
+            j.out = (float*)m_d_out.p + pl.off;
+            j.ld_out = m_nticks;
+            j.mask = (unsigned char*)m_d_mask.p + pl.off;
+            j.ld_mask = m_nticks;
+            j.rms = (float*)m_d_rms.p + pl.rms_off;
+            j.bad_ptr = dbad + ptr_off[ip];
+            j.bad_lo = dbad + lo_off[ip];
+            j.bad_hi = dbad + hi_off[ip];
+            j.lf_tight = (const float*)pl.lf_tight.p;
+            j.lf_tighter = (const float*)pl.lf_tighter.p;
+            j.th_factor = pl.th_factor;
+            j.restore_baseline = 1;
+            j.save_mode = m_use_roi_debug_mode ? 0 : 1;
+            jobs.push_back(j);
+        }
+        const size_t ws = wct_decon_workspace_size(jobs.data(), (int)jobs.size());
+        m_d_ws.ensure(std::max<size_t>(ws, 1));
+        hcheck(wct_decon_plan_create(&m_plan, jobs.data(), (int)jobs.size(), m_d_ws.p, m_d_ws.bytes), "plan");
+    }
+
+    // configuration (reference keys and defaults,
+    // sigproc/inc/WireCellSigProc/OmnibusSigProc.h:100-234, except where
+    // noted: the anode stand-in and the dft default name this build's
+    // components, per_chan_resp defaults to "" as the PDSP layer sets it)
+    std::string m_anode_tn{"AnodeChannels"};
+    std::string m_dft_tn{"HipDFT"};
+    std::string m_field_response{"FieldResponse"};
+    std::string m_elecresponse_tn{"ColdElecResponse"};
+    std::string m_per_chan_resp{""};
+    double m_fine_time_offset{0.0 * units::microsecond};
+    double m_coarse_time_offset{-8.0 * units::microsecond};
+    double m_inter_gain{1.2};
+    double m_ADC_mV{4096 / (2000. * units::mV)};
+    float m_th_factor_ind{3};
+    float m_th_factor_col{5};
+    int m_pad{5};
+    float m_asy{0.1f};
+    std::vector<int> m_process_planes{0, 1, 2};
+    std::string m_wiener_tag{"wiener"};
+    std::string m_gauss_tag{"gauss"};
+    std::string m_frame_tag{"sigproc"};
+    std::string m_tight_lf_tag{"tight_lf"};
+    bool m_use_roi_debug_mode{false};
+    bool m_use_roi_refinement{true};
+    bool m_sparse{false};
+    std::string m_tight_roi_mask{"tight_roi"};
+    double m_spectral_cut{1e-7};
+    int m_device{0};
+
+    std::vector<int> m_channels[3];
+    std::unordered_map<int, std::pair<int, int>> m_chmap;
+
+    Plane m_planes[3];
+    void* m_stream{nullptr};
+    wct_decon_plan m_plan{nullptr};
+    DevBuf m_d_adc, m_d_out, m_d_mask, m_d_rms, m_d_bad, m_d_ws;
+    HostBuf m_h_adc, m_h_out, m_h_mask, m_h_rms;
+    size_t m_adc_floats{0}, m_rms_floats{0};
+    int m_nticks{0};
+    int m_built_nticks{-1};
+    double m_built_period{0};
+    std::vector<size_t> m_bad_layout;
+};
+
+}  // namespace Hip
+}  // namespace WireCell
+
+WIRECELL_HIP_FACTORY(HipOmnibusSigProc, WireCell::Hip::HipOmnibusSigProc)

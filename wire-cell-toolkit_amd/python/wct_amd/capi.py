@@ -39,6 +39,10 @@ _SIGS = {
     "wct_memcpy_dtoh": (_I, [_V, _V, _Z, _V]),
     "wct_memcpy_dtod": (_I, [_V, _V, _Z, _V]),
     "wct_memset": (_I, [_V, _I, _Z, _V]),
+    "wct_host_malloc": (_I, [C.POINTER(C.c_void_p), _Z]),
+    "wct_host_free": (_I, [_V]),
+    "wct_memcpy_htod_async": (_I, [_V, _V, _Z, _V]),
+    "wct_memcpy_dtoh_async": (_I, [_V, _V, _Z, _V]),
     "wct_stream_create": (_I, [C.POINTER(C.c_void_p)]),
     "wct_stream_destroy": (_I, [_V]),
     "wct_stream_sync": (_I, [_V]),

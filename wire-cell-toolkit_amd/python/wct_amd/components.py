@@ -1,0 +1,185 @@
+"""The C++ component layer of libWireCellHip.so from Python.
+
+configure() takes the same list of {"type", "name", "data"} objects the
+reference's application reads (apps/src/Main.cxx:330-350); dft() calls
+an IDFT component through its virtual interface; Frame / frame_filter()
+push an IFrame through an IFrameFilter (IFunctionNode.h:50) and read
+back the traces, tags, summaries and channel masks of the result.
+"""
+import ctypes as C
+import json
+
+import numpy as np
+
+from .capi import WctError
+from .host import lib
+
+OPS = {"fwd1d": 0, "inv1d": 1, "fwd1b": 2, "inv1b": 3, "fwd2d": 4, "inv2d": 5,
+       "transpose_real": 6, "transpose_complex": 7}
+
+
+def _check(rc):
+    if rc != 0:
+        raise WctError(lib().wcth_last_error().decode())
+
+
+def configure(cfg_list):
+    _check(lib().wcth_configure(json.dumps(cfg_list).encode()))
+
+
+def reset():
+    _check(lib().wcth_reset())
+
+
+def known_types():
+    n = lib().wcth_known_types(None, 0)
+    buf = C.create_string_buffer(n + 1)
+    lib().wcth_known_types(buf, n + 1)
+    return buf.value.decode().split()
+
+
+def dft(tn, op, arr, axis=1):
+    """Runs IDFT op on a host array: complex64 for the transforms (1-d or
+    2-d) and complex64 or float32 for transpose.  Returns a new array."""
+    a = np.ascontiguousarray(arr)
+    if op == "transpose":
+        op = "transpose_real" if a.dtype == np.float32 else "transpose_complex"
+        if a.dtype not in (np.float32, np.complex64):
+            raise TypeError("transpose takes float32 or complex64")
+    elif a.dtype != np.complex64:
+        raise TypeError("IDFT transforms take complex64")
+    if a.ndim == 1:
+        nrows, ncols = a.shape[0], 1
+    else:
+        nrows, ncols = a.shape
+    out = np.empty_like(a)
+    _check(lib().wcth_dft(tn.encode(), OPS[op], a.ctypes.data, out.ctypes.data, nrows, ncols, axis))
+    if op.startswith("transpose"):
+        out = out.reshape(ncols, nrows)
+    return out
+
+
+def filter_waveform(tn, n):
+    out = np.zeros(n, np.float32)
+    _check(lib().wcth_filter_waveform(tn.encode(), n, out.ctypes.data))
+    return out
+
+
+class Frame:
+    """An IFrame handle (input under construction, or a filter result)."""
+
+    def __init__(self, ident=0, time=0.0, tick=500.0, _handle=None):
+        self.h = _handle if _handle is not None else lib().wcth_frame_new(ident, time, tick)
+
+    def add_trace(self, channel, charge, tbin=0):
+        q = np.ascontiguousarray(charge, np.float32)
+        _check(lib().wcth_frame_add_trace(self.h, int(channel), int(tbin), q.size, q.ctypes.data))
+
+    def add_mask(self, name, channel, first, second):
+        _check(lib().wcth_frame_add_mask(self.h, name.encode(), int(channel), int(first), int(second)))
+
+    def info(self):
+        ident, ntr = C.c_int(), C.c_int()
+        time, tick = C.c_double(), C.c_double()
+        _check(lib().wcth_frame_info(self.h, C.byref(ident), C.byref(time), C.byref(tick), C.byref(ntr)))
+        return dict(ident=ident.value, time=time.value, tick=tick.value, ntraces=ntr.value)
+
+    def trace(self, i):
+        ch, tb, n = C.c_int(), C.c_int(), C.c_int()
+        p = C.POINTER(C.c_float)()
+        _check(lib().wcth_frame_trace(self.h, i, C.byref(ch), C.byref(tb), C.byref(n), C.byref(p)))
+        q = np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else np.zeros(0, np.float32)
+        return ch.value, tb.value, q
+
+    def traces(self):
+        return [self.trace(i) for i in range(self.info()["ntraces"])]
+
+    def _tags(self, which):
+        n = lib().wcth_frame_tags(self.h, which, -1, None)
+        out = []
+        for i in range(n):
+            s = C.c_char_p()
+            lib().wcth_frame_tags(self.h, which, i, C.byref(s))
+            out.append(s.value.decode())
+        return out
+
+    def frame_tags(self):
+        return self._tags(0)
+
+    def trace_tags(self):
+        return self._tags(1)
+
+    def tagged_traces(self, tag):
+        p = C.POINTER(C.c_size_t)()
+        n = C.c_int()
+        _check(lib().wcth_frame_tagged(self.h, tag.encode(), C.byref(p), C.byref(n)))
+        return np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else np.zeros(0, np.uint64)
+
+    def trace_summary(self, tag):
+        p = C.POINTER(C.c_double)()
+        n = C.c_int()
+        _check(lib().wcth_frame_summary(self.h, tag.encode(), C.byref(p), C.byref(n)))
+        return np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else np.zeros(0)
+
+    def masks(self, name):
+        """{channel: [(first, second), ...]} of one channel-mask-map entry."""
+        n = lib().wcth_frame_masks(self.h, name.encode(), None, None, None, 0)
+        if n < 0:
+            raise WctError("bad frame handle")
+        ch = np.zeros(max(n, 1), np.int32)
+        lo = np.zeros(max(n, 1), np.int32)
+        hi = np.zeros(max(n, 1), np.int32)
+        lib().wcth_frame_masks(self.h, name.encode(), ch.ctypes.data, lo.ctypes.data, hi.ctypes.data, n)
+        out = {}
+        for c, a, b in zip(ch[:n], lo[:n], hi[:n]):
+            out.setdefault(int(c), []).append((int(a), int(b)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().wcth_frame_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def frame_filter(tn, frame):
+    """Calls IFrameFilter tn on frame (None = end of stream)."""
+    out = C.c_void_p()
+    _check(lib().wcth_frame_filter(tn.encode(), frame.h if frame is not None else None, C.byref(out)))
+    return Frame(_handle=out.value) if out.value else None
+
+
+def roi_lists(mask, pad=5, asy=0.1, cap=1 << 22):
+    """Tight ROI lists (inclusive tick pairs per wire) from a threshold mask."""
+    m = np.ascontiguousarray(mask, np.uint8)
+    nw, nt = m.shape
+    cnt = np.zeros(nw, np.int32)
+    pairs = np.zeros(2 * cap, np.int32)
+    tot = lib().wcth_roi_lists(m.ctypes.data, nw, nt, nt, pad, asy, cnt.ctypes.data, pairs.ctypes.data, cap)
+    if tot < 0:
+        raise WctError("roi_lists failed" if tot == -1 else "ROI capacity exceeded")
+    out, k = [], 0
+    for w in range(nw):
+        out.append([(int(pairs[2 * (k + i)]), int(pairs[2 * (k + i) + 1])) for i in range(cnt[w])])
+        k += cnt[w]
+    return out
+
+
+def field_response_json(planes, origin, tstart, period, speed):
+    """A field response in the reference's JSON schema
+    (util/src/Response.cxx:32-72).  planes: list of (planeid, location,
+    pitch, pitchpos[], current[npaths, nsamp])."""
+    pl = []
+    for planeid, location, pitch, pos, cur in planes:
+        paths = [{"PathResponse": {"current": {"array": {"shape": [len(c)], "elements": [float(x) for x in c]}},
+                                   "pitchpos": float(p), "wirepos": 0.0}}
+                 for p, c in zip(pos, cur)]
+        pl.append({"PlaneResponse": {"paths": paths, "planeid": planeid, "location": location,
+                                     "pitch": pitch}})
+    return {"FieldResponse": {"planes": pl, "axis": [1.0, 0.0, 0.0], "origin": origin, "tstart": tstart,
+                              "period": period, "speed": speed}}

@@ -18,6 +18,30 @@ _SIGS = {
     "wcth_fft_best_length": (C.c_long, [C.c_long, C.c_int]),
     "wcth_wire_region_average": (C.c_int, [C.c_int, _f64, _f64, C.c_int, C.c_double, C.c_int,
                                            _i32, _f64]),
+    "wcth_last_error": (C.c_char_p, []),
+    "wcth_configure": (C.c_int, [C.c_char_p]),
+    "wcth_reset": (C.c_int, []),
+    "wcth_known_types": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "wcth_dft": (C.c_int, [C.c_char_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "wcth_filter_waveform": (C.c_int, [C.c_char_p, C.c_int, C.c_void_p]),
+    "wcth_frame_new": (C.c_void_p, [C.c_int, C.c_double, C.c_double]),
+    "wcth_frame_free": (None, [C.c_void_p]),
+    "wcth_frame_add_trace": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "wcth_frame_add_mask": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int]),
+    "wcth_frame_filter": (C.c_int, [C.c_char_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "wcth_frame_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "wcth_frame_trace": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_int), C.POINTER(C.POINTER(C.c_float))]),
+    "wcth_frame_tags": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_char_p)]),
+    "wcth_frame_tagged": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.POINTER(C.c_size_t)),
+                                    C.POINTER(C.c_int)]),
+    "wcth_frame_summary": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.POINTER(C.c_double)),
+                                     C.POINTER(C.c_int)]),
+    "wcth_frame_masks": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_int]),
+    "wcth_roi_lists": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_long, C.c_int, C.c_float,
+                                 C.c_void_p, C.c_void_p, C.c_int]),
 }
 EXPORTED = sorted(_SIGS)
 MISSING = []
