@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--cut", type=float, default=1e-7, help="spectral cut tolerance")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--plain-apas", type=int, default=4,
+                    help="APAs timed through the plain per-plane path (0 = skip)")
     return ap.parse_args()
 
 
@@ -123,6 +125,34 @@ def cpu_baseline(planes, nthreads):
                       f"CPU: {cpu}"}
 
 
+def plain_leg(planes, frames, outs, masks, rmss, kern, napas, reps=2):
+    """The plain per-plane device path (one launch per reference step,
+    FP32, response spectrum rebuilt per call) over the first napas APAs
+    of the same resident frames, timed on the same stream."""
+    import torch
+    from wct_amd import decon
+    n = min(len(planes), 3 * napas)
+    pp = {t: decon.PlainPlane(kern[t].spec, resp=kern[t].resp) for t in sorted(set(planes[:n]))}
+
+    def run():
+        for i in range(n):
+            pp[planes[i]].run(frames[i], outs[i], mask=masks[i], rms=rmss[i])
+
+    run()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        run()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    ct = sum(nw * nt for _, nw, nt in planes[:n])
+    return ct / (ms / 1e3), ms, n
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -182,6 +212,14 @@ def main():
     achieved = B_ALG * ch_ticks / (gpu_ms / 1e3) / 1e9
     log(f"bench: {ms:.3f} ms/step wall, {gpu_ms:.3f} ms/step on the stream")
 
+    plain = None
+    if args.plain_apas > 0:
+        pv, pms, pn = plain_leg(planes, frames, outs, masks, rmss, kern, args.plain_apas)
+        log(f"bench: plain path {pms:.2f} ms for {pn} planes -> {pv:.3e} channel-ticks/s")
+        plain = {"value": pv, "unit": "channel-ticks/s", "ratio": value / pv,
+                 "sample": f"{pn} planes ({pn // 3} APAs) of the same frames, one launch per "
+                           f"reference step, {pms:.2f} ms per pass"}
+
     cpu = None
     if not args.no_cpu:
         n = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -189,7 +227,7 @@ def main():
         cpu = cpu_baseline(planes, n)
 
     res = {
-        "metric": "deconvolved channel-ticks/s (batched fused, 1 GPU) at <=1e-4 rel RMS err",
+        "metric": "deconvolved channel-ticks/s (batched vs plain, 1 GPU) at <=1e-4 rel RMS err",
         "value": value,
         "unit": "channel-ticks/s",
         "n_gpus": 1,
@@ -220,6 +258,8 @@ def main():
             "traffic": None,
             "kernel": "fused decon pipeline (row r2c, wire pass, row c2r+ROI) per step",
         },
+        "plain_gpu": plain,
+        "batched_vs_plain": None if plain is None else plain["ratio"],
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

@@ -180,6 +180,28 @@ int wct_decon_plan_destroy(wct_decon_plan plan);
 int wct_decon_run(const wct_decon_job* jobs, int njobs, void* workspace, size_t ws_bytes,
                   void* stream);
 
+/* ---- the plain per-plane path ----
+ * OmnibusSigProc's per-plane control flow (load_data, decon_2D_init,
+ * decon_2D_tighterROI/tightROI, restore_baseline, cal_RMS + threshold,
+ * save_data; sigproc/src/OmnibusSigProc.cxx:304-472,944-1160,
+ * ROI_formation.cxx:364-456) with one device launch per reference step,
+ * full c2c transforms (aux/src/DftTools.cxx:127-139), FP32 throughout
+ * and the response spectrum rebuilt on every call (:988-998).  All
+ * pointers are device memory:
+ *   resp           nr x nt overall response rows (nr <= nw)
+ *   wire_filter    nw values; tight_filter / tighter_filter nt values
+ *                  (tighter NULL for a collection plane)
+ *   out, mask, rms as for wct_decon_job (mask needs rms); save_mode as
+ *                  there.
+ * The workspace holds wct_plain_workspace_size(nw, nt) bytes. */
+size_t wct_plain_workspace_size(int nw, int nt);
+int wct_plain_decon_plane(const float* adc, int ld_adc, int nw, int nt, const float* resp, int nr,
+                          const float* wire_filter, const float* tight_filter, const float* tighter_filter,
+                          int wire_shift, int time_shift, float th_factor, const int* bad_ptr,
+                          const int* bad_lo, const int* bad_hi, float* out, int ld_out,
+                          unsigned char* mask, int ld_mask, float* rms, int save_mode, void* workspace,
+                          size_t ws_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

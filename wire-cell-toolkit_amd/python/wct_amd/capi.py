@@ -68,6 +68,9 @@ _SIGS = {
     "wct_decon_plan_create": (_I, [C.POINTER(C.c_void_p), C.POINTER(DeconJob), _I, _V, _Z]),
     "wct_decon_plan_run": (_I, [_V, _V]),
     "wct_decon_plan_destroy": (_I, [_V]),
+    "wct_plain_workspace_size": (_Z, [_I, _I]),
+    "wct_plain_decon_plane": (_I, [_V, _I, _I, _I, _V, _I, _V, _V, _V, _I, _I, _F, _V, _V, _V, _V, _I, _V,
+                                   _I, _V, _I, _V, _Z, _V]),
 }
 
 # Every symbol include/wct_hip.h declares (checked by the CPU tests).

@@ -155,3 +155,46 @@ class ForwardKernel(PlaneKernel):
 def convolve(fwd_kernel, q, out):
     """out = forward response convolution of q (torch tensors on device)."""
     Batch([make_job(fwd_kernel, q, out, restore=0)]).run()
+
+
+class PlainPlane:
+    """The plain per-plane path (wct_plain_decon_plane): the reference's
+    per-step control flow, FP32, one launch per step.  Holds the plane's
+    FP32 response rows and filters on the device; the tight filter is
+    Wiener (x ROI_tight_lf on induction planes) and the tighter filter
+    Wiener x ROI_tighter_lf (induction only), multiplied in float as the
+    reference multiplies its filter waveforms."""
+
+    def __init__(self, spec, resp=None, device=None):
+        import torch
+        dev = device or "cuda"
+        self.spec = spec
+        resp = overall_response(spec) if resp is None else np.asarray(resp, np.float64)
+        f32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(dev)
+        self.resp = f32(resp)
+        self.nr = resp.shape[0]
+        self.wire = f32(spec.wire_filter)
+        tf = np.asarray(spec.time_filter, np.float32)
+        self.tighter = None
+        if spec.kind == "ind" and spec.lf_tight is not None and not spec.lf_folded:
+            self.tight = f32(tf * np.asarray(spec.lf_tight, np.float32))
+            self.tighter = f32(tf * np.asarray(spec.lf_tighter, np.float32))
+        else:   # collection, or the tight LF already folded into time_filter
+            self.tight = f32(tf)
+        self.ws_bytes = lib().wct_plain_workspace_size(spec.nw, spec.nt)
+        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=dev)
+
+    def run(self, adc, out, mask=None, rms=None, th_factor=W.TH_FACTOR, save_mode=0, bad=None,
+            tighter=True, stream=None):
+        sp = self.spec
+        bp = blo = bhi = None
+        if bad is not None:
+            bp, blo, bhi = (b.data_ptr() for b in bad)
+        check(lib().wct_plain_decon_plane(
+            adc.data_ptr(), adc.stride(0), sp.nw, sp.nt, self.resp.data_ptr(), self.nr,
+            self.wire.data_ptr(), self.tight.data_ptr(),
+            self.tighter.data_ptr() if (tighter and self.tighter is not None) else None,
+            sp.wire_shift, sp.time_shift, th_factor, bp, blo, bhi, out.data_ptr(), out.stride(0),
+            _ptr(mask), mask.stride(0) if mask is not None else 0, _ptr(rms), save_mode,
+            self.ws.data_ptr(), self.ws_bytes, _stream() if stream is None else stream))
+

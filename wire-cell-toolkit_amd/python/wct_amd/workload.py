@@ -76,6 +76,7 @@ class PlaneSpec:
     time_filter: np.ndarray = field(repr=False, default=None)   # Wiener (x tight LF if folded)
     lf_tight: np.ndarray = field(repr=False, default=None)
     lf_tighter: np.ndarray = field(repr=False, default=None)
+    lf_folded: bool = False          # time_filter holds Wiener x ROI_tight_lf
 
 
 def plane_spec(kind, nw, nt, fold_lf=True):
@@ -94,6 +95,7 @@ def plane_spec(kind, nw, nt, fold_lf=True):
         sp.lf_tight = lt
         sp.lf_tighter = host.lf_filter(nt, **LF_TIGHTER)
         sp.time_filter = wiener * lt if fold_lf else wiener
+        sp.lf_folded = bool(fold_lf)
     else:
         sp.time_filter = wiener
     return sp
