@@ -44,6 +44,11 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--plain-apas", type=int, default=4,
                     help="APAs timed through the plain per-plane path (0 = skip)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (use --chunk, default wire tile and grid)")
+    ap.add_argument("--eager", action="store_true", help="time plan runs without the hipGraph")
+    ap.add_argument("--adjoint-frames", type=int, default=16,
+                    help="frames of 800 x 6000 in the timed adjoint gradient step (0 = skip)")
     return ap.parse_args()
 
 
@@ -125,6 +130,72 @@ def cpu_baseline(planes, nthreads):
                       f"CPU: {cpu}"}
 
 
+def calibration_copy(dev, mib=1024):
+    """One elementwise copy of known size (mib MiB read, mib MiB written):
+    its FETCH_SIZE / WRITE_SIZE counters scale the decon kernels' to
+    bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def hbm_traffic():
+    """HBM bytes per channel-tick of the fused pipeline from the
+    committed counter summary (FETCH_SIZE + WRITE_SIZE of one 8-APA
+    chunk's launches, scaled by the calibration copy), as (bytes,
+    channel-ticks), or None when absent."""
+    path = os.path.join(REPO, "profiles", "decon_fused_chunk8apa_hbm_traffic.md")
+    try:
+        vals = {}
+        with open(path) as f:
+            for line in f:
+                if line.startswith("- ") and ":" in line:
+                    k, v = line[2:].split(":", 1)
+                    vals[k.strip()] = v.strip().split()[0]
+        return float(vals["bytes_per_chunk"]), int(vals["channel_ticks_per_chunk"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def adjoint_leg(dev, nframes, reps=3):
+    """One gradient step of the charge fit on nframes induction frames of
+    800 x 6000: forward response convolution, filtered decon, least-squares
+    loss, backward through the hand-written adjoints to the raster and
+    both filter widths (six fused passes)."""
+    import numpy as np
+    import torch
+    from wct_amd import autograd as A, workload as W
+    m = A.Model(W.plane_spec("ind", 800, 6000))
+    rng = np.random.default_rng(3)
+    q_true = torch.from_numpy(np.stack([W.charge_raster(rng, 800, 6000) for _ in range(nframes)])).to(dev)
+    T = m.forward_conv(q_true) + torch.randn_like(q_true) * W.NOISE_ADC
+    s_t = torch.tensor(1e-4, dtype=torch.float64, requires_grad=True)
+    s_w = torch.tensor(0.75, dtype=torch.float64, requires_grad=True)
+
+    def one():
+        q = torch.zeros_like(q_true, requires_grad=True)
+        y = A.decon(A.forward_conv(q, m), s_t, s_w, m)
+        L = 0.5 * ((y - T) ** 2).sum()
+        L.backward()
+        return L
+
+    one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        one()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    m.close()
+    ct = nframes * 800 * 6000
+    return {"ms": round(ms, 3), "frames": nframes, "shape": "800x6000",
+            "passes": 6, "channel_ticks_per_s": ct / (ms / 1e3)}
+
+
 def plain_leg(planes, frames, outs, masks, rmss, kern, napas, reps=2):
     """The plain per-plane device path (one launch per reference step,
     FP32, response spectrum rebuilt per call) over the first napas APAs
@@ -163,6 +234,7 @@ def main():
     dev = torch.device("cuda:0")
     planes = W.config_planes(args.workload)
     types = sorted(set(planes))
+    calibration_copy(dev)
     log(f"bench: {args.workload}: {len(planes)} planes, plane types {types}")
     kern, fwd = {}, {}
     for t in types:
@@ -173,23 +245,35 @@ def main():
     outs = [torch.empty_like(f) for f in frames]
     masks = [torch.empty(f.shape, dtype=torch.uint8, device=dev) for f in frames]
     rmss = [torch.empty(f.shape[0], dtype=torch.float32, device=dev) for f in frames]
-    per_chunk = 3 * args.chunk
-    batches = []
-    ws = None
-    for c0 in range(0, len(planes), per_chunk):
-        jobs = [decon.make_job(kern[planes[i]], frames[i], outs[i], mask=masks[i], rms=rmss[i],
-                               restore=1) for i in range(c0, min(c0 + per_chunk, len(planes)))]
-        # all chunks share one workspace, sized by the first (full) chunk
-        b = decon.Batch(jobs, workspace=ws)
-        ws = b.ws
-        batches.append(b)
+    def make_jobs(idx):
+        return [decon.make_job(kern[planes[i]], frames[i], outs[i], mask=masks[i], rms=rmss[i],
+                               restore=1) for i in idx]
+
+    tuning = {"chunk_apas": args.chunk, "wire_tile": 0, "row_grid_pct": 0, "tuned": False}
+    if not args.no_tune:
+        from wct_amd import tune
+        t0 = time.perf_counter()
+        best, res = tune.tune(make_jobs, len(planes), log=log)
+        tuning = {"chunk_apas": best[0], "wire_tile": best[1], "row_grid_pct": best[2], "tuned": True,
+                  "candidates": len(res), "tuning_s": round(time.perf_counter() - t0, 2),
+                  "best_ms": round(res[best], 3),
+                  "worst_ms": round(max(v for v in res.values() if isinstance(v, float)), 3)}
+        log(f"bench: tuned {tuning}")
+    from wct_amd import tune
+    batches = tune.build_step(make_jobs, len(planes), tuning["chunk_apas"], tuning["wire_tile"],
+                              tuning["row_grid_pct"])
+    graph = None if args.eager else decon.Graph(batches)
     ch_ticks = sum(nw * nt for _, nw, nt in planes)
     log(f"bench: {ch_ticks / 1e9:.3f} G channel-ticks per step, {len(batches)} chunks of "
-        f"{args.chunk} APAs, kc={ {str(t): kern[t].kc for t in types} }")
+        f"{tuning['chunk_apas']} APAs, kc={ {str(t): kern[t].kc for t in types} }, "
+        f"{'eager' if graph is None else 'one hipGraph'} per step")
 
     def step():
-        for b in batches:
-            b.run()
+        if graph is not None:
+            graph.launch()
+        else:
+            for b in batches:
+                b.run()
 
     for i in range(args.warmup):
         step()
@@ -220,11 +304,21 @@ def main():
                  "sample": f"{pn} planes ({pn // 3} APAs) of the same frames, one launch per "
                            f"reference step, {pms:.2f} ms per pass"}
 
+    adjoint = None
+    if args.adjoint_frames > 0:
+        adjoint = adjoint_leg(dev, args.adjoint_frames)
+        log(f"bench: adjoint step {adjoint}")
+
     cpu = None
     if not args.no_cpu:
         n = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"bench: CPU baseline on {n} threads")
         cpu = cpu_baseline(planes, n)
+
+    traffic = None
+    tr = hbm_traffic()
+    if tr is not None:
+        traffic = tr[0] * ch_ticks / tr[1]     # bytes per step
 
     res = {
         "metric": "deconvolved channel-ticks/s (batched vs plain, 1 GPU) at <=1e-4 rel RMS err",
@@ -245,7 +339,9 @@ def main():
                 else f"{len(planes)} planes"),
             "planes": len(planes),
             "channel_ticks": ch_ticks,
-            "chunk_apas": args.chunk,
+            "chunk_apas": tuning["chunk_apas"],
+            "tuning": tuning,
+            "hipgraph": graph is not None,
             "spectral_cut": args.cut,
             "kept_bins": {f"{k}{nw}x{nt}": kern[(k, nw, nt)].kc for (k, nw, nt) in types},
         },
@@ -255,10 +351,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "kernel": "fused decon pipeline (row r2c, wire pass, row c2r+ROI) per step",
         },
         "plain_gpu": plain,
+        "adjoint_step": adjoint,
         "batched_vs_plain": None if plain is None else plain["ratio"],
         "cpu_baseline": cpu,
     }

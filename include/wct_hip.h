@@ -159,6 +159,9 @@ typedef struct {
     float th_factor;
     int restore_baseline;
     int save_mode;
+    int adjoint;        /* 1: apply the adjoint operator (conj(K)); the
+                           job must then have no bad ranges, baseline,
+                           mask or save_mode (linear part only) */
 } wct_decon_job;
 
 /* Device scratch needed to run the jobs (spectrum buffers). */
@@ -174,6 +177,26 @@ int wct_decon_plan_create(wct_decon_plan* out, const wct_decon_job* jobs, int nj
                           void* workspace, size_t ws_bytes);
 int wct_decon_plan_run(wct_decon_plan plan, void* stream);
 int wct_decon_plan_destroy(wct_decon_plan plan);
+/* Launch-shape choices of a plan (0 = the default for that field):
+ *   wire_tile     bins per wire-pass workgroup: 4, 8 or 16 (16 and 8
+ *                 need <= 1023 wires, 4 takes up to 2047; default 8 or
+ *                 4 by wire count)
+ *   row_grid_pct  persistent grid of the row passes as a percentage of
+ *                 the most workgroups the device keeps resident
+ * The autotuner (wct_amd.tune) times these per workload. */
+typedef struct {
+    int wire_tile;
+    int row_grid_pct;
+} wct_decon_tuning;
+int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, int njobs,
+                                void* workspace, size_t ws_bytes, const wct_decon_tuning* tuning);
+/* hipGraph of a sequence of plan runs (captured on stream, or on a
+ * private stream when stream is NULL); a launch replays every stage of
+ * every plan.  The plans and their buffers must outlive the graph. */
+typedef struct wct_decon_graph_s* wct_decon_graph;
+int wct_decon_graph_create(wct_decon_graph* out, const wct_decon_plan* plans, int nplans, void* stream);
+int wct_decon_graph_launch(wct_decon_graph graph, void* stream);
+int wct_decon_graph_destroy(wct_decon_graph graph);
 /* One-shot convenience: create a plan, run it, wait, destroy it.
  * jobs is a host array; workspace is device memory of at least
  * wct_decon_workspace_size bytes. */

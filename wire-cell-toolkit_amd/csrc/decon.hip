@@ -37,7 +37,14 @@ struct wct_decon_plan_s {
     int Mmax = 0, nwmax = 0, tk = 8;
     int twM = 0, twW = 0;        // largest twiddle tables (complex entries)
     int ncu = 0;                 // compute units of the device
+    int row_pct = 100;           // persistent row grids: percent of the resident maximum
     bool smooth5 = true;         // every length is 2^a 3^b 5^c
+};
+
+// A captured sequence of plan runs, replayed with one launch.
+struct wct_decon_graph_s {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace wct {
@@ -65,7 +72,7 @@ struct DJob {
     int ld_adc, ld_out, ld_mask;
     int nw, nt, M, kc, ldk;
     float th_factor;
-    int restore, save_mode;
+    int restore, save_mode, adjoint;
     FftPlan planM;
     FftPlan planW;
 };
@@ -249,7 +256,11 @@ __global__ __launch_bounds__(NT) void k_wire_pass(const DJob* __restrict__ jobs,
         float2 o;
         o.x = 0.f;
         o.y = 0.f;
-        if (k < J.kc) o = cmul(col[pidx(m)], Kk[m]);
+        if (k < J.kc) {
+            float2 kk = Kk[m];
+            if (J.adjoint) kk.y = -kk.y;   // adjoint of a real circular filter: conj(K)
+            o = cmul(col[pidx(m)], kk);
+        }
         col[pidx(m)] = o;
     }
     __syncthreads();
@@ -540,12 +551,13 @@ __global__ void k_resp_mul_mirror(double2* __restrict__ rows, const double2* __r
 // Persistent grid: as many blocks as the device keeps resident at once
 // (a larger grid leaves a tail of blocks that start only as others end).
 template <typename K>
-static int resident_grid(K kernel, int nt, size_t lds, int ncu, long work, int* grid)
+static int resident_grid(K kernel, int nt, size_t lds, int ncu, long work, int pct, int* grid)
 {
     int per_cu = 0;
     WCT_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, nt, lds));
     if (per_cu < 1) per_cu = 1;
-    *grid = (int)std::max<long>(1, std::min<long>(work, (long)per_cu * ncu));
+    const long full = std::max<long>(1, (long)per_cu * ncu * pct / 100);
+    *grid = (int)std::max<long>(1, std::min<long>(work, full));
     return WCT_OK;
 }
 
@@ -593,7 +605,7 @@ int plan_run(wct_decon_plan p, hipStream_t s)
         int r_ = allow_lds(k_row_r2c<kNT, VH, S5>, lds);
         if (r_) return r_;
         int grid = 0;
-        r_ = resident_grid(k_row_r2c<kNT, VH, S5>, kNT, lds, p->ncu, rows, &grid);
+        r_ = resident_grid(k_row_r2c<kNT, VH, S5>, kNT, lds, p->ncu, rows, p->row_pct, &grid);
         if (r_) return r_;
         hipLaunchKernelGGL((k_row_r2c<kNT, VH, S5>), dim3(grid), dim3(kNT), lds, s, dj, njobs, rows, lay);
         return WCT_OK;
@@ -602,7 +614,12 @@ int plan_run(wct_decon_plan p, hipStream_t s)
     WCT_HIP_CHECK(hipGetLastError());
     const int ldc = padded_len(p->nwmax) | 1;
     const size_t lds2 = ((size_t)p->tk * ldc + p->twW) * sizeof(float2);
-    if (p->tk == 8) {
+    if (p->tk == 16) {
+        rc = allow_lds(k_wire_pass<kNT, 16, 1024, S5>, lds2);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_wire_pass<kNT, 16, 1024, S5>), dim3(tiles), dim3(kNT), lds2, s, dj, njobs,
+                           tiles, ldc);
+    } else if (p->tk == 8) {
         rc = allow_lds(k_wire_pass<kNT, 8, 1024, S5>, lds2);
         if (rc) return rc;
         hipLaunchKernelGGL((k_wire_pass<kNT, 8, 1024, S5>), dim3(tiles), dim3(kNT), lds2, s, dj, njobs,
@@ -623,7 +640,7 @@ int plan_run(wct_decon_plan p, hipStream_t s)
         int r_ = allow_lds(k_row_c2r_roi<kNT, VH, S5>, lds);
         if (r_) return r_;
         int grid = 0;
-        r_ = resident_grid(k_row_c2r_roi<kNT, VH, S5>, kNT, lds, p->ncu, rows, &grid);
+        r_ = resident_grid(k_row_c2r_roi<kNT, VH, S5>, kNT, lds, p->ncu, rows, p->row_pct, &grid);
         if (r_) return r_;
         hipLaunchKernelGGL((k_row_c2r_roi<kNT, VH, S5>), dim3(grid), dim3(kNT), lds, s, dj, njobs,
                            rows, lay3);
@@ -715,7 +732,7 @@ static int kernel_create(wct_decon_kernel* out, const double* resp, int nr, int 
     WCT_REQUIRE(nt % 2 == 0, WCT_ERR_UNSUPPORTED, "odd tick count %d not supported", nt);
     WCT_REQUIRE(fft_supported_length(nw) && fft_supported_length(nt / 2), WCT_ERR_UNSUPPORTED,
                 "plane %d x %d: extents need prime factors <= 13", nw, nt);
-    WCT_REQUIRE(nw <= 2048, WCT_ERR_UNSUPPORTED, "more than 2048 wires in a plane");
+    WCT_REQUIRE(nw <= 2047, WCT_ERR_UNSUPPORTED, "more than 2047 wires in a plane");
     *out = nullptr;
     hipStream_t s = as_stream(stream);
     const int kh = nt / 2 + 1;
@@ -831,11 +848,11 @@ int wct_decon_kernel_destroy(wct_decon_kernel k)
     return WCT_OK;
 }
 
-static int job_ldk(const wct_decon_job& j)
-{
-    const int tk = pick_tk(j.kernel->nw);
-    return (j.kernel->kc + tk - 1) / tk * tk;
-}
+// Spectrum row length: kc rounded up to 16 bins, a multiple of every
+// wire-pass tile width.
+static int job_ldk(const wct_decon_job& j) { return (j.kernel->kc + 15) / 16 * 16; }
+
+static int wire_tile_max_nw(int tk) { return tk == 4 ? 2047 : 1023; }
 
 size_t wct_decon_workspace_size(const wct_decon_job* jobs, int njobs)
 {
@@ -852,6 +869,12 @@ size_t wct_decon_workspace_size(const wct_decon_job* jobs, int njobs)
 int wct_decon_plan_create(wct_decon_plan* out, const wct_decon_job* jobs, int njobs,
                           void* workspace, size_t ws_bytes)
 {
+    return wct_decon_plan_create_tuned(out, jobs, njobs, workspace, ws_bytes, nullptr);
+}
+
+int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, int njobs,
+                                void* workspace, size_t ws_bytes, const wct_decon_tuning* tuning)
+{
     WCT_REQUIRE(out && jobs && njobs > 0, WCT_ERR_ARG, "decon_plan: no jobs");
     *out = nullptr;
     const size_t need = wct_decon_workspace_size(jobs, njobs);
@@ -861,13 +884,31 @@ int wct_decon_plan_create(wct_decon_plan* out, const wct_decon_job* jobs, int nj
     std::vector<DJob> h(njobs);
     size_t off = 0;
     long rows = 0, tiles = 0;
-    int Mmax = 0, tk = pick_tk(jobs[0].kernel->nw), nwmax = 0, twM = 0, twW = 0;
+    int nw_all = 0;
+    for (int i = 0; i < njobs; ++i) {
+        WCT_REQUIRE(jobs[i].kernel, WCT_ERR_ARG, "decon_plan: job %d has no kernel", i);
+        nw_all = std::max(nw_all, jobs[i].kernel->nw);
+    }
+    int Mmax = 0, tk = pick_tk(nw_all), nwmax = 0, twM = 0, twW = 0;
+    int row_pct = 100;
+    if (tuning) {
+        if (tuning->wire_tile) {
+            WCT_REQUIRE(tuning->wire_tile == 4 || tuning->wire_tile == 8 || tuning->wire_tile == 16,
+                        WCT_ERR_ARG, "decon_plan: wire tile %d not in {4, 8, 16}", tuning->wire_tile);
+            tk = tuning->wire_tile;
+        }
+        if (tuning->row_grid_pct) {
+            WCT_REQUIRE(tuning->row_grid_pct >= 10 && tuning->row_grid_pct <= 100, WCT_ERR_ARG,
+                        "decon_plan: row grid percent %d not in [10, 100]", tuning->row_grid_pct);
+            row_pct = tuning->row_grid_pct;
+        }
+    }
     for (int i = 0; i < njobs; ++i) {
         const wct_decon_job& j = jobs[i];
         const wct_decon_kernel K = j.kernel;
-        WCT_REQUIRE(pick_tk(K->nw) == tk, WCT_ERR_UNSUPPORTED,
-                    "decon_plan: planes of one batch need the same wire tile (%d vs %d wires)",
-                    jobs[0].kernel->nw, K->nw);
+        WCT_REQUIRE(K->nw <= wire_tile_max_nw(tk), WCT_ERR_UNSUPPORTED,
+                    "decon_plan: %d wires exceed the %d-bin wire tile's limit of %d", K->nw, tk,
+                    wire_tile_max_nw(tk));
         WCT_REQUIRE(j.adc && j.out && j.ld_adc >= K->nt && j.ld_out >= K->nt && j.ld_adc % 2 == 0 &&
                         j.ld_out % 2 == 0 && (!j.mask || (j.ld_mask >= K->nt && j.ld_mask % 2 == 0)),
                     WCT_ERR_ARG, "decon_plan: job %d has bad pointers or strides", i);
@@ -875,6 +916,9 @@ int wct_decon_plan_create(wct_decon_plan* out, const wct_decon_job* jobs, int nj
                         (reinterpret_cast<uintptr_t>(j.out) & 7) == 0 &&
                         (!j.mask || (reinterpret_cast<uintptr_t>(j.mask) & 1) == 0),
                     WCT_ERR_ARG, "decon_plan: job %d buffers are misaligned", i);
+        WCT_REQUIRE(!j.adjoint || (!j.bad_ptr && !j.restore_baseline && !j.mask && !j.save_mode),
+                    WCT_ERR_ARG, "decon_plan: adjoint job %d must be linear (no bad ranges, baseline, "
+                    "mask or save_mode)", i);
         DJob& d = h[i];
         memset(&d, 0, sizeof(d));
         d.adc = j.adc;
@@ -898,6 +942,7 @@ int wct_decon_plan_create(wct_decon_plan* out, const wct_decon_job* jobs, int nj
         d.th_factor = j.th_factor;
         d.restore = j.restore_baseline;
         d.save_mode = j.save_mode;
+        d.adjoint = j.adjoint ? 1 : 0;
         d.S = reinterpret_cast<float2*>(static_cast<char*>(workspace) + off);
         off += (size_t)d.nw * d.ldk * sizeof(float2);
         off = (off + 255) & ~(size_t)255;
@@ -942,6 +987,7 @@ int wct_decon_plan_create(wct_decon_plan* out, const wct_decon_job* jobs, int nj
     p->Mmax = Mmax;
     p->nwmax = nwmax;
     p->tk = tk;
+    p->row_pct = row_pct;
     hipError_t e = hipMalloc((void**)&p->jobs, njobs * sizeof(DJob));
     if (e == hipSuccess) e = hipMemcpy(p->jobs, h.data(), njobs * sizeof(DJob), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -965,6 +1011,67 @@ int wct_decon_plan_destroy(wct_decon_plan p)
     if (!p) return WCT_OK;
     if (p->jobs) WCT_HIP_CHECK(hipFree(p->jobs));
     delete p;
+    return WCT_OK;
+}
+
+int wct_decon_graph_create(wct_decon_graph* out, const wct_decon_plan* plans, int nplans, void* stream)
+{
+    WCT_REQUIRE(out && plans && nplans > 0, WCT_ERR_ARG, "decon_graph: no plans");
+    *out = nullptr;
+    for (int i = 0; i < nplans; ++i) WCT_REQUIRE(plans[i] && plans[i]->jobs, WCT_ERR_ARG, "decon_graph: null plan");
+    hipStream_t s = as_stream(stream);
+    hipStream_t own = nullptr;
+    if (!s) {   // the legacy default stream cannot be captured
+        WCT_HIP_CHECK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+        s = own;
+    }
+    auto* g = new wct_decon_graph_s();
+    int rc = WCT_OK;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        for (int i = 0; i < nplans && rc == WCT_OK; ++i) rc = wct_decon_plan_run(plans[i], s);
+        hipGraph_t graph = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(s, &graph);
+        g->graph = graph;
+        if (rc == WCT_OK && e2 != hipSuccess) {
+            set_error("decon_graph: capture failed: %s", hipGetErrorString(e2));
+            rc = WCT_ERR_HIP;
+        }
+        if (rc == WCT_OK) {
+            e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                set_error("decon_graph: instantiate failed: %s", hipGetErrorString(e));
+                rc = WCT_ERR_HIP;
+            }
+        }
+    } else {
+        set_error("decon_graph: begin capture failed: %s", hipGetErrorString(e));
+        rc = WCT_ERR_HIP;
+    }
+    if (own) (void)hipStreamDestroy(own);
+    if (rc != WCT_OK) {
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return WCT_OK;
+}
+
+int wct_decon_graph_launch(wct_decon_graph g, void* stream)
+{
+    WCT_REQUIRE(g && g->exec, WCT_ERR_ARG, "decon_graph_launch: null graph");
+    WCT_HIP_CHECK(hipGraphLaunch(g->exec, as_stream(stream)));
+    return WCT_OK;
+}
+
+int wct_decon_graph_destroy(wct_decon_graph g)
+{
+    if (!g) return WCT_OK;
+    if (g->exec) WCT_HIP_CHECK(hipGraphExecDestroy(g->exec));
+    if (g->graph) WCT_HIP_CHECK(hipGraphDestroy(g->graph));
+    delete g;
     return WCT_OK;
 }
 

@@ -11,6 +11,10 @@ class WctError(RuntimeError):
     pass
 
 
+class DeconTuning(C.Structure):
+    _fields_ = [("wire_tile", C.c_int), ("row_grid_pct", C.c_int)]
+
+
 class DeconJob(C.Structure):
     _fields_ = [
         ("kernel", C.c_void_p),
@@ -23,6 +27,7 @@ class DeconJob(C.Structure):
         ("th_factor", C.c_float),
         ("restore_baseline", C.c_int),
         ("save_mode", C.c_int),
+        ("adjoint", C.c_int),
     ]
 
 
@@ -68,6 +73,11 @@ _SIGS = {
     "wct_decon_plan_create": (_I, [C.POINTER(C.c_void_p), C.POINTER(DeconJob), _I, _V, _Z]),
     "wct_decon_plan_run": (_I, [_V, _V]),
     "wct_decon_plan_destroy": (_I, [_V]),
+    "wct_decon_plan_create_tuned": (_I, [C.POINTER(C.c_void_p), C.POINTER(DeconJob), _I, _V, _Z,
+                                         C.POINTER(DeconTuning)]),
+    "wct_decon_graph_create": (_I, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), _I, _V]),
+    "wct_decon_graph_launch": (_I, [_V, _V]),
+    "wct_decon_graph_destroy": (_I, [_V]),
     "wct_plain_workspace_size": (_Z, [_I, _I]),
     "wct_plain_decon_plane": (_I, [_V, _I, _I, _I, _V, _I, _V, _V, _V, _I, _I, _F, _V, _V, _V, _V, _I, _V,
                                    _I, _V, _I, _V, _Z, _V]),

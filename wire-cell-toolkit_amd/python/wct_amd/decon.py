@@ -10,7 +10,7 @@ import ctypes as C
 
 import numpy as np
 
-from .capi import DeconJob, check, lib
+from .capi import DeconJob, DeconTuning, check, lib
 from . import workload as W
 
 
@@ -74,7 +74,7 @@ def _ptr(t):
 
 
 def make_job(kernel, adc, out, mask=None, rms=None, th_factor=W.TH_FACTOR, restore=1,
-             save_mode=0, lf_tight=None, lf_tighter=None, bad=None):
+             save_mode=0, lf_tight=None, lf_tighter=None, bad=None, adjoint=False):
     """One wct_decon_job.  adc/out/mask/rms are torch tensors (2D for the
     frames); bad is (ptr, lo, hi) int32 device tensors or None."""
     j = DeconJob()
@@ -93,6 +93,7 @@ def make_job(kernel, adc, out, mask=None, rms=None, th_factor=W.TH_FACTOR, resto
     j.th_factor = th_factor
     j.restore_baseline = restore
     j.save_mode = save_mode
+    j.adjoint = int(bool(adjoint))
     return j
 
 
@@ -100,7 +101,7 @@ class Batch:
     """A fixed list of plane jobs and its workspace; run() launches the
     three stages once over all of them."""
 
-    def __init__(self, jobs, device=None, workspace=None):
+    def __init__(self, jobs, device=None, workspace=None, wire_tile=0, row_grid_pct=0):
         import torch
         self.jobs = (DeconJob * len(jobs))(*jobs)
         self.n = len(jobs)
@@ -113,8 +114,9 @@ class Batch:
             self.ws = workspace
 
         h = C.c_void_p()
-        check(lib().wct_decon_plan_create(C.byref(h), self.jobs, self.n, self.ws.data_ptr(),
-                                          self.ws.numel()))
+        tun = DeconTuning(int(wire_tile), int(row_grid_pct))
+        check(lib().wct_decon_plan_create_tuned(C.byref(h), self.jobs, self.n, self.ws.data_ptr(),
+                                                self.ws.numel(), C.byref(tun)))
         self.plan = h
 
     def run(self, stream=None):
@@ -126,6 +128,40 @@ class Batch:
             torch.cuda.synchronize()
             lib().wct_decon_plan_destroy(self.plan)
             self.plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """A hipGraph replaying the runs of several batches in order
+    (wct_decon_graph_create), captured on a side stream; launch() replays
+    it on the current torch stream."""
+
+    def __init__(self, batches, stream=None):
+        import torch
+        self.batches = list(batches)     # the plans must outlive the graph
+        arr = (C.c_void_p * len(self.batches))(*[b.plan for b in self.batches])
+        h = C.c_void_p()
+        if stream is None:
+            stream = torch.cuda.Stream()
+        self._capture_stream = stream
+        torch.cuda.synchronize()
+        check(lib().wct_decon_graph_create(C.byref(h), arr, len(self.batches), stream.cuda_stream))
+        self.graph = h
+
+    def launch(self, stream=None):
+        check(lib().wct_decon_graph_launch(self.graph, _stream() if stream is None else stream))
+
+    def close(self):
+        if getattr(self, "graph", None):
+            import torch
+            torch.cuda.synchronize()
+            lib().wct_decon_graph_destroy(self.graph)
+            self.graph = None
 
     def __del__(self):
         try:

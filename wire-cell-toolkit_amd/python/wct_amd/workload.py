@@ -135,6 +135,20 @@ def tracks_raster(rng, ntracks, nw, nt, q_per_tick=Q_PER_TICK):
     return q
 
 
+def charge_raster(rng, nw, nt, occupancy=0.02, ntracks=None, blip_mean=20000.0):
+    """Sparse tracks plus exponential-energy point blips (electrons per
+    pixel) until the given fraction of pixels is nonzero (float32)."""
+    q = tracks_raster(rng, tracks_per_plane(nw) if ntracks is None else ntracks, nw, nt)
+    want = int(occupancy * nw * nt)
+    have = int(np.count_nonzero(q))
+    if want > have:
+        n = want - have
+        w = rng.integers(0, nw, n)
+        t = rng.integers(0, nt, n)
+        np.add.at(q, (w, t), rng.exponential(blip_mean, n).astype(np.float32))
+    return q
+
+
 def tracks_per_plane(nw, per_480=20):
     """Track count at the density of 20 tracks per 480 wires."""
     return max(1, int(round(per_480 * nw / 480.0)))
