@@ -30,8 +30,19 @@ struct wct_decon_kernel_s {
 
 namespace wct { struct DJob; }
 
+// Jobs of equal length, launched together with that length's kernels.
+struct wct_decon_group {
+    int len;          // M (row passes) or nw (wire pass)
+    long begin, end;  // rows or tiles
+    int twcount;      // twiddle entries of the length's plan
+    bool smooth5;
+    bool bad;         // some job of the group has bad-channel ranges
+};
+
 struct wct_decon_plan_s {
-    wct::DJob* jobs = nullptr;   // device job table
+    wct::DJob* jobs = nullptr;   // device job table, grouped by row length M (row passes)
+    wct::DJob* jobs_t = nullptr; // the same jobs grouped by wire count (wire pass)
+    std::vector<wct_decon_group> rgroups, tgroups;
     int njobs = 0;
     long rows = 0, tiles = 0;
     int Mmax = 0, nwmax = 0, tk = 8;
@@ -150,7 +161,7 @@ inline RowLds row_lds(int Mmax, int twmax, bool stats)
 // restages twiddles only when the row's length changes.
 template <int NT, int VH, bool S5>
 __global__ __launch_bounds__(NT) void k_row_r2c(const DJob* __restrict__ jobs, int njobs,
-                                                long nrows, RowLds lay)
+                                                long rbeg, long rend, RowLds lay)
 {
     constexpr int MMAX = NT * VH;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -170,8 +181,8 @@ __global__ __launch_bounds__(NT) void k_row_r2c(const DJob* __restrict__ jobs, i
             if (n < Jn.M) z[i] = src[n];
         }
     };
-    if (blockIdx.x < nrows) fetch(blockIdx.x);
-    for (long r = blockIdx.x; r < nrows; r += gridDim.x) {
+    if (rbeg + blockIdx.x < rend) fetch(rbeg + blockIdx.x);
+    for (long r = rbeg + blockIdx.x; r < rend; r += gridDim.x) {
         const DJob& J = jobs[find_job_rows(jobs, njobs, r)];
         const int w = (int)(r - J.row0);
         const int M = J.M;
@@ -192,7 +203,7 @@ __global__ __launch_bounds__(NT) void k_row_r2c(const DJob* __restrict__ jobs, i
                 buf[pidx(n)] = v;
             }
         }
-        if (r + gridDim.x < nrows) fetch(r + gridDim.x);
+        if (r + gridDim.x < rend) fetch(r + gridDim.x);
         __syncthreads();
         fft_lds<float2, NT, MMAX, S5>(buf, J.planM, twl, -1, threadIdx.x);
         float2* dst = J.S + (long)w * J.ldk;
@@ -218,20 +229,20 @@ __global__ __launch_bounds__(NT) void k_row_r2c(const DJob* __restrict__ jobs, i
 }
 
 // ---- K2: wire FFT, multiply by K, inverse wire FFT ------------------
-template <int NT, int TK, int NWMAX, bool S5>
+template <int NT, int TK, int NWMAX, bool S5, int NWS>
 __global__ __launch_bounds__(NT) void k_wire_pass(const DJob* __restrict__ jobs, int njobs,
-                                                  long ntiles, int ldc)
+                                                  long tbeg, long ntiles, int ldc)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float2* buf = reinterpret_cast<float2*>(smem_raw);
     float2* twl = buf + (size_t)TK * ldc;
-    const long t = xcd_remap(blockIdx.x, ntiles);
+    const long t = tbeg + xcd_remap(blockIdx.x, ntiles);
     const DJob& J = jobs[find_job_tiles(jobs, njobs, t)];
     const int k0 = (int)(t - J.tile0) * TK;
-    const int nw = J.nw;
+    const int nw = NWS ? NWS : J.nw;
     float2* S = J.S;
     stage_twiddles(twl, J.twW, J.planW.twcount, threadIdx.x, NT);
-    constexpr int NLD = (NWMAX * TK + NT - 1) / NT;   // elements per thread, all loads in flight
+    constexpr int NLD = ((NWS ? NWS : NWMAX) * TK + NT - 1) / NT;   // elements per thread, all loads in flight
     {
         float2 t[NLD];
 #pragma unroll
@@ -249,7 +260,10 @@ __global__ __launch_bounds__(NT) void k_wire_pass(const DJob* __restrict__ jobs,
     constexpr int SL = NT / TK;
     const int slice = threadIdx.x / SL, st = threadIdx.x % SL;
     float2* col = buf + slice * ldc;
-    fft_lds<float2, SL, NWMAX, S5>(col, J.planW, twl, -1, st);
+    if constexpr (NWS > 0)
+        fft_static<float2, SL, NWS>(col, twl, -1, st);
+    else
+        fft_lds<float2, SL, NWMAX, S5>(col, J.planW, twl, -1, st);
     const int k = k0 + slice;
     const float2* Kk = J.K + (long)k * nw;
     for (int m = st; m < nw; m += SL) {
@@ -264,7 +278,10 @@ __global__ __launch_bounds__(NT) void k_wire_pass(const DJob* __restrict__ jobs,
         col[pidx(m)] = o;
     }
     __syncthreads();
-    fft_lds<float2, SL, NWMAX, S5>(col, J.planW, twl, +1, st);
+    if constexpr (NWS > 0)
+        fft_static<float2, SL, NWS>(col, twl, +1, st);
+    else
+        fft_lds<float2, SL, NWMAX, S5>(col, J.planW, twl, +1, st);
     for (int idx = threadIdx.x; idx < nw * TK; idx += NT) {
         const int cc = idx % TK, w = idx / TK;
         S[(long)w * J.ldk + k0 + cc] = buf[cc * ldc + pidx(w)];
@@ -313,22 +330,25 @@ __device__ __forceinline__ void c2r_stage(const DJob& J, const float2* __restric
 }
 
 // ---- K3: inverse real FFT, baseline, RMS, threshold, outputs ---------
-template <int NT, int VH, bool S5>
-__global__ __launch_bounds__(NT, 2) void k_row_c2r_roi(const DJob* __restrict__ jobs, int njobs,
-                                                    long nrows, RowLds lay)
+// BAD: the group has bad-channel ranges (rows with excluded samples take
+// the general statistics path); without, that path is not compiled in,
+// which keeps the kernel within the registers of three workgroups per CU.
+template <int NT, int VH, bool S5, bool BAD>
+__global__ __launch_bounds__(NT, 3) void k_row_c2r_roi(const DJob* __restrict__ jobs, int njobs,
+                                                    long rbeg, long rend, RowLds lay)
 {
     constexpr int MMAX = NT * VH;
     constexpr int VPT = 2 * VH;            // real samples per thread
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float2* buf = reinterpret_cast<float2*>(smem_raw);
     RowScratch* sh = reinterpret_cast<RowScratch*>(buf);   // reused once the row is in registers
-    for (long r = blockIdx.x; r < nrows; r += gridDim.x) {
+    for (long r = rbeg + blockIdx.x; r < rend; r += gridDim.x) {
         const DJob& J = jobs[find_job_rows(jobs, njobs, r)];
         const int w = (int)(r - J.row0);
         const int M = J.M;
         const float2* twl = J.twM;
         const float2* Y = J.S + (long)w * J.ldk;
-        const bool has_bad = J.bad_ptr && J.bad_ptr[w + 1] > J.bad_ptr[w];
+        const bool has_bad = BAD && J.bad_ptr && J.bad_ptr[w + 1] > J.bad_ptr[w];
 
         c2r_stage<NT, VH>(J, Y, J.lf1, buf);
         __syncthreads();
@@ -363,7 +383,7 @@ __global__ __launch_bounds__(NT, 2) void k_row_c2r_roi(const DJob* __restrict__ 
             const bool row_bad = has_bad && (__syncthreads_or(pm != valid) != 0);
             if (!row_bad) {
                 rms = row_stats_fast<NT, VPT>(val, valid, J.restore != 0, true, sh);
-            } else {
+            } else if constexpr (BAD) {
                 if (J.restore) row_restore_baseline<NT, VPT>(val, valid, sh);
                 rms = row_cal_rms<NT, VPT>(val, pm, sh);
             }
@@ -591,64 +611,114 @@ static int dispatch_vh(int Mmax, F&& f)
 
 namespace wct {
 
-template <bool S5>
-int plan_run(wct_decon_plan p, hipStream_t s)
+// Wire counts with wire-pass kernels compiled for that exact count (the
+// compile-time FFT plan saves the pass bookkeeping of the column
+// transforms); other counts take the runtime-plan kernel.
+template <typename F>
+static int dispatch_static_nw(int nw, int tk, F&& f)
 {
-    DJob* dj = p->jobs;
-    const int njobs = p->njobs;
-    const long rows = p->rows, tiles = p->tiles;
-    const RowLds lay = row_lds(p->Mmax, p->twM, false);
-    const RowLds lay3 = row_lds(p->Mmax, p->twM, true);
-    int rc = dispatch_vh(p->Mmax, [&](auto V) -> int {
-        constexpr int VH = decltype(V)::value;
-        const size_t lds = lay.bytes();
-        int r_ = allow_lds(k_row_r2c<kNT, VH, S5>, lds);
-        if (r_) return r_;
-        int grid = 0;
-        r_ = resident_grid(k_row_r2c<kNT, VH, S5>, kNT, lds, p->ncu, rows, p->row_pct, &grid);
-        if (r_) return r_;
-        hipLaunchKernelGGL((k_row_r2c<kNT, VH, S5>), dim3(grid), dim3(kNT), lds, s, dj, njobs, rows, lay);
-        return WCT_OK;
-    });
-    if (rc) return rc;
-    WCT_HIP_CHECK(hipGetLastError());
-    const int ldc = padded_len(p->nwmax) | 1;
-    const size_t lds2 = ((size_t)p->tk * ldc + p->twW) * sizeof(float2);
-    if (p->tk == 16) {
-        rc = allow_lds(k_wire_pass<kNT, 16, 1024, S5>, lds2);
-        if (rc) return rc;
-        hipLaunchKernelGGL((k_wire_pass<kNT, 16, 1024, S5>), dim3(tiles), dim3(kNT), lds2, s, dj, njobs,
-                           tiles, ldc);
-    } else if (p->tk == 8) {
-        rc = allow_lds(k_wire_pass<kNT, 8, 1024, S5>, lds2);
-        if (rc) return rc;
-        hipLaunchKernelGGL((k_wire_pass<kNT, 8, 1024, S5>), dim3(tiles), dim3(kNT), lds2, s, dj, njobs,
-                           tiles, ldc);
-    } else if (p->tk == 4) {
-        rc = allow_lds(k_wire_pass<kNT, 4, 2048, S5>, lds2);
-        if (rc) return rc;
-        hipLaunchKernelGGL((k_wire_pass<kNT, 4, 2048, S5>), dim3(tiles), dim3(kNT), lds2, s, dj, njobs,
-                           tiles, ldc);
-    } else {
-        set_error("decon_plan_run: %d wires not supported", p->nwmax);
-        return WCT_ERR_UNSUPPORTED;
+    if (tk == 8) {
+        if (nw == 800) return f(std::integral_constant<int, 800>());
+        if (nw == 960) return f(std::integral_constant<int, 960>());
     }
-    WCT_HIP_CHECK(hipGetLastError());
-    rc = dispatch_vh(p->Mmax, [&](auto V) -> int {
-        constexpr int VH = decltype(V)::value;
-        const size_t lds = lay3.bytes();
-        int r_ = allow_lds(k_row_c2r_roi<kNT, VH, S5>, lds);
-        if (r_) return r_;
-        int grid = 0;
-        r_ = resident_grid(k_row_c2r_roi<kNT, VH, S5>, kNT, lds, p->ncu, rows, p->row_pct, &grid);
-        if (r_) return r_;
-        hipLaunchKernelGGL((k_row_c2r_roi<kNT, VH, S5>), dim3(grid), dim3(kNT), lds, s, dj, njobs,
-                           rows, lay3);
-        return WCT_OK;
-    });
+    return f(std::integral_constant<int, 0>());
+}
+
+template <int VH, bool S5>
+static int launch_rows_r2c(wct_decon_plan p, const wct_decon_group& g, hipStream_t s)
+{
+    const RowLds lay = row_lds(g.len, g.twcount, false);
+    const size_t lds = lay.bytes();
+    int rc = allow_lds(k_row_r2c<kNT, VH, S5>, lds);
     if (rc) return rc;
+    int grid = 0;
+    rc = resident_grid(k_row_r2c<kNT, VH, S5>, kNT, lds, p->ncu, g.end - g.begin, p->row_pct, &grid);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_row_r2c<kNT, VH, S5>), dim3(grid), dim3(kNT), lds, s, p->jobs, p->njobs, g.begin,
+                       g.end, lay);
     WCT_HIP_CHECK(hipGetLastError());
     return WCT_OK;
+}
+
+template <int VH, bool S5, bool BAD>
+static int launch_rows_c2r(wct_decon_plan p, const wct_decon_group& g, hipStream_t s)
+{
+    const RowLds lay = row_lds(g.len, g.twcount, true);
+    const size_t lds = lay.bytes();
+    int rc = allow_lds(k_row_c2r_roi<kNT, VH, S5, BAD>, lds);
+    if (rc) return rc;
+    int grid = 0;
+    rc = resident_grid(k_row_c2r_roi<kNT, VH, S5, BAD>, kNT, lds, p->ncu, g.end - g.begin, p->row_pct, &grid);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_row_c2r_roi<kNT, VH, S5, BAD>), dim3(grid), dim3(kNT), lds, s, p->jobs, p->njobs,
+                       g.begin, g.end, lay);
+    WCT_HIP_CHECK(hipGetLastError());
+    return WCT_OK;
+}
+
+template <int TK, int NWMAX, bool S5, int NWS>
+static int launch_wire(wct_decon_plan p, const wct_decon_group& g, hipStream_t s)
+{
+    const int ldc = padded_len(g.len) | 1;
+    const size_t lds = ((size_t)TK * ldc + g.twcount) * sizeof(float2);
+    int rc = allow_lds(k_wire_pass<kNT, TK, NWMAX, S5, NWS>, lds);
+    if (rc) return rc;
+    const long nt = g.end - g.begin;
+    hipLaunchKernelGGL((k_wire_pass<kNT, TK, NWMAX, S5, NWS>), dim3(nt), dim3(kNT), lds, s, p->jobs_t, p->njobs,
+                       g.begin, nt, ldc);
+    WCT_HIP_CHECK(hipGetLastError());
+    return WCT_OK;
+}
+
+template <bool S5, int VH>
+static int row_pass(wct_decon_plan p, const wct_decon_group& g, hipStream_t s, bool c2r)
+{
+    if (!c2r) return launch_rows_r2c<VH, S5>(p, g, s);
+    return g.bad ? launch_rows_c2r<VH, S5, true>(p, g, s) : launch_rows_c2r<VH, S5, false>(p, g, s);
+}
+
+static int run_rows(wct_decon_plan p, hipStream_t s, bool c2r)
+{
+    for (const auto& g : p->rgroups) {
+        const int rc = dispatch_vh(g.len, [&](auto V) -> int {
+            constexpr int VH = decltype(V)::value;
+            return g.smooth5 ? row_pass<true, VH>(p, g, s, c2r) : row_pass<false, VH>(p, g, s, c2r);
+        });
+        if (rc) return rc;
+    }
+    return WCT_OK;
+}
+
+static int run_wire(wct_decon_plan p, hipStream_t s)
+{
+    for (const auto& g : p->tgroups) {
+        int rc = WCT_OK;
+        if (p->tk == 16)
+            rc = g.smooth5 ? launch_wire<16, 1024, true, 0>(p, g, s) : launch_wire<16, 1024, false, 0>(p, g, s);
+        else if (p->tk == 8)
+            rc = dispatch_static_nw(g.len, 8, [&](auto N) -> int {
+                constexpr int NWS = decltype(N)::value;
+                if (NWS > 0 || g.smooth5) return launch_wire<8, 1024, true, NWS>(p, g, s);
+                return launch_wire<8, 1024, false, 0>(p, g, s);
+            });
+        else if (p->tk == 4)
+            rc = g.smooth5 ? launch_wire<4, 2048, true, 0>(p, g, s) : launch_wire<4, 2048, false, 0>(p, g, s);
+        else {
+            set_error("decon_plan_run: wire tile %d not supported", p->tk);
+            rc = WCT_ERR_UNSUPPORTED;
+        }
+        if (rc) return rc;
+    }
+    return WCT_OK;
+}
+
+int plan_run(wct_decon_plan p, hipStream_t s)
+{
+    int rc = run_rows(p, s, false);
+    if (rc) return rc;
+    rc = run_wire(p, s);
+    if (rc) return rc;
+    return run_rows(p, s, true);
 }
 
 }  // namespace wct
@@ -946,8 +1016,6 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
         d.S = reinterpret_cast<float2*>(static_cast<char*>(workspace) + off);
         off += (size_t)d.nw * d.ldk * sizeof(float2);
         off = (off + 255) & ~(size_t)255;
-        d.row0 = rows;
-        d.tile0 = tiles;
         rows += d.nw;
         tiles += d.ldk / tk;
         const DevPlan* pM = nullptr;
@@ -988,10 +1056,43 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
     p->nwmax = nwmax;
     p->tk = tk;
     p->row_pct = row_pct;
+    // two orderings of the same jobs: by row length for the row passes,
+    // by wire count for the wire pass, each with its own cumulative row
+    // and tile starts and its groups of equal length
+    std::vector<int> ord(njobs);
+    for (int i = 0; i < njobs; ++i) ord[i] = i;
+    std::vector<DJob> hr(njobs), ht(njobs);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return h[a].M < h[b].M; });
+    long acc = 0;
+    for (int i = 0; i < njobs; ++i) {
+        hr[i] = h[ord[i]];
+        hr[i].row0 = acc;
+        if (p->rgroups.empty() || p->rgroups.back().len != hr[i].M)
+            p->rgroups.push_back({hr[i].M, acc, acc, hr[i].planM.twcount, true, false});
+        acc += hr[i].nw;
+        p->rgroups.back().end = acc;
+        if (!is_smooth5(hr[i].M)) p->rgroups.back().smooth5 = false;
+        if (hr[i].bad_ptr) p->rgroups.back().bad = true;
+    }
+    for (int i = 0; i < njobs; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return h[a].nw < h[b].nw; });
+    acc = 0;
+    for (int i = 0; i < njobs; ++i) {
+        ht[i] = h[ord[i]];
+        ht[i].tile0 = acc;
+        if (p->tgroups.empty() || p->tgroups.back().len != ht[i].nw)
+            p->tgroups.push_back({ht[i].nw, acc, acc, ht[i].planW.twcount, true, false});
+        acc += ht[i].ldk / tk;
+        p->tgroups.back().end = acc;
+        if (!is_smooth5(ht[i].nw)) p->tgroups.back().smooth5 = false;
+    }
     hipError_t e = hipMalloc((void**)&p->jobs, njobs * sizeof(DJob));
-    if (e == hipSuccess) e = hipMemcpy(p->jobs, h.data(), njobs * sizeof(DJob), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void**)&p->jobs_t, njobs * sizeof(DJob));
+    if (e == hipSuccess) e = hipMemcpy(p->jobs, hr.data(), njobs * sizeof(DJob), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->jobs_t, ht.data(), njobs * sizeof(DJob), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         if (p->jobs) (void)hipFree(p->jobs);
+        if (p->jobs_t) (void)hipFree(p->jobs_t);
         delete p;
         set_error("decon_plan: job table upload failed: %s", hipGetErrorString(e));
         return WCT_ERR_HIP;
@@ -1003,13 +1104,14 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
 int wct_decon_plan_run(wct_decon_plan p, void* stream)
 {
     WCT_REQUIRE(p && p->jobs, WCT_ERR_ARG, "decon_plan_run: null plan");
-    return p->smooth5 ? plan_run<true>(p, as_stream(stream)) : plan_run<false>(p, as_stream(stream));
+    return plan_run(p, as_stream(stream));
 }
 
 int wct_decon_plan_destroy(wct_decon_plan p)
 {
     if (!p) return WCT_OK;
     if (p->jobs) WCT_HIP_CHECK(hipFree(p->jobs));
+    if (p->jobs_t) WCT_HIP_CHECK(hipFree(p->jobs_t));
     delete p;
     return WCT_OK;
 }

@@ -278,4 +278,122 @@ __device__ __forceinline__ void stage_twiddles(T2* dst, const T2* __restrict__ s
     for (int i = tid; i < count; i += nthreads) dst[i] = src[i];
 }
 
+// ---- compile-time plans -------------------------------------------------
+// The radix schedule of a length known at compile time, in the order the
+// host plan uses (8s, then a 4 or a 2, then 5, 3, 7, 11, 13), so the
+// twiddle table built for the runtime plan serves both.  With n, each
+// radix and each span constant, a pass needs no plan arrays, no radix
+// switch and no runtime division.
+
+constexpr int sf_radix(int n, int p)
+{
+    int m = n, idx = 0;
+    while (m % 8 == 0 && m > 1) {
+        if (idx == p) return 8;
+        ++idx;
+        m /= 8;
+    }
+    if (m % 4 == 0) {
+        if (idx == p) return 4;
+        ++idx;
+        m /= 4;
+    }
+    if (m % 2 == 0) {
+        if (idx == p) return 2;
+        ++idx;
+        m /= 2;
+    }
+    const int odd[5] = {5, 3, 7, 11, 13};
+    for (int q = 0; q < 5; ++q)
+        while (m % odd[q] == 0) {
+            if (idx == p) return odd[q];
+            ++idx;
+            m /= odd[q];
+        }
+    return 0;
+}
+
+constexpr int sf_npass(int n)
+{
+    int p = 0;
+    while (sf_radix(n, p)) ++p;
+    return p;
+}
+
+constexpr int sf_span(int n, int p)
+{
+    int ns = 1;
+    for (int q = 0; q < p; ++q) ns *= sf_radix(n, q);
+    return ns;
+}
+
+constexpr int sf_twoff(int n, int p)
+{
+    int off = 0, ns = 1;
+    for (int q = 0; q < p; ++q) {
+        off += ns * (sf_radix(n, q) - 1);
+        ns *= sf_radix(n, q);
+    }
+    return off;
+}
+
+// One Stockham pass with every size a constant.
+template <typename T2, int R, int NT, int N, int NS>
+__device__ __forceinline__ void stockham_pass_c(T2* buf, const T2* __restrict__ tw, const T2* __restrict__ gtab,
+                                                int sign, int tid)
+{
+    constexpr int m = N / R;
+    constexpr int MAXB = (m + NT - 1) / NT;
+    T2 v[MAXB][R];
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int j = tid + b * NT;
+        if (m % NT == 0 || j < m) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = buf[pidx(j + r * m)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int j = tid + b * NT;
+        if (m % NT == 0 || j < m) {
+            const int k = NS == 1 ? 0 : j % NS;
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    T2 w = tw[k * (R - 1) + (r - 1)];
+                    if (sign > 0) w.y = -w.y;
+                    v[b][r] = cmul(v[b][r], w);
+                }
+            }
+            Dft<T2, R>::run(v[b], sign, gtab + generic_tab_offset(R));
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[pidx(base + r * NS)] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T2, int NT, int N, int P>
+__device__ __forceinline__ void fft_static_from(T2* buf, const T2* __restrict__ tw, int sign, int tid)
+{
+    if constexpr (P < sf_npass(N)) {
+        constexpr int R = sf_radix(N, P);
+        stockham_pass_c<T2, R, NT, N, sf_span(N, P)>(buf, tw + sf_twoff(N, P), tw + sf_twoff(N, sf_npass(N)),
+                                                     sign, tid);
+        fft_static_from<T2, NT, N, P + 1>(buf, tw, sign, tid);
+    }
+}
+
+// Full transform of length N (a compile-time constant) in LDS; same
+// contract as fft_lds, with the twiddle table of the runtime plan of N.
+template <typename T2, int NT, int N>
+__device__ __forceinline__ void fft_static(T2* buf, const T2* __restrict__ tw, int sign, int tid)
+{
+    static_assert(sf_npass(N) > 0 && sf_npass(N) <= kMaxPasses, "length with prime factors above 13");
+    fft_static_from<T2, NT, N, 0>(buf, tw, sign, tid);
+}
+
 }  // namespace wct

@@ -1,8 +1,7 @@
 // Block-wide row statistics for one wire row held in registers:
-// exact order statistics by radix select over LDS histograms,
-// restore_baseline (sigproc/src/OmnibusSigProc.cxx:867-906), the robust
-// RMS of ROI_formation::cal_RMS (sigproc/src/ROI_formation.cxx:364-389)
-// and its threshold.
+// exact order statistics, restore_baseline (sigproc/src/
+// OmnibusSigProc.cxx:867-906), the robust RMS of ROI_formation::cal_RMS
+// (sigproc/src/ROI_formation.cxx:364-389) and its threshold.
 //
 // Order statistics follow Waveform::percentile (util/src/Waveform.cxx:
 // 85-106): the value at index min((size_t)(p * (float)n), n-1) of the
@@ -19,18 +18,18 @@
 //    so the 16/50/84% points of the row are either 0 or fl(z - b) for the
 //    element z of Z at a rank shifted by the zero count.
 // Z is therefore counting-sorted into LDS once, by linear bins spanning
-// mean +- 8 sd of Z (plus an underflow and an overflow bin; the bin of a
-// sample is a monotone function of it, so bins are contiguous in sorted
-// order).  Each statistic then finds its bin from the offsets and selects
-// exactly inside that bin's keys (rank counting for bins up to a block,
-// radix passes above).
+// +-8 clipped standard deviations around the clipped mean of Z (moments
+// of the samples within 3 sd of the mean, so signal on the row does not
+// widen the bins of its noise), plus an underflow and an overflow bin.
+// The bin of a sample is a monotone function of it, so bins are
+// contiguous in sorted order.  Each statistic then finds its bin from the
+// offsets and selects exactly inside that bin's keys (rank counting for
+// bins up to a block, 8-bit radix passes above).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace wct {
-
-constexpr int kHistBins = 2048;
 
 __device__ __forceinline__ uint32_t f2key(float f)
 {
@@ -43,21 +42,30 @@ __device__ __forceinline__ float key2f(uint32_t k)
     return __uint_as_float(u);
 }
 
-constexpr int kLinBins = 2048;          // interior linear bins
-constexpr int kLinHist = 2560;          // histogram length (underflow, bins, overflow, padding)
+constexpr int kLinBins = 1024;          // interior linear bins
+constexpr int kLinHist = 1280;          // histogram length (underflow, bins, overflow, padding)
+constexpr int kRadixBins = 256;         // 8-bit digits of the radix selections
 
 // LDS scratch of the row-statistics routines.  The bin-sorted keys of
 // the row follow the struct (row_keys()).
 struct RowScratch {
-    uint32_t off[kLinHist + 1];  // linear-bin counts, then exclusive offsets
-    uint32_t cur[kLinHist];      // scatter cursors; reused as the radix histogram
-    uint32_t wsum[32];
-    uint32_t csum[4 * 32];
-    double dsum[32];
+    // Linear-bin counts, then exclusive offsets; once the keys are
+    // scattered, off[b] holds the end of bin b (the start of bin b + 1).
+    uint32_t off[kLinHist + 1];
+    uint32_t hist[kRadixBins];   // radix histogram
+    uint32_t wsum[8];
+    uint32_t csum[4][8];
+    double dsum[8];
     uint32_t sel_bucket;
     uint32_t sel_rank;
-    uint32_t list_n;
-    uint32_t pad_;
+    // Fast path: every phase of a row reduces into its own slots, so no
+    // phase has to wait for the reads of the one before (phases of
+    // consecutive rows are separated by the row's other barriers).
+    float fpart[4][8];           // per-wave float sums
+    uint32_t upart[5][2][8];     // [phase][counter][wave]
+    double dpart[8];
+    uint32_t sbin[5];            // located bins of the selections
+    uint32_t skey[5];            // selected keys
 };
 
 __device__ __forceinline__ uint32_t* row_keys(RowScratch* sh)
@@ -84,7 +92,22 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
     return v;
 }
 
-// Sums of up to four counters over the block.
+__device__ __forceinline__ float wave_sum_f(float v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sums of up to four counters over the block (two barriers; the slots
+// are reused).
 template <int NT, int NC>
 __device__ __forceinline__ void block_sums(uint32_t (&c)[NC], RowScratch* sh)
 {
@@ -95,14 +118,14 @@ __device__ __forceinline__ void block_sums(uint32_t (&c)[NC], RowScratch* sh)
     __syncthreads();
     if (lane == 0) {
 #pragma unroll
-        for (int q = 0; q < NC; ++q) sh->csum[q * 32 + wid] = c[q];
+        for (int q = 0; q < NC; ++q) sh->csum[q][wid] = c[q];
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
         uint32_t t = 0;
 #pragma unroll
-        for (int w = 0; w < NW; ++w) t += sh->csum[q * 32 + w];
+        for (int w = 0; w < NW; ++w) t += sh->csum[q][w];
         c[q] = t;
     }
 }
@@ -120,8 +143,7 @@ __device__ __forceinline__ double block_sum_f64(double v, RowScratch* sh)
 {
     constexpr int NW = NT / 64;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    v = wave_sum_d(v);
     __syncthreads();
     if (lane == 0) sh->dsum[wid] = v;
     __syncthreads();
@@ -131,11 +153,34 @@ __device__ __forceinline__ double block_sum_f64(double v, RowScratch* sh)
     return tot;
 }
 
-// Finds the bin of hist (nb bins) holding the rank-th element and the
-// rank inside it; every thread gets them.
+// Sums of NC counters over the block through the slots of one phase:
+// one barrier.
+template <int NT, int NC>
+__device__ __forceinline__ void block_sums_slot(uint32_t (&c)[NC], uint32_t (*slot)[8])
+{
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) c[q] = wave_sum(c[q]);
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NC; ++q) slot[q][wid] = c[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) t += slot[q][w];
+        c[q] = t;
+    }
+}
+
+// Finds the bin of hist (nb bins, a multiple of NT) holding the rank-th
+// element and the rank inside it; every thread gets them.
 template <int NT>
 __device__ __forceinline__ void hist_locate(const uint32_t* hist, int nb, uint32_t rank, uint32_t& bucket,
-                            uint32_t& sub, RowScratch* sh)
+                                            uint32_t& sub, RowScratch* sh)
 {
     constexpr int NW = NT / 64;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -180,35 +225,6 @@ __device__ __forceinline__ uint32_t lin_bin(float x, LinBins lb)
     return t < 0.f ? 0u : (t >= (float)kLinBins ? (uint32_t)kLinBins + 1u : 1u + (uint32_t)t);
 }
 
-// Exclusive scan of a[0..n) in place (n a multiple of NT), total in a[n].
-template <int NT>
-__device__ __forceinline__ void block_excl_scan(uint32_t* a, int n, RowScratch* sh)
-{
-    constexpr int NW = NT / 64;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int per = n / NT;
-    uint32_t* my = a + threadIdx.x * per;
-    uint32_t local = 0;
-    for (int b = 0; b < per; ++b) local += my[b];
-    const uint32_t incl = wave_incl_scan(local);
-    if (lane == 63) sh->wsum[wid] = incl;
-    __syncthreads();
-    uint32_t base = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-        base += (w < wid) ? sh->wsum[w] : 0u;
-        tot += sh->wsum[w];
-    }
-    uint32_t run = base + incl - local;
-    for (int b = 0; b < per; ++b) {
-        const uint32_t c = my[b];
-        my[b] = run;
-        run += c;
-    }
-    if (threadIdx.x == 0) a[n] = tot;
-    __syncthreads();
-}
-
 // Exact rank-th smallest of seg[0..cnt) (unordered keys in LDS).
 template <int NT>
 __device__ __forceinline__ uint32_t seg_select(const uint32_t* seg, uint32_t cnt, uint32_t rank, RowScratch* sh)
@@ -229,54 +245,82 @@ __device__ __forceinline__ uint32_t seg_select(const uint32_t* seg, uint32_t cnt
         __syncthreads();
         return key;
     }
-    // three radix passes over the segment (11/11/10 bits)
-    uint32_t* h = sh->cur;
+    // four radix passes over the segment, 8 bits each
+    uint32_t* h = sh->hist;
     uint32_t prefix = 0, pmask = 0;
-    const int shifts[3] = {21, 10, 0};
-    const uint32_t dmasks[3] = {0x7ffu, 0x7ffu, 0x3ffu};
 #pragma unroll 1
-    for (int pass = 0; pass < 3; ++pass) {
-        const int sh_ = shifts[pass];
-        const uint32_t dm = dmasks[pass];
-        for (int b = threadIdx.x; b < kHistBins; b += NT) h[b] = 0;
+    for (int sh_ = 24; sh_ >= 0; sh_ -= 8) {
+        for (int b = threadIdx.x; b < kRadixBins; b += NT) h[b] = 0;
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < cnt; i += NT) {
             const uint32_t k = seg[i];
-            if ((k & pmask) == prefix) atomicAdd(&h[(k >> sh_) & dm], 1u);
+            if ((k & pmask) == prefix) atomicAdd(&h[(k >> sh_) & 0xffu], 1u);
         }
         __syncthreads();
         uint32_t bucket, sub;
-        hist_locate<NT>(h, pass == 2 ? 1024 : kHistBins, rank, bucket, sub, sh);
+        hist_locate<NT>(h, kRadixBins, rank, bucket, sub, sh);
         rank = sub;
         prefix |= bucket << sh_;
-        pmask |= dm << sh_;
+        pmask |= 0xffu << sh_;
     }
     return prefix;
-}
-
-// rank-th smallest of the bin-sorted keys (offsets in sh->off).
-template <int NT>
-__device__ __forceinline__ uint32_t sorted_select(uint32_t rank, RowScratch* sh)
-{
-    // the bin b with off[b] <= rank < off[b+1]
-    const int per = kLinHist / NT;
-    for (int q = 0; q < per; ++q) {
-        const int b = threadIdx.x * per + q;
-        if (sh->off[b] <= rank && rank < sh->off[b + 1]) {
-            sh->sel_rank = b;
-        }
-    }
-    __syncthreads();
-    const uint32_t b = sh->sel_rank;
-    const uint32_t o = sh->off[b], cnt = sh->off[b + 1] - o;
-    __syncthreads();
-    return seg_select<NT>(row_keys(sh) + o, cnt, rank - o, sh);
 }
 
 __device__ __forceinline__ uint32_t pct_index(float p, uint32_t n)
 {
     uint32_t mid = (uint32_t)(p * (float)n);
     return mid < n - 1 ? mid : n - 1;
+}
+
+// Bin b of the scattered keys: off[b] is its end, off[b-1] its start.
+__device__ __forceinline__ uint32_t bin_start(const RowScratch* sh, uint32_t b)
+{
+    return b ? sh->off[b - 1] : 0u;
+}
+
+// The thread owning the bin that holds sorted rank r writes that bin.
+template <int NT>
+__device__ __forceinline__ void locate_write(uint32_t r, uint32_t* dst, const RowScratch* sh)
+{
+    constexpr int per = kLinHist / NT;
+    const int b0 = threadIdx.x * per;
+    if (r >= bin_start(sh, b0) && r < sh->off[b0 + per - 1]) {
+        for (int q = 0; q < per; ++q)
+            if (r < sh->off[b0 + q]) {
+                *dst = b0 + q;
+                break;
+            }
+    }
+}
+
+// Rank counting inside a segment of at most a block of keys: the thread
+// holding the key at sorted position `rank` writes it.
+__device__ __forceinline__ void seg_rank_write(const uint32_t* seg, uint32_t cnt, uint32_t rank, uint32_t* dst)
+{
+    if (threadIdx.x < cnt) {
+        const uint32_t k = seg[threadIdx.x];
+        uint32_t pos = 0;
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const uint32_t o = seg[j];
+            pos += (o < k) || (o == k && j < threadIdx.x);
+        }
+        if (pos == rank) *dst = k;
+    }
+}
+
+// rank-th smallest of the bin-sorted keys through selection slot q:
+// two barriers (more only for a bin holding more than a block of keys).
+template <int NT>
+__device__ __forceinline__ uint32_t select_slot(uint32_t rank, int q, RowScratch* sh)
+{
+    locate_write<NT>(rank, &sh->sbin[q], sh);
+    __syncthreads();
+    const uint32_t b = sh->sbin[q];
+    const uint32_t o = bin_start(sh, b), cnt = sh->off[b] - o;
+    if (cnt > (uint32_t)NT) return seg_select<NT>(row_keys(sh) + o, cnt, rank - o, sh);
+    seg_rank_write(row_keys(sh) + o, cnt, rank - o, &sh->skey[q]);
+    __syncthreads();
+    return sh->skey[q];
 }
 
 // Fast path: restore_baseline (if restore) and cal_RMS over all valid
@@ -286,50 +330,129 @@ __device__ __forceinline__ uint32_t pct_index(float p, uint32_t n)
 // the baselined samples in val.
 template <int NT, int VPT>
 __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint64_t valid, bool restore, bool want_rms,
-                                RowScratch* sh)
+                                                RowScratch* sh)
 {
+    constexpr int NW = NT / 64;
+    constexpr int per = kLinHist / NT;
+    static_assert(kLinHist % NT == 0 && kRadixBins % NT == 0, "bins per thread");
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const bool stats = restore || want_rms;
+    // moments of the non-zero samples (float: they only place the bins)
     uint64_t nz = 0;
-    double s1 = 0, s2 = 0;
+    float s1 = 0.f, s2 = 0.f;
 #pragma unroll
     for (int i = 0; i < VPT; ++i)
         if (((valid >> i) & 1ull) && val[i] != 0.0f) {
             nz |= 1ull << i;
             s1 += val[i];
-            s2 += (double)val[i] * val[i];
+            s2 = fmaf(val[i], val[i], s2);
         }
-    uint32_t c2[2] = {(uint32_t)__popcll(nz), (uint32_t)__popcll(valid)};
-    block_sums<NT, 2>(c2, sh);
-    const uint32_t n_nz = c2[0], n_valid = c2[1];
-    if (n_nz > 0 && (restore || want_rms)) {
-        const double S1 = block_sum_f64<NT>(s1, sh);
-        const double S2 = block_sum_f64<NT>(s2, sh);
-        const double mean = S1 / n_nz;
-        const double sd = sqrt(fmax(S2 / n_nz - mean * mean, 0.0));
-        LinBins lb;
-        lb.lo = (float)(mean - 8.0 * sd);
-        const double iw = sd > 0 ? (double)kLinBins / (16.0 * sd) : 0.0;
-        lb.inv_w = (isfinite(lb.lo) && isfinite(iw)) ? (float)iw : 0.f;
-        if (!isfinite(lb.lo)) lb.lo = 0.f;
-        // counting sort of the non-zero keys by bin (bins recomputed in the
-        // scatter rather than kept in registers)
-        for (int b = threadIdx.x; b < kLinHist; b += NT) sh->off[b] = 0;
-        __syncthreads();
+    if (stats) {
+        for (int q = 0; q < per; ++q) sh->off[threadIdx.x * per + q] = 0;
+    }
+    {
+        const uint32_t nn = wave_sum((uint32_t)__popcll(nz));
+        const uint32_t nv = wave_sum((uint32_t)__popcll(valid));
+        s1 = wave_sum_f(s1);
+        s2 = wave_sum_f(s2);
+        if (lane == 0) {
+            sh->upart[0][0][wid] = nn;
+            sh->upart[0][1][wid] = nv;
+            sh->fpart[0][wid] = s1;
+            sh->fpart[1][wid] = s2;
+        }
+    }
+    __syncthreads();
+    uint32_t n_nz = 0, n_valid = 0;
+    float S1 = 0.f, S2 = 0.f;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        n_nz += sh->upart[0][0][w];
+        n_valid += sh->upart[0][1][w];
+        S1 += sh->fpart[0][w];
+        S2 += sh->fpart[1][w];
+    }
+    if (n_nz == 0 || !stats) return 0.0f;   // an all-zero row: RMS 0, nothing to subtract
+
+    float mean = S1 / (float)n_nz;
+    float sd = sqrtf(fmaxf(S2 / (float)n_nz - mean * mean, 0.0f));
+    {   // clipped moments: the samples within 3 sd of the mean
+        const float lim = 3.0f * sd;
+        float c1 = 0.f, c2 = 0.f;
+        uint32_t cn = 0;
 #pragma unroll
         for (int i = 0; i < VPT; ++i)
-            if ((nz >> i) & 1ull) atomicAdd(&sh->off[lin_bin(val[i], lb)], 1u);
+            if (((nz >> i) & 1ull) && fabsf(val[i] - mean) < lim) {
+                c1 += val[i];
+                c2 = fmaf(val[i], val[i], c2);
+                ++cn;
+            }
+        cn = wave_sum(cn);
+        c1 = wave_sum_f(c1);
+        c2 = wave_sum_f(c2);
+        if (lane == 0) {
+            sh->upart[1][0][wid] = cn;
+            sh->fpart[2][wid] = c1;
+            sh->fpart[3][wid] = c2;
+        }
         __syncthreads();
-        block_excl_scan<NT>(sh->off, kLinHist, sh);
-        for (int b = threadIdx.x; b < kLinHist; b += NT) sh->cur[b] = sh->off[b];
-        __syncthreads();
-        uint32_t* keys = row_keys(sh);
+        uint32_t CN = 0;
+        float C1 = 0.f, C2 = 0.f;
 #pragma unroll
-        for (int i = 0; i < VPT; ++i)
-            if ((nz >> i) & 1ull) keys[atomicAdd(&sh->cur[lin_bin(val[i], lb)], 1u)] = f2key(val[i]);
+        for (int w = 0; w < NW; ++w) {
+            CN += sh->upart[1][0][w];
+            C1 += sh->fpart[2][w];
+            C2 += sh->fpart[3][w];
+        }
+        if (CN >= 2) {
+            const float cm = C1 / (float)CN;
+            const float cs = sqrtf(fmaxf(C2 / (float)CN - cm * cm, 0.0f));
+            if (cs > 0.f) {
+                mean = cm;
+                sd = cs;
+            }
+        }
+    }
+    LinBins lb;
+    lb.lo = mean - 8.0f * sd;
+    lb.inv_w = sd > 0.f ? (float)kLinBins / (16.0f * sd) : 0.f;
+    if (!isfinite(lb.lo) || !isfinite(lb.inv_w)) {
+        lb.lo = 0.f;
+        lb.inv_w = 0.f;
+    }
+    // counting sort of the non-zero keys by bin
+#pragma unroll
+    for (int i = 0; i < VPT; ++i)
+        if ((nz >> i) & 1ull) atomicAdd(&sh->off[lin_bin(val[i], lb)], 1u);
+    __syncthreads();
+    {
+        uint32_t* my = sh->off + threadIdx.x * per;
+        uint32_t local = 0;
+        for (int q = 0; q < per; ++q) local += my[q];
+        const uint32_t incl = wave_incl_scan(local);
+        if (lane == 63) sh->wsum[wid] = incl;
+        __syncthreads();
+        uint32_t base = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) base += (w < wid) ? sh->wsum[w] : 0u;
+        uint32_t run = base + incl - local;
+        for (int q = 0; q < per; ++q) {
+            const uint32_t c = my[q];
+            my[q] = run;
+            run += c;
+        }
         __syncthreads();
     }
+    // scatter: the offsets advance to the ends of their bins
+    uint32_t* keys = row_keys(sh);
+#pragma unroll
+    for (int i = 0; i < VPT; ++i)
+        if ((nz >> i) & 1ull) keys[atomicAdd(&sh->off[lin_bin(val[i], lb)], 1u)] = f2key(val[i]);
+    __syncthreads();
+
     float b = 0.0f;
-    if (restore && n_nz > 0) {
-        const float m1 = key2f(sorted_select<NT>(pct_index(0.5f, n_nz), sh));
+    if (restore) {
+        const float m1 = key2f(select_slot<NT>(pct_index(0.5f, n_nz), 0, sh));
         uint32_t c[2] = {0, 0};
 #pragma unroll
         for (int i = 0; i < VPT; ++i) {
@@ -339,12 +462,12 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint64_t vali
                 }
             }
         }
-        block_sums<NT, 2>(c, sh);
+        block_sums_slot<NT, 2>(c, sh->upart[2]);
         const uint32_t n2 = n_nz - c[0] - c[1];
-        b = key2f(sorted_select<NT>(c[0] + pct_index(0.5f, n2), sh));
+        b = key2f(select_slot<NT>(c[0] + pct_index(0.5f, n2), 1, sh));
     }
     float rms = 0.0f;
-    if (want_rms && n_valid > 0) {
+    if (want_rms) {
         // ranks of the 16/50/84% points among the baselined row
         uint32_t c[2] = {0, 0};
 #pragma unroll
@@ -355,20 +478,41 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint64_t vali
                 else if (y == 0.0f) ++c[1];
             }
         }
-        block_sums<NT, 2>(c, sh);
+        block_sums_slot<NT, 2>(c, sh->upart[3]);
         const uint32_t c_neg = c[0], c_eq = c[1], n_zero = n_valid - n_nz;
         const float ps[3] = {(float)(0.5 - 0.34), 0.5f, (float)(0.5 + 0.34)};
-        float par[3];
+        uint32_t sr[3];     // rank among the non-zero keys, or ~0u for a zero
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const uint32_t r = pct_index(ps[q], n_valid);
-            if (r < c_neg) {
-                par[q] = key2f(sorted_select<NT>(r, sh)) - b;
-            } else if (r < c_neg + n_zero + c_eq) {
-                par[q] = 0.0f;
-            } else {
-                par[q] = key2f(sorted_select<NT>(r - n_zero, sh)) - b;
+            sr[q] = r < c_neg ? r : (r < c_neg + n_zero + c_eq ? ~0u : r - n_zero);
+            if (sr[q] != ~0u) locate_write<NT>(sr[q], &sh->sbin[2 + q], sh);
+        }
+        __syncthreads();
+        uint32_t o[3] = {0, 0, 0}, cnt[3] = {0, 0, 0};
+        bool big = false;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            if (sr[q] != ~0u) {
+                const uint32_t bq = sh->sbin[2 + q];
+                o[q] = bin_start(sh, bq);
+                cnt[q] = sh->off[bq] - o[q];
+                if (cnt[q] > (uint32_t)NT) big = true;
+                else seg_rank_write(keys + o[q], cnt[q], sr[q] - o[q], &sh->skey[2 + q]);
             }
+        }
+        __syncthreads();
+        float par[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            par[q] = 0.0f;
+            if (sr[q] != ~0u && cnt[q] <= (uint32_t)NT) par[q] = key2f(sh->skey[2 + q]) - b;
+        }
+        if (big) {   // a bin larger than a block: radix passes, one statistic at a time
+#pragma unroll 1
+            for (int q = 0; q < 3; ++q)
+                if (sr[q] != ~0u && cnt[q] > (uint32_t)NT)
+                    par[q] = key2f(seg_select<NT>(keys + o[q], cnt[q], sr[q] - o[q], sh)) - b;
         }
         const double d21 = (double)(par[2] - par[1]), d10 = (double)(par[1] - par[0]);
         const float r0 = (float)sqrt((d21 * d21 + d10 * d10) / 2.);
@@ -385,9 +529,21 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint64_t vali
                 }
             }
         }
-        const double tot = block_sum_f64<NT>(s, sh);
-        const uint32_t cnt = block_sum_u32<NT>(k, sh);
-        rms = cnt ? sqrtf((float)(tot / (double)cnt)) : 0.0f;
+        s = wave_sum_d(s);
+        k = wave_sum(k);
+        if (lane == 0) {
+            sh->dpart[wid] = s;
+            sh->upart[4][0][wid] = k;
+        }
+        __syncthreads();
+        double tot = 0;
+        uint32_t cntk = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            tot += sh->dpart[w];
+            cntk += sh->upart[4][0][w];
+        }
+        rms = cntk ? sqrtf((float)(tot / (double)cntk)) : 0.0f;
     }
     if (restore) {
 #pragma unroll
@@ -399,33 +555,29 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint64_t vali
 
 // ---- general path: filtered radix select (rows with excluded samples) --
 
-// rank-th smallest key among the flagged samples, three full passes.
+// rank-th smallest key among the flagged samples, four 8-bit passes.
 template <int NT, int VPT>
 __device__ __forceinline__ uint32_t block_select(const float (&val)[VPT], const uint64_t fl, uint32_t rank,
-                                 RowScratch* sh)
+                                                 RowScratch* sh)
 {
     uint32_t prefix = 0, pmask = 0;
-    const int shifts[3] = {21, 10, 0};
-    const uint32_t dmasks[3] = {0x7ffu, 0x7ffu, 0x3ffu};
 #pragma unroll 1
-    for (int pass = 0; pass < 3; ++pass) {
-        const int sh_ = shifts[pass];
-        const uint32_t dm = dmasks[pass];
-        for (int b = threadIdx.x; b < kHistBins; b += NT) sh->cur[b] = 0;
+    for (int sh_ = 24; sh_ >= 0; sh_ -= 8) {
+        for (int b = threadIdx.x; b < kRadixBins; b += NT) sh->hist[b] = 0;
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < VPT; ++i) {
             if ((fl >> i) & 1ull) {
                 const uint32_t k = f2key(val[i]);
-                if ((k & pmask) == prefix) atomicAdd(&sh->cur[(k >> sh_) & dm], 1u);
+                if ((k & pmask) == prefix) atomicAdd(&sh->hist[(k >> sh_) & 0xffu], 1u);
             }
         }
         __syncthreads();
         uint32_t bucket, sub;
-        hist_locate<NT>(sh->cur, pass == 2 ? 1024 : kHistBins, rank, bucket, sub, sh);
+        hist_locate<NT>(sh->hist, kRadixBins, rank, bucket, sub, sh);
         rank = sub;
         prefix |= bucket << sh_;
-        pmask |= dm << sh_;
+        pmask |= 0xffu << sh_;
     }
     return prefix;
 }
