@@ -157,8 +157,9 @@ inline RowLds row_lds(int Mmax, int twmax, bool stats)
 }
 
 // ---- K1: real FFT along ticks --------------------------------------
-// Persistent: each block walks rows blockIdx.x, +gridDim.x, ... and
-// restages twiddles only when the row's length changes.
+// Persistent: each block walks rows blockIdx.x, +gridDim.x, ...; the
+// twiddles are read through the caches (LDS holds only the row, which
+// leaves room for four workgroups per CU).
 template <int NT, int VH, bool S5>
 __global__ __launch_bounds__(NT) void k_row_r2c(const DJob* __restrict__ jobs, int njobs,
                                                 long rbeg, long rend, RowLds lay)
@@ -166,8 +167,7 @@ __global__ __launch_bounds__(NT) void k_row_r2c(const DJob* __restrict__ jobs, i
     constexpr int MMAX = NT * VH;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float2* buf = reinterpret_cast<float2*>(smem_raw);
-    float2* twl = buf + lay.buf_len;
-    int curM = -1;
+    (void)lay;
     // the next row's samples are fetched into registers while the current
     // row is transformed, so the loads of a row are all in flight at once
     float2 z[VH];
@@ -186,10 +186,7 @@ __global__ __launch_bounds__(NT) void k_row_r2c(const DJob* __restrict__ jobs, i
         const DJob& J = jobs[find_job_rows(jobs, njobs, r)];
         const int w = (int)(r - J.row0);
         const int M = J.M;
-        if (M != curM) {
-            stage_twiddles(twl, J.twM, J.planM.twcount, threadIdx.x, NT);
-            curM = M;
-        }
+
         const bool has_bad = J.bad_ptr && J.bad_ptr[w + 1] > J.bad_ptr[w];
 #pragma unroll
         for (int i = 0; i < VH; ++i) {
@@ -204,8 +201,8 @@ __global__ __launch_bounds__(NT) void k_row_r2c(const DJob* __restrict__ jobs, i
             }
         }
         if (r + gridDim.x < rend) fetch(r + gridDim.x);
-        __syncthreads();
-        fft_lds<float2, NT, MMAX, S5>(buf, J.planM, twl, -1, threadIdx.x);
+        lds_barrier();
+        fft_lds<float2, NT, MMAX, S5>(buf, J.planM, J.twM, -1, threadIdx.x);
         float2* dst = J.S + (long)w * J.ldk;
         for (int k = threadIdx.x; k < J.ldk; k += NT) {
             float2 X;
@@ -224,7 +221,7 @@ __global__ __launch_bounds__(NT) void k_row_r2c(const DJob* __restrict__ jobs, i
             }
             dst[k] = X;
         }
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -256,7 +253,7 @@ __global__ __launch_bounds__(NT) void k_wire_pass(const DJob* __restrict__ jobs,
             if (idx < nw * TK) buf[(idx % TK) * ldc + pidx(idx / TK)] = t[q];
         }
     }
-    __syncthreads();
+    lds_barrier();
     constexpr int SL = NT / TK;
     const int slice = threadIdx.x / SL, st = threadIdx.x % SL;
     float2* col = buf + slice * ldc;
@@ -277,7 +274,7 @@ __global__ __launch_bounds__(NT) void k_wire_pass(const DJob* __restrict__ jobs,
         }
         col[pidx(m)] = o;
     }
-    __syncthreads();
+    lds_barrier();
     if constexpr (NWS > 0)
         fft_static<float2, SL, NWS>(col, twl, +1, st);
     else
@@ -351,7 +348,7 @@ __global__ __launch_bounds__(NT, 3) void k_row_c2r_roi(const DJob* __restrict__ 
         const bool has_bad = BAD && J.bad_ptr && J.bad_ptr[w + 1] > J.bad_ptr[w];
 
         c2r_stage<NT, VH>(J, Y, J.lf1, buf);
-        __syncthreads();
+        lds_barrier();
         fft_lds<float2, NT, MMAX, S5>(buf, J.planM, twl, +1, threadIdx.x);
 
         float val[VPT];
@@ -382,7 +379,7 @@ __global__ __launch_bounds__(NT, 3) void k_row_c2r_roi(const DJob* __restrict__ 
             // the fast path needs every valid sample in the RMS set
             const bool row_bad = has_bad && (__syncthreads_or(pm != valid) != 0);
             if (!row_bad) {
-                rms = row_stats_fast<NT, VPT>(val, valid, J.restore != 0, true, sh);
+                rms = row_stats_fast<NT, VPT>(val, 2u * (uint32_t)M, J.restore != 0, true, sh);
             } else if constexpr (BAD) {
                 if (J.restore) row_restore_baseline<NT, VPT>(val, valid, sh);
                 rms = row_cal_rms<NT, VPT>(val, pm, sh);
@@ -392,8 +389,8 @@ __global__ __launch_bounds__(NT, 3) void k_row_c2r_roi(const DJob* __restrict__ 
 
         uint64_t m1 = 0, m2 = 0;   // bit i: sample i above threshold (tight, tighter)
 #pragma unroll
-        for (int i = 0; i < VPT; ++i)
-            if (((pm >> i) & 1ull) && (double)val[i] > th) m1 |= 1ull << i;
+        for (int i = 0; i < VPT; ++i)   // slots past the row hold 0 < th
+            if ((!has_bad || ((pm >> i) & 1ull)) && (double)val[i] > th) m1 |= 1ull << i;
 
         float* o = J.out + (long)w * J.ld_out;
 #pragma unroll
@@ -417,9 +414,9 @@ __global__ __launch_bounds__(NT, 3) void k_row_c2r_roi(const DJob* __restrict__ 
         }
 
         if (J.lf2) {   // the "tighter" variant for the mask (decon_2D_tighterROI, :1120-1160)
-            __syncthreads();
+            lds_barrier();
             c2r_stage<NT, VH>(J, Y, J.lf2, buf);
-            __syncthreads();
+            lds_barrier();
             fft_lds<float2, NT, MMAX, S5>(buf, J.planM, twl, +1, threadIdx.x);
 #pragma unroll
             for (int i = 0; i < VH; ++i) {
@@ -430,10 +427,10 @@ __global__ __launch_bounds__(NT, 3) void k_row_c2r_roi(const DJob* __restrict__ 
                 val[2 * i] = z.x;
                 val[2 * i + 1] = z.y;
             }
-            if (J.restore) row_stats_fast<NT, VPT>(val, valid, true, false, sh);
+            if (J.restore) row_stats_fast<NT, VPT>(val, 2u * (uint32_t)M, true, false, sh);
 #pragma unroll
             for (int i = 0; i < VPT; ++i)
-                if (((pm >> i) & 1ull) && (double)val[i] > th) m2 |= 1ull << i;
+                if ((!has_bad || ((pm >> i) & 1ull)) && (double)val[i] > th) m2 |= 1ull << i;
         }
 
         if (J.mask) {
@@ -449,7 +446,7 @@ __global__ __launch_bounds__(NT, 3) void k_row_c2r_roi(const DJob* __restrict__ 
             }
         }
         if (J.rms && threadIdx.x == 0) J.rms[w] = rms;
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -512,10 +509,10 @@ __global__ void k_colmax(const double* __restrict__ Kabs, int nw, int kh, double
     double m = 0;
     for (int i = threadIdx.x; i < nw; i += kNT) m = fmax(m, Kabs[(long)k * nw + i]);
     red[threadIdx.x] = m;
-    __syncthreads();
+    lds_barrier();
     for (int s = kNT / 2; s > 0; s >>= 1) {
         if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-        __syncthreads();
+        lds_barrier();
     }
     if (threadIdx.x == 0) cmax[k] = red[0];
 }
@@ -627,7 +624,7 @@ static int dispatch_static_nw(int nw, int tk, F&& f)
 template <int VH, bool S5>
 static int launch_rows_r2c(wct_decon_plan p, const wct_decon_group& g, hipStream_t s)
 {
-    const RowLds lay = row_lds(g.len, g.twcount, false);
+    const RowLds lay = row_lds(g.len, 0, false);   // twiddles read through the caches
     const size_t lds = lay.bytes();
     int rc = allow_lds(k_row_r2c<kNT, VH, S5>, lds);
     if (rc) return rc;

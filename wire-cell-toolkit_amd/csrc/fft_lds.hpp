@@ -17,6 +17,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "sync.hpp"
+
 namespace wct {
 
 constexpr int kMaxPasses = 24;
@@ -205,7 +207,7 @@ __device__ __forceinline__ void stockham_pass(T2* buf, int n, int Ns, const T2* 
             for (int r = 0; r < R; ++r) v[b][r] = buf[pidx(j + r * m)];
         }
     }
-    __syncthreads();
+    lds_barrier();
     // k = j mod Ns, stepped from the first butterfly (one division per pass)
     int k = tid % Ns;
     const int kstep = NT % Ns;
@@ -229,7 +231,7 @@ __device__ __forceinline__ void stockham_pass(T2* buf, int n, int Ns, const T2* 
         k += kstep;
         if (k >= Ns) k -= Ns;
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 // SMOOTH5 instantiates only the radix 2/3/4/5/8 passes (lengths whose
@@ -353,7 +355,7 @@ __device__ __forceinline__ void stockham_pass_c(T2* buf, const T2* __restrict__ 
             for (int r = 0; r < R; ++r) v[b][r] = buf[pidx(j + r * m)];
         }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
         const int j = tid + b * NT;
@@ -373,7 +375,7 @@ __device__ __forceinline__ void stockham_pass_c(T2* buf, const T2* __restrict__ 
             for (int r = 0; r < R; ++r) buf[pidx(base + r * NS)] = v[b][r];
         }
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 template <typename T2, int NT, int N, int P>

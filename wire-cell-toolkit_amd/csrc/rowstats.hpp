@@ -29,6 +29,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "sync.hpp"
+
 namespace wct {
 
 __device__ __forceinline__ uint32_t f2key(float f)
@@ -115,12 +117,12 @@ __device__ __forceinline__ void block_sums(uint32_t (&c)[NC], RowScratch* sh)
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NC; ++q) c[q] = wave_sum(c[q]);
-    __syncthreads();
+    lds_barrier();
     if (lane == 0) {
 #pragma unroll
         for (int q = 0; q < NC; ++q) sh->csum[q][wid] = c[q];
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
         uint32_t t = 0;
@@ -144,9 +146,9 @@ __device__ __forceinline__ double block_sum_f64(double v, RowScratch* sh)
     constexpr int NW = NT / 64;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     v = wave_sum_d(v);
-    __syncthreads();
+    lds_barrier();
     if (lane == 0) sh->dsum[wid] = v;
-    __syncthreads();
+    lds_barrier();
     double tot = 0;
 #pragma unroll
     for (int w = 0; w < NW; ++w) tot += sh->dsum[w];
@@ -166,7 +168,7 @@ __device__ __forceinline__ void block_sums_slot(uint32_t (&c)[NC], uint32_t (*sl
 #pragma unroll
         for (int q = 0; q < NC; ++q) slot[q][wid] = c[q];
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
         uint32_t t = 0;
@@ -189,7 +191,7 @@ __device__ __forceinline__ void hist_locate(const uint32_t* hist, int nb, uint32
     for (int b = 0; b < bpt; ++b) local += hist[threadIdx.x * bpt + b];
     const uint32_t incl = wave_incl_scan(local);
     if (lane == 63) sh->wsum[wid] = incl;
-    __syncthreads();
+    lds_barrier();
     uint32_t base = 0;
 #pragma unroll
     for (int w = 0; w < NW; ++w) base += (w < wid) ? sh->wsum[w] : 0u;
@@ -206,10 +208,10 @@ __device__ __forceinline__ void hist_locate(const uint32_t* hist, int nb, uint32
             c += h;
         }
     }
-    __syncthreads();
+    lds_barrier();
     bucket = sh->sel_bucket;
     sub = sh->sel_rank;
-    __syncthreads();
+    lds_barrier();
 }
 
 // Linear binning of a row: bin 0 below lo, 1..kLinBins inside, kLinBins+1
@@ -240,9 +242,9 @@ __device__ __forceinline__ uint32_t seg_select(const uint32_t* seg, uint32_t cnt
             }
             if (pos == rank) sh->sel_bucket = k;
         }
-        __syncthreads();
+        lds_barrier();
         const uint32_t key = sh->sel_bucket;
-        __syncthreads();
+        lds_barrier();
         return key;
     }
     // four radix passes over the segment, 8 bits each
@@ -251,12 +253,12 @@ __device__ __forceinline__ uint32_t seg_select(const uint32_t* seg, uint32_t cnt
 #pragma unroll 1
     for (int sh_ = 24; sh_ >= 0; sh_ -= 8) {
         for (int b = threadIdx.x; b < kRadixBins; b += NT) h[b] = 0;
-        __syncthreads();
+        lds_barrier();
         for (uint32_t i = threadIdx.x; i < cnt; i += NT) {
             const uint32_t k = seg[i];
             if ((k & pmask) == prefix) atomicAdd(&h[(k >> sh_) & 0xffu], 1u);
         }
-        __syncthreads();
+        lds_barrier();
         uint32_t bucket, sub;
         hist_locate<NT>(h, kRadixBins, rank, bucket, sub, sh);
         rank = sub;
@@ -314,22 +316,25 @@ template <int NT>
 __device__ __forceinline__ uint32_t select_slot(uint32_t rank, int q, RowScratch* sh)
 {
     locate_write<NT>(rank, &sh->sbin[q], sh);
-    __syncthreads();
+    lds_barrier();
     const uint32_t b = sh->sbin[q];
     const uint32_t o = bin_start(sh, b), cnt = sh->off[b] - o;
     if (cnt > (uint32_t)NT) return seg_select<NT>(row_keys(sh) + o, cnt, rank - o, sh);
     seg_rank_write(row_keys(sh) + o, cnt, rank - o, &sh->skey[q]);
-    __syncthreads();
+    lds_barrier();
     return sh->skey[q];
 }
 
-// Fast path: restore_baseline (if restore) and cal_RMS over all valid
-// samples of the row.  The non-zero samples are counting-sorted into LDS
-// by linear bin once; every order statistic is then a short exact select
-// inside one bin.  Returns the RMS (0 when want_rms is false) and leaves
-// the baselined samples in val.
+// Fast path: restore_baseline (if restore) and cal_RMS over the n_valid
+// samples of the row; the register slots past them hold zeros (a zero is
+// skipped by every statistic but the RMS count, which is corrected).  The
+// non-zero samples are counting-sorted into LDS by linear bin once; every
+// order statistic is then a short exact select inside one bin.  Returns
+// the RMS (0 when want_rms is false) and leaves the baselined samples in
+// val.  Working from values rather than per-slot validity bits keeps the
+// compiler from holding one lane mask per slot across the whole routine.
 template <int NT, int VPT>
-__device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint64_t valid, bool restore, bool want_rms,
+__device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint32_t n_valid, bool restore, bool want_rms,
                                                 RowScratch* sh)
 {
     constexpr int NW = NT / 64;
@@ -338,37 +343,31 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint64_t vali
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const bool stats = restore || want_rms;
     // moments of the non-zero samples (float: they only place the bins)
-    uint64_t nz = 0;
+    uint32_t nn = 0;
     float s1 = 0.f, s2 = 0.f;
 #pragma unroll
-    for (int i = 0; i < VPT; ++i)
-        if (((valid >> i) & 1ull) && val[i] != 0.0f) {
-            nz |= 1ull << i;
-            s1 += val[i];
-            s2 = fmaf(val[i], val[i], s2);
-        }
+    for (int i = 0; i < VPT; ++i) {
+        nn += val[i] != 0.0f;
+        s1 += val[i];
+        s2 = fmaf(val[i], val[i], s2);
+    }
     if (stats) {
         for (int q = 0; q < per; ++q) sh->off[threadIdx.x * per + q] = 0;
     }
-    {
-        const uint32_t nn = wave_sum((uint32_t)__popcll(nz));
-        const uint32_t nv = wave_sum((uint32_t)__popcll(valid));
-        s1 = wave_sum_f(s1);
-        s2 = wave_sum_f(s2);
-        if (lane == 0) {
-            sh->upart[0][0][wid] = nn;
-            sh->upart[0][1][wid] = nv;
-            sh->fpart[0][wid] = s1;
-            sh->fpart[1][wid] = s2;
-        }
+    nn = wave_sum(nn);
+    s1 = wave_sum_f(s1);
+    s2 = wave_sum_f(s2);
+    if (lane == 0) {
+        sh->upart[0][0][wid] = nn;
+        sh->fpart[0][wid] = s1;
+        sh->fpart[1][wid] = s2;
     }
-    __syncthreads();
-    uint32_t n_nz = 0, n_valid = 0;
+    lds_barrier();
+    uint32_t n_nz = 0;
     float S1 = 0.f, S2 = 0.f;
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
         n_nz += sh->upart[0][0][w];
-        n_valid += sh->upart[0][1][w];
         S1 += sh->fpart[0][w];
         S2 += sh->fpart[1][w];
     }
@@ -382,7 +381,7 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint64_t vali
         uint32_t cn = 0;
 #pragma unroll
         for (int i = 0; i < VPT; ++i)
-            if (((nz >> i) & 1ull) && fabsf(val[i] - mean) < lim) {
+            if (val[i] != 0.0f && fabsf(val[i] - mean) < lim) {
                 c1 += val[i];
                 c2 = fmaf(val[i], val[i], c2);
                 ++cn;
@@ -395,7 +394,7 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint64_t vali
             sh->fpart[2][wid] = c1;
             sh->fpart[3][wid] = c2;
         }
-        __syncthreads();
+        lds_barrier();
         uint32_t CN = 0;
         float C1 = 0.f, C2 = 0.f;
 #pragma unroll
@@ -423,15 +422,15 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint64_t vali
     // counting sort of the non-zero keys by bin
 #pragma unroll
     for (int i = 0; i < VPT; ++i)
-        if ((nz >> i) & 1ull) atomicAdd(&sh->off[lin_bin(val[i], lb)], 1u);
-    __syncthreads();
+        if (val[i] != 0.0f) atomicAdd(&sh->off[lin_bin(val[i], lb)], 1u);
+    lds_barrier();
     {
         uint32_t* my = sh->off + threadIdx.x * per;
         uint32_t local = 0;
         for (int q = 0; q < per; ++q) local += my[q];
         const uint32_t incl = wave_incl_scan(local);
         if (lane == 63) sh->wsum[wid] = incl;
-        __syncthreads();
+        lds_barrier();
         uint32_t base = 0;
 #pragma unroll
         for (int w = 0; w < NW; ++w) base += (w < wid) ? sh->wsum[w] : 0u;
@@ -441,14 +440,14 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint64_t vali
             my[q] = run;
             run += c;
         }
-        __syncthreads();
+        lds_barrier();
     }
     // scatter: the offsets advance to the ends of their bins
     uint32_t* keys = row_keys(sh);
 #pragma unroll
     for (int i = 0; i < VPT; ++i)
-        if ((nz >> i) & 1ull) keys[atomicAdd(&sh->off[lin_bin(val[i], lb)], 1u)] = f2key(val[i]);
-    __syncthreads();
+        if (val[i] != 0.0f) keys[atomicAdd(&sh->off[lin_bin(val[i], lb)], 1u)] = f2key(val[i]);
+    lds_barrier();
 
     float b = 0.0f;
     if (restore) {
@@ -456,15 +455,17 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint64_t vali
         uint32_t c[2] = {0, 0};
 #pragma unroll
         for (int i = 0; i < VPT; ++i) {
-            if ((nz >> i) & 1ull) {
-                if (!(fabsf(val[i] - m1) < 500.0f)) {
-                    if (val[i] < m1) ++c[0]; else ++c[1];
-                }
+            if (val[i] != 0.0f && !(fabsf(val[i] - m1) < 500.0f)) {
+                if (val[i] < m1) ++c[0]; else ++c[1];
             }
         }
         block_sums_slot<NT, 2>(c, sh->upart[2]);
-        const uint32_t n2 = n_nz - c[0] - c[1];
-        b = key2f(select_slot<NT>(c[0] + pct_index(0.5f, n2), 1, sh));
+        if (c[0] + c[1] == 0) {
+            b = m1;   // every sample within 500 of m1: the second median is the first
+        } else {
+            const uint32_t n2 = n_nz - c[0] - c[1];
+            b = key2f(select_slot<NT>(c[0] + pct_index(0.5f, n2), 1, sh));
+        }
     }
     float rms = 0.0f;
     if (want_rms) {
@@ -472,7 +473,7 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint64_t vali
         uint32_t c[2] = {0, 0};
 #pragma unroll
         for (int i = 0; i < VPT; ++i) {
-            if ((nz >> i) & 1ull) {
+            if (val[i] != 0.0f) {
                 const float y = val[i] - b;
                 if (y < 0.0f) ++c[0];
                 else if (y == 0.0f) ++c[1];
@@ -488,7 +489,7 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint64_t vali
             sr[q] = r < c_neg ? r : (r < c_neg + n_zero + c_eq ? ~0u : r - n_zero);
             if (sr[q] != ~0u) locate_write<NT>(sr[q], &sh->sbin[2 + q], sh);
         }
-        __syncthreads();
+        lds_barrier();
         uint32_t o[3] = {0, 0, 0}, cnt[3] = {0, 0, 0};
         bool big = false;
 #pragma unroll
@@ -501,7 +502,7 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint64_t vali
                 else seg_rank_write(keys + o[q], cnt[q], sr[q] - o[q], &sh->skey[2 + q]);
             }
         }
-        __syncthreads();
+        lds_barrier();
         float par[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
@@ -521,12 +522,10 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint64_t vali
         uint32_t k = 0;
 #pragma unroll
         for (int i = 0; i < VPT; ++i) {
-            if ((valid >> i) & 1ull) {
-                const float y = ((nz >> i) & 1ull) ? val[i] - b : val[i];
-                if ((double)fabsf(y) < lim) {
-                    s += (double)y * (double)y;
-                    ++k;
-                }
+            const float y = val[i] != 0.0f ? val[i] - b : 0.0f;
+            if ((double)fabsf(y) < lim) {
+                s += (double)y * (double)y;
+                ++k;
             }
         }
         s = wave_sum_d(s);
@@ -535,7 +534,7 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint64_t vali
             sh->dpart[wid] = s;
             sh->upart[4][0][wid] = k;
         }
-        __syncthreads();
+        lds_barrier();
         double tot = 0;
         uint32_t cntk = 0;
 #pragma unroll
@@ -543,12 +542,14 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint64_t vali
             tot += sh->dpart[w];
             cntk += sh->upart[4][0][w];
         }
+        // the slots past n_valid are zeros that passed |0| < lim
+        if (lim > 0.0) cntk -= (uint32_t)(NT * VPT) - n_valid;
         rms = cntk ? sqrtf((float)(tot / (double)cntk)) : 0.0f;
     }
     if (restore) {
 #pragma unroll
         for (int i = 0; i < VPT; ++i)
-            if ((nz >> i) & 1ull) val[i] -= b;
+            if (val[i] != 0.0f) val[i] -= b;
     }
     return rms;
 }
@@ -564,7 +565,7 @@ __device__ __forceinline__ uint32_t block_select(const float (&val)[VPT], const 
 #pragma unroll 1
     for (int sh_ = 24; sh_ >= 0; sh_ -= 8) {
         for (int b = threadIdx.x; b < kRadixBins; b += NT) sh->hist[b] = 0;
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int i = 0; i < VPT; ++i) {
             if ((fl >> i) & 1ull) {
@@ -572,7 +573,7 @@ __device__ __forceinline__ uint32_t block_select(const float (&val)[VPT], const 
                 if ((k & pmask) == prefix) atomicAdd(&sh->hist[(k >> sh_) & 0xffu], 1u);
             }
         }
-        __syncthreads();
+        lds_barrier();
         uint32_t bucket, sub;
         hist_locate<NT>(sh->hist, kRadixBins, rank, bucket, sub, sh);
         rank = sub;
