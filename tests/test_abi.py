@@ -68,3 +68,12 @@ def test_abi_version_and_error_string():
     assert L.wct_abi_version() == 1
     assert L.wct_dft_supported(6000) == 1 and L.wct_dft_supported(17) == 0
     assert isinstance(L.wct_last_error(), bytes)
+
+
+def test_torch_extension_loads():
+    """The libtorch extension (autograd Functions over the C ABI) is built
+    in-tree and imports without a device."""
+    from wct_amd import autograd as A
+    mod = A.ext()
+    for name in ("Model", "forward_conv", "decon", "apply"):
+        assert hasattr(mod, name)
