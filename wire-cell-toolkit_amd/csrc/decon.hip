@@ -376,6 +376,9 @@ __global__ __launch_bounds__(NT, 3) void k_row_c2r_roi(const DJob* __restrict__ 
         }
         float rms = 0.0f;
         if (stats) {
+            // the statistics scratch overlays the row buffer: every
+            // thread holds its samples before any scratch write
+            lds_barrier();
             // the fast path needs every valid sample in the RMS set
             const bool row_bad = has_bad && (__syncthreads_or(pm != valid) != 0);
             if (!row_bad) {
@@ -427,7 +430,10 @@ __global__ __launch_bounds__(NT, 3) void k_row_c2r_roi(const DJob* __restrict__ 
                 val[2 * i] = z.x;
                 val[2 * i + 1] = z.y;
             }
-            if (J.restore) row_stats_fast<NT, VPT>(val, 2u * (uint32_t)M, true, false, sh);
+            if (J.restore) {
+                lds_barrier();   // samples in registers before the scratch is written
+                row_stats_fast<NT, VPT>(val, 2u * (uint32_t)M, true, false, sh);
+            }
 #pragma unroll
             for (int i = 0; i < VPT; ++i)
                 if ((!has_bad || ((pm >> i) & 1ull)) && (double)val[i] > th) m2 |= 1ull << i;

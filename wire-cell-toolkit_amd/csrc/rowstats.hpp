@@ -47,6 +47,7 @@ __device__ __forceinline__ float key2f(uint32_t k)
 constexpr int kLinBins = 1024;          // interior linear bins
 constexpr int kLinHist = 1280;          // histogram length (underflow, bins, overflow, padding)
 constexpr int kRadixBins = 256;         // 8-bit digits of the radix selections
+constexpr int kSortSeg = 256;           // largest bin kept in sorted order
 
 // LDS scratch of the row-statistics routines.  The bin-sorted keys of
 // the row follow the struct (row_keys()).
@@ -68,6 +69,13 @@ struct RowScratch {
     double dpart[8];
     uint32_t sbin[5];            // located bins of the selections
     uint32_t skey[5];            // selected keys
+    uint32_t sless[5];           // keys of the bin below the selected one
+    uint32_t seq[5];             // keys equal to the selected one
+    // The bin of the first median in sorted order: each key with the
+    // number of keys of the bin below it and equal to it.
+    uint32_t mkey[kSortSeg];
+    uint32_t mlt[kSortSeg];
+    uint32_t met[kSortSeg];
 };
 
 __device__ __forceinline__ uint32_t* row_keys(RowScratch* sh)
@@ -75,37 +83,74 @@ __device__ __forceinline__ uint32_t* row_keys(RowScratch* sh)
     return reinterpret_cast<uint32_t*>(sh + 1);
 }
 
-// Inclusive scan inside a wavefront (64 lanes).
+// Cross-lane steps through DPP (a VALU operand modifier: no LDS round
+// trip).  DPP controls of the GFX9 encoding: row_shr:n = 0x110 + n
+// (lane i of a row of 16 reads lane i - n of the same row), row_bcast:15
+// = 0x142 (lane 15 of each row feeds the next row), row_bcast:31 = 0x143
+// (lane 31 feeds rows 2 and 3).  update_dpp(old = 0, ...) leaves 0 in a
+// lane whose source is outside its row or whose row is masked, the
+// identity of the sums below.  Callers run these with the whole
+// wavefront active.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+
+// Inclusive scan over the 64 lanes: prefix sums inside each row of 16
+// (shifts 1, 2, 4, 8), rows 1 and 3 then add the last lane of the row
+// before them, rows 2 and 3 add lane 31.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t t = __shfl_up(v, off, 64);
-        if (lane >= off) v += t;
-    }
+    v += (uint32_t)dpp_i<0x111, 0xf>((int)v);
+    v += (uint32_t)dpp_i<0x112, 0xf>((int)v);
+    v += (uint32_t)dpp_i<0x114, 0xf>((int)v);
+    v += (uint32_t)dpp_i<0x118, 0xf>((int)v);
+    v += (uint32_t)dpp_i<0x142, 0xa>((int)v);
+    v += (uint32_t)dpp_i<0x143, 0xc>((int)v);
     return v;
 }
 
+// Sums over the wavefront; the total (lane 63 of the scan) is returned
+// to every lane.
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(v), 63);
+}
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __int_as_float(dpp_i<CTRL, ROWS>(__float_as_int(v)));
 }
 
 __device__ __forceinline__ float wave_sum_f(float v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_f<0x111, 0xf>(v);
+    v += dpp_f<0x112, 0xf>(v);
+    v += dpp_f<0x114, 0xf>(v);
+    v += dpp_f<0x118, 0xf>(v);
+    v += dpp_f<0x142, 0xa>(v);
+    v += dpp_f<0x143, 0xc>(v);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    return __hiloint2double(dpp_i<CTRL, ROWS>(__double2hiint(v)), dpp_i<CTRL, ROWS>(__double2loint(v)));
 }
 
 __device__ __forceinline__ double wave_sum_d(double v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_d<0x111, 0xf>(v);
+    v += dpp_d<0x112, 0xf>(v);
+    v += dpp_d<0x114, 0xf>(v);
+    v += dpp_d<0x118, 0xf>(v);
+    v += dpp_d<0x142, 0xa>(v);
+    v += dpp_d<0x143, 0xc>(v);
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                            __builtin_amdgcn_readlane(__double2loint(v), 63));
 }
 
 // Sums of up to four counters over the block (two barriers; the slots
@@ -295,33 +340,88 @@ __device__ __forceinline__ void locate_write(uint32_t r, uint32_t* dst, const Ro
     }
 }
 
-// Rank counting inside a segment of at most a block of keys: the thread
-// holding the key at sorted position `rank` writes it.
-__device__ __forceinline__ void seg_rank_write(const uint32_t* seg, uint32_t cnt, uint32_t rank, uint32_t* dst)
+// Rank counting inside a segment of keys: thread t (< cnt) takes key
+// seg[t] and counts the keys below it and equal to it; the thread at
+// sorted position `rank` writes its key and, when asked, those counts.
+__device__ __forceinline__ void seg_rank_write(const uint32_t* seg, uint32_t cnt, uint32_t rank, uint32_t t,
+                                               uint32_t* dst, uint32_t* less = nullptr, uint32_t* eq = nullptr)
 {
-    if (threadIdx.x < cnt) {
-        const uint32_t k = seg[threadIdx.x];
-        uint32_t pos = 0;
+    if (t < cnt) {
+        const uint32_t k = seg[t];
+        uint32_t lt = 0, et = 0, eb = 0;   // below; equal; equal and before t
+#pragma unroll 4
         for (uint32_t j = 0; j < cnt; ++j) {
             const uint32_t o = seg[j];
-            pos += (o < k) || (o == k && j < threadIdx.x);
+            lt += o < k;
+            et += o == k;
+            eb += (o == k) & (j < t);
         }
-        if (pos == rank) *dst = k;
+        if (lt + eb == rank) {
+            *dst = k;
+            if (less) {
+                *less = lt;
+                *eq = et;
+            }
+        }
+    }
+}
+
+// Exact rank-th key of a bin larger than a block (radix passes) and the
+// numbers of keys below and equal to it (o: keys of the bins before).
+template <int NT>
+__device__ __forceinline__ uint32_t big_select(const uint32_t* seg, uint32_t cnt, uint32_t rank, uint32_t o,
+                                               RowScratch* sh, uint32_t& less, uint32_t& eq)
+{
+    const uint32_t key = seg_select<NT>(seg, cnt, rank, sh);
+    uint32_t c[2] = {0, 0};
+    for (uint32_t j = threadIdx.x; j < cnt; j += NT) {
+        c[0] += seg[j] < key;
+        c[1] += seg[j] == key;
+    }
+    block_sums<NT, 2>(c, sh);
+    less = o + c[0];
+    eq = c[1];
+    return key;
+}
+
+// Sorts a bin of at most kSortSeg keys into sh->mkey (with the counts of
+// keys below and equal to each in mlt/met): thread t places seg[t] at
+// its sorted position (ties in index order).
+__device__ __forceinline__ void seg_sort_write(const uint32_t* seg, uint32_t cnt, RowScratch* sh)
+{
+    const uint32_t t = threadIdx.x;
+    if (t < cnt) {
+        const uint32_t k = seg[t];
+        uint32_t lt = 0, et = 0, eb = 0;
+#pragma unroll 4
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const uint32_t o = seg[j];
+            lt += o < k;
+            et += o == k;
+            eb += (o == k) & (j < t);
+        }
+        sh->mkey[lt + eb] = k;
+        sh->mlt[lt + eb] = lt;
+        sh->met[lt + eb] = et;
     }
 }
 
 // rank-th smallest of the bin-sorted keys through selection slot q:
 // two barriers (more only for a bin holding more than a block of keys).
+// less/eq return how many keys lie below the selected one and equal it.
 template <int NT>
-__device__ __forceinline__ uint32_t select_slot(uint32_t rank, int q, RowScratch* sh)
+__device__ __forceinline__ uint32_t select_slot(uint32_t rank, int q, RowScratch* sh, uint32_t& less, uint32_t& eq)
 {
     locate_write<NT>(rank, &sh->sbin[q], sh);
     lds_barrier();
     const uint32_t b = sh->sbin[q];
     const uint32_t o = bin_start(sh, b), cnt = sh->off[b] - o;
-    if (cnt > (uint32_t)NT) return seg_select<NT>(row_keys(sh) + o, cnt, rank - o, sh);
-    seg_rank_write(row_keys(sh) + o, cnt, rank - o, &sh->skey[q]);
+    const uint32_t* seg = row_keys(sh) + o;
+    if (cnt > (uint32_t)NT) return big_select<NT>(seg, cnt, rank - o, o, sh, less, eq);
+    seg_rank_write(seg, cnt, rank - o, threadIdx.x, &sh->skey[q], &sh->sless[q], &sh->seq[q]);
     lds_barrier();
+    less = o + sh->sless[q];
+    eq = sh->seq[q];
     return sh->skey[q];
 }
 
@@ -340,6 +440,7 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint32_t n_va
     constexpr int NW = NT / 64;
     constexpr int per = kLinHist / NT;
     static_assert(kLinHist % NT == 0 && kRadixBins % NT == 0, "bins per thread");
+    static_assert(NT >= 192, "the three percentile ranks run on three wavefronts");
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const bool stats = restore || want_rms;
     // moments of the non-zero samples (float: they only place the bins)
@@ -450,8 +551,29 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint32_t n_va
     lds_barrier();
 
     float b = 0.0f;
+    uint32_t b_less = 0, b_eq = 0;   // non-zero samples below b and equal to b
+    // the first median's bin, kept sorted when it holds at most a block
+    uint32_t o1 = 0, n1 = 0;
+    bool sorted1 = false;
     if (restore) {
-        const float m1 = key2f(select_slot<NT>(pct_index(0.5f, n_nz), 0, sh));
+        const uint32_t r1 = pct_index(0.5f, n_nz);
+        locate_write<NT>(r1, &sh->sbin[0], sh);
+        lds_barrier();
+        const uint32_t B1 = sh->sbin[0];
+        o1 = bin_start(sh, B1);
+        n1 = sh->off[B1] - o1;
+        sorted1 = n1 <= (uint32_t)NT;
+        float m1;
+        uint32_t l1, e1;
+        if (sorted1) {
+            seg_sort_write(keys + o1, n1, sh);
+            lds_barrier();
+            m1 = key2f(sh->mkey[r1 - o1]);
+            l1 = o1 + sh->mlt[r1 - o1];
+            e1 = sh->met[r1 - o1];
+        } else {
+            m1 = key2f(big_select<NT>(keys + o1, n1, r1 - o1, o1, sh, l1, e1));
+        }
         uint32_t c[2] = {0, 0};
 #pragma unroll
         for (int i = 0; i < VPT; ++i) {
@@ -460,75 +582,104 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint32_t n_va
             }
         }
         block_sums_slot<NT, 2>(c, sh->upart[2]);
-        if (c[0] + c[1] == 0) {
-            b = m1;   // every sample within 500 of m1: the second median is the first
+        const uint32_t r2 = c[0] + pct_index(0.5f, n_nz - c[0] - c[1]);
+        if (c[0] + c[1] == 0 || r2 == r1) {
+            b = m1;   // the second median is the first
+            b_less = l1;
+            b_eq = e1;
+        } else if (sorted1 && r2 - o1 < n1) {
+            b = key2f(sh->mkey[r2 - o1]);   // inside the sorted bin
+            b_less = o1 + sh->mlt[r2 - o1];
+            b_eq = sh->met[r2 - o1];
         } else {
-            const uint32_t n2 = n_nz - c[0] - c[1];
-            b = key2f(select_slot<NT>(c[0] + pct_index(0.5f, n2), 1, sh));
+            b = key2f(select_slot<NT>(r2, 1, sh, b_less, b_eq));
         }
     }
     float rms = 0.0f;
     if (want_rms) {
-        // ranks of the 16/50/84% points among the baselined row
-        uint32_t c[2] = {0, 0};
+        // ranks of the 16/50/84% points among the baselined row.  For a
+        // non-zero sample x, fl(x - b) < 0 exactly when x < b and == 0
+        // exactly when x == b, so with b a sample of Z the counts are
+        // those of its selection.
+        uint32_t c[2] = {b_less, b_eq};
+        if (!restore) {
+            uint32_t neg[1] = {0};
 #pragma unroll
-        for (int i = 0; i < VPT; ++i) {
-            if (val[i] != 0.0f) {
-                const float y = val[i] - b;
-                if (y < 0.0f) ++c[0];
-                else if (y == 0.0f) ++c[1];
-            }
+            for (int i = 0; i < VPT; ++i) neg[0] += val[i] < 0.0f;
+            block_sums_slot<NT, 1>(neg, sh->upart[3]);
+            c[0] = neg[0];
         }
-        block_sums_slot<NT, 2>(c, sh->upart[3]);
         const uint32_t c_neg = c[0], c_eq = c[1], n_zero = n_valid - n_nz;
         const float ps[3] = {(float)(0.5 - 0.34), 0.5f, (float)(0.5 + 0.34)};
         uint32_t sr[3];     // rank among the non-zero keys, or ~0u for a zero
+        float par[3];
+        bool need[3], any = false;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const uint32_t r = pct_index(ps[q], n_valid);
             sr[q] = r < c_neg ? r : (r < c_neg + n_zero + c_eq ? ~0u : r - n_zero);
-            if (sr[q] != ~0u) locate_write<NT>(sr[q], &sh->sbin[2 + q], sh);
-        }
-        lds_barrier();
-        uint32_t o[3] = {0, 0, 0}, cnt[3] = {0, 0, 0};
-        bool big = false;
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            if (sr[q] != ~0u) {
-                const uint32_t bq = sh->sbin[2 + q];
-                o[q] = bin_start(sh, bq);
-                cnt[q] = sh->off[bq] - o[q];
-                if (cnt[q] > (uint32_t)NT) big = true;
-                else seg_rank_write(keys + o[q], cnt[q], sr[q] - o[q], &sh->skey[2 + q]);
+            par[q] = 0.0f;
+            need[q] = false;
+            if (sr[q] == ~0u) continue;
+            if (sorted1 && sr[q] - o1 < n1) {
+                par[q] = key2f(sh->mkey[sr[q] - o1]) - b;   // inside the median's sorted bin
+            } else {
+                need[q] = any = true;
+                locate_write<NT>(sr[q], &sh->sbin[2 + q], sh);
             }
         }
-        lds_barrier();
-        float par[3];
+        if (any) {
+            lds_barrier();
+            uint32_t o[3] = {0, 0, 0}, cnt[3] = {0, 0, 0};
+            bool big = false, wide = false;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            par[q] = 0.0f;
-            if (sr[q] != ~0u && cnt[q] <= (uint32_t)NT) par[q] = key2f(sh->skey[2 + q]) - b;
-        }
-        if (big) {   // a bin larger than a block: radix passes, one statistic at a time
-#pragma unroll 1
+            for (int q = 0; q < 3; ++q) {
+                if (need[q]) {
+                    const uint32_t bq = sh->sbin[2 + q];
+                    o[q] = bin_start(sh, bq);
+                    cnt[q] = sh->off[bq] - o[q];
+                    if (cnt[q] > (uint32_t)NT) big = true;
+                    if (cnt[q] > 64u) wide = true;
+                }
+            }
+            // bins of at most a wavefront of keys: statistic q on wavefront q
+            const uint32_t t0 = threadIdx.x;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                if (need[q] && cnt[q] <= (uint32_t)NT)
+                    seg_rank_write(keys + o[q], cnt[q], sr[q] - o[q], wide ? t0 : t0 - 64u * q, &sh->skey[2 + q]);
+            }
+            lds_barrier();
+#pragma unroll
             for (int q = 0; q < 3; ++q)
-                if (sr[q] != ~0u && cnt[q] > (uint32_t)NT)
-                    par[q] = key2f(seg_select<NT>(keys + o[q], cnt[q], sr[q] - o[q], sh)) - b;
+                if (need[q] && cnt[q] <= (uint32_t)NT) par[q] = key2f(sh->skey[2 + q]) - b;
+            if (big) {   // a bin larger than a block: radix passes, one statistic at a time
+#pragma unroll 1
+                for (int q = 0; q < 3; ++q)
+                    if (need[q] && cnt[q] > (uint32_t)NT)
+                        par[q] = key2f(seg_select<NT>(keys + o[q], cnt[q], sr[q] - o[q], sh)) - b;
+            }
         }
         const double d21 = (double)(par[2] - par[1]), d10 = (double)(par[1] - par[0]);
         const float r0 = (float)sqrt((d21 * d21 + d10 * d10) / 2.);
         const double lim = 5.0 * (double)r0;
-        double s = 0;
+        // For a float y, (double)|y| < lim exactly when |y| < L, L the
+        // float nearest lim if that is not below it, else the next one up.
+        float L = (float)lim;
+        if ((double)L < lim) L = __uint_as_float(__float_as_uint(L) + 1u);
+        // per-thread partial sums in FP32 (at most VPT terms), the block
+        // sum in FP64
+        float sf = 0.0f;
         uint32_t k = 0;
 #pragma unroll
         for (int i = 0; i < VPT; ++i) {
             const float y = val[i] != 0.0f ? val[i] - b : 0.0f;
-            if ((double)fabsf(y) < lim) {
-                s += (double)y * (double)y;
-                ++k;
-            }
+            const bool in = fabsf(y) < L;
+            const float yi = in ? y : 0.0f;
+            sf = fmaf(yi, yi, sf);
+            k += in;
         }
-        s = wave_sum_d(s);
+        double s = wave_sum_d((double)sf);
         k = wave_sum(k);
         if (lane == 0) {
             sh->dpart[wid] = s;
