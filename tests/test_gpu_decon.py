@@ -137,12 +137,14 @@ def test_baseline_rms_mask(dev, kind, nw, nt, ntr):
     # exactness on identical data
     ybl_f, _ = O.restore_baseline(raw, dtype=np.float32)
     assert np.array_equal(y, ybl_f)
-    # the device accumulates sum(x^2) in double: compare with the oracle's
-    # double accumulation on the same data, and loosely with the
-    # reference's float accumulation (which drifts ~1e-6 on 6000 samples)
+    # the device sums x^2 in FP32 per thread (at most 24 terms, relative
+    # error <= 24 * 2^-24) and in double across threads: compare with the
+    # oracle's double accumulation on the same data (the RMS, a square
+    # root, halves that bound), and loosely with the reference's float
+    # accumulation (which drifts ~1e-6 on 6000 samples)
     rms_d, _, _ = O.roi_threshold(ybl_f.astype(np.float64), W.TH_FACTOR, dtype=np.float64)
     rms_f, th_f, mask_f = O.roi_threshold(ybl_f, W.TH_FACTOR, dtype=np.float32)
-    assert np.all(np.abs(r - rms_d) <= 2e-7 * rms_d + 1e-30)
+    assert np.all(np.abs(r - rms_d) <= 1e-6 * rms_d + 1e-30)
     assert np.all(np.abs(r - rms_f) <= 2e-5 * rms_f + 1e-30)
     mism = np.argwhere((m & 1) != mask_f)
     for (w, t) in mism:

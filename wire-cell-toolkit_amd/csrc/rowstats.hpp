@@ -44,8 +44,8 @@ __device__ __forceinline__ float key2f(uint32_t k)
     return __uint_as_float(u);
 }
 
-constexpr int kLinBins = 1024;          // interior linear bins
-constexpr int kLinHist = 1280;          // histogram length (underflow, bins, overflow, padding)
+constexpr int kLinBins = 2048;          // interior linear bins
+constexpr int kLinHist = 2304;          // histogram length (underflow, bins, overflow, padding)
 constexpr int kRadixBins = 256;         // 8-bit digits of the radix selections
 constexpr int kSortSeg = 256;           // largest bin kept in sorted order
 
@@ -425,6 +425,31 @@ __device__ __forceinline__ uint32_t select_slot(uint32_t rank, int q, RowScratch
     return sh->skey[q];
 }
 
+// restore_baseline's window (OmnibusSigProc.cxx:867-906): the numbers of
+// non-zero samples x with !(|fl(x - m1)| < 500) below and above m1.  When
+// the underflow and overflow bins are empty, every non-zero sample lies
+// in the linear bins [lo, lo + kLinBins / inv_w) up to float rounding; if
+// that span sits inside the window with a margin far above the rounding,
+// both counts are 0 without a pass over the row.
+template <int NT, int VPT>
+__device__ __forceinline__ void far_counts(const float (&val)[VPT], float m1, LinBins lb, RowScratch* sh,
+                                           uint32_t (&c)[2])
+{
+    c[0] = c[1] = 0;
+    if (lb.inv_w > 0.f && sh->off[0] == 0 && sh->off[kLinBins + 1] == sh->off[kLinBins]) {
+        const double lo = lb.lo, hi = (double)lb.lo + (double)kLinBins / (double)lb.inv_w;
+        const double m = m1, margin = 1e-3 * (fabs(lo) + fabs(hi) + fabs(m) + 500.0);
+        if (lo - m > -500.0 + margin && hi - m < 500.0 - margin) return;
+    }
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+        if (val[i] != 0.0f && !(fabsf(val[i] - m1) < 500.0f)) {
+            if (val[i] < m1) ++c[0]; else ++c[1];
+        }
+    }
+    block_sums_slot<NT, 2>(c, sh->upart[2]);
+}
+
 // Fast path: restore_baseline (if restore) and cal_RMS over the n_valid
 // samples of the row; the register slots past them hold zeros (a zero is
 // skipped by every statistic but the RMS count, which is corrected).  The
@@ -555,33 +580,67 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint32_t n_va
     // the first median's bin, kept sorted when it holds at most a block
     uint32_t o1 = 0, n1 = 0;
     bool sorted1 = false;
-    if (restore) {
+    const float ps[3] = {(float)(0.5 - 0.34), 0.5f, (float)(0.5 + 0.34)};
+    float par[3] = {0.0f, 0.0f, 0.0f};   // the 16/50/84% points of the baselined row
+    if (restore && want_rms && n_nz == n_valid) {
+        // No zero in the row: the percentile ranks do not depend on the
+        // baseline (the r-th smallest of fl(z - b) is fl(z_(r) - b)), so
+        // the median and the 16/84% points are selected in one round.
         const uint32_t r1 = pct_index(0.5f, n_nz);
+        const uint32_t rq[2] = {pct_index(ps[0], n_nz), pct_index(ps[2], n_nz)};
         locate_write<NT>(r1, &sh->sbin[0], sh);
+        locate_write<NT>(rq[0], &sh->sbin[2], sh);
+        locate_write<NT>(rq[1], &sh->sbin[3], sh);
         lds_barrier();
         const uint32_t B1 = sh->sbin[0];
         o1 = bin_start(sh, B1);
         n1 = sh->off[B1] - o1;
-        sorted1 = n1 <= (uint32_t)NT;
+        uint32_t oq[2], nq[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const uint32_t bq = sh->sbin[2 + q];
+            oq[q] = bin_start(sh, bq);
+            nq[q] = sh->off[bq] - oq[q];
+        }
+        uint32_t kq[2];
+        if (n1 <= 64u && nq[0] <= 64u && nq[1] <= 64u) {
+            // the median's bin sorted on wavefront 0, the 16/84% ranks on 1 and 2
+            seg_sort_write(keys + o1, n1, sh);
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+                seg_rank_write(keys + oq[q], nq[q], rq[q] - oq[q], threadIdx.x - 64u * (q + 1), &sh->skey[2 + q]);
+            lds_barrier();
+            sorted1 = true;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) kq[q] = sh->skey[2 + q];
+        } else {   // wider bins: one selection at a time over the block
+            sorted1 = n1 <= (uint32_t)NT;
+            if (sorted1) {
+                seg_sort_write(keys + o1, n1, sh);
+                lds_barrier();
+            }
+#pragma unroll 1
+            for (int q = 0; q < 2; ++q) {
+                if (nq[q] > (uint32_t)NT) {
+                    kq[q] = seg_select<NT>(keys + oq[q], nq[q], rq[q] - oq[q], sh);
+                } else {
+                    seg_rank_write(keys + oq[q], nq[q], rq[q] - oq[q], threadIdx.x, &sh->skey[2 + q]);
+                    lds_barrier();
+                    kq[q] = sh->skey[2 + q];
+                }
+            }
+        }
         float m1;
         uint32_t l1, e1;
         if (sorted1) {
-            seg_sort_write(keys + o1, n1, sh);
-            lds_barrier();
             m1 = key2f(sh->mkey[r1 - o1]);
             l1 = o1 + sh->mlt[r1 - o1];
             e1 = sh->met[r1 - o1];
         } else {
             m1 = key2f(big_select<NT>(keys + o1, n1, r1 - o1, o1, sh, l1, e1));
         }
-        uint32_t c[2] = {0, 0};
-#pragma unroll
-        for (int i = 0; i < VPT; ++i) {
-            if (val[i] != 0.0f && !(fabsf(val[i] - m1) < 500.0f)) {
-                if (val[i] < m1) ++c[0]; else ++c[1];
-            }
-        }
-        block_sums_slot<NT, 2>(c, sh->upart[2]);
+        uint32_t c[2];
+        far_counts<NT, VPT>(val, m1, lb, sh, c);
         const uint32_t r2 = c[0] + pct_index(0.5f, n_nz - c[0] - c[1]);
         if (c[0] + c[1] == 0 || r2 == r1) {
             b = m1;   // the second median is the first
@@ -594,72 +653,110 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint32_t n_va
         } else {
             b = key2f(select_slot<NT>(r2, 1, sh, b_less, b_eq));
         }
+        par[0] = key2f(kq[0]) - b;
+        par[1] = m1 - b;
+        par[2] = key2f(kq[1]) - b;
+    } else {
+        if (restore) {
+            const uint32_t r1 = pct_index(0.5f, n_nz);
+            locate_write<NT>(r1, &sh->sbin[0], sh);
+            lds_barrier();
+            const uint32_t B1 = sh->sbin[0];
+            o1 = bin_start(sh, B1);
+            n1 = sh->off[B1] - o1;
+            sorted1 = n1 <= (uint32_t)NT;
+            float m1;
+            uint32_t l1, e1;
+            if (sorted1) {
+                seg_sort_write(keys + o1, n1, sh);
+                lds_barrier();
+                m1 = key2f(sh->mkey[r1 - o1]);
+                l1 = o1 + sh->mlt[r1 - o1];
+                e1 = sh->met[r1 - o1];
+            } else {
+                m1 = key2f(big_select<NT>(keys + o1, n1, r1 - o1, o1, sh, l1, e1));
+            }
+            uint32_t c[2];
+            far_counts<NT, VPT>(val, m1, lb, sh, c);
+            const uint32_t r2 = c[0] + pct_index(0.5f, n_nz - c[0] - c[1]);
+            if (c[0] + c[1] == 0 || r2 == r1) {
+                b = m1;   // the second median is the first
+                b_less = l1;
+                b_eq = e1;
+            } else if (sorted1 && r2 - o1 < n1) {
+                b = key2f(sh->mkey[r2 - o1]);   // inside the sorted bin
+                b_less = o1 + sh->mlt[r2 - o1];
+                b_eq = sh->met[r2 - o1];
+            } else {
+                b = key2f(select_slot<NT>(r2, 1, sh, b_less, b_eq));
+            }
+        }
+        if (want_rms) {
+            // ranks of the 16/50/84% points among the baselined row.  For a
+            // non-zero sample x, fl(x - b) < 0 exactly when x < b and == 0
+            // exactly when x == b, so with b a sample of Z the counts are
+            // those of its selection.
+            uint32_t c[2] = {b_less, b_eq};
+            if (!restore) {
+                uint32_t neg[1] = {0};
+#pragma unroll
+                for (int i = 0; i < VPT; ++i) neg[0] += val[i] < 0.0f;
+                block_sums_slot<NT, 1>(neg, sh->upart[3]);
+                c[0] = neg[0];
+            }
+            const uint32_t c_neg = c[0], c_eq = c[1], n_zero = n_valid - n_nz;
+            uint32_t sr[3];     // rank among the non-zero keys, or ~0u for a zero
+            bool need[3], any = false;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const uint32_t r = pct_index(ps[q], n_valid);
+                sr[q] = r < c_neg ? r : (r < c_neg + n_zero + c_eq ? ~0u : r - n_zero);
+                par[q] = 0.0f;
+                need[q] = false;
+                if (sr[q] == ~0u) continue;
+                if (sorted1 && sr[q] - o1 < n1) {
+                    par[q] = key2f(sh->mkey[sr[q] - o1]) - b;   // inside the median's sorted bin
+                } else {
+                    need[q] = any = true;
+                    locate_write<NT>(sr[q], &sh->sbin[2 + q], sh);
+                }
+            }
+            if (any) {
+                lds_barrier();
+                uint32_t o[3] = {0, 0, 0}, cnt[3] = {0, 0, 0};
+                bool big = false, wide = false;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    if (need[q]) {
+                        const uint32_t bq = sh->sbin[2 + q];
+                        o[q] = bin_start(sh, bq);
+                        cnt[q] = sh->off[bq] - o[q];
+                        if (cnt[q] > (uint32_t)NT) big = true;
+                        if (cnt[q] > 64u) wide = true;
+                    }
+                }
+                // bins of at most a wavefront of keys: statistic q on wavefront q
+                const uint32_t t0 = threadIdx.x;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    if (need[q] && cnt[q] <= (uint32_t)NT)
+                        seg_rank_write(keys + o[q], cnt[q], sr[q] - o[q], wide ? t0 : t0 - 64u * q, &sh->skey[2 + q]);
+                }
+                lds_barrier();
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    if (need[q] && cnt[q] <= (uint32_t)NT) par[q] = key2f(sh->skey[2 + q]) - b;
+                if (big) {   // a bin larger than a block: radix passes, one statistic at a time
+#pragma unroll 1
+                    for (int q = 0; q < 3; ++q)
+                        if (need[q] && cnt[q] > (uint32_t)NT)
+                            par[q] = key2f(seg_select<NT>(keys + o[q], cnt[q], sr[q] - o[q], sh)) - b;
+                }
+            }
+        }
     }
     float rms = 0.0f;
     if (want_rms) {
-        // ranks of the 16/50/84% points among the baselined row.  For a
-        // non-zero sample x, fl(x - b) < 0 exactly when x < b and == 0
-        // exactly when x == b, so with b a sample of Z the counts are
-        // those of its selection.
-        uint32_t c[2] = {b_less, b_eq};
-        if (!restore) {
-            uint32_t neg[1] = {0};
-#pragma unroll
-            for (int i = 0; i < VPT; ++i) neg[0] += val[i] < 0.0f;
-            block_sums_slot<NT, 1>(neg, sh->upart[3]);
-            c[0] = neg[0];
-        }
-        const uint32_t c_neg = c[0], c_eq = c[1], n_zero = n_valid - n_nz;
-        const float ps[3] = {(float)(0.5 - 0.34), 0.5f, (float)(0.5 + 0.34)};
-        uint32_t sr[3];     // rank among the non-zero keys, or ~0u for a zero
-        float par[3];
-        bool need[3], any = false;
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const uint32_t r = pct_index(ps[q], n_valid);
-            sr[q] = r < c_neg ? r : (r < c_neg + n_zero + c_eq ? ~0u : r - n_zero);
-            par[q] = 0.0f;
-            need[q] = false;
-            if (sr[q] == ~0u) continue;
-            if (sorted1 && sr[q] - o1 < n1) {
-                par[q] = key2f(sh->mkey[sr[q] - o1]) - b;   // inside the median's sorted bin
-            } else {
-                need[q] = any = true;
-                locate_write<NT>(sr[q], &sh->sbin[2 + q], sh);
-            }
-        }
-        if (any) {
-            lds_barrier();
-            uint32_t o[3] = {0, 0, 0}, cnt[3] = {0, 0, 0};
-            bool big = false, wide = false;
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                if (need[q]) {
-                    const uint32_t bq = sh->sbin[2 + q];
-                    o[q] = bin_start(sh, bq);
-                    cnt[q] = sh->off[bq] - o[q];
-                    if (cnt[q] > (uint32_t)NT) big = true;
-                    if (cnt[q] > 64u) wide = true;
-                }
-            }
-            // bins of at most a wavefront of keys: statistic q on wavefront q
-            const uint32_t t0 = threadIdx.x;
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                if (need[q] && cnt[q] <= (uint32_t)NT)
-                    seg_rank_write(keys + o[q], cnt[q], sr[q] - o[q], wide ? t0 : t0 - 64u * q, &sh->skey[2 + q]);
-            }
-            lds_barrier();
-#pragma unroll
-            for (int q = 0; q < 3; ++q)
-                if (need[q] && cnt[q] <= (uint32_t)NT) par[q] = key2f(sh->skey[2 + q]) - b;
-            if (big) {   // a bin larger than a block: radix passes, one statistic at a time
-#pragma unroll 1
-                for (int q = 0; q < 3; ++q)
-                    if (need[q] && cnt[q] > (uint32_t)NT)
-                        par[q] = key2f(seg_select<NT>(keys + o[q], cnt[q], sr[q] - o[q], sh)) - b;
-            }
-        }
         const double d21 = (double)(par[2] - par[1]), d10 = (double)(par[1] - par[0]);
         const float r0 = (float)sqrt((d21 * d21 + d10 * d10) / 2.);
         const double lim = 5.0 * (double)r0;
