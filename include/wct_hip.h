@@ -162,6 +162,17 @@ typedef struct {
     int adjoint;        /* 1: apply the adjoint operator (conj(K)); the
                            job must then have no bad ranges, baseline,
                            mask or save_mode (linear part only) */
+    /* Spectrum mode, the tensor-set Decon2DResponse
+     * (sig/src/Decon2DResponse.cxx:203-390): when set, the third stage
+     * writes, instead of out/mask/rms, the full time spectrum of the
+     * deconvolved and rolled frame, FFT_t(X) with X the reference's
+     * r_data before its final fwd_r2c: nw rows of nt complex64 values
+     * (interleaved floats), row stride ld_spectrum complex elements.  The
+     * kernel then carries no time filter (all ones) and no spectral cut
+     * (cut_tol 0); out, mask, rms, bad ranges, lf filters, baseline and
+     * save_mode must be unset.  A plan's jobs all use the mode or none. */
+    float* spectrum;
+    int ld_spectrum;
 } wct_decon_job;
 
 /* Device scratch needed to run the jobs (spectrum buffers). */
@@ -202,6 +213,19 @@ int wct_decon_graph_destroy(wct_decon_graph graph);
  * wct_decon_workspace_size bytes. */
 int wct_decon_run(const wct_decon_job* jobs, int njobs, void* workspace, size_t ws_bytes,
                   void* stream);
+
+/* Decon2DFilter's device pass (sig/src/Decon2DFilter.cxx:90-125):
+ * per row of a complex64 spectrum (nw rows of nt complex values,
+ * interleaved floats, row stride ld_spectrum complex elements), multiply
+ * bin k by filter[k], inverse real transform as DftTools::inv_c2r does
+ * (Hermitian mirror of bins 0..nt/2 with real DC and Nyquist, 1/nt
+ * normalisation; aux/src/DftTools.cxx:133-139, Spectrum.h:56-89), then
+ * Sig::restore_baseline when restore_baseline is set
+ * (sig/src/Util.cxx:5-35).  filter holds nt/2 + 1 floats (bins above
+ * nt/2 do not reach the output).  All pointers are device memory; out
+ * has nw rows of nt floats, row stride ld_out.  nt even, up to 16384. */
+int wct_spectrum_filter(const float* spectrum, int ld_spectrum, int nw, int nt, const float* filter,
+                        int restore_baseline, float* out, int ld_out, void* stream);
 
 /* ---- the plain per-plane path ----
  * OmnibusSigProc's per-plane control flow (load_data, decon_2D_init,

@@ -302,6 +302,35 @@ def decon_fused(data, resp, wire_filter, time_filter, wire_shift, time_shift):
     return out
 
 
+# ---- tensor-set variant (sig/src/Decon2DResponse.cxx, Decon2DFilter.cxx) ----
+
+def decon2d_response(data, resp, wire_filter, wire_shift, time_shift, dtype=np.float64):
+    """Sig::Decon2DResponse::operator() (sig/src/Decon2DResponse.cxx:203-390):
+    fwd_r2c along ticks, fwd along wires, divide by the response
+    spectrum, NaN -> -0 and Inf -> 0, wire filter, inverse along wires,
+    inv_c2r along ticks, roll by wire_shift (and time_shift when > 0),
+    fwd_r2c along ticks: the steps of decon_2D_init
+    (OmnibusSigProc.cxx:944-1051) on the same response, so the complex
+    (nw, nt) time spectrum of the rolled frame."""
+    return decon_init(data, resp, wire_filter, wire_shift, time_shift, dtype)
+
+
+def decon2d_filter(spec, time_filter, cmm=(), restore=True, dtype=np.float64):
+    """Sig::Decon2DFilter::operator() (sig/src/Decon2DFilter.cxx:90-145):
+    each spectrum column times the filter product, inv_c2r along ticks,
+    Sig::restore_baseline (sig/src/Util.cxx:5-35), then row ch zeroed
+    over [tmin, tmax) for every (ch, tmin, tmax) of bad:cmm_range with
+    ch below the row count."""
+    r = decon_apply_filter(spec, time_filter, dtype)
+    if restore:
+        r, _ = restore_baseline(r, dtype)
+    r = np.array(r)
+    for ch, tmin, tmax in cmm:
+        if ch < r.shape[0]:
+            r[int(ch), int(tmin):int(tmax)] = 0.0
+    return r
+
+
 def forward_conv(q, resp, wire_shift, time_shift):
     qq = np.ascontiguousarray(q, np.float64)
     r = np.ascontiguousarray(resp, np.float64)

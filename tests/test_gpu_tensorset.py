@@ -1,0 +1,134 @@
+"""Device paths of the tensor-set deconvolution (sig/src/Decon2DResponse.cxx,
+Decon2DFilter.cxx): the spectrum mode of the fused plan and
+wct_spectrum_filter, against the FP64 oracle on seeded frames.
+
+Tolerances:
+  spectrum (Decon2DResponse)     exact Hermitian symmetry (real DC and
+      Nyquist, bin nt - k the conjugate of bin k); weighted by the time
+      filter the spectrum later meets in Decon2DFilter, relative RMS
+      error <= 1e-4 against FP64.  Unweighted, the spectrum near Nyquist
+      is the noise divided by a response that has fallen to ~1e-20 of
+      its peak, below the FP64 transform's own rounding floor (1e-16 of
+      the peak): there the FP64 oracle, this path and the reference's
+      FP32 arithmetic all divide by rounding noise and cannot agree.
+      Weighted, the bins past the fused path's spectral cut (filtered
+      kernel below 1e-7 of its peak, where the response is near that
+      floor) still leave a few 1e-5 of disagreement between any two
+      evaluations; the fused path drops them (2-3e-7), this mode keeps
+      them as the reference does.
+  filter pass (Decon2DFilter)    relative RMS error <= 5e-5 against the
+      FP64 oracle on the same spectrum (restore_baseline's median can
+      land on a neighbouring sample, which shifts a row)
+  the pair, with the Wiener and LF filters, against the FP64 fused
+      decon + restore_baseline: relative RMS error <= 5e-5 (as for the
+      filter pass; measured 3-5e-7)
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("ind", 800, 6000, 33), ("col", 960, 6000, 40), ("ind", 96, 1000, 6)]
+
+
+def _rel_rms(a, b):
+    return float(np.sqrt(np.sum(np.abs(a - b) ** 2) / np.sum(np.abs(b) ** 2)))
+
+
+def _setup(kind, nw, nt, ntr):
+    from wct_amd import workload as W
+    spec = W.plane_spec(kind, nw, nt)
+    resp = O.overall_response(spec.fr_rows, spec.elec, W.FINE_PERIOD, W.TICK, W.INTER_GAIN,
+                              W.ADC_MV, W.FTOFFSET, nt, dtype=np.float64)
+    seed = 3 * nw + nt
+    q = O.tracks(seed, 1, ntr, nw, nt, W.Q_PER_TICK)
+    adc = O.forward_conv(q, resp, spec.wire_shift, spec.time_shift)
+    adc += O.normal(seed, 2, nw * nt, W.NOISE_ADC).reshape(nw, nt)
+    return spec, resp, adc.astype(np.float32)
+
+
+def _gpu_spectrum(spec, resp, adc, dev):
+    import torch
+    from wct_amd import decon
+    pk = decon.PlaneKernel(spec, cut_tol=0.0, time_filter=np.ones(spec.nt), resp=resp)
+    assert pk.kc == spec.nt // 2 + 1
+    a = torch.from_numpy(adc).to(dev)
+    sp = torch.empty((spec.nw, spec.nt), dtype=torch.complex64, device=dev)
+    b = decon.Batch([decon.make_job(pk, a, None, spectrum=sp)])
+    b.run()
+    torch.cuda.synchronize()
+    b.close()
+    return sp
+
+
+def _time_filter(spec, kind):
+    # the tight Wiener filter, times the tight LF filter on induction
+    # planes (plane_spec folds it in): a Decon2DFilter "filters" list of
+    # HfFilter:Wiener_tight_* (+ LfFilter:ROI_tight_lf)
+    return np.asarray(spec.time_filter, np.float64)
+
+
+@pytest.mark.parametrize("kind,nw,nt,ntr", CASES)
+def test_spectrum_mode_vs_fp64(dev, kind, nw, nt, ntr):
+    print(f"tensor-set spectrum {kind} {nw}x{nt}", flush=True)
+    spec, resp, adc = _setup(kind, nw, nt, ntr)
+    sp = _gpu_spectrum(spec, resp, adc, dev).cpu().numpy()
+    assert np.all(np.isfinite(sp))
+    k = np.arange(nt)
+    assert np.array_equal(sp[:, (-k) % nt], np.conj(sp))
+    assert np.all(sp[:, 0].imag == 0) and np.all(sp[:, nt // 2].imag == 0)
+    ref = O.decon2d_response(adc.astype(np.float64), resp, spec.wire_filter, spec.wire_shift, spec.time_shift)
+    w = _time_filter(spec, kind)[None, :]
+    e = _rel_rms(sp.astype(np.complex128) * w, ref * w)
+    print(f"  filter-weighted spectrum rel_rms={e:.3e}")
+    assert e <= 1e-4
+
+
+@pytest.mark.parametrize("kind,nw,nt,ntr", CASES)
+def test_spectrum_filter_vs_fp64(dev, kind, nw, nt, ntr):
+    import torch
+    from wct_amd import decon
+    print(f"tensor-set filter {kind} {nw}x{nt}", flush=True)
+    spec, resp, adc = _setup(kind, nw, nt, ntr)
+    sp = _gpu_spectrum(spec, resp, adc, dev)
+    tf = _time_filter(spec, kind)
+    ft = torch.from_numpy(tf[: nt // 2 + 1].astype(np.float32)).to(dev)
+    out = torch.empty((nw, nt), device=dev)
+    decon.spectrum_filter(sp, ft, out, restore=1)
+    torch.cuda.synchronize()
+    y = out.cpu().numpy().astype(np.float64)
+    # the same spectrum through the FP64 oracle
+    ref_same = O.decon2d_filter(sp.cpu().numpy().astype(np.complex128), tf)
+    e1 = _rel_rms(y, ref_same)
+    # the pair against the FP64 fused decon + restore_baseline
+    y64 = O.decon_fused(adc.astype(np.float64), resp, spec.wire_filter, tf, spec.wire_shift, spec.time_shift)
+    ref_pair, _ = O.restore_baseline(y64, np.float64)
+    e2 = _rel_rms(y, ref_pair)
+    print(f"  filter rel_rms={e1:.3e} pair vs fused rel_rms={e2:.3e}")
+    assert e1 <= 5e-5 and e2 <= 5e-5
+    # without the baseline the pass is linear: a scaled filter scales the output
+    out2 = torch.empty_like(out)
+    decon.spectrum_filter(sp, ft * 2.0, out2, restore=0)
+    decon.spectrum_filter(sp, ft, out, restore=0)
+    torch.cuda.synchronize()
+    assert torch.allclose(out2, 2.0 * out, rtol=1e-6, atol=1e-6 * float(out.abs().max()))
+
+
+def test_spectrum_mode_rejects_mixed_jobs(dev):
+    import torch
+    from wct_amd import decon
+    from wct_amd.capi import WctError
+    spec, resp, adc = _setup("ind", 96, 1000, 2)
+    pk = decon.PlaneKernel(spec, cut_tol=0.0, time_filter=np.ones(spec.nt), resp=resp)
+    a = torch.from_numpy(adc).to(dev)
+    sp = torch.empty((96, 1000), dtype=torch.complex64, device=dev)
+    o = torch.empty((96, 1000), device=dev)
+    with pytest.raises(WctError, match="spectrum mode"):
+        decon.Batch([decon.make_job(pk, a, None, spectrum=sp), decon.make_job(pk, a, o)])
+    bad = decon.make_job(pk, a, None, spectrum=sp)
+    bad.out = o.data_ptr()
+    with pytest.raises(WctError, match="outputs of the decon mode"):
+        decon.Batch([bad])
+    pk.close()

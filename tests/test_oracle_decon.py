@@ -134,3 +134,35 @@ def test_linearity_and_shift():
     assert np.abs(yab - (2 * ya - yb)).max() < 1e-12
     ys = O.decon_fused(np.roll(a, (3, 5), axis=(0, 1)), resp, wf, tf, 1, 4)
     assert np.abs(ys - np.roll(ya, (3, 5), axis=(0, 1))).max() < 1e-12
+
+
+def test_tensorset_pair_composes_to_the_fused_decon():
+    """Decon2DResponse then Decon2DFilter (sig/src/Decon2DResponse.cxx,
+    Decon2DFilter.cxx) is the fused decon with the filter product as the
+    time filter; the response stage's spectrum is Hermitian (the FFT of a
+    real frame) and equals the FFT of the unit-time-filter decon; the
+    cmm ranges zero [tmin, tmax) of row ch and skip rows past the end."""
+    rng = np.random.default_rng(14)
+    nw, nt, nr = 20, 48, 5
+    resp = _rand_resp(nr, nt, rng)
+    d = rng.standard_normal((nw, nt)) + 3.0
+    wf = O.hf_filter(nw, 1.0, 0.75, 2, False)
+    tf = O.hf_filter(nt, 1 * MHZ, 0.3 * MHZ, 2, True) * O.lf_filter(nt, 1 * MHZ, 0.02 * MHZ)
+    for ts in (0, 6):
+        spec = O.decon2d_response(d, resp, wf, nr // 2, ts)
+        assert spec.shape == (nw, nt) and np.iscomplexobj(spec)
+        herm = np.conj(spec[:, (-np.arange(nt)) % nt])
+        assert np.abs(spec - herm).max() < 1e-10 * np.abs(spec).max()
+        y1 = O.decon_fused(d, resp, wf, np.ones(nt), nr // 2, ts)
+        assert np.abs(spec - np.fft.fft(y1, axis=1)).max() < 1e-10 * np.abs(spec).max()
+        y = O.decon2d_filter(spec, tf, restore=False)
+        want = O.decon_fused(d, resp, wf, tf, nr // 2, ts)
+        assert np.abs(y - want).max() < 1e-10 * np.abs(want).max()
+        yb = O.decon2d_filter(spec, tf)
+        wb, _ = O.restore_baseline(want, np.float64)
+        assert np.abs(yb - wb).max() < 1e-10 * np.abs(wb).max()
+        yc = O.decon2d_filter(spec, tf, cmm=[(3, 5, 17), (nw + 2, 0, 4), (0, 40, 48)])
+        assert np.all(yc[3, 5:17] == 0) and np.all(yc[0, 40:48] == 0)
+        keep = np.ones((nw, nt), bool)
+        keep[3, 5:17] = keep[0, 40:48] = False
+        assert np.array_equal(yc[keep], yb[keep])

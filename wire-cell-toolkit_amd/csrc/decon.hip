@@ -50,6 +50,7 @@ struct wct_decon_plan_s {
     int ncu = 0;                 // compute units of the device
     int row_pct = 100;           // persistent row grids: percent of the resident maximum
     bool smooth5 = true;         // every length is 2^a 3^b 5^c
+    bool spectrum = false;       // the third stage writes time spectra
 };
 
 // A captured sequence of plan runs, replayed with one launch.
@@ -84,6 +85,8 @@ struct DJob {
     int nw, nt, M, kc, ldk;
     float th_factor;
     int restore, save_mode, adjoint;
+    float2* spec;     // spectrum mode: full time spectrum rows (else null)
+    int ld_spec;
     FftPlan planM;
     FftPlan planW;
 };
@@ -322,6 +325,34 @@ __device__ __forceinline__ void c2r_stage(const DJob& J, const float2* __restric
             zz.x = sr - pi;
             zz.y = si + pr;
             buf[pidx(k)] = zz;
+        }
+    }
+}
+
+// ---- spectrum mode: the time spectrum of the deconvolved frame -------
+// X = c2r(Y) unnormalised (K carries 1/(nw nt)), so FFT_t(X)[k] is nt
+// times the Hermitian extension of the kept half spectrum Y: bins
+// 0..nt/2 from Y (zero past kc) with real DC and Nyquist, bin nt - k the
+// conjugate of bin k.  One workgroup per row, grid-stride over rows.
+__global__ __launch_bounds__(256) void k_row_spectrum(const DJob* __restrict__ jobs, int njobs, long rbeg,
+                                                      long rend)
+{
+    for (long r = rbeg + blockIdx.x; r < rend; r += gridDim.x) {
+        const DJob& J = jobs[find_job_rows(jobs, njobs, r)];
+        const int w = (int)(r - J.row0);
+        const float2* Y = J.S + (long)w * J.ldk;
+        float2* o = J.spec + (long)w * J.ld_spec;
+        const float sc = (float)J.nt;
+        for (int k = threadIdx.x; k < J.nt; k += blockDim.x) {
+            const int kk = k <= J.M ? k : J.nt - k;
+            float2 v;
+            v.x = v.y = 0.f;
+            if (kk < J.kc) v = Y[kk];
+            if (kk == 0 || kk == J.M) v.y = 0.f;
+            if (k > J.M) v.y = -v.y;
+            v.x *= sc;
+            v.y *= sc;
+            o[k] = v;
         }
     }
 }
@@ -682,6 +713,13 @@ static int row_pass(wct_decon_plan p, const wct_decon_group& g, hipStream_t s, b
 
 static int run_rows(wct_decon_plan p, hipStream_t s, bool c2r)
 {
+    if (c2r && p->spectrum) {
+        const long rows = p->rows;
+        const int grid = (int)std::min<long>(rows, (long)p->ncu * 8);
+        hipLaunchKernelGGL(k_row_spectrum, dim3(grid), dim3(256), 0, s, p->jobs, p->njobs, 0L, rows);
+        WCT_HIP_CHECK(hipGetLastError());
+        return WCT_OK;
+    }
     for (const auto& g : p->rgroups) {
         const int rc = dispatch_vh(g.len, [&](auto V) -> int {
             constexpr int VH = decltype(V)::value;
@@ -722,6 +760,21 @@ int plan_run(wct_decon_plan p, hipStream_t s)
     rc = run_wire(p, s);
     if (rc) return rc;
     return run_rows(p, s, true);
+}
+
+// The job of a one-off launch, passed by value and stored in stream-
+// ordered scratch (no host buffer has to outlive the call).
+__global__ void k_put_job(const DJob job, DJob* dst)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = job;
+}
+
+// g[k] = f[k] * sc: inv_c2r's 1/nt folded into the per-bin filter that
+// the c2r stage applies.
+__global__ void k_scale_filter(const float* __restrict__ f, float* __restrict__ g, int n, float sc)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) g[k] = f[k] * sc;
 }
 
 }  // namespace wct
@@ -982,13 +1035,26 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
         WCT_REQUIRE(K->nw <= wire_tile_max_nw(tk), WCT_ERR_UNSUPPORTED,
                     "decon_plan: %d wires exceed the %d-bin wire tile's limit of %d", K->nw, tk,
                     wire_tile_max_nw(tk));
-        WCT_REQUIRE(j.adc && j.out && j.ld_adc >= K->nt && j.ld_out >= K->nt && j.ld_adc % 2 == 0 &&
-                        j.ld_out % 2 == 0 && (!j.mask || (j.ld_mask >= K->nt && j.ld_mask % 2 == 0)),
-                    WCT_ERR_ARG, "decon_plan: job %d has bad pointers or strides", i);
-        WCT_REQUIRE((reinterpret_cast<uintptr_t>(j.adc) & 7) == 0 &&
-                        (reinterpret_cast<uintptr_t>(j.out) & 7) == 0 &&
-                        (!j.mask || (reinterpret_cast<uintptr_t>(j.mask) & 1) == 0),
-                    WCT_ERR_ARG, "decon_plan: job %d buffers are misaligned", i);
+        const bool spec = j.spectrum != nullptr;
+        WCT_REQUIRE(spec == (jobs[0].spectrum != nullptr), WCT_ERR_ARG,
+                    "decon_plan: jobs 0 and %d differ in spectrum mode (a plan uses it for all jobs or none)", i);
+        if (spec) {
+            WCT_REQUIRE(j.adc && j.ld_adc >= K->nt && j.ld_adc % 2 == 0 && j.ld_spectrum >= K->nt &&
+                            (reinterpret_cast<uintptr_t>(j.adc) & 7) == 0 &&
+                            (reinterpret_cast<uintptr_t>(j.spectrum) & 7) == 0,
+                        WCT_ERR_ARG, "decon_plan: spectrum job %d has bad pointers or strides", i);
+            WCT_REQUIRE(!j.out && !j.mask && !j.rms && !j.bad_ptr && !j.lf_tight && !j.lf_tighter &&
+                            !j.restore_baseline && !j.save_mode,
+                        WCT_ERR_ARG, "decon_plan: spectrum job %d sets outputs of the decon mode", i);
+        } else {
+            WCT_REQUIRE(j.adc && j.out && j.ld_adc >= K->nt && j.ld_out >= K->nt && j.ld_adc % 2 == 0 &&
+                            j.ld_out % 2 == 0 && (!j.mask || (j.ld_mask >= K->nt && j.ld_mask % 2 == 0)),
+                        WCT_ERR_ARG, "decon_plan: job %d has bad pointers or strides", i);
+            WCT_REQUIRE((reinterpret_cast<uintptr_t>(j.adc) & 7) == 0 &&
+                            (reinterpret_cast<uintptr_t>(j.out) & 7) == 0 &&
+                            (!j.mask || (reinterpret_cast<uintptr_t>(j.mask) & 1) == 0),
+                        WCT_ERR_ARG, "decon_plan: job %d buffers are misaligned", i);
+        }
         WCT_REQUIRE(!j.adjoint || (!j.bad_ptr && !j.restore_baseline && !j.mask && !j.save_mode),
                     WCT_ERR_ARG, "decon_plan: adjoint job %d must be linear (no bad ranges, baseline, "
                     "mask or save_mode)", i);
@@ -1016,6 +1082,8 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
         d.restore = j.restore_baseline;
         d.save_mode = j.save_mode;
         d.adjoint = j.adjoint ? 1 : 0;
+        d.spec = reinterpret_cast<float2*>(j.spectrum);
+        d.ld_spec = j.ld_spectrum;
         d.S = reinterpret_cast<float2*>(static_cast<char*>(workspace) + off);
         off += (size_t)d.nw * d.ldk * sizeof(float2);
         off = (off + 255) & ~(size_t)255;
@@ -1043,6 +1111,7 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
     WCT_REQUIRE(Mmax <= 8192, WCT_ERR_UNSUPPORTED, "decon_plan: %d ticks above 16384", 2 * Mmax);
     auto* p = new wct_decon_plan_s();
     p->smooth5 = true;
+    p->spectrum = jobs[0].spectrum != nullptr;
     for (int i = 0; i < njobs; ++i)
         if (!is_smooth5(h[i].M) || !is_smooth5(h[i].nw)) p->smooth5 = false;
     p->twM = twM;
@@ -1196,6 +1265,75 @@ int wct_decon_run(const wct_decon_job* jobs, int njobs, void* workspace, size_t 
         return WCT_ERR_HIP;
     }
     return rc2;
+}
+
+int wct_spectrum_filter(const float* spectrum, int ld_spectrum, int nw, int nt, const float* filter,
+                        int restore_baseline, float* out, int ld_out, void* stream)
+{
+    WCT_REQUIRE(spectrum && filter && out && nw > 0 && nt > 0 && nt % 2 == 0, WCT_ERR_ARG,
+                "spectrum_filter: bad arguments (nw %d, nt %d)", nw, nt);
+    const int M = nt / 2;
+    WCT_REQUIRE(M <= 8192, WCT_ERR_UNSUPPORTED, "spectrum_filter: %d ticks above 16384", nt);
+    WCT_REQUIRE(ld_spectrum >= M + 1 && ld_out >= nt && ld_out % 2 == 0 &&
+                    (reinterpret_cast<uintptr_t>(spectrum) & 7) == 0 && (reinterpret_cast<uintptr_t>(out) & 7) == 0,
+                WCT_ERR_ARG, "spectrum_filter: bad strides or misaligned buffers");
+    hipStream_t s = as_stream(stream);
+    const DevPlan* pM = nullptr;
+    int rc = get_plan(M, false, &pM);
+    if (rc) return rc;
+    const void* line = nullptr;
+    rc = get_twiddle_line(nt, M + 1, false, &line);
+    if (rc) return rc;
+    const size_t jb = (sizeof(DJob) + 255) & ~(size_t)255;
+    void* scratch = nullptr;
+    WCT_HIP_CHECK(hipMallocAsync(&scratch, jb + sizeof(float) * (size_t)(M + 1), s));
+    float* g = reinterpret_cast<float*>(static_cast<char*>(scratch) + jb);
+    DJob d;
+    memset(&d, 0, sizeof(d));
+    d.out = out;
+    d.ld_out = ld_out;
+    d.S = const_cast<float2*>(reinterpret_cast<const float2*>(spectrum));
+    d.ldk = ld_spectrum;
+    d.lf1 = g;
+    d.nw = nw;
+    d.nt = nt;
+    d.M = M;
+    d.kc = M + 1;
+    d.restore = restore_baseline ? 1 : 0;
+    d.planM = pM->plan;
+    d.twM = (const float2*)pM->tw;
+    d.wN = (const float2*)line;
+    hipLaunchKernelGGL(k_put_job, dim3(1), dim3(1), 0, s, d, static_cast<DJob*>(scratch));
+    hipLaunchKernelGGL(k_scale_filter, dim3((M + 1 + 255) / 256), dim3(256), 0, s, filter, g, M + 1,
+                       1.0f / (float)nt);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        // K3's c2r stage over the caller's spectrum rows, through a
+        // transient one-job plan on the stack
+        wct_decon_plan_s tmp;
+        tmp.jobs = static_cast<DJob*>(scratch);
+        tmp.njobs = 1;
+        tmp.rows = nw;
+        tmp.Mmax = M;
+        tmp.twM = d.planM.twcount;
+        tmp.rgroups.push_back({M, 0, (long)nw, d.planM.twcount, is_smooth5(M), false});
+        int dev = 0;
+        e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&tmp.ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess) rc = run_rows(&tmp, s, true);
+        tmp.jobs = nullptr;
+    }
+    const hipError_t ef = hipFreeAsync(scratch, s);
+    if (e != hipSuccess) {
+        set_error("spectrum_filter: %s", hipGetErrorString(e));
+        return WCT_ERR_HIP;
+    }
+    if (rc) return rc;
+    if (ef != hipSuccess) {
+        set_error("spectrum_filter: %s", hipGetErrorString(ef));
+        return WCT_ERR_HIP;
+    }
+    return WCT_OK;
 }
 
 }  // extern "C"

@@ -28,6 +28,7 @@ class DeconJob(C.Structure):
         ("restore_baseline", C.c_int),
         ("save_mode", C.c_int),
         ("adjoint", C.c_int),
+        ("spectrum", C.c_void_p), ("ld_spectrum", C.c_int),
     ]
 
 
@@ -70,6 +71,7 @@ _SIGS = {
     "wct_decon_kernel_destroy": (_I, [_V]),
     "wct_decon_workspace_size": (_Z, [C.POINTER(DeconJob), _I]),
     "wct_decon_run": (_I, [C.POINTER(DeconJob), _I, _V, _Z, _V]),
+    "wct_spectrum_filter": (_I, [_V, _I, _I, _I, _V, _I, _V, _I, _V]),
     "wct_decon_plan_create": (_I, [C.POINTER(C.c_void_p), C.POINTER(DeconJob), _I, _V, _Z]),
     "wct_decon_plan_run": (_I, [_V, _V]),
     "wct_decon_plan_destroy": (_I, [_V]),

@@ -74,13 +74,19 @@ def _ptr(t):
 
 
 def make_job(kernel, adc, out, mask=None, rms=None, th_factor=W.TH_FACTOR, restore=1,
-             save_mode=0, lf_tight=None, lf_tighter=None, bad=None, adjoint=False):
+             save_mode=0, lf_tight=None, lf_tighter=None, bad=None, adjoint=False, spectrum=None):
     """One wct_decon_job.  adc/out/mask/rms are torch tensors (2D for the
-    frames); bad is (ptr, lo, hi) int32 device tensors or None."""
+    frames); bad is (ptr, lo, hi) int32 device tensors or None.  With
+    spectrum (a complex64 (nw, nt) tensor) the job writes the time
+    spectrum of the deconvolved frame instead (out must be None)."""
     j = DeconJob()
     j.kernel = kernel.handle
     j.adc = adc.data_ptr()
     j.ld_adc = adc.stride(0)
+    if spectrum is not None:
+        j.spectrum = spectrum.data_ptr()
+        j.ld_spectrum = spectrum.stride(0)
+        return j
     j.out = out.data_ptr()
     j.ld_out = out.stride(0)
     j.mask = _ptr(mask)
@@ -234,3 +240,11 @@ class PlainPlane:
             _ptr(mask), mask.stride(0) if mask is not None else 0, _ptr(rms), save_mode,
             self.ws.data_ptr(), self.ws_bytes, _stream() if stream is None else stream))
 
+
+def spectrum_filter(spec, filt, out, restore=1, stream=None):
+    """wct_spectrum_filter: Decon2DFilter's inverse pass on a complex64
+    (nw, nt) device spectrum; filt holds nt/2 + 1 float32 device values;
+    out is a float32 (nw, nt) device tensor."""
+    nw, nt = out.shape
+    check(lib().wct_spectrum_filter(spec.data_ptr(), spec.stride(0), nw, nt, filt.data_ptr(), int(restore),
+                                    out.data_ptr(), out.stride(0), _stream() if stream is None else stream))
