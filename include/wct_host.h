@@ -69,6 +69,27 @@ int wcth_frame_summary(void* frame, const char* tag, const double** val, int* n)
  * the first cap are written). */
 int wcth_frame_masks(void* frame, const char* name, int* channels, int* firsts, int* seconds, int cap);
 
+/* Tensor sets (ITensorSet / ITensor, iface/inc/WireCellIface/
+ * ITensorSet.h:9-20, ITensor.h:14-65): build an input set, run an
+ * ITensorSetFilter ("type:name", e.g. Decon2DResponse or Decon2DFilter,
+ * sig/src/Decon2DResponse.cxx, Decon2DFilter.cxx) on it (in = NULL is
+ * end-of-stream; *out = NULL then), read the result.  Metadata is JSON;
+ * dtype strings follow util/src/Dtype.cxx ("f4", "f8", "c8", "i4", ...);
+ * tensor data is copied in, and read back in place (valid while the set
+ * lives).  The metadata calls return the JSON length (only cap - 1
+ * characters are written). */
+void* wcth_tensorset_new(int ident, const char* metadata_json);
+void wcth_tensorset_free(void* set);
+int wcth_tensorset_add(void* set, const char* dtype, int ndim, const size_t* shape, const void* data,
+                       const char* metadata_json);
+int wcth_tensorset_filter(const char* tn, void* in, void** out);
+int wcth_tensorset_info(void* set, int* ident, int* ntensors, char* metadata_json, size_t cap);
+/* Tensor i: dtype (up to 7 characters), ndim, shape (up to 8 extents),
+ * data pointer and size in bytes. */
+int wcth_tensorset_tensor(void* set, int i, char* dtype, int* ndim, size_t* shape, const void** data,
+                          size_t* nbytes);
+int wcth_tensorset_tensor_md(void* set, int i, char* metadata_json, size_t cap);
+
 /* ROI_formation::find_ROI_by_decon_itself's run scan, extend_ROI_self
  * and create_ROI_connect_info (sigproc/src/ROI_formation.cxx:167-360,
  * 458-531) over a threshold mask (nonzero = above threshold), nw rows

@@ -175,3 +175,108 @@ def test_sigproc_full_configuration_parses_field_response(tmp_path):
     assert "e-310" in text or "e-309" in text
     K.configure(cfg)
     assert K.frame_filter("HipOmnibusSigProc:osp", None) is None
+
+
+# ---- tensor-set deconvolution components (no device work) --------------
+
+def _tensorset_config(tmpdir, resp_extra=None, filt_extra=None):
+    import json as _json
+    import os as _os
+    from wct_amd import components as K
+    from wct_amd import workload as W
+    planes = []
+    for ip, kind in enumerate(("ind", "ind", "col")):
+        pos, cur = W.fr_paths(kind)
+        planes.append((ip, 0.0, W.PITCH, pos, cur))
+    path = _os.path.join(str(tmpdir), "fr.json")
+    with open(path, "w") as f:
+        _json.dump(K.field_response_json(planes, W.ORIGIN, 0.0, W.FINE_PERIOD, W.SPEED), f)
+    rdata = {"anode": "AnodeChannels:apa0", "field_response": "FieldResponse:synth", "per_chan_resp": ""}
+    rdata.update(resp_extra or {})
+    fdata = {"filters": ["HfFilter:Wiener_tight_U", "LfFilter:ROI_tight_lf"]}
+    fdata.update(filt_extra or {})
+    return [
+        {"type": "AnodeChannels", "name": "apa0", "data": {"ident": 0, "plane_ranges": [[0, 96], [96, 96], [192, 96]]}},
+        {"type": "FieldResponse", "name": "synth", "data": {"filename": path}},
+        {"type": "HfFilter", "name": "Wiener_tight_U", "data": W.WIENER},
+        {"type": "HfFilter", "name": "Wire_ind", "data": dict(W.WIRE)},
+        {"type": "HfFilter", "name": "Wire_col", "data": dict(W.WIRE)},
+        {"type": "LfFilter", "name": "ROI_tight_lf", "data": W.LF_TIGHT},
+        {"type": "Decon2DResponse", "name": "r", "data": rdata},
+        {"type": "Decon2DFilter", "name": "f", "data": fdata},
+    ]
+
+
+def test_tensorset_handles_roundtrip():
+    from wct_amd import components as K
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((5, 7)).astype(np.float32)
+    s = K.TensorSet(ident=42, metadata={"tick": 500.0, "note": "x"})
+    s.add_array(a, "f4", {"tag": "t", "type": "waveform"})
+    s.add("i4", (5,), np.arange(5, dtype=np.int32), {"tag": "t", "type": "channels"})
+    info = s.info()
+    assert info["ident"] == 42 and info["ntensors"] == 2 and info["metadata"]["tick"] == 500.0
+    dt, shape, flat, md = s.tensor(0)
+    assert dt == "f4" and shape == (5, 7) and md == {"tag": "t", "type": "waveform"}
+    assert np.array_equal(flat, a.ravel(order="F"))       # column by column
+    assert np.array_equal(s.array(0), a)
+    assert s.find("t", "channels") == 1 and s.find("t", "summary") is None
+    # the C front refuses an unknown element type
+    import ctypes
+    from wct_amd.host import lib as hlib
+    one = (ctypes.c_size_t * 1)(1)
+    z = np.zeros(1, np.float32)
+    assert hlib().wcth_tensorset_add(s.h, b"zz", 1, one, z.ctypes.data, b"{}") == -1
+
+
+def test_decon2d_configuration_errors(tmp_path):
+    from wct_amd import components as K
+    from wct_amd.capi import WctError
+    K.reset()
+    cfg = _tensorset_config(tmp_path)
+    bad = [dict(c) for c in cfg]
+    bad[-2] = {"type": "Decon2DResponse", "name": "r", "data": {"per_chan_resp": ""}}   # anode "Anode"
+    with pytest.raises(WctError, match="no anode"):
+        K.configure(bad)
+    K.reset()
+    bad[-2] = {"type": "Decon2DResponse", "name": "r", "data": {"anode": "AnodeChannels:apa0"}}
+    with pytest.raises(WctError, match="per-channel response"):
+        K.configure(bad)
+    K.reset()
+    bad[-2] = {"type": "Decon2DResponse", "name": "r",
+               "data": {"anode": "AnodeChannels:apa0", "per_chan_resp": "", "field_response": "Nope:x"}}
+    with pytest.raises(WctError, match="no field response"):
+        K.configure(bad)
+    K.reset()
+
+
+def test_decon2d_eos_and_input_errors(tmp_path):
+    from wct_amd import components as K
+    from wct_amd.capi import WctError
+    K.reset()
+    K.configure(_tensorset_config(tmp_path))
+    assert K.tensorset_filter("Decon2DResponse:r", None) is None
+    assert K.tensorset_filter("Decon2DFilter:f", None) is None
+    s = K.TensorSet(0, {"tick": 500.0})
+    s.add("i4", (4,), np.arange(4, dtype=np.int32), {"tag": "trace_tag", "type": "channels"})
+    with pytest.raises(WctError, match="no waveform tensor"):
+        K.tensorset_filter("Decon2DResponse:r", s)
+    s = K.TensorSet(0, {})
+    s.add_array(np.ones((4, 10), np.float32), "f4", {"tag": "trace_tag", "type": "waveform"})
+    with pytest.raises(WctError, match="no channels tensor"):
+        K.tensorset_filter("Decon2DResponse:r", s)
+    with pytest.raises(WctError, match="dtype f4, expected c8"):
+        K.tensorset_filter("Decon2DFilter:f", s)
+    s = K.TensorSet(0, {})
+    s.add_array(np.ones((4, 11), np.complex64), "c8", {"tag": "trace_tag", "type": "waveform"})
+    s.add("i4", (4,), np.arange(4, dtype=np.int32), {"tag": "trace_tag", "type": "channels"})
+    with pytest.raises(WctError, match="odd tick count"):
+        K.tensorset_filter("Decon2DFilter:f", s)
+    K.reset()
+    K.configure(_tensorset_config(tmp_path, filt_extra={"filters": ["NoSuchFilter:x"]}))
+    s = K.TensorSet(0, {})
+    s.add_array(np.ones((4, 10), np.complex64), "c8", {"tag": "trace_tag", "type": "waveform"})
+    s.add("i4", (4,), np.arange(4, dtype=np.int32), {"tag": "trace_tag", "type": "channels"})
+    with pytest.raises(WctError, match="no filter"):
+        K.tensorset_filter("Decon2DFilter:f", s)
+    K.reset()

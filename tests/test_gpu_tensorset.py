@@ -132,3 +132,85 @@ def test_spectrum_mode_rejects_mixed_jobs(dev):
     with pytest.raises(WctError, match="outputs of the decon mode"):
         decon.Batch([bad])
     pk.close()
+
+
+# ---- the components through the C front (column-by-column tensors) ------
+
+def _components_setup(K, tmp_path, nch=96, nt=1000, ntr=6):
+    from wct_amd import workload as W
+    from test_components import _tensorset_config
+    K.reset()
+    K.configure(_tensorset_config(tmp_path, resp_extra={
+        "gain": W.GAIN, "shaping_time": W.SHAPING, "inter_gain": W.INTER_GAIN, "ADC_mV": W.ADC_MV,
+        "ftoffset": W.FTOFFSET, "ctoffset": W.CTOFFSET, "iplane": 0}))
+    spec, resp, adc = _setup("ind", nch, nt, ntr)
+    s = K.TensorSet(7, {"tick": W.TICK, "run": 3})
+    s.add_array(adc, "f4", {"tag": "trace_tag", "type": "waveform", "tbin": 0.0, "pad": 0.0})
+    s.add("i4", (nch,), np.arange(100, 100 + nch, dtype=np.int32), {"tag": "trace_tag", "type": "channels"})
+    s.add("f8", (nch,), np.linspace(1.0, 2.0, nch), {"tag": "trace_tag", "type": "summary"})
+    return spec, resp, adc, s
+
+
+def test_tensorset_components_match_device_paths(dev, tmp_path):
+    import torch
+    from wct_amd import components as K
+    from wct_amd import decon
+    print("Decon2DResponse -> Decon2DFilter, 96 x 1000", flush=True)
+    spec, resp, adc, s = _components_setup(K, tmp_path)
+    r = K.tensorset_filter("Decon2DResponse:r", s)
+    info = r.info()
+    assert info["ident"] == 7 and info["metadata"]["tags"] == ["trace_tag"] and info["metadata"]["run"] == 3
+    iw = r.find("trace_tag", "waveform")
+    dt, shape, _, md = r.tensor(iw)
+    assert dt == "c8" and shape == (96, 1000) and md["type"] == "waveform" and md["tag"] == "trace_tag"
+    assert r.find("trace_tag", "channels") is not None and r.find("trace_tag", "summary") is not None
+    sp = r.array(iw)
+    # the device path on the same frame and response
+    want = _gpu_spectrum(spec, decon.overall_response(spec), adc, dev).cpu().numpy()
+    assert np.abs(sp - want).max() <= 1e-5 * np.abs(want).max()
+    f = K.tensorset_filter("Decon2DFilter:f", r)
+    iy = f.find("trace_tag", "waveform")
+    assert f.tensor(iy)[0] == "f4" and f.find("trace_tag", "channels") is not None
+    y = f.array(iy).astype(np.float64)
+    tf = _time_filter(spec, "ind")
+    ref_same = O.decon2d_filter(sp.astype(np.complex128), tf)
+    y64 = O.decon_fused(adc.astype(np.float64), resp, spec.wire_filter, tf, spec.wire_shift, spec.time_shift)
+    ref_pair, _ = O.restore_baseline(y64, np.float64)
+    e1, e2 = _rel_rms(y, ref_same), _rel_rms(y, ref_pair)
+    print(f"  component filter rel_rms={e1:.3e} pair vs fused rel_rms={e2:.3e}")
+    assert e1 <= 5e-5 and e2 <= 5e-5
+    assert K.tensorset_filter("Decon2DResponse:r", None) is None
+    K.reset()
+
+
+def test_decon2d_filter_cmm_ranges_and_empty_rows(dev, tmp_path):
+    from wct_amd import components as K
+    from wct_amd.capi import WctError
+    print("Decon2DFilter cmm ranges", flush=True)
+    _, _, _, s = _components_setup(K, tmp_path)
+    r = K.tensorset_filter("Decon2DResponse:r", s)
+    sp = r.array(r.find("trace_tag", "waveform"))
+    base = K.tensorset_filter("Decon2DFilter:f", r)
+    y0 = base.array(base.find("trace_tag", "waveform"))
+    t = K.TensorSet(8, {})
+    t.add_array(sp, "c8", {"tag": "trace_tag", "type": "waveform"})
+    t.add("i4", (96,), np.arange(96, dtype=np.int32), {"tag": "trace_tag", "type": "channels"})
+    ranges = np.array([[5, 40], [0, 1000], [10, 20]], np.float64)     # (tmin, tmax) per range
+    chans = np.array([3.0, 95.0, 500.0])                               # 500: past the last row, skipped
+    t.add_array(ranges, "f8", {"tag": "trace_tag", "type": "bad:cmm_range"})
+    t.add("f8", (3,), chans, {"tag": "trace_tag", "type": "bad:cmm_channel"})
+    o = K.tensorset_filter("Decon2DFilter:f", t)
+    y = o.array(o.find("trace_tag", "waveform"))
+    keep = np.ones_like(y, bool)
+    keep[3, 5:40] = False
+    keep[95, :] = False
+    assert np.all(y[~keep] == 0) and np.array_equal(y[keep], y0[keep])
+    # Sig::restore_baseline refuses a row with no non-zero sample
+    z = sp.copy()
+    z[17, :] = 0
+    u = K.TensorSet(9, {})
+    u.add_array(z, "c8", {"tag": "trace_tag", "type": "waveform"})
+    u.add("i4", (96,), np.arange(96, dtype=np.int32), {"tag": "trace_tag", "type": "channels"})
+    with pytest.raises(WctError, match="empty waveform"):
+        K.tensorset_filter("Decon2DFilter:f", u)
+    K.reset()

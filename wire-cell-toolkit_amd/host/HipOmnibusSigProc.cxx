@@ -37,6 +37,7 @@
 #include "Frame.h"
 #include "Interface.h"
 #include "Response.h"
+#include "Buffers.h"
 #include "RoiFormation.h"
 
 namespace WireCell {
@@ -50,42 +51,6 @@ static void hcheck(int rc, const char* what)
 {
     if (rc != WCT_OK) throw SigProcError(std::string("HipOmnibusSigProc ") + what + ": " + wct_last_error());
 }
-
-// Device allocation owned by a component (freed on destruction).
-struct DevBuf {
-    void* p{nullptr};
-    size_t bytes{0};
-    void ensure(size_t n)
-    {
-        if (n <= bytes && p) return;
-        release();
-        hcheck(wct_malloc(&p, n), "device allocation");
-        bytes = n;
-    }
-    void release()
-    {
-        if (p) wct_free(p);
-        p = nullptr;
-        bytes = 0;
-    }
-};
-struct HostBuf {
-    void* p{nullptr};
-    size_t bytes{0};
-    void ensure(size_t n)
-    {
-        if (n <= bytes && p) return;
-        release();
-        hcheck(wct_host_malloc(&p, n), "pinned allocation");
-        bytes = n;
-    }
-    void release()
-    {
-        if (p) wct_host_free(p);
-        p = nullptr;
-        bytes = 0;
-    }
-};
 
 class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
   public:

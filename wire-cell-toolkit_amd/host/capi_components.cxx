@@ -11,6 +11,7 @@
 #include "Frame.h"
 #include "Interface.h"
 #include "RoiFormation.h"
+#include "Tensor.h"
 
 using namespace WireCell;
 
@@ -40,6 +41,34 @@ struct FrameHandle {
         return frame;
     }
 };
+// A tensor set under construction (input side) or a filter's result.
+struct TensorSetHandle {
+    int ident{0};
+    Configuration md{Configuration::makeObject()};
+    std::vector<ITensor::pointer> tensors;
+    ITensorSet::pointer set;   // set once built or when it is an output
+    ITensorSet::pointer get()
+    {
+        if (!set) set = std::make_shared<Aux::SimpleTensorSet>(ident, md, std::make_shared<ITensor::vector>(tensors));
+        return set;
+    }
+};
+
+int copy_json(const Configuration& c, char* buf, size_t cap)
+{
+    const std::string s = c.dump();
+    if (buf && cap) {
+        std::strncpy(buf, s.c_str(), cap - 1);
+        buf[cap - 1] = 0;
+    }
+    return (int)s.size();
+}
+
+Configuration parse_md(const char* json)
+{
+    if (!json || !*json) return Configuration::makeObject();
+    return Configuration::parse(json);
+}
 }  // namespace
 
 extern "C" {
@@ -273,4 +302,103 @@ int wcth_roi_lists(const unsigned char* mask, int nw, int nt, long ld, int pad, 
     return over ? -2 : tot;
 }
 
+
+void* wcth_tensorset_new(int ident, const char* metadata_json)
+{
+    try {
+        auto h = new TensorSetHandle;
+        h->ident = ident;
+        h->md = parse_md(metadata_json);
+        return h;
+    } catch (const std::exception& e) {
+        fail(e);
+        return nullptr;
+    }
+}
+
+void wcth_tensorset_free(void* set) { delete (TensorSetHandle*)set; }
+
+int wcth_tensorset_add(void* set, const char* dtype, int ndim, const size_t* shape, const void* data,
+                       const char* metadata_json)
+{
+    try {
+        auto h = (TensorSetHandle*)set;
+        if (!h || h->set || !dtype || ndim < 0 || (ndim && !shape)) {
+            g_err = "tensorset_add: bad arguments (or the set was already used)";
+            return -1;
+        }
+        ITensor::shape_t sh(shape, shape + ndim);
+        h->tensors.push_back(std::make_shared<Aux::SimpleTensor>(dtype, sh, data, parse_md(metadata_json)));
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int wcth_tensorset_filter(const char* tn, void* in, void** out)
+{
+    try {
+        if (!out) {
+            g_err = "null output handle";
+            return -1;
+        }
+        *out = nullptr;
+        auto filt = Factory::find_tn<ITensorSetFilter>(tn);
+        ITensorSet::pointer sin = in ? ((TensorSetHandle*)in)->get() : nullptr;
+        ITensorSet::pointer sout;
+        if (!(*filt)(sin, sout)) {
+            g_err = "tensor set filter returned false";
+            return -1;
+        }
+        if (sout) {
+            auto h = new TensorSetHandle;
+            h->set = sout;
+            *out = h;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int wcth_tensorset_info(void* set, int* ident, int* ntensors, char* metadata_json, size_t cap)
+{
+    auto h = (TensorSetHandle*)set;
+    if (!h) return -1;
+    auto s = h->get();
+    if (ident) *ident = s->ident();
+    auto tv = s->tensors();
+    if (ntensors) *ntensors = tv ? (int)tv->size() : 0;
+    return copy_json(s->metadata(), metadata_json, cap);
+}
+
+int wcth_tensorset_tensor(void* set, int i, char* dtype, int* ndim, size_t* shape, const void** data,
+                          size_t* nbytes)
+{
+    auto h = (TensorSetHandle*)set;
+    if (!h) return -1;
+    auto tv = h->get()->tensors();
+    if (!tv || i < 0 || i >= (int)tv->size()) return -1;
+    const auto& t = tv->at(i);
+    const auto sh = t->shape();
+    if (dtype) {
+        std::strncpy(dtype, t->dtype().c_str(), 7);
+        dtype[7] = 0;
+    }
+    if (ndim) *ndim = (int)sh.size();
+    if (shape)
+        for (size_t k = 0; k < sh.size() && k < 8; ++k) shape[k] = sh[k];
+    if (data) *data = t->data();
+    if (nbytes) *nbytes = t->size();
+    return 0;
+}
+
+int wcth_tensorset_tensor_md(void* set, int i, char* metadata_json, size_t cap)
+{
+    auto h = (TensorSetHandle*)set;
+    if (!h) return -1;
+    auto tv = h->get()->tensors();
+    if (!tv || i < 0 || i >= (int)tv->size()) return -1;
+    return copy_json(tv->at(i)->metadata(), metadata_json, cap);
+}
 }  // extern "C"

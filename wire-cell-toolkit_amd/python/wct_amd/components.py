@@ -154,6 +154,94 @@ def frame_filter(tn, frame):
     return Frame(_handle=out.value) if out.value else None
 
 
+class TensorSet:
+    """An ITensorSet handle (input under construction, or a filter
+    result).  2D waveform tensors of shape (nrows, ncols) keep their
+    elements column by column, as the reference's tensor-set nodes map
+    them (aux/inc/WireCellAux/Util.h:119-151): add_array / array take a
+    numpy (nrows, ncols) array and do that layout."""
+
+    _DT = {"f4": np.float32, "f8": np.float64, "c8": np.complex64, "c16": np.complex128, "i4": np.int32,
+           "i8": np.int64, "u1": np.uint8, "i2": np.int16, "u4": np.uint32}
+
+    def __init__(self, ident=0, metadata=None, _handle=None):
+        if _handle is not None:
+            self.h = _handle
+        else:
+            self.h = lib().wcth_tensorset_new(int(ident), json.dumps(metadata or {}).encode())
+            if not self.h:
+                _check(-1)
+
+    def add(self, dtype, shape, data, metadata=None):
+        """Adds a tensor whose store is the bytes of data (C order of the
+        given array, copied)."""
+        a = np.ascontiguousarray(data, self._DT[dtype])
+        sh = (C.c_size_t * max(len(shape), 1))(*shape)
+        _check(lib().wcth_tensorset_add(self.h, dtype.encode(), len(shape), sh, a.ctypes.data if a.size else None,
+                                        json.dumps(metadata or {}).encode()))
+
+    def add_array(self, arr, dtype, metadata=None):
+        """A 2D (nrows, ncols) array stored column by column."""
+        a = np.asarray(arr)
+        self.add(dtype, a.shape, np.asfortranarray(a, self._DT[dtype]).ravel(order="F"), metadata)
+
+    def info(self):
+        ident, n = C.c_int(), C.c_int()
+        ln = lib().wcth_tensorset_info(self.h, C.byref(ident), C.byref(n), None, 0)
+        if ln < 0:
+            raise WctError("bad tensor set handle")
+        buf = C.create_string_buffer(ln + 1)
+        lib().wcth_tensorset_info(self.h, None, None, buf, ln + 1)
+        return dict(ident=ident.value, ntensors=n.value, metadata=json.loads(buf.value.decode() or "null"))
+
+    def tensor(self, i):
+        """(dtype, shape, flat data copy, metadata) of tensor i."""
+        dt = C.create_string_buffer(8)
+        nd = C.c_int()
+        sh = (C.c_size_t * 8)()
+        p, nb = C.c_void_p(), C.c_size_t()
+        _check(lib().wcth_tensorset_tensor(self.h, int(i), dt, C.byref(nd), sh, C.byref(p), C.byref(nb)))
+        dtype = dt.value.decode()
+        shape = tuple(sh[k] for k in range(nd.value))
+        raw = C.string_at(p.value, nb.value) if nb.value else b""
+        flat = np.frombuffer(raw, self._DT[dtype]).copy()
+        ln = lib().wcth_tensorset_tensor_md(self.h, int(i), None, 0)
+        buf = C.create_string_buffer(ln + 1)
+        lib().wcth_tensorset_tensor_md(self.h, int(i), buf, ln + 1)
+        return dtype, shape, flat, json.loads(buf.value.decode() or "null")
+
+    def array(self, i):
+        """Tensor i as a (nrows, ncols) array (column-by-column store)."""
+        dtype, shape, flat, _ = self.tensor(i)
+        return flat.reshape(shape, order="F")
+
+    def find(self, tag, type_):
+        """Index of the first tensor with metadata tag and type, or None."""
+        for i in range(self.info()["ntensors"]):
+            md = self.tensor(i)[3] or {}
+            if md.get("tag") == tag and md.get("type") == type_:
+                return i
+        return None
+
+    def close(self):
+        if self.h:
+            lib().wcth_tensorset_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tensorset_filter(tn, tset):
+    """Calls ITensorSetFilter tn on tset (None = end of stream)."""
+    out = C.c_void_p()
+    _check(lib().wcth_tensorset_filter(tn.encode(), tset.h if tset is not None else None, C.byref(out)))
+    return TensorSet(_handle=out.value) if out.value else None
+
+
 def roi_lists(mask, pad=5, asy=0.1, cap=1 << 22):
     """Tight ROI lists (inclusive tick pairs per wire) from a threshold mask."""
     m = np.ascontiguousarray(mask, np.uint8)

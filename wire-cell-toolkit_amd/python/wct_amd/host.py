@@ -42,6 +42,15 @@ _SIGS = {
                                    C.c_int]),
     "wcth_roi_lists": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_long, C.c_int, C.c_float,
                                  C.c_void_p, C.c_void_p, C.c_int]),
+    "wcth_tensorset_new": (C.c_void_p, [C.c_int, C.c_char_p]),
+    "wcth_tensorset_free": (None, [C.c_void_p]),
+    "wcth_tensorset_add": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_void_p, C.c_void_p, C.c_char_p]),
+    "wcth_tensorset_filter": (C.c_int, [C.c_char_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "wcth_tensorset_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p,
+                                      C.c_size_t]),
+    "wcth_tensorset_tensor": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.POINTER(C.c_int), C.c_void_p,
+                                        C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]),
+    "wcth_tensorset_tensor_md": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_size_t]),
 }
 EXPORTED = sorted(_SIGS)
 MISSING = []
