@@ -325,19 +325,39 @@ __device__ __forceinline__ uint32_t bin_start(const RowScratch* sh, uint32_t b)
     return b ? sh->off[b - 1] : 0u;
 }
 
-// The thread owning the bin that holds sorted rank r writes that bin.
+// The ends of the bins a thread owns (its share of the offsets), read
+// once; write() lets the thread owning the bin that holds sorted rank r
+// write that bin.  The ends are non-decreasing, so the bin is the first
+// one whose end exceeds r: b0 + the number of ends <= r (no early exit,
+// no per-bin branch).
+template <int NT>
+struct OwnBins {
+    static constexpr int per = kLinHist / NT;
+    int b0;
+    uint32_t lo;
+    uint32_t e[per];
+    __device__ __forceinline__ explicit OwnBins(const RowScratch* sh)
+    {
+        b0 = threadIdx.x * per;
+        lo = bin_start(sh, b0);
+#pragma unroll
+        for (int q = 0; q < per; ++q) e[q] = sh->off[b0 + q];
+    }
+    __device__ __forceinline__ void write(uint32_t r, uint32_t* dst) const
+    {
+        if (r >= lo && r < e[per - 1]) {
+            uint32_t c = 0;
+#pragma unroll
+            for (int q = 0; q < per - 1; ++q) c += e[q] <= r;
+            *dst = b0 + c;
+        }
+    }
+};
+
 template <int NT>
 __device__ __forceinline__ void locate_write(uint32_t r, uint32_t* dst, const RowScratch* sh)
 {
-    constexpr int per = kLinHist / NT;
-    const int b0 = threadIdx.x * per;
-    if (r >= bin_start(sh, b0) && r < sh->off[b0 + per - 1]) {
-        for (int q = 0; q < per; ++q)
-            if (r < sh->off[b0 + q]) {
-                *dst = b0 + q;
-                break;
-            }
-    }
+    OwnBins<NT>(sh).write(r, dst);
 }
 
 // Rank counting inside a segment of keys: thread t (< cnt) takes key
@@ -588,9 +608,12 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint32_t n_va
         // the median and the 16/84% points are selected in one round.
         const uint32_t r1 = pct_index(0.5f, n_nz);
         const uint32_t rq[2] = {pct_index(ps[0], n_nz), pct_index(ps[2], n_nz)};
-        locate_write<NT>(r1, &sh->sbin[0], sh);
-        locate_write<NT>(rq[0], &sh->sbin[2], sh);
-        locate_write<NT>(rq[1], &sh->sbin[3], sh);
+        {
+            const OwnBins<NT> own(sh);
+            own.write(r1, &sh->sbin[0]);
+            own.write(rq[0], &sh->sbin[2]);
+            own.write(rq[1], &sh->sbin[3]);
+        }
         lds_barrier();
         const uint32_t B1 = sh->sbin[0];
         o1 = bin_start(sh, B1);
