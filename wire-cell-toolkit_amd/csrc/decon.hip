@@ -145,11 +145,13 @@ struct RowLds {
     size_t bytes() const { return (size_t)(buf_len + tw_len) * sizeof(float2); }
 };
 
-inline RowLds row_lds(int Mmax, int twmax, bool stats)
+// nkeys: the row statistics' key capacity (every register slot of the
+// row, padding included).
+inline RowLds row_lds(int Mmax, int twmax, bool stats, int nkeys = 0)
 {
     RowLds l;
     if (stats) {
-        const size_t scratch = sizeof(RowScratch) + (size_t)2 * Mmax * sizeof(uint32_t);
+        const size_t scratch = sizeof(RowScratch) + (size_t)std::max(2 * Mmax, nkeys) * sizeof(uint32_t);
         l.buf_len = std::max(padded_len(Mmax), (int)((scratch + sizeof(float2) - 1) / sizeof(float2)));
         l.tw_len = 0;
     } else {
@@ -677,7 +679,7 @@ static int launch_rows_r2c(wct_decon_plan p, const wct_decon_group& g, hipStream
 template <int VH, bool S5, bool BAD>
 static int launch_rows_c2r(wct_decon_plan p, const wct_decon_group& g, hipStream_t s)
 {
-    const RowLds lay = row_lds(g.len, g.twcount, true);
+    const RowLds lay = row_lds(g.len, g.twcount, true, 2 * kNT * VH);
     const size_t lds = lay.bytes();
     int rc = allow_lds(k_row_c2r_roi<kNT, VH, S5, BAD>, lds);
     if (rc) return rc;

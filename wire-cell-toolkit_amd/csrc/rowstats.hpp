@@ -452,32 +452,37 @@ __device__ __forceinline__ uint32_t select_slot(uint32_t rank, int q, RowScratch
 // that span sits inside the window with a margin far above the rounding,
 // both counts are 0 without a pass over the row.
 template <int NT, int VPT>
-__device__ __forceinline__ void far_counts(const float (&val)[VPT], float m1, LinBins lb, RowScratch* sh,
-                                           uint32_t (&c)[2])
+__device__ __forceinline__ void far_counts(const float (&val)[VPT], float m1, LinBins lb, uint32_t nzero,
+                                           RowScratch* sh, uint32_t (&c)[2])
 {
     c[0] = c[1] = 0;
-    if (lb.inv_w > 0.f && sh->off[0] == 0 && sh->off[kLinBins + 1] == sh->off[kLinBins]) {
+    // the nzero zero slots hold +inf and sit in the overflow bin
+    if (lb.inv_w > 0.f && sh->off[0] == 0 && sh->off[kLinBins + 1] - sh->off[kLinBins] == nzero) {
         const double lo = lb.lo, hi = (double)lb.lo + (double)kLinBins / (double)lb.inv_w;
         const double m = m1, margin = 1e-3 * (fabs(lo) + fabs(hi) + fabs(m) + 500.0);
         if (lo - m > -500.0 + margin && hi - m < 500.0 - margin) return;
     }
 #pragma unroll
     for (int i = 0; i < VPT; ++i) {
-        if (val[i] != 0.0f && !(fabsf(val[i] - m1) < 500.0f)) {
+        if (!(fabsf(val[i] - m1) < 500.0f)) {
             if (val[i] < m1) ++c[0]; else ++c[1];
         }
     }
     block_sums_slot<NT, 2>(c, sh->upart[2]);
+    c[1] -= nzero;
 }
 
 // Fast path: restore_baseline (if restore) and cal_RMS over the n_valid
-// samples of the row; the register slots past them hold zeros (a zero is
-// skipped by every statistic but the RMS count, which is corrected).  The
-// non-zero samples are counting-sorted into LDS by linear bin once; every
-// order statistic is then a short exact select inside one bin.  Returns
-// the RMS (0 when want_rms is false) and leaves the baselined samples in
-// val.  Working from values rather than per-slot validity bits keeps the
-// compiler from holding one lane mask per slot across the whole routine.
+// samples of the row; the register slots past them hold zeros.  After the
+// moments every zero slot (padding or a zero sample) holds +inf: it sorts
+// above every non-zero sample, so no rank below n_nz moves, it falls out
+// of the RMS window and of the clipped moments by itself, and the counts
+// that see it (the +-500 window, the RMS count of zero samples) are
+// corrected by their known number; no pass needs a per-sample validity
+// test.  The zeros are restored at the end.  The non-zero samples are
+// counting-sorted into LDS by linear bin once; every order statistic is
+// then a short exact select inside one bin.  Returns the RMS (0 when
+// want_rms is false) and leaves the baselined samples in val.
 template <int NT, int VPT>
 __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint32_t n_valid, bool restore, bool want_rms,
                                                 RowScratch* sh)
@@ -521,13 +526,17 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint32_t n_va
 
     float mean = S1 / (float)n_nz;
     float sd = sqrtf(fmaxf(S2 / (float)n_nz - mean * mean, 0.0f));
+    const float kInf = __int_as_float(0x7f800000);
+    const uint32_t nzero = (uint32_t)(NT * VPT) - n_nz;   // zero slots, padding included
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) val[i] = val[i] != 0.0f ? val[i] : kInf;
     {   // clipped moments: the samples within 3 sd of the mean
         const float lim = 3.0f * sd;
         float c1 = 0.f, c2 = 0.f;
         uint32_t cn = 0;
 #pragma unroll
         for (int i = 0; i < VPT; ++i)
-            if (val[i] != 0.0f && fabsf(val[i] - mean) < lim) {
+            if (fabsf(val[i] - mean) < lim) {
                 c1 += val[i];
                 c2 = fmaf(val[i], val[i], c2);
                 ++cn;
@@ -568,7 +577,7 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint32_t n_va
     // counting sort of the non-zero keys by bin
 #pragma unroll
     for (int i = 0; i < VPT; ++i)
-        if (val[i] != 0.0f) atomicAdd(&sh->off[lin_bin(val[i], lb)], 1u);
+        atomicAdd(&sh->off[lin_bin(val[i], lb)], 1u);
     lds_barrier();
     {
         uint32_t* my = sh->off + threadIdx.x * per;
@@ -592,7 +601,7 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint32_t n_va
     uint32_t* keys = row_keys(sh);
 #pragma unroll
     for (int i = 0; i < VPT; ++i)
-        if (val[i] != 0.0f) keys[atomicAdd(&sh->off[lin_bin(val[i], lb)], 1u)] = f2key(val[i]);
+        keys[atomicAdd(&sh->off[lin_bin(val[i], lb)], 1u)] = f2key(val[i]);
     lds_barrier();
 
     float b = 0.0f;
@@ -663,7 +672,7 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint32_t n_va
             m1 = key2f(big_select<NT>(keys + o1, n1, r1 - o1, o1, sh, l1, e1));
         }
         uint32_t c[2];
-        far_counts<NT, VPT>(val, m1, lb, sh, c);
+        far_counts<NT, VPT>(val, m1, lb, nzero, sh, c);
         const uint32_t r2 = c[0] + pct_index(0.5f, n_nz - c[0] - c[1]);
         if (c[0] + c[1] == 0 || r2 == r1) {
             b = m1;   // the second median is the first
@@ -700,7 +709,7 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint32_t n_va
                 m1 = key2f(big_select<NT>(keys + o1, n1, r1 - o1, o1, sh, l1, e1));
             }
             uint32_t c[2];
-            far_counts<NT, VPT>(val, m1, lb, sh, c);
+            far_counts<NT, VPT>(val, m1, lb, nzero, sh, c);
             const uint32_t r2 = c[0] + pct_index(0.5f, n_nz - c[0] - c[1]);
             if (c[0] + c[1] == 0 || r2 == r1) {
                 b = m1;   // the second median is the first
@@ -793,7 +802,7 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint32_t n_va
         uint32_t k = 0;
 #pragma unroll
         for (int i = 0; i < VPT; ++i) {
-            const float y = val[i] != 0.0f ? val[i] - b : 0.0f;
+            const float y = val[i] - b;   // +inf for a zero slot: outside the window
             const bool in = fabsf(y) < L;
             const float yi = in ? y : 0.0f;
             sf = fmaf(yi, yi, sf);
@@ -813,15 +822,13 @@ __device__ __forceinline__ float row_stats_fast(float (&val)[VPT], uint32_t n_va
             tot += sh->dpart[w];
             cntk += sh->upart[4][0][w];
         }
-        // the slots past n_valid are zeros that passed |0| < lim
-        if (lim > 0.0) cntk -= (uint32_t)(NT * VPT) - n_valid;
+        // the zero samples of the row (not the padding) are in the
+        // window whenever it is not empty
+        if (lim > 0.0) cntk += n_valid - n_nz;
         rms = cntk ? sqrtf((float)(tot / (double)cntk)) : 0.0f;
     }
-    if (restore) {
 #pragma unroll
-        for (int i = 0; i < VPT; ++i)
-            if (val[i] != 0.0f) val[i] -= b;
-    }
+    for (int i = 0; i < VPT; ++i) val[i] = val[i] != kInf ? val[i] - b : 0.0f;   // b = 0 without restore
     return rms;
 }
 
