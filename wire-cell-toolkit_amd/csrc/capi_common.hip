@@ -50,7 +50,9 @@ static void factorize(int n, FftPlan& pl)
     while (m % 8 == 0) { pl.rad[np++] = 8; m /= 8; }
     if (m % 4 == 0) { pl.rad[np++] = 4; m /= 4; }
     if (m % 2 == 0) { pl.rad[np++] = 2; m /= 2; }
-    const int odd[] = {5, 3, 7, 11, 13};
+    // a 5 and a 3 go together as one radix-15 pass (3 x 5 prime-factor
+    // butterfly): one LDS round trip and one round of butterflies fewer
+    const int odd[] = {15, 5, 3, 7, 11, 13};
     for (int p : odd)
         while (m % p == 0) { pl.rad[np++] = p; m /= p; }
     pl.npass = np;

@@ -156,6 +156,30 @@ template <typename T2> struct Dft<T2, 8> {
     }
 };
 
+// 15 = 3 x 5 by the prime-factor map, so no inner twiddles: input
+// (5 n1 + 3 n2) mod 15 is element n2 of 5-point DFT n1; bin k1 of the
+// 3-point DFT over n1 of those DFTs' bin k2 is output (10 k1 + 6 k2)
+// mod 15 (W15^(n k) = W3^(n1 k1) W5^(n2 k2) under these maps).
+template <typename T2> struct Dft<T2, 15> {
+    static __device__ __forceinline__ void run(T2* v, int sign, const T2*)
+    {
+        T2 a[3][5];
+#pragma unroll
+        for (int n1 = 0; n1 < 3; ++n1) {
+#pragma unroll
+            for (int n2 = 0; n2 < 5; ++n2) a[n1][n2] = v[(5 * n1 + 3 * n2) % 15];
+            Dft<T2, 5>::run(a[n1], sign, nullptr);
+        }
+#pragma unroll
+        for (int k2 = 0; k2 < 5; ++k2) {
+            T2 b[3] = {a[0][k2], a[1][k2], a[2][k2]};
+            Dft<T2, 3>::run(b, sign, nullptr);
+#pragma unroll
+            for (int k1 = 0; k1 < 3; ++k1) v[(10 * k1 + 6 * k2) % 15] = b[k1];
+        }
+    }
+};
+
 // Generic odd prime radix (7, 11, 13) from a small table of
 // exp(-2 pi i q / R), q < R (forward sign).
 template <typename T2, int R> struct DftGeneric {
@@ -234,7 +258,7 @@ __device__ __forceinline__ void stockham_pass(T2* buf, int n, int Ns, const T2* 
     lds_barrier();
 }
 
-// SMOOTH5 instantiates only the radix 2/3/4/5/8 passes (lengths whose
+// SMOOTH5 instantiates only the radix 2/3/4/5/8/15 passes (lengths whose
 // prime factors are 2, 3 and 5); the generic 7/11/13 passes carry large
 // constant tables that would inflate the register allocation of every
 // kernel that includes them.
@@ -256,6 +280,7 @@ __device__ __forceinline__ void fft_lds(T2* buf, const FftPlan& pl, const T2* __
         case 3: stockham_pass<T2, 3, NT, (NMAX / 3 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
         case 4: stockham_pass<T2, 4, NT, (NMAX / 4 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
         case 5: stockham_pass<T2, 5, NT, (NMAX / 5 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
+        case 15: stockham_pass<T2, 15, NT, (NMAX / 15 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
         case 8: stockham_pass<T2, 8, NT, (NMAX / 8 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
         case 7:
             if (!SMOOTH5) stockham_pass<T2, 7, NT, (NMAX / 7 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid);
@@ -282,7 +307,7 @@ __device__ __forceinline__ void stage_twiddles(T2* dst, const T2* __restrict__ s
 
 // ---- compile-time plans -------------------------------------------------
 // The radix schedule of a length known at compile time, in the order the
-// host plan uses (8s, then a 4 or a 2, then 5, 3, 7, 11, 13), so the
+// host plan uses (8s, then a 4 or a 2, then 15, 5, 3, 7, 11, 13), so the
 // twiddle table built for the runtime plan serves both.  With n, each
 // radix and each span constant, a pass needs no plan arrays, no radix
 // switch and no runtime division.
@@ -305,8 +330,8 @@ constexpr int sf_radix(int n, int p)
         ++idx;
         m /= 2;
     }
-    const int odd[5] = {5, 3, 7, 11, 13};
-    for (int q = 0; q < 5; ++q)
+    const int odd[6] = {15, 5, 3, 7, 11, 13};
+    for (int q = 0; q < 6; ++q)
         while (m % odd[q] == 0) {
             if (idx == p) return odd[q];
             ++idx;
