@@ -58,6 +58,13 @@ void* wcth_frame_new(int ident, double time, double tick);
 void wcth_frame_free(void* frame);
 int wcth_frame_add_trace(void* frame, int channel, int tbin, int n, const float* charge);
 int wcth_frame_add_mask(void* frame, const char* name, int channel, int first, int second);
+/* Tags of an input frame, applied when it is first used
+ * (Aux::SimpleFrame::tag_frame / tag_traces, aux/inc/WireCellAux/
+ * SimpleFrame.h:42-70): a frame tag, or a trace tag over trace indices
+ * with an optional per-trace summary. */
+int wcth_frame_tag_frame(void* frame, const char* tag);
+int wcth_frame_tag_traces(void* frame, const char* tag, const size_t* idx, int n, const double* summary,
+                          int nsummary);
 int wcth_frame_filter(const char* tn, void* in, void** out);
 int wcth_frame_info(void* frame, int* ident, double* time, double* tick, int* ntraces);
 int wcth_frame_trace(void* frame, int i, int* channel, int* tbin, int* n, const float** charge);
@@ -68,6 +75,16 @@ int wcth_frame_summary(void* frame, const char* tag, const double** val, int* n)
 /* Ranges of one channel-mask-map entry; returns the total count (only
  * the first cap are written). */
 int wcth_frame_masks(void* frame, const char* name, int* channels, int* firsts, int* seconds, int cap);
+
+/* Frame sinks and sources (IFrameSink / IFrameSource, iface/inc/
+ * WireCellIface/IFrameSink.h:11-21, IFrameSource.h:12-20), e.g. the
+ * frame files FrameFileSink / FrameFileSource (sio/src/FrameFileSink.cxx,
+ * FrameFileSource.cxx).  wcth_frame_sink passes frame (NULL = end of
+ * stream) to the sink "type:name".  wcth_frame_source sets *out to the
+ * next frame, or NULL at end of stream; it returns 1 once the source has
+ * reported end of stream and is called again. */
+int wcth_frame_sink(const char* tn, void* frame);
+int wcth_frame_source(const char* tn, void** out);
 
 /* Tensor sets (ITensorSet / ITensor, iface/inc/WireCellIface/
  * ITensorSet.h:9-20, ITensor.h:14-65): build an input set, run an

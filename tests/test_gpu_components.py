@@ -275,3 +275,56 @@ def test_sigproc_debug_and_sparse(K, tmp_path):
         rebuilt.setdefault(ch, np.zeros(NT, np.float32))[tb:tb + q.size] = q
     for ch, q in dense.items():
         np.testing.assert_array_equal(rebuilt.get(ch, np.zeros(NT, np.float32)), np.maximum(q, 0))
+
+
+def test_sigproc_between_frame_files(K, tmp_path):
+    """FrameFileSource -> HipOmnibusSigProc -> FrameFileSink: an ADC
+    archive in the reference's layout (int16 frame, column by column, a
+    bad-channel mask) gives an output archive whose wiener frame, channels
+    and RMS summary equal the component's in-memory output."""
+    import io
+    import tarfile
+    print("frame files around sigproc, 3 planes", NW, "x", NT, flush=True)
+    adc = _adc_planes(5)
+    adc16 = [np.round(a * 8).astype(np.int16) for a in adc]   # integer ADC counts
+    bad = {(1, 4): [(200, 260)]}
+    src, dst = tmp_path / "adc.tar.gz", tmp_path / "sp.tar.gz"
+    rows = np.concatenate(adc16)
+    chans = np.concatenate([np.arange(CH0[ip], CH0[ip] + NW[ip], dtype=np.int32) for ip in range(3)])
+    members = [("frame_orig_17.npy", np.asfortranarray(rows)), ("channels_orig_17.npy", chans),
+               ("tickinfo_orig_17.npy", np.array([3.0, 500.0, 0.0])),
+               ("chanmask_bad_17.npy", np.asfortranarray(np.array([[CH0[1] + 4, 200, 260]], np.int32)))]
+    with tarfile.open(src, "w:gz") as tf:
+        for name, arr in members:
+            b = io.BytesIO()
+            np.save(b, arr)
+            ti = tarfile.TarInfo(name)
+            ti.size = b.tell()
+            b.seek(0)
+            tf.addfile(ti, b)
+    cfg = _sigproc_config(tmp_path) + [
+        {"type": "FrameFileSource", "name": "in", "data": {"inname": str(src), "tags": ["orig"]}},
+        {"type": "FrameFileSink", "name": "out", "data": {"outname": str(dst), "tags": ["wiener"]}}]
+    K.configure(cfg)
+    f = K.frame_source("FrameFileSource:in")
+    out = K.frame_filter("HipOmnibusSigProc:osp", f)
+    K.frame_sink("FrameFileSink:out", out)
+    assert K.frame_source("FrameFileSource:in") is None
+    K.frame_sink("FrameFileSink:out", None)
+    # the same frame built in memory
+    direct = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, [a.astype(np.float32) for a in adc16], bad))
+    want = np.stack([q for _, _, q in direct.traces()])
+    got = {}
+    with tarfile.open(dst) as tf:
+        for m in tf.getmembers():
+            got[m.name] = np.load(io.BytesIO(tf.extractfile(m).read()))
+    assert sorted(got) == ["channels_wiener_17.npy", "chanmask_bad_17.npy", "chanmask_tight_roi_17.npy",
+                           "frame_wiener_17.npy", "summary_wiener_17.npy", "tickinfo_wiener_17.npy"]
+    np.testing.assert_array_equal(got["frame_wiener_17.npy"], want)
+    np.testing.assert_array_equal(got["channels_wiener_17.npy"], chans)
+    np.testing.assert_array_equal(got["summary_wiener_17.npy"], direct.trace_summary("wiener"))
+    assert list(got["tickinfo_wiener_17.npy"]) == [3.0, 500.0, 0.0]
+    assert not np.any(got["frame_wiener_17.npy"][NW[0] + 4, 200:260])
+    rois = direct.masks("tight_roi")
+    flat = sorted((c, a, b) for c, rl in rois.items() for (a, b) in rl)
+    assert sorted(map(tuple, got["chanmask_tight_roi_17.npy"].tolist())) == flat and flat

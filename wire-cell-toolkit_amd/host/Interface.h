@@ -11,6 +11,8 @@
 //   IFilterWaveform iface/inc/WireCellIface/IFilterWaveform.h:12-18
 //   IWaveform       iface/inc/WireCellIface/IWaveform.h:16-29
 //   IFieldResponse  iface/inc/WireCellIface/IFieldResponse.h
+//   IFrameSink, IFrameSource  iface/inc/WireCellIface/IFrameSink.h:11-21,
+//                   IFrameSource.h:12-20
 // IAnodeChannels stands in for the part of IAnodePlane that
 // OmnibusSigProc reads (its channel -> (plane, wire) map,
 // sigproc/src/OmnibusSigProc.cxx:171-216); wire geometry is not part of
@@ -121,6 +123,22 @@ class IFrameFilter : public IFunctionNode<IFrame, IFrame>, public IComponent<IFr
   public:
     typedef std::shared_ptr<IFrameFilter> pointer;
     virtual ~IFrameFilter() = default;
+};
+
+// Sink and source nodes of frames (iface/inc/WireCellIface/IFrameSink.h,
+// IFrameSource.h over ISinkNode / ISourceNode): a sink consumes a frame
+// (nullptr is end-of-stream); a source produces the next frame, nullptr
+// once at end-of-stream, and returns false after that.
+class IFrameSink : public IComponent<IFrameSink> {
+  public:
+    virtual ~IFrameSink() = default;
+    virtual bool operator()(const IFrame::pointer& frame) = 0;
+};
+
+class IFrameSource : public IComponent<IFrameSource> {
+  public:
+    virtual ~IFrameSource() = default;
+    virtual bool operator()(IFrame::pointer& frame) = 0;
 };
 
 class IFilterWaveform : public IComponent<IFilterWaveform> {

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <tuple>
 
 #include "../../include/wct_host.h"
 #include "Factory.h"
@@ -32,11 +33,17 @@ struct FrameHandle {
     Waveform::ChannelMaskMap masks;
     IFrame::pointer frame;   // set once built or when it is an output
     std::vector<std::string> frame_tags, trace_tags;
+    // input side: tags to apply when the frame is built
+    std::vector<std::string> in_frame_tags;
+    std::vector<std::tuple<std::string, IFrame::trace_list_t, IFrame::trace_summary_t>> in_trace_tags;
     IFrame::pointer get()
     {
         if (!frame) {
             auto tv = std::make_shared<ITrace::vector>(traces);
-            frame = std::make_shared<Aux::SimpleFrame>(ident, time, tv, tick, masks);
+            auto sf = std::make_shared<Aux::SimpleFrame>(ident, time, tv, tick, masks);
+            for (const auto& t : in_frame_tags) sf->tag_frame(t);
+            for (const auto& [t, idx, sum] : in_trace_tags) sf->tag_traces(t, idx, sum);
+            frame = sf;
         }
         return frame;
     }
@@ -400,5 +407,66 @@ int wcth_tensorset_tensor_md(void* set, int i, char* metadata_json, size_t cap)
     auto tv = h->get()->tensors();
     if (!tv || i < 0 || i >= (int)tv->size()) return -1;
     return copy_json(tv->at(i)->metadata(), metadata_json, cap);
+}
+
+int wcth_frame_sink(const char* tn, void* frame)
+{
+    try {
+        auto sink = Factory::find_tn<IFrameSink>(tn);
+        IFrame::pointer f = frame ? ((FrameHandle*)frame)->get() : nullptr;
+        if (!(*sink)(f)) {
+            g_err = "frame sink returned false";
+            return -1;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int wcth_frame_source(const char* tn, void** out)
+{
+    try {
+        if (!out) {
+            g_err = "null output handle";
+            return -1;
+        }
+        *out = nullptr;
+        auto src = Factory::find_tn<IFrameSource>(tn);
+        IFrame::pointer f;
+        if (!(*src)(f)) return 1;
+        if (f) {
+            auto h = new FrameHandle;
+            h->frame = f;
+            *out = h;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int wcth_frame_tag_frame(void* frame, const char* tag)
+{
+    auto f = (FrameHandle*)frame;
+    if (!f || f->frame || !tag) {
+        g_err = "frame_tag_frame: bad handle (or the frame was already used)";
+        return -1;
+    }
+    f->in_frame_tags.push_back(tag);
+    return 0;
+}
+
+int wcth_frame_tag_traces(void* frame, const char* tag, const size_t* idx, int n, const double* summary,
+                          int nsummary)
+{
+    auto f = (FrameHandle*)frame;
+    if (!f || f->frame || !tag || n < 0 || (n && !idx) || nsummary < 0 || (nsummary && !summary)) {
+        g_err = "frame_tag_traces: bad arguments (or the frame was already used)";
+        return -1;
+    }
+    f->in_trace_tags.emplace_back(tag, IFrame::trace_list_t(idx, idx + n),
+                                  IFrame::trace_summary_t(summary, summary + nsummary));
+    return 0;
 }
 }  // extern "C"

@@ -78,6 +78,15 @@ class Frame:
     def add_mask(self, name, channel, first, second):
         _check(lib().wcth_frame_add_mask(self.h, name.encode(), int(channel), int(first), int(second)))
 
+    def tag_frame(self, tag):
+        _check(lib().wcth_frame_tag_frame(self.h, tag.encode()))
+
+    def tag_traces(self, tag, indices, summary=None):
+        idx = np.ascontiguousarray(indices, np.uint64)
+        sm = np.ascontiguousarray(summary if summary is not None else [], np.float64)
+        _check(lib().wcth_frame_tag_traces(self.h, tag.encode(), idx.ctypes.data if idx.size else None, idx.size,
+                                           sm.ctypes.data if sm.size else None, sm.size))
+
     def info(self):
         ident, ntr = C.c_int(), C.c_int()
         time, tick = C.c_double(), C.c_double()
@@ -240,6 +249,22 @@ def tensorset_filter(tn, tset):
     out = C.c_void_p()
     _check(lib().wcth_tensorset_filter(tn.encode(), tset.h if tset is not None else None, C.byref(out)))
     return TensorSet(_handle=out.value) if out.value else None
+
+
+def frame_sink(tn, frame):
+    """Passes frame (None = end of stream) to IFrameSink tn."""
+    _check(lib().wcth_frame_sink(tn.encode(), frame.h if frame is not None else None))
+
+
+def frame_source(tn):
+    """The next frame of IFrameSource tn: a Frame, None at end of stream,
+    and False once the source has nothing more after that."""
+    out = C.c_void_p()
+    rc = lib().wcth_frame_source(tn.encode(), C.byref(out))
+    if rc == 1:
+        return False
+    _check(rc)
+    return Frame(_handle=out.value) if out.value else None
 
 
 def roi_lists(mask, pad=5, asy=0.1, cap=1 << 22):

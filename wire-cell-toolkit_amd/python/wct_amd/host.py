@@ -42,6 +42,10 @@ _SIGS = {
                                    C.c_int]),
     "wcth_roi_lists": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_long, C.c_int, C.c_float,
                                  C.c_void_p, C.c_void_p, C.c_int]),
+    "wcth_frame_tag_frame": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "wcth_frame_tag_traces": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
+    "wcth_frame_sink": (C.c_int, [C.c_char_p, C.c_void_p]),
+    "wcth_frame_source": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "wcth_tensorset_new": (C.c_void_p, [C.c_int, C.c_char_p]),
     "wcth_tensorset_free": (None, [C.c_void_p]),
     "wcth_tensorset_add": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_void_p, C.c_void_p, C.c_char_p]),
