@@ -42,19 +42,15 @@ int lds_limit_bytes()
     return lim;
 }
 
-// Radix order: 8s, then a 4 or 2 for the rest of the power of two, then
-// 5, 3, 7, 11, 13.  Larger radices mean fewer passes through LDS.
+// The radix schedule is sf_radix's (fft_lds.hpp), shared with the
+// compile-time plans.
 static void factorize(int n, FftPlan& pl)
 {
-    int m = n, np = 0;
-    while (m % 8 == 0) { pl.rad[np++] = 8; m /= 8; }
-    if (m % 4 == 0) { pl.rad[np++] = 4; m /= 4; }
-    if (m % 2 == 0) { pl.rad[np++] = 2; m /= 2; }
-    // a 5 and a 3 go together as one radix-15 pass (3 x 5 prime-factor
-    // butterfly): one LDS round trip and one round of butterflies fewer
-    const int odd[] = {15, 5, 3, 7, 11, 13};
-    for (int p : odd)
-        while (m % p == 0) { pl.rad[np++] = p; m /= p; }
+    int np = 0;
+    while (np < kMaxPasses && sf_radix(n, np)) {
+        pl.rad[np] = sf_radix(n, np);
+        ++np;
+    }
     pl.npass = np;
     pl.n = n;
 }

@@ -156,29 +156,44 @@ template <typename T2> struct Dft<T2, 8> {
     }
 };
 
-// 15 = 3 x 5 by the prime-factor map, so no inner twiddles: input
-// (5 n1 + 3 n2) mod 15 is element n2 of 5-point DFT n1; bin k1 of the
-// 3-point DFT over n1 of those DFTs' bin k2 is output (10 k1 + 6 k2)
-// mod 15 (W15^(n k) = W3^(n1 k1) W5^(n2 k2) under these maps).
-template <typename T2> struct Dft<T2, 15> {
+// Prime-factor (Good-Thomas) butterfly of N = N1 N2 with N1, N2 coprime:
+// no inner twiddles.  Input (N2 n1 + N1 n2) mod N is element n2 of the
+// N2-point DFT n1; bin k1 of the N1-point DFT over n1 of those DFTs' bin
+// k2 is output (N2 e1 k1 + N1 e2 k2) mod N, e1 = N2^-1 mod N1 and
+// e2 = N1^-1 mod N2 (then W_N^(n k) = W_N1^(n1 k1) W_N2^(n2 k2)).
+constexpr int inv_mod(int a, int m)
+{
+    for (int x = 1; x < m; ++x)
+        if ((a * x) % m == 1) return x;
+    return 1;
+}
+
+template <typename T2, int N1, int N2> struct DftPfa {
     static __device__ __forceinline__ void run(T2* v, int sign, const T2*)
     {
-        T2 a[3][5];
+        constexpr int N = N1 * N2;
+        constexpr int o1 = N2 * inv_mod(N2 % N1, N1), o2 = N1 * inv_mod(N1 % N2, N2);
+        T2 a[N1][N2];
 #pragma unroll
-        for (int n1 = 0; n1 < 3; ++n1) {
+        for (int n1 = 0; n1 < N1; ++n1) {
 #pragma unroll
-            for (int n2 = 0; n2 < 5; ++n2) a[n1][n2] = v[(5 * n1 + 3 * n2) % 15];
-            Dft<T2, 5>::run(a[n1], sign, nullptr);
+            for (int n2 = 0; n2 < N2; ++n2) a[n1][n2] = v[(N2 * n1 + N1 * n2) % N];
+            Dft<T2, N2>::run(a[n1], sign, nullptr);
         }
 #pragma unroll
-        for (int k2 = 0; k2 < 5; ++k2) {
-            T2 b[3] = {a[0][k2], a[1][k2], a[2][k2]};
-            Dft<T2, 3>::run(b, sign, nullptr);
+        for (int k2 = 0; k2 < N2; ++k2) {
+            T2 b[N1];
 #pragma unroll
-            for (int k1 = 0; k1 < 3; ++k1) v[(10 * k1 + 6 * k2) % 15] = b[k1];
+            for (int n1 = 0; n1 < N1; ++n1) b[n1] = a[n1][k2];
+            Dft<T2, N1>::run(b, sign, nullptr);
+#pragma unroll
+            for (int k1 = 0; k1 < N1; ++k1) v[(o1 * k1 + o2 * k2) % N] = b[k1];
         }
     }
 };
+template <typename T2> struct Dft<T2, 10> : DftPfa<T2, 2, 5> {};
+template <typename T2> struct Dft<T2, 15> : DftPfa<T2, 3, 5> {};
+template <typename T2> struct Dft<T2, 20> : DftPfa<T2, 4, 5> {};
 
 // Generic odd prime radix (7, 11, 13) from a small table of
 // exp(-2 pi i q / R), q < R (forward sign).
@@ -258,7 +273,7 @@ __device__ __forceinline__ void stockham_pass(T2* buf, int n, int Ns, const T2* 
     lds_barrier();
 }
 
-// SMOOTH5 instantiates only the radix 2/3/4/5/8/15 passes (lengths whose
+// SMOOTH5 instantiates only the radix 2/3/4/5/8/10/15/20 passes (lengths whose
 // prime factors are 2, 3 and 5); the generic 7/11/13 passes carry large
 // constant tables that would inflate the register allocation of every
 // kernel that includes them.
@@ -280,7 +295,9 @@ __device__ __forceinline__ void fft_lds(T2* buf, const FftPlan& pl, const T2* __
         case 3: stockham_pass<T2, 3, NT, (NMAX / 3 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
         case 4: stockham_pass<T2, 4, NT, (NMAX / 4 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
         case 5: stockham_pass<T2, 5, NT, (NMAX / 5 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
+        case 10: stockham_pass<T2, 10, NT, (NMAX / 10 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
         case 15: stockham_pass<T2, 15, NT, (NMAX / 15 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
+        case 20: stockham_pass<T2, 20, NT, (NMAX / 20 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
         case 8: stockham_pass<T2, 8, NT, (NMAX / 8 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
         case 7:
             if (!SMOOTH5) stockham_pass<T2, 7, NT, (NMAX / 7 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid);
@@ -305,38 +322,44 @@ __device__ __forceinline__ void stage_twiddles(T2* dst, const T2* __restrict__ s
     for (int i = tid; i < count; i += nthreads) dst[i] = src[i];
 }
 
-// ---- compile-time plans -------------------------------------------------
-// The radix schedule of a length known at compile time, in the order the
-// host plan uses (8s, then a 4 or a 2, then 15, 5, 3, 7, 11, 13), so the
-// twiddle table built for the runtime plan serves both.  With n, each
-// radix and each span constant, a pass needs no plan arrays, no radix
-// switch and no runtime division.
-
+// ---- radix schedule ------------------------------------------------------
+// The one radix rule, used by the host plans and the compile-time plans
+// alike (so a twiddle table built for one serves the other).  The fewest
+// passes come from pairing factors into the largest butterflies that
+// stay cheap in registers: a 3 with a 5 (radix 15), each remaining 5 with
+// one or two of the 2s (radix 10 or 20), the rest of the 2s in 8s plus a
+// 4 or a 2.  The 5s take only as many 2s as save a pass, preferring
+// radix 10 over 20 (fewer registers for the same count).  Order: 8s, the
+// 4 or 2, 20s, 10s, 15s, 5s, 3s, 7s, 11s, 13s.  Returns 0 past the last
+// pass (and for a length with a prime factor above 13).
 constexpr int sf_radix(int n, int p)
 {
-    int m = n, idx = 0;
-    while (m % 8 == 0 && m > 1) {
-        if (idx == p) return 8;
-        ++idx;
-        m /= 8;
+    if (n < 2) return 0;
+    int a = 0, b = 0, c = 0, m = n;
+    while (m % 2 == 0) { ++a; m /= 2; }
+    while (m % 3 == 0) { ++b; m /= 3; }
+    while (m % 5 == 0) { ++c; m /= 5; }
+    int e7 = 0, e11 = 0, e13 = 0;
+    while (m % 7 == 0) { ++e7; m /= 7; }
+    while (m % 11 == 0) { ++e11; m /= 11; }
+    while (m % 13 == 0) { ++e13; m /= 13; }
+    if (m != 1) return 0;
+    const int n15 = b < c ? b : c;
+    const int c1 = c - n15;                       // 5s left after the 15s
+    const int xmax = a < 2 * c1 ? a : 2 * c1;     // 2s the 5s could take
+    const int lp = (a - xmax + 2) / 3;            // passes for the 2s left over
+    const int x = a - 3 * lp > 0 ? a - 3 * lp : 0;   // 2s the 5s do take
+    const int n20 = x > c1 ? x - c1 : 0;
+    const int n10 = x - 2 * n20;
+    const int n5 = c1 - n10 - n20;
+    const int l2 = a - x;
+    const int cnt[11] = {l2 / 3, l2 % 3 == 2 ? 1 : 0, l2 % 3 == 1 ? 1 : 0, n20, n10, n15, n5, b - n15, e7, e11, e13};
+    const int rad[11] = {8, 4, 2, 20, 10, 15, 5, 3, 7, 11, 13};
+    int idx = p;
+    for (int q = 0; q < 11; ++q) {
+        if (idx < cnt[q]) return rad[q];
+        idx -= cnt[q];
     }
-    if (m % 4 == 0) {
-        if (idx == p) return 4;
-        ++idx;
-        m /= 4;
-    }
-    if (m % 2 == 0) {
-        if (idx == p) return 2;
-        ++idx;
-        m /= 2;
-    }
-    const int odd[6] = {15, 5, 3, 7, 11, 13};
-    for (int q = 0; q < 6; ++q)
-        while (m % odd[q] == 0) {
-            if (idx == p) return odd[q];
-            ++idx;
-            m /= odd[q];
-        }
     return 0;
 }
 
