@@ -145,10 +145,10 @@ def calibration_copy(dev, mib=1024):
 
 def hbm_traffic():
     """HBM bytes per channel-tick of the fused pipeline from the
-    committed counter summary (FETCH_SIZE + WRITE_SIZE of one 8-APA
-    chunk's launches, scaled by the calibration copy), as (bytes,
-    channel-ticks), or None when absent."""
-    path = os.path.join(REPO, "profiles", "decon_fused_chunk8apa_hbm_traffic.md")
+    committed counter summary (FETCH_SIZE + WRITE_SIZE of the launches of
+    one 150-APA step run as one plan, scaled by the calibration copy), as
+    (bytes, channel-ticks), or None when absent."""
+    path = os.path.join(REPO, "profiles", "decon_fused_apa150_hbm_traffic.md")
     try:
         vals = {}
         with open(path) as f:
