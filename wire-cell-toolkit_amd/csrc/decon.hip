@@ -207,7 +207,7 @@ __global__ __launch_bounds__(NT, r2c_min_blocks(VH, S5)) void k_row_r2c(const DJ
                     if (in_bad(J, w, 2 * n, false)) v.x = 0.f;
                     if (in_bad(J, w, 2 * n + 1, false)) v.y = 0.f;
                 }
-                buf[pidx(n)] = v;
+                buf[pidx_slot<NT>(pidx(threadIdx.x), i)] = v;
             }
         }
         if (r + gridDim.x < rend) fetch(r + gridDim.x);
@@ -250,6 +250,7 @@ __global__ __launch_bounds__(NT) void k_wire_pass(const DJob* __restrict__ jobs,
     float2* S = J.S;
     stage_twiddles(twl, J.twW, J.planW.twcount, threadIdx.x, NT);
     constexpr int NLD = ((NWS ? NWS : NWMAX) * TK + NT - 1) / NT;   // elements per thread, all loads in flight
+    static_assert(NT % TK == 0 && (NT / TK) % 16 == 0, "wire tile layout");
     {
         float2 t[NLD];
 #pragma unroll
@@ -257,10 +258,12 @@ __global__ __launch_bounds__(NT) void k_wire_pass(const DJob* __restrict__ jobs,
             const int idx = threadIdx.x + q * NT;
             if (idx < nw * TK) t[q] = S[(long)(idx / TK) * J.ldk + k0 + idx % TK];
         }
+        // idx / TK = threadIdx.x / TK + q NT/TK and idx % TK = threadIdx.x % TK
+        const int lbase = (threadIdx.x % TK) * ldc + pidx(threadIdx.x / TK);
 #pragma unroll
         for (int q = 0; q < NLD; ++q) {
             const int idx = threadIdx.x + q * NT;
-            if (idx < nw * TK) buf[(idx % TK) * ldc + pidx(idx / TK)] = t[q];
+            if (idx < nw * TK) buf[lbase + q * ((NT / TK) + (NT / TK) / 16)] = t[q];
         }
     }
     lds_barrier();
@@ -331,7 +334,7 @@ __device__ __forceinline__ void c2r_stage(const DJob& J, const float2* __restric
             float2 zz;
             zz.x = sr - pi;
             zz.y = si + pr;
-            buf[pidx(k)] = zz;
+            buf[pidx_slot<NT>(pidx(threadIdx.x), i)] = zz;
         }
     }
 }
@@ -397,7 +400,7 @@ __global__ __launch_bounds__(NT, 3) void k_row_c2r_roi(const DJob* __restrict__ 
             float2 z;
             z.x = z.y = 0.f;
             if (n < M) {
-                z = buf[pidx(n)];
+                z = buf[pidx_slot<NT>(pidx(threadIdx.x), i)];
                 valid |= 3ull << (2 * i);
             }
             val[2 * i] = z.x;
@@ -464,7 +467,7 @@ __global__ __launch_bounds__(NT, 3) void k_row_c2r_roi(const DJob* __restrict__ 
                 const int n = threadIdx.x + i * NT;
                 float2 z;
                 z.x = z.y = 0.f;
-                if (n < M) z = buf[pidx(n)];
+                if (n < M) z = buf[pidx_slot<NT>(pidx(threadIdx.x), i)];
                 val[2 * i] = z.x;
                 val[2 * i + 1] = z.y;
             }

@@ -39,6 +39,16 @@ struct FftPlan {
 __host__ __device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
 __host__ __device__ __forceinline__ int padded_len(int n) { return n + (n >> 4) + 1; }
 
+// pidx(t + i NT) for NT a multiple of 16 equals pidx(t) + i (NT + NT/16):
+// a row walked with stride NT is one base address plus constant offsets
+// (immediate offsets of the LDS instructions, no per-slot address register).
+template <int NT>
+__host__ __device__ __forceinline__ int pidx_slot(int pt, int i)
+{
+    static_assert(NT % 16 == 0, "stride must be a multiple of the pad period");
+    return pt + i * (NT + NT / 16);
+}
+
 template <typename T> struct C2;
 template <> struct C2<float> { using type = float2; };
 template <> struct C2<double> { using type = double2; };
@@ -229,6 +239,17 @@ __host__ __device__ constexpr int generic_tab_offset(int R)
 constexpr int kGenericTabSize = 7 + 11 + 13;
 
 // ---- one Stockham pass ------------------------------------------------
+
+// LDS index of input r of butterfly j = tid + b NT: pidx(j + r m), written
+// as pidx(tid + r m) plus a constant when NT is a multiple of 16.
+template <int NT>
+__device__ __forceinline__ int pass_read_idx(int tid, int b, int rm)
+{
+    if constexpr (NT % 16 == 0)
+        return pidx_slot<NT>(pidx(tid + rm), b);
+    else
+        return pidx(tid + b * NT + rm);
+}
 // buf: LDS sequence of length n.  The group has NT threads, this thread
 // is tid.  MAXB bounds the butterflies one thread owns in this pass.
 
@@ -243,7 +264,7 @@ __device__ __forceinline__ void stockham_pass(T2* buf, int n, int Ns, const T2* 
         const int j = tid + b * NT;
         if (j < m) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[b][r] = buf[pidx(j + r * m)];
+            for (int r = 0; r < R; ++r) v[b][r] = buf[pass_read_idx<NT>(tid, b, r * m)];
         }
     }
     lds_barrier();
@@ -400,7 +421,7 @@ __device__ __forceinline__ void stockham_pass_c(T2* buf, const T2* __restrict__ 
         const int j = tid + b * NT;
         if (m % NT == 0 || j < m) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[b][r] = buf[pidx(j + r * m)];
+            for (int r = 0; r < R; ++r) v[b][r] = buf[pass_read_idx<NT>(tid, b, r * m)];
         }
     }
     lds_barrier();
