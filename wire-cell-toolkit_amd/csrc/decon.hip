@@ -661,7 +661,7 @@ namespace wct {
 template <typename F>
 static int dispatch_static_nw(int nw, int tk, F&& f)
 {
-    if (tk == 8) {
+    if (tk == 8 || tk == 4 || tk == 2) {
         if (nw == 800) return f(std::integral_constant<int, 800>());
         if (nw == 960) return f(std::integral_constant<int, 960>());
     }
@@ -753,7 +753,17 @@ static int run_wire(wct_decon_plan p, hipStream_t s)
                 return launch_wire<8, 1024, false, 0>(p, g, s);
             });
         else if (p->tk == 4)
-            rc = g.smooth5 ? launch_wire<4, 2048, true, 0>(p, g, s) : launch_wire<4, 2048, false, 0>(p, g, s);
+            rc = dispatch_static_nw(g.len, 4, [&](auto N) -> int {
+                constexpr int NWS = decltype(N)::value;
+                if (NWS > 0 || g.smooth5) return launch_wire<4, 2048, true, NWS>(p, g, s);
+                return launch_wire<4, 2048, false, 0>(p, g, s);
+            });
+        else if (p->tk == 2)
+            rc = dispatch_static_nw(g.len, 2, [&](auto N) -> int {
+                constexpr int NWS = decltype(N)::value;
+                if (NWS > 0 || g.smooth5) return launch_wire<2, 2048, true, NWS>(p, g, s);
+                return launch_wire<2, 2048, false, 0>(p, g, s);
+            });
         else {
             set_error("decon_plan_run: wire tile %d not supported", p->tk);
             rc = WCT_ERR_UNSUPPORTED;
@@ -988,7 +998,7 @@ int wct_decon_kernel_destroy(wct_decon_kernel k)
 // wire-pass tile width.
 static int job_ldk(const wct_decon_job& j) { return (j.kernel->kc + 15) / 16 * 16; }
 
-static int wire_tile_max_nw(int tk) { return tk == 4 ? 2047 : 1023; }
+static int wire_tile_max_nw(int tk) { return tk <= 4 ? 2047 : 1023; }
 
 size_t wct_decon_workspace_size(const wct_decon_job* jobs, int njobs)
 {
@@ -1029,8 +1039,9 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
     int row_pct = 100;
     if (tuning) {
         if (tuning->wire_tile) {
-            WCT_REQUIRE(tuning->wire_tile == 4 || tuning->wire_tile == 8 || tuning->wire_tile == 16,
-                        WCT_ERR_ARG, "decon_plan: wire tile %d not in {4, 8, 16}", tuning->wire_tile);
+            WCT_REQUIRE(tuning->wire_tile == 2 || tuning->wire_tile == 4 || tuning->wire_tile == 8 ||
+                            tuning->wire_tile == 16,
+                        WCT_ERR_ARG, "decon_plan: wire tile %d not in {2, 4, 8, 16}", tuning->wire_tile);
             tk = tuning->wire_tile;
         }
         if (tuning->row_grid_pct) {

@@ -16,7 +16,7 @@ from . import decon as D
 # A half-size persistent row grid measured 50-60 % slower than the full one
 # on the 150-APA workload, so the default space keeps the full grid and
 # spends the candidates on plan size and wire tile instead.
-DEFAULT_SPACE = {"chunk_apas": (8, 16, 32, 64, 150), "wire_tile": (4, 8, 16), "row_grid_pct": (100,)}
+DEFAULT_SPACE = {"chunk_apas": (8, 16, 32, 64, 150), "wire_tile": (2, 4, 8, 16), "row_grid_pct": (100,)}
 
 
 def build_step(make_jobs, nplanes, chunk_apas, wire_tile, row_grid_pct, planes_per_apa=3):
