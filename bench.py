@@ -45,7 +45,9 @@ def parse():
     ap.add_argument("--plain-apas", type=int, default=4,
                     help="APAs timed through the plain per-plane path (0 = skip)")
     ap.add_argument("--no-tune", action="store_true",
-                    help="skip the autotuner (use --chunk, default wire tile and grid)")
+                    help="skip the autotuner (use --chunk and --wire-tile, full row grid)")
+    ap.add_argument("--wire-tile", type=int, default=0,
+                    help="bins per wire-pass tile with --no-tune (0 = the plan's default)")
     ap.add_argument("--eager", action="store_true", help="time plan runs without the hipGraph")
     ap.add_argument("--adjoint-frames", type=int, default=16,
                     help="frames of 800 x 6000 in the timed adjoint gradient step (0 = skip)")
@@ -249,7 +251,7 @@ def main():
         return [decon.make_job(kern[planes[i]], frames[i], outs[i], mask=masks[i], rms=rmss[i],
                                restore=1) for i in idx]
 
-    tuning = {"chunk_apas": args.chunk, "wire_tile": 0, "row_grid_pct": 0, "tuned": False}
+    tuning = {"chunk_apas": args.chunk, "wire_tile": args.wire_tile, "row_grid_pct": 0, "tuned": False}
     if not args.no_tune:
         from wct_amd import tune
         t0 = time.perf_counter()
