@@ -190,8 +190,7 @@ int wct_decon_plan_run(wct_decon_plan plan, void* stream);
 int wct_decon_plan_destroy(wct_decon_plan plan);
 /* Launch-shape choices of a plan (0 = the default for that field):
  *   wire_tile     bins per wire-pass workgroup: 2, 4, 8 or 16 (16 and 8
- *                 need <= 1023 wires, 2 and 4 take up to 2047; default
- *                 8 or 4 by wire count)
+ *                 need <= 1023 wires, 2 and 4 take up to 2047; default 4)
  *   row_grid_pct  persistent grid of the row passes as a percentage of
  *                 the most workgroups the device keeps resident
  * The autotuner (wct_amd.tune) times these per workload. */

@@ -632,7 +632,10 @@ static bool is_smooth5(int n)
     return n == 1;
 }
 
-static int pick_tk(int nw) { return nw + 1 <= 1024 ? 8 : (nw + 1 <= 2048 ? 4 : 2); }
+// Default bins per wire-pass tile: 4 (the autotuner's pick on the
+// 150-APA workload, 21.9 against 22.3 ms per step with 8; tiles of 2, 4
+// and 8 give the same bits).  Planes above 2047 wires are refused.
+static int pick_tk(int) { return 4; }
 
 // Complex slots per thread of the row kernels for rows of up to Mmax
 // complex samples (M = nt/2), at kNT threads per row.
