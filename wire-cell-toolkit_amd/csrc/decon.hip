@@ -165,10 +165,10 @@ inline RowLds row_lds(int Mmax, int twmax, bool stats, int nkeys = 0)
 // Persistent: each block walks rows blockIdx.x, +gridDim.x, ...; the
 // twiddles are read through the caches (LDS holds only the row, which
 // leaves room for four workgroups per CU).
-// Four workgroups per CU for rows of up to 12 slots per thread with the
-// smooth radix set (the register cap then costs a few spills and buys a
-// quarter more resident rows); the larger instances keep every register.
-constexpr int r2c_min_blocks(int VH, bool S5) { return S5 && VH <= 12 ? 4 : 1; }
+// Five workgroups per CU for rows of up to 12 slots per thread with the
+// smooth radix set (a 96-register cap: a few spills for a quarter more
+// resident rows than four); the larger instances keep every register.
+constexpr int r2c_min_blocks(int VH, bool S5) { return S5 && VH <= 12 ? 5 : 1; }
 
 template <int NT, int VH, bool S5>
 __global__ __launch_bounds__(NT, r2c_min_blocks(VH, S5)) void k_row_r2c(const DJob* __restrict__ jobs, int njobs,
