@@ -266,10 +266,14 @@ struct LinBins {
     float inv_w;
 };
 
+// A NaN position (an +inf zero-slot sentinel times inv_w == 0, the bins
+// of a constant row) goes to the overflow bin with the other sentinels;
+// no NaN reaches the float-to-integer conversion.
 __device__ __forceinline__ uint32_t lin_bin(float x, LinBins lb)
 {
     const float t = (x - lb.lo) * lb.inv_w;
-    return t < 0.f ? 0u : (t >= (float)kLinBins ? (uint32_t)kLinBins + 1u : 1u + (uint32_t)t);
+    if (!(t < (float)kLinBins)) return (uint32_t)kLinBins + 1u;
+    return t < 0.f ? 0u : 1u + (uint32_t)t;
 }
 
 // Exact rank-th smallest of seg[0..cnt) (unordered keys in LDS).

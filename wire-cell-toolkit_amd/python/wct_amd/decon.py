@@ -73,20 +73,79 @@ def _ptr(t):
     return None if t is None else t.data_ptr()
 
 
+def _check_tensor(name, t, shape, dtypes, device):
+    """Raises ValueError unless t is a tensor of the given shape (None
+    entries match any extent), one of dtypes, on device, with unit inner
+    stride: the C side trusts the pointers and strides it is given."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if t.dim() != len(shape) or any(w is not None and w != g for w, g in zip(shape, t.shape)):
+        raise ValueError(f"{name}: shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if t.dtype not in dtypes:
+        raise ValueError(f"{name}: dtype {t.dtype}, expected one of {dtypes}")
+    if t.device != device:
+        raise ValueError(f"{name}: on {t.device}, the job's frames are on {device}")
+    if t.dim() and t.stride(-1) != 1:
+        raise ValueError(f"{name}: inner stride {t.stride(-1)}, expected 1")
+    if t.dim() == 2 and t.shape[0] > 1 and t.stride(0) < t.shape[1]:
+        raise ValueError(f"{name}: row stride {t.stride(0)} below the row length {t.shape[1]}")
+
+
+def _check_gpu(dev):
+    if dev.type != "cuda":
+        raise ValueError(f"the job's tensors are on {dev}, expected a GPU device")
+
+
 def make_job(kernel, adc, out, mask=None, rms=None, th_factor=W.TH_FACTOR, restore=1,
              save_mode=0, lf_tight=None, lf_tighter=None, bad=None, adjoint=False, spectrum=None):
     """One wct_decon_job.  adc/out/mask/rms are torch tensors (2D for the
     frames); bad is (ptr, lo, hi) int32 device tensors or None.  With
     spectrum (a complex64 (nw, nt) tensor) the job writes the time
-    spectrum of the deconvolved frame instead (out must be None)."""
+    spectrum of the deconvolved frame instead (out must be None).
+
+    Every tensor is checked against the kernel's (nw, nt) before any
+    launch (ValueError on a mismatch), and the job keeps references to
+    them: a Batch built from it keeps them alive as long as its plan."""
+    import torch
+    nw, nt, kc = kernel.nw, kernel.nt, kernel.kc
+    _check_tensor("adc", adc, (nw, nt), (torch.float32,), adc.device)
+    dev = adc.device
     j = DeconJob()
     j.kernel = kernel.handle
     j.adc = adc.data_ptr()
     j.ld_adc = adc.stride(0)
+    keep = [kernel, adc]
     if spectrum is not None:
+        if out is not None:
+            raise ValueError("a spectrum-mode job has no out frame")
+        _check_tensor("spectrum", spectrum, (nw, nt), (torch.complex64,), dev)
         j.spectrum = spectrum.data_ptr()
         j.ld_spectrum = spectrum.stride(0)
+        keep.append(spectrum)
+        _check_gpu(dev)
+        j._keep = keep
         return j
+    _check_tensor("out", out, (nw, nt), (torch.float32,), dev)
+    keep.append(out)
+    if mask is not None:
+        _check_tensor("mask", mask, (nw, nt), (torch.uint8,), dev)
+        keep.append(mask)
+    if rms is not None:
+        _check_tensor("rms", rms, (nw,), (torch.float32,), dev)
+        keep.append(rms)
+    for name, f in (("lf_tight", lf_tight), ("lf_tighter", lf_tighter)):
+        if f is not None:
+            _check_tensor(name, f, (None,), (torch.float32,), dev)
+            if f.numel() < kc:
+                raise ValueError(f"{name}: {f.numel()} values, the kernel keeps {kc} bins")
+            keep.append(f)
+    if bad is not None:
+        bp, blo, bhi = bad
+        _check_tensor("bad_ptr", bp, (nw + 1,), (torch.int32,), dev)
+        _check_tensor("bad_lo", blo, (None,), (torch.int32,), dev)
+        _check_tensor("bad_hi", bhi, (blo.shape[0],), (torch.int32,), dev)
+        keep.extend(bad)
     j.out = out.data_ptr()
     j.ld_out = out.stride(0)
     j.mask = _ptr(mask)
@@ -100,6 +159,8 @@ def make_job(kernel, adc, out, mask=None, rms=None, th_factor=W.TH_FACTOR, resto
     j.restore_baseline = restore
     j.save_mode = save_mode
     j.adjoint = int(bool(adjoint))
+    _check_gpu(dev)
+    j._keep = keep
     return j
 
 
@@ -109,6 +170,8 @@ class Batch:
 
     def __init__(self, jobs, device=None, workspace=None, wire_tile=0, row_grid_pct=0):
         import torch
+        # the jobs' tensors must outlive the plan (and any graph of it)
+        self._keep = [getattr(j, "_keep", None) for j in jobs]
         self.jobs = (DeconJob * len(jobs))(*jobs)
         self.n = len(jobs)
         self.ws_bytes = lib().wct_decon_workspace_size(self.jobs, self.n)

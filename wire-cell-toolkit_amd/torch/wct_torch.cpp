@@ -70,6 +70,9 @@ class Model {
           m_cut(cut_tol), m_cap(cache)
     {
         if ((int)m_resp.size() != nr * nt) throw std::invalid_argument("wct_torch: response size != nr * nt");
+        // a backward pass holds three kernels at once (value, d/ds_t,
+        // d/ds_w); the eviction below never drops the newest entry
+        if (cache < 3) throw std::invalid_argument("wct_torch: the kernel cache must hold at least 3 kernels");
         check(wct_forward_kernel_create(&m_fwd, m_resp.data(), nr, nw, nt, wire_shift, time_shift,
                                         reinterpret_cast<void*>(stream)),
               "forward kernel");
