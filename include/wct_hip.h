@@ -125,6 +125,54 @@ int wct_decon_kernel_info(wct_decon_kernel k, int* nw, int* nt, int* kc);
 int wct_decon_kernel_download(wct_decon_kernel k, float* host_out);
 int wct_decon_kernel_destroy(wct_decon_kernel k);
 
+/* The kernel of a job whose outputs each apply their whole time filter
+ * (the job's lf_tight for the tight output, wct_decon_output.filter for
+ * the others), so one spectrum serves filters whose ratios are not
+ * bounded (Wiener_tight, Wiener_wide, Gaus_wide; OmnibusSigProc.cxx:
+ * 1053-1380):
+ *   K(m,k) = Fw[m] exp(-2 pi i (m ws/nw + k ts/nt)) / (R(m,k) nw nt)
+ * built in FP64 as above.  envelope (nt host doubles, or NULL) bounds
+ * the outputs' filters: bins k >= kc with
+ * max_m |K(m,k)| envelope[k] <= cut_tol * max are dropped.  r_floor
+ * regularises the inverse: K = 0 where |R| <= r_floor * max|R|
+ * (r_floor = 0: only where R == 0, the reference's NaN/Inf fix-up).  A
+ * response that falls below FP64 rounding at high frequency (the
+ * synthetic Gaussian responses) needs r_floor > 0 for a filter that
+ * does not fall faster than the response (Gaus_wide). */
+int wct_decon_kernel_create_unfiltered(wct_decon_kernel* out, const double* resp, int nr, int nw, int nt,
+                                       const double* wire_filter, const double* envelope, int wire_shift,
+                                       int time_shift, double cut_tol, double r_floor, void* stream);
+
+/* An extra output of a decon job: the same deconvolved spectrum through
+ * another per-bin time filter, as OmnibusSigProc's other decon variants
+ * of a plane (decon_2D_tighterROI :1120-1160, decon_2D_looseROI
+ * :1162-1238, decon_2D_ROI_refine :1053-1075, decon_2D_hits :1316-1347,
+ * decon_2D_charge :1349-1380), then optionally restore_baseline, the
+ * per-wire cal_RMS and save_data's zeroing.
+ *   filter       device, kc floats: per-bin time factor (times the
+ *                kernel's own)
+ *   filter_alt   device, kc floats, or NULL: the factor of the rows
+ *                flagged in alt_rows (decon_2D_looseROI's tight-LF filter
+ *                for channels next to bad or lf_noisy ones, :1216-1231)
+ *   alt_rows     device, nw bytes, or NULL
+ *   out          device, nw x ld_out floats, or NULL
+ *   rms          device, nw floats, or NULL: cal_RMS of this output with
+ *                the job's bad ranges excluded (ROI_formation.cxx:364-389)
+ *   restore_baseline, save_mode: as for the job's own output
+ * (The tighter ROI of the mask is the job's lf_tighter, not an output.) */
+typedef struct {
+    const float* filter;
+    const float* filter_alt;
+    const unsigned char* alt_rows;
+    float* out;
+    int ld_out;
+    float* rms;
+    int restore_baseline;
+    int save_mode;
+} wct_decon_output;
+/* Extra outputs of one job. */
+#define WCT_DECON_MAX_OUTPUTS 6
+
 /* One plane of one frame.  Replaces, per plane, load_data + decon_2D_init
  * + decon_2D_tightROI (+ decon_2D_tighterROI) + restore_baseline +
  * ROI_formation::find_ROI_by_decon_itself's RMS and threshold
@@ -173,6 +221,10 @@ typedef struct {
      * save_mode must be unset.  A plan's jobs all use the mode or none. */
     float* spectrum;
     int ld_spectrum;
+    /* Extra outputs (host array of noutputs entries, copied when the
+     * plan is created; at most WCT_DECON_MAX_OUTPUTS). */
+    const wct_decon_output* outputs;
+    int noutputs;
 } wct_decon_job;
 
 /* Device scratch needed to run the jobs (spectrum buffers). */

@@ -65,7 +65,7 @@ def test_plugin_factories_exported():
 def test_abi_version_and_error_string():
     from wct_amd import capi
     L = capi.lib()
-    assert L.wct_abi_version() == 2
+    assert L.wct_abi_version() == 3
     assert L.wct_dft_supported(6000) == 1 and L.wct_dft_supported(17) == 0
     assert isinstance(L.wct_last_error(), bytes)
 

@@ -54,8 +54,12 @@ __global__ __launch_bounds__(256) void k_row_spectrum(const DJob* __restrict__ j
 // ---- FP64 kernel build ------------------------------------------------
 
 // K(m,k) for k <= nt/2, stored k-major in double with |K| per entry.
+// Ft NULL: no time filter in K (the outputs apply theirs); Fenv (or
+// NULL) then weights |K| for the spectral cut.  |R|^2 <= den_floor gives
+// K = 0 (den_floor 0: only R == 0).
 __global__ void k_build_K(const double2* __restrict__ R, int nw, int nt, int kh,
                           const double* __restrict__ Fw, const double* __restrict__ Ft,
+                          const double* __restrict__ Fenv, double den_floor,
                           int wshift, int tshift, double2* __restrict__ Kd,
                           double* __restrict__ Kabs)
 {
@@ -66,12 +70,12 @@ __global__ void k_build_K(const double2* __restrict__ R, int nw, int nt, int kh,
     const double den = r.x * r.x + r.y * r.y;
     double2 out;
     out.x = out.y = 0.0;
-    if (den != 0.0) {
+    if (den > den_floor) {
         const long qm = ((long)m * wshift) % nw, qk = ((long)k * tshift) % nt;
         const double ang = -2.0 * M_PI * ((double)qm / nw + (double)qk / nt);
         double sn, cs;
         sincos(ang, &sn, &cs);
-        const double g = Fw[m] * Ft[k] / (den * (double)nw * (double)nt);
+        const double g = Fw[m] * (Ft ? Ft[k] : 1.0) / (den * (double)nw * (double)nt);
         // phase / R = phase * conj(R) / |R|^2
         const double pr = cs * r.x + sn * r.y;   // (cs + i sn)(r.x - i r.y)
         const double pi = sn * r.x - cs * r.y;
@@ -79,7 +83,23 @@ __global__ void k_build_K(const double2* __restrict__ R, int nw, int nt, int kh,
         out.y = pi * g;
     }
     Kd[i] = out;
-    Kabs[i] = sqrt(out.x * out.x + out.y * out.y);
+    Kabs[i] = sqrt(out.x * out.x + out.y * out.y) * (Fenv ? fabs(Fenv[k]) : 1.0);
+}
+
+// Per-block maxima of |R|^2 over n entries (grid-stride).
+__global__ void k_den_max(const double2* __restrict__ R, long n, double* __restrict__ bmax)
+{
+    __shared__ double red[kNT];
+    double m = 0;
+    for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT)
+        m = fmax(m, R[i].x * R[i].x + R[i].y * R[i].y);
+    red[threadIdx.x] = m;
+    lds_barrier();
+    for (int s = kNT / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        lds_barrier();
+    }
+    if (threadIdx.x == 0) bmax[blockIdx.x] = red[0];
 }
 
 // Forward response spectrum for the synthetic-frame generator and the
@@ -320,12 +340,16 @@ int wct_overall_response(const double* fr_avg, int nr, int nin, const double* el
     return WCT_OK;
 }
 
+// forward: the response convolution.  Otherwise the decon kernel, with
+// time_filter baked in, or (time_filter NULL) without a time filter and
+// envelope (or NULL) weighting the spectral cut.
 static int kernel_create(wct_decon_kernel* out, const double* resp, int nr, int nw, int nt,
-                         const double* wire_filter, const double* time_filter, int wire_shift,
-                         int time_shift, double cut_tol, bool forward, void* stream)
+                         const double* wire_filter, const double* time_filter, const double* envelope,
+                         int wire_shift, int time_shift, double cut_tol, double r_floor, bool forward,
+                         void* stream)
 {
-    WCT_REQUIRE(out && resp && nr > 0 && nw > 0 && nt > 1 &&
-                    (forward || (wire_filter && time_filter)),
+    WCT_REQUIRE(out && resp && nr > 0 && nw > 0 && nt > 1 && (forward || wire_filter) && r_floor >= 0 &&
+                    r_floor < 1,
                 WCT_ERR_ARG, "kernel_create: bad arguments");
     WCT_REQUIRE(nt % 2 == 0, WCT_ERR_UNSUPPORTED, "odd tick count %d not supported", nt);
     WCT_REQUIRE(fft_supported_length(nw) && fft_supported_length(nt / 2), WCT_ERR_UNSUPPORTED,
@@ -350,10 +374,27 @@ static int kernel_create(wct_decon_kernel* out, const double* resp, int nr, int 
     if (rc) return rc;
     rc = launch_cols_any<double>(R, R, nw, nt, -1, 1.0, s);
     if (rc) return rc;
-    double *Fw = nullptr, *Ft = nullptr, *Kabs = nullptr, *cmax = nullptr;
+    // |R|^2 floor of the regularised inverse: (r_floor max|R|)^2
+    double den_floor = 0.0;
+    if (r_floor > 0 && !forward) {
+        constexpr int nb = 512;
+        double* bmax = nullptr;
+        WCT_HIP_CHECK(hipMallocAsync((void**)&bmax, nb * sizeof(double), s));
+        hipLaunchKernelGGL(k_den_max, dim3(nb), dim3(kNT), 0, s, R, (long)N, bmax);
+        WCT_HIP_CHECK(hipGetLastError());
+        std::vector<double> hb(nb);
+        WCT_HIP_CHECK(hipMemcpyAsync(hb.data(), bmax, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+        WCT_HIP_CHECK(hipFreeAsync(bmax, s));
+        WCT_HIP_CHECK(hipStreamSynchronize(s));
+        double dmax = 0;
+        for (double v : hb) dmax = std::max(dmax, v);
+        den_floor = r_floor * r_floor * dmax;
+    }
+    double *Fw = nullptr, *Ft = nullptr, *Fe = nullptr, *Kabs = nullptr, *cmax = nullptr;
     double2* Kd = nullptr;
     WCT_HIP_CHECK(hipMallocAsync((void**)&Fw, nw * sizeof(double), s));
     WCT_HIP_CHECK(hipMallocAsync((void**)&Ft, nt * sizeof(double), s));
+    WCT_HIP_CHECK(hipMallocAsync((void**)&Fe, nt * sizeof(double), s));
     WCT_HIP_CHECK(hipMallocAsync((void**)&Kd, (size_t)kh * nw * sizeof(double2), s));
     WCT_HIP_CHECK(hipMallocAsync((void**)&Kabs, (size_t)kh * nw * sizeof(double), s));
     WCT_HIP_CHECK(hipMallocAsync((void**)&cmax, kh * sizeof(double), s));
@@ -363,8 +404,12 @@ static int kernel_create(wct_decon_kernel* out, const double* resp, int nr, int 
                            wire_shift, time_shift > 0 ? time_shift : 0, Kd, Kabs);
     } else {
         WCT_HIP_CHECK(hipMemcpyAsync(Fw, wire_filter, nw * sizeof(double), hipMemcpyHostToDevice, s));
-        WCT_HIP_CHECK(hipMemcpyAsync(Ft, time_filter, nt * sizeof(double), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_build_K, dim3((nk + 255) / 256), dim3(256), 0, s, R, nw, nt, kh, Fw, Ft,
+        if (time_filter)
+            WCT_HIP_CHECK(hipMemcpyAsync(Ft, time_filter, nt * sizeof(double), hipMemcpyHostToDevice, s));
+        if (!time_filter && envelope)
+            WCT_HIP_CHECK(hipMemcpyAsync(Fe, envelope, nt * sizeof(double), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_build_K, dim3((nk + 255) / 256), dim3(256), 0, s, R, nw, nt, kh, Fw,
+                           time_filter ? Ft : nullptr, (!time_filter && envelope) ? Fe : nullptr, den_floor,
                            wire_shift, time_shift > 0 ? time_shift : 0, Kd, Kabs);
     }
     WCT_HIP_CHECK(hipGetLastError());
@@ -399,6 +444,7 @@ static int kernel_create(wct_decon_kernel* out, const double* resp, int nr, int 
     WCT_HIP_CHECK(hipFreeAsync(tmp, s));
     WCT_HIP_CHECK(hipFreeAsync(Fw, s));
     WCT_HIP_CHECK(hipFreeAsync(Ft, s));
+    WCT_HIP_CHECK(hipFreeAsync(Fe, s));
     WCT_HIP_CHECK(hipFreeAsync(Kd, s));
     WCT_HIP_CHECK(hipFreeAsync(Kabs, s));
     WCT_HIP_CHECK(hipFreeAsync(cmax, s));
@@ -411,14 +457,23 @@ int wct_decon_kernel_create(wct_decon_kernel* out, const double* resp, int nr, i
                             const double* wire_filter, const double* time_filter, int wire_shift,
                             int time_shift, double cut_tol, void* stream)
 {
-    return kernel_create(out, resp, nr, nw, nt, wire_filter, time_filter, wire_shift, time_shift,
-                         cut_tol, false, stream);
+    WCT_REQUIRE(time_filter, WCT_ERR_ARG, "kernel_create: no time filter");
+    return kernel_create(out, resp, nr, nw, nt, wire_filter, time_filter, nullptr, wire_shift, time_shift,
+                         cut_tol, 0.0, false, stream);
+}
+
+int wct_decon_kernel_create_unfiltered(wct_decon_kernel* out, const double* resp, int nr, int nw, int nt,
+                                       const double* wire_filter, const double* envelope, int wire_shift,
+                                       int time_shift, double cut_tol, double r_floor, void* stream)
+{
+    return kernel_create(out, resp, nr, nw, nt, wire_filter, nullptr, envelope, wire_shift, time_shift,
+                         cut_tol, r_floor, false, stream);
 }
 
 int wct_forward_kernel_create(wct_decon_kernel* out, const double* resp, int nr, int nw, int nt,
                               int wire_shift, int time_shift, void* stream)
 {
-    return kernel_create(out, resp, nr, nw, nt, nullptr, nullptr, wire_shift, time_shift, 0.0,
+    return kernel_create(out, resp, nr, nw, nt, nullptr, nullptr, nullptr, wire_shift, time_shift, 0.0, 0.0,
                          true, stream);
 }
 
@@ -528,9 +583,22 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
                             (!j.mask || (reinterpret_cast<uintptr_t>(j.mask) & 1) == 0),
                         WCT_ERR_ARG, "decon_plan: job %d buffers are misaligned", i);
         }
-        WCT_REQUIRE(!j.adjoint || (!j.bad_ptr && !j.restore_baseline && !j.mask && !j.save_mode),
+        WCT_REQUIRE(!j.adjoint || (!j.bad_ptr && !j.restore_baseline && !j.mask && !j.save_mode &&
+                                   j.noutputs == 0),
                     WCT_ERR_ARG, "decon_plan: adjoint job %d must be linear (no bad ranges, baseline, "
-                    "mask or save_mode)", i);
+                    "mask, save_mode or extra outputs)", i);
+        WCT_REQUIRE(j.noutputs >= 0 && (j.noutputs == 0 || j.outputs) && j.noutputs <= WCT_DECON_MAX_OUTPUTS,
+                    WCT_ERR_ARG, "decon_plan: job %d asks for %d extra outputs (at most %d)", i, j.noutputs,
+                    WCT_DECON_MAX_OUTPUTS);
+        WCT_REQUIRE(!spec || j.noutputs == 0, WCT_ERR_ARG,
+                    "decon_plan: spectrum job %d sets outputs of the decon mode", i);
+        for (int q = 0; q < j.noutputs; ++q) {
+            const wct_decon_output& o = j.outputs[q];
+            WCT_REQUIRE(o.filter && (!o.alt_rows || o.filter_alt) &&
+                            (!o.out || (o.ld_out >= K->nt && o.ld_out % 2 == 0 &&
+                                        (reinterpret_cast<uintptr_t>(o.out) & 7) == 0)),
+                        WCT_ERR_ARG, "decon_plan: job %d output %d has a bad filter or frame", i, q);
+        }
         DJob& d = h[i];
         memset(&d, 0, sizeof(d));
         d.adc = j.adc;
@@ -542,6 +610,18 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
         d.bad_hi = j.bad_hi;
         d.lf1 = j.lf_tight;
         d.lf2 = j.lf_tighter;
+        for (int q = 0; q < j.noutputs; ++q) {
+            const wct_decon_output& o = j.outputs[q];
+            DVar& v = d.var[d.nvar++];
+            v.g = o.filter;
+            v.g_alt = o.filter_alt;
+            v.alt_rows = o.alt_rows;
+            v.out = o.out;
+            v.ld_out = o.ld_out;
+            v.rms = o.rms;
+            v.restore = o.restore_baseline;
+            v.save_mode = o.save_mode;
+        }
         d.K = K->K;
         d.ld_adc = j.ld_adc;
         d.ld_out = j.ld_out;
@@ -613,11 +693,12 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
         hr[i] = h[ord[i]];
         hr[i].row0 = acc;
         if (p->rgroups.empty() || p->rgroups.back().len != hr[i].M)
-            p->rgroups.push_back({hr[i].M, acc, acc, hr[i].planM.twcount, true, false});
+            p->rgroups.push_back({hr[i].M, acc, acc, hr[i].planM.twcount, true, false, false});
         acc += hr[i].nw;
         p->rgroups.back().end = acc;
         if (!is_smooth5(hr[i].M)) p->rgroups.back().smooth5 = false;
         if (hr[i].bad_ptr) p->rgroups.back().bad = true;
+        if (hr[i].nvar) p->rgroups.back().var = true;
     }
     for (int i = 0; i < njobs; ++i) ord[i] = i;
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return h[a].nw < h[b].nw; });
@@ -626,7 +707,7 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
         ht[i] = h[ord[i]];
         ht[i].tile0 = acc;
         if (p->tgroups.empty() || p->tgroups.back().len != ht[i].nw)
-            p->tgroups.push_back({ht[i].nw, acc, acc, ht[i].planW.twcount, true, false});
+            p->tgroups.push_back({ht[i].nw, acc, acc, ht[i].planW.twcount, true, false, false});
         acc += ht[i].ldk / tk;
         p->tgroups.back().end = acc;
         if (!is_smooth5(ht[i].nw)) p->tgroups.back().smooth5 = false;
@@ -789,7 +870,7 @@ int wct_spectrum_filter(const float* spectrum, int ld_spectrum, int nw, int nt, 
         tmp.rows = nw;
         tmp.Mmax = M;
         tmp.twM = d.planM.twcount;
-        tmp.rgroups.push_back({M, 0, (long)nw, d.planM.twcount, is_smooth5(M), false});
+        tmp.rgroups.push_back({M, 0, (long)nw, d.planM.twcount, is_smooth5(M), false, false});
         int dev = 0;
         e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&tmp.ncu, hipDeviceAttributeMultiprocessorCount, dev);

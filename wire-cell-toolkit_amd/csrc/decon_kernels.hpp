@@ -27,6 +27,7 @@ struct wct_decon_group {
     int twcount;      // twiddle entries of the length's plan
     bool smooth5;
     bool bad;         // some job of the group has bad-channel ranges
+    bool var;         // some job of the group has extra outputs (K4)
 };
 
 struct wct_decon_plan_s {
@@ -53,6 +54,20 @@ namespace wct {
 
 constexpr int kNT = 256;
 
+// An extra output of a job (K4): the same spectrum row through another
+// per-bin time filter (rows flagged in alt_rows take g_alt), then
+// optionally restore_baseline, cal_RMS and save_data's zeroing.
+constexpr int kMaxVar = 6;
+struct DVar {
+    const float* g;
+    const float* g_alt;
+    const unsigned char* alt_rows;
+    float* out;
+    float* rms;
+    int ld_out;
+    int restore, save_mode;
+};
+
 // Device-side description of one plane job of a batch.
 struct DJob {
     const float* adc;
@@ -77,6 +92,8 @@ struct DJob {
     int restore, save_mode, adjoint;
     float2* spec;     // spectrum mode: full time spectrum rows (else null)
     int ld_spec;
+    int nvar;         // extra outputs (K4)
+    DVar var[kMaxVar];
     FftPlan planM;
     FftPlan planW;
 };
@@ -459,6 +476,98 @@ __global__ __launch_bounds__(NT, 3) void k_row_c2r_roi(const DJob* __restrict__ 
     }
 }
 
+// ---- K4: the extra outputs of a job (DVar) ----------------------------
+// Per row and per extra output: the kept bins through the output's
+// filter, the inverse real FFT, then restore_baseline / cal_RMS (with the
+// job's bad ranges excluded) / save_data's zeroing as the output asks.
+// A launch of its own after K3 (the spectrum rows are still in the
+// workspace), so the registers of the tight pass are not shared with it;
+// two workgroups per CU leave it every register it asks for.
+template <int NT, int VH, bool S5, bool BAD>
+__global__ __launch_bounds__(NT, 2) void k_row_c2r_var(const DJob* __restrict__ jobs, int njobs,
+                                                    long rbeg, long rend, RowLds lay)
+{
+    constexpr int MMAX = NT * VH;
+    constexpr int VPT = 2 * VH;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* buf = reinterpret_cast<float2*>(smem_raw);
+    RowScratch* sh = reinterpret_cast<RowScratch*>(buf);
+    for (long r = rbeg + blockIdx.x; r < rend; r += gridDim.x) {
+        const DJob& J = jobs[find_job_rows(jobs, njobs, r)];
+        if (J.nvar == 0) continue;   // uniform over the block
+        const int w = (int)(r - J.row0);
+        const int M = J.M;
+        const float2* Y = J.S + (long)w * J.ldk;
+        const bool has_bad = BAD && J.bad_ptr && J.bad_ptr[w + 1] > J.bad_ptr[w];
+        for (int v = 0; v < J.nvar; ++v) {
+            const DVar& V = J.var[v];
+            const float* g = (V.alt_rows && V.alt_rows[w]) ? V.g_alt : V.g;
+            c2r_stage<NT, VH>(J, Y, g, buf);
+            lds_barrier();
+            fft_lds<float2, NT, MMAX, S5>(buf, J.planM, J.twM, +1, threadIdx.x);
+            float val[VPT];
+            uint64_t valid = 0;
+#pragma unroll
+            for (int i = 0; i < VH; ++i) {
+                const int n = threadIdx.x + i * NT;
+                float2 z;
+                z.x = z.y = 0.f;
+                if (n < M) {
+                    z = buf[pidx_slot<NT>(pidx(threadIdx.x), i)];
+                    valid |= 3ull << (2 * i);
+                }
+                val[2 * i] = z.x;
+                val[2 * i + 1] = z.y;
+            }
+            const bool want_rms = V.rms != nullptr;
+            uint64_t pm = valid;
+            if (want_rms && has_bad) {
+#pragma unroll
+                for (int i = 0; i < VPT; ++i) {
+                    const int pos = 2 * (threadIdx.x + (i / 2) * NT) + (i & 1);
+                    if (((valid >> i) & 1ull) && in_bad(J, w, pos, true)) pm &= ~(1ull << i);
+                }
+            }
+            float rms = 0.0f;
+            // the statistics scratch overlays the row buffer
+            lds_barrier();
+            if (V.restore || want_rms) {
+                const bool row_bad = want_rms && has_bad && (__syncthreads_or(pm != valid) != 0);
+                if (!row_bad) {
+                    rms = row_stats_fast<NT, VPT>(val, 2u * (uint32_t)M, V.restore != 0, want_rms, sh);
+                } else if constexpr (BAD) {
+                    if (V.restore) row_restore_baseline<NT, VPT>(val, valid, sh);
+                    rms = row_cal_rms<NT, VPT>(val, pm, sh);
+                }
+            }
+            if (V.out) {
+                float* o = V.out + (long)w * V.ld_out;
+#pragma unroll
+                for (int i = 0; i < VH; ++i) {
+                    const int n = threadIdx.x + i * NT;
+                    if (n < M) {
+                        float a = val[2 * i], b = val[2 * i + 1];
+                        if (V.save_mode) {   // save_data (:383-418)
+                            a = a > 0.f ? a : 0.f;
+                            b = b > 0.f ? b : 0.f;
+                            if (has_bad) {
+                                if (in_bad(J, w, 2 * n, false)) a = 0.f;
+                                if (in_bad(J, w, 2 * n + 1, false)) b = 0.f;
+                            }
+                        }
+                        float2 ab;
+                        ab.x = a;
+                        ab.y = b;
+                        *reinterpret_cast<float2*>(o + 2 * n) = ab;
+                    }
+                }
+            }
+            if (V.rms && threadIdx.x == 0) V.rms[w] = rms;
+            lds_barrier();
+        }
+    }
+}
+
 // Persistent grid: as many blocks as the device keeps resident at once
 // (a larger grid leaves a tail of blocks that start only as others end).
 template <typename K>
@@ -519,6 +628,15 @@ static int launch_rows_c2r(wct_decon_plan p, const wct_decon_group& g, hipStream
     rc = resident_grid(k_row_c2r_roi<kNT, VH, S5, BAD>, kNT, lds, p->ncu, g.end - g.begin, p->row_pct, &grid);
     if (rc) return rc;
     hipLaunchKernelGGL((k_row_c2r_roi<kNT, VH, S5, BAD>), dim3(grid), dim3(kNT), lds, s, p->jobs, p->njobs,
+                       g.begin, g.end, lay);
+    WCT_HIP_CHECK(hipGetLastError());
+    if (!g.var) return WCT_OK;
+    // the jobs' extra outputs, after the tight pass
+    rc = allow_lds(k_row_c2r_var<kNT, VH, S5, BAD>, lds);
+    if (rc) return rc;
+    rc = resident_grid(k_row_c2r_var<kNT, VH, S5, BAD>, kNT, lds, p->ncu, g.end - g.begin, p->row_pct, &grid);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_row_c2r_var<kNT, VH, S5, BAD>), dim3(grid), dim3(kNT), lds, s, p->jobs, p->njobs,
                        g.begin, g.end, lay);
     WCT_HIP_CHECK(hipGetLastError());
     return WCT_OK;

@@ -15,6 +15,19 @@ class DeconTuning(C.Structure):
     _fields_ = [("wire_tile", C.c_int), ("row_grid_pct", C.c_int)]
 
 
+class DeconOutput(C.Structure):
+    _fields_ = [
+        ("filter", C.c_void_p), ("filter_alt", C.c_void_p), ("alt_rows", C.c_void_p),
+        ("out", C.c_void_p), ("ld_out", C.c_int),
+        ("rms", C.c_void_p),
+        ("restore_baseline", C.c_int),
+        ("save_mode", C.c_int),
+    ]
+
+
+MAX_OUTPUTS = 6
+
+
 class DeconJob(C.Structure):
     _fields_ = [
         ("kernel", C.c_void_p),
@@ -29,6 +42,7 @@ class DeconJob(C.Structure):
         ("save_mode", C.c_int),
         ("adjoint", C.c_int),
         ("spectrum", C.c_void_p), ("ld_spectrum", C.c_int),
+        ("outputs", C.POINTER(DeconOutput)), ("noutputs", C.c_int),
     ]
 
 
@@ -66,6 +80,8 @@ _SIGS = {
     "wct_overall_response": (_I, [_V, _I, _I, _V, _I, _D, _D, _D, _D, _D, _I, _V, _V]),
     "wct_decon_kernel_create": (_I, [C.POINTER(C.c_void_p), _V, _I, _I, _I, _V, _V, _I, _I, _D, _V]),
     "wct_forward_kernel_create": (_I, [C.POINTER(C.c_void_p), _V, _I, _I, _I, _I, _I, _V]),
+    "wct_decon_kernel_create_unfiltered": (_I, [C.POINTER(C.c_void_p), _V, _I, _I, _I, _V, _V, _I, _I, _D,
+                                                _D, _V]),
     "wct_decon_kernel_info": (_I, [_V, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "wct_decon_kernel_download": (_I, [_V, _V]),
     "wct_decon_kernel_destroy": (_I, [_V]),

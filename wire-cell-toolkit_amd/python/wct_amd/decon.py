@@ -10,7 +10,7 @@ import ctypes as C
 
 import numpy as np
 
-from .capi import DeconJob, DeconTuning, check, lib
+from .capi import MAX_OUTPUTS, DeconJob, DeconOutput, DeconTuning, check, lib
 from . import workload as W
 
 
@@ -34,19 +34,34 @@ def overall_response(spec, nt=None, stream=None):
 
 
 class PlaneKernel:
-    """Device-resident K(m, k) for one plane type (see include/wct_hip.h)."""
+    """Device-resident K(m, k) for one plane type (see include/wct_hip.h).
 
-    def __init__(self, spec, cut_tol=1e-7, time_filter=None, resp=None, stream=None):
+    unfiltered=True builds K without a time filter
+    (wct_decon_kernel_create_unfiltered): each output of a job then
+    applies its whole time filter, envelope (nt values, default the
+    spec's time filter) sets the spectral cut and r_floor regularises
+    the inverse."""
+
+    def __init__(self, spec, cut_tol=1e-7, time_filter=None, resp=None, stream=None, unfiltered=False,
+                 envelope=None, r_floor=0.0):
         self.spec = spec
+        self.unfiltered = bool(unfiltered)
         self.resp = overall_response(spec) if resp is None else np.ascontiguousarray(resp, np.float64)
         tf = spec.time_filter if time_filter is None else time_filter
         tf = np.ascontiguousarray(tf, np.float64)
         wf = np.ascontiguousarray(spec.wire_filter, np.float64)
         h = C.c_void_p()
-        check(lib().wct_decon_kernel_create(
-            C.byref(h), self.resp.ctypes.data, self.resp.shape[0], spec.nw, spec.nt,
-            wf.ctypes.data, tf.ctypes.data, spec.wire_shift, spec.time_shift, cut_tol,
-            _stream() if stream is None else stream))
+        st = _stream() if stream is None else stream
+        if self.unfiltered:
+            env = tf if envelope is None else np.ascontiguousarray(envelope, np.float64)
+            check(lib().wct_decon_kernel_create_unfiltered(
+                C.byref(h), self.resp.ctypes.data, self.resp.shape[0], spec.nw, spec.nt,
+                wf.ctypes.data, env.ctypes.data, spec.wire_shift, spec.time_shift, cut_tol,
+                float(r_floor), st))
+        else:
+            check(lib().wct_decon_kernel_create(
+                C.byref(h), self.resp.ctypes.data, self.resp.shape[0], spec.nw, spec.nt,
+                wf.ctypes.data, tf.ctypes.data, spec.wire_shift, spec.time_shift, cut_tol, st))
         self.handle = h
         nw, nt, kc = C.c_int(), C.c_int(), C.c_int()
         check(lib().wct_decon_kernel_info(h, C.byref(nw), C.byref(nt), C.byref(kc)))
@@ -92,13 +107,26 @@ def _check_tensor(name, t, shape, dtypes, device):
         raise ValueError(f"{name}: row stride {t.stride(0)} below the row length {t.shape[1]}")
 
 
+class Output:
+    """One extra output of a job (wct_decon_output): the job's spectrum
+    through filter (device float32, >= kc values; rows flagged in
+    alt_rows, a device uint8 (nw,) tensor, take filter_alt), optionally
+    baselined, with its per-wire cal_RMS and save_data's zeroing."""
+
+    def __init__(self, filter, out=None, rms=None, restore=1, save_mode=0, filter_alt=None, alt_rows=None):
+        self.filter, self.out, self.rms = filter, out, rms
+        self.restore, self.save_mode = int(restore), int(save_mode)
+        self.filter_alt, self.alt_rows = filter_alt, alt_rows
+
+
 def _check_gpu(dev):
     if dev.type != "cuda":
         raise ValueError(f"the job's tensors are on {dev}, expected a GPU device")
 
 
 def make_job(kernel, adc, out, mask=None, rms=None, th_factor=W.TH_FACTOR, restore=1,
-             save_mode=0, lf_tight=None, lf_tighter=None, bad=None, adjoint=False, spectrum=None):
+             save_mode=0, lf_tight=None, lf_tighter=None, bad=None, adjoint=False, spectrum=None,
+             outputs=()):
     """One wct_decon_job.  adc/out/mask/rms are torch tensors (2D for the
     frames); bad is (ptr, lo, hi) int32 device tensors or None.  With
     spectrum (a complex64 (nw, nt) tensor) the job writes the time
@@ -159,6 +187,42 @@ def make_job(kernel, adc, out, mask=None, rms=None, th_factor=W.TH_FACTOR, resto
     j.restore_baseline = restore
     j.save_mode = save_mode
     j.adjoint = int(bool(adjoint))
+    outputs = list(outputs)
+    if len(outputs) > MAX_OUTPUTS:
+        raise ValueError(f"{len(outputs)} outputs exceed {MAX_OUTPUTS}")
+    if outputs:
+        arr = (DeconOutput * len(outputs))()
+        for q, o in enumerate(outputs):
+            _check_tensor(f"outputs[{q}].filter", o.filter, (None,), (torch.float32,), dev)
+            if o.filter.numel() < kc:
+                raise ValueError(f"outputs[{q}].filter: {o.filter.numel()} values, the kernel keeps {kc} bins")
+            d = arr[q]
+            d.filter = o.filter.data_ptr()
+            keep.append(o.filter)
+            if o.filter_alt is not None or o.alt_rows is not None:
+                if o.filter_alt is None or o.alt_rows is None:
+                    raise ValueError(f"outputs[{q}]: filter_alt and alt_rows go together")
+                _check_tensor(f"outputs[{q}].filter_alt", o.filter_alt, (None,), (torch.float32,), dev)
+                if o.filter_alt.numel() < kc:
+                    raise ValueError(f"outputs[{q}].filter_alt: {o.filter_alt.numel()} values, the kernel "
+                                     f"keeps {kc} bins")
+                _check_tensor(f"outputs[{q}].alt_rows", o.alt_rows, (nw,), (torch.uint8,), dev)
+                d.filter_alt = o.filter_alt.data_ptr()
+                d.alt_rows = o.alt_rows.data_ptr()
+                keep += [o.filter_alt, o.alt_rows]
+            if o.out is not None:
+                _check_tensor(f"outputs[{q}].out", o.out, (nw, nt), (torch.float32,), dev)
+                d.out = o.out.data_ptr()
+                d.ld_out = o.out.stride(0)
+                keep.append(o.out)
+            if o.rms is not None:
+                _check_tensor(f"outputs[{q}].rms", o.rms, (nw,), (torch.float32,), dev)
+                d.rms = o.rms.data_ptr()
+                keep.append(o.rms)
+            d.restore_baseline, d.save_mode = o.restore, o.save_mode
+        j.outputs = arr
+        j.noutputs = len(outputs)
+        keep.append(arr)
     _check_gpu(dev)
     j._keep = keep
     return j

@@ -43,6 +43,11 @@ WIENER = dict(max_freq=1 * MHZ, sigma=0.1 * MHZ, power=3.35324, flag=True)
 WIRE = dict(max_freq=1.0, sigma=0.75, power=2, flag=False)
 LF_TIGHT = dict(max_freq=1 * MHZ, tau=0.014 * MHZ)
 LF_TIGHTER = dict(max_freq=1 * MHZ, tau=0.06 * MHZ)
+# the other decon variants' filters, PDSP values
+# (cfg/pgrapher/experiment/pdsp/sp-filters.jsonnet:79-114)
+LF_LOOSE = dict(max_freq=1 * MHZ, tau=0.002 * MHZ)
+WIENER_WIDE = dict(max_freq=1 * MHZ, sigma=0.175722 * MHZ, power=4.37928, flag=True)
+GAUS_WIDE = dict(max_freq=1 * MHZ, sigma=0.12 * MHZ, power=2, flag=True)
 TH_FACTOR = 3.0
 NOISE_ADC = 1.5
 Q_PER_TICK = 5000.0
