@@ -143,7 +143,9 @@ def test_sigproc_eos_and_empty_frame():
 @pytest.mark.parametrize("extra,msg", [({"per_chan_resp": "PerChannelResponse"}, "per_chan_resp"),
                                        ({"fft_flag": 1}, "fft_flag"),
                                        ({"process_planes": [0, 3]}, "process_planes"),
-                                       ({"anode": "AnodeChannels:missing_planes"}, "plane")])
+                                       ({"anode": "AnodeChannels:missing_planes"}, "plane"),
+                                       ({"use_multi_plane_protection": True}, "refinement"),
+                                       ({"response_floor": 1.5}, "response_floor")])
 def test_sigproc_configuration_errors(extra, msg):
     from wct_amd import components as K
     from wct_amd.capi import WctError

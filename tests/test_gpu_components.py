@@ -7,10 +7,14 @@ calls from several threads at once.
 
 HipOmnibusSigProc (IFrameFilter): a three-plane anode frame pushed
 through the component must give exactly what the C ABI gives on the
-same data (dense wiener traces, per-wire RMS summary), save_data's
+same data (dense tight_lf traces with the per-wire RMS summary, and the
+debug mode's tight_lf / loose_lf / decon_charge outputs), save_data's
 post-processing (negatives and bad ranges zeroed; debug mode keeps
-negatives; sparse runs), and tight ROI lists equal to the oracle's
-run/extend/connect rules applied to the device mask.
+negatives; sparse runs), tight ROI lists equal to the oracle's
+run/extend/connect rules applied to the device mask, the wiener tag only
+on request (the ROI refinement that makes the reference's wiener traces
+is not run), and no traces without ROI refinement or the debug mode, as
+the reference saves none then.
 """
 import json
 import os
@@ -135,6 +139,8 @@ def _sigproc_config(tmpdir, extra=None):
         {"type": "HfFilter", "name": "Wire_col", "data": wire},
         {"type": "LfFilter", "name": "ROI_tight_lf", "data": W.LF_TIGHT},
         {"type": "LfFilter", "name": "ROI_tighter_lf", "data": W.LF_TIGHTER},
+        {"type": "LfFilter", "name": "ROI_loose_lf", "data": W.LF_LOOSE},
+        {"type": "HfFilter", "name": "Gaus_wide", "data": W.GAUS_WIDE},
         {"type": "HipDFT", "name": ""},
         {"type": "HipOmnibusSigProc", "name": "osp", "data": data},
     ]
@@ -152,15 +158,37 @@ def _adc_planes(seed):
     return out
 
 
-def _expected(adc, bad, th):
+def _filters(kind, nt, debug=False, tight=True):
+    """The component's output filters of one plane, in FP64: the base
+    (tight, or all ones for an induction plane without ROI refinement),
+    the tighter one, and in debug mode the loose and charge ones."""
+    from wct_amd import host, workload as W
+    wiener = host.hf_filter(nt, **W.WIENER)
+    f = {}
+    if tight:
+        f["base"] = wiener * host.lf_filter(nt, **W.LF_TIGHT) if kind == "ind" else wiener
+        if kind == "ind":
+            f["tighter"] = wiener * host.lf_filter(nt, **W.LF_TIGHTER)
+    else:
+        f["base"] = np.ones(nt)
+    if debug:
+        if kind == "ind":
+            f["loose"] = wiener * host.lf_filter(nt, **W.LF_LOOSE)
+        f["charge"] = host.hf_filter(nt, **W.GAUS_WIDE)
+    return f
+
+
+def _expected(adc, bad, th, debug=False, refine=True):
     """The same planes through the C ABI (workload's spec, padded to
-    plane 0's 21 response regions as the reference sizes them)."""
+    plane 0's 21 response regions as the reference sizes them), with the
+    component's unfiltered kernels and output filters.  Returns per
+    plane (base, mask, rms, {extra name: frame})."""
     import torch
     from wct_amd import decon as D
     from wct_amd import workload as W
     dev = torch.device("cuda:0")
     kinds = ("ind", "ind", "col")
-    specs, kernels, jobs, keep = [], [], [], []
+    jobs, keep, res = [], [], []
     nfine_rows = len(W.fr_paths("ind")[0])
     for ip, kind in enumerate(kinds):
         sp = W.plane_spec(kind, NW[ip], NT, fold_lf=False)
@@ -168,11 +196,15 @@ def _expected(adc, bad, th):
         rows[:sp.fr_rows.shape[0]] = sp.fr_rows
         sp.fr_rows = rows
         sp.wire_shift = (nfine_rows - 1) // 2
-        pk = D.PlaneKernel(sp)
+        tight = kind == "col" or refine
+        flt = _filters(kind, NT, debug, tight)
+        env = np.max(np.abs(np.stack(list(flt.values()))), axis=0)
+        pk = D.PlaneKernel(sp, unfiltered=True, envelope=env)
+        g = {k: torch.from_numpy(v[:pk.kc].astype(np.float32)).to(dev) for k, v in flt.items()}
         a = torch.from_numpy(adc[ip]).to(dev)
         y = torch.empty_like(a)
-        m = torch.empty(a.shape, dtype=torch.uint8, device=dev)
-        r = torch.empty(NW[ip], dtype=torch.float32, device=dev)
+        m = torch.empty(a.shape, dtype=torch.uint8, device=dev) if tight else None
+        r = torch.empty(NW[ip], dtype=torch.float32, device=dev) if tight else None
         ptr, lo, hi = [0], [], []
         for w in range(NW[ip]):
             for (b0, b1) in bad.get((ip, w), []):
@@ -180,20 +212,21 @@ def _expected(adc, bad, th):
                 hi.append(b1)
             ptr.append(len(lo))
         badt = tuple(torch.tensor(v if v else [0], dtype=torch.int32, device=dev) for v in (ptr, lo, hi))
-        lt = lth = None
-        if kind == "ind":
-            lt = torch.from_numpy(sp.lf_tight[:pk.kc].astype(np.float32)).to(dev)
-            lth = torch.from_numpy(sp.lf_tighter[:pk.kc].astype(np.float32)).to(dev)
-        jobs.append(D.make_job(pk, a, y, mask=m, rms=r, th_factor=th[ip], restore=1, save_mode=1,
-                               lf_tight=lt, lf_tighter=lth, bad=badt))
-        keep += [pk, a, y, m, r, badt, lt, lth]
-        specs.append((y, m, r))
+        extra = {k: torch.empty_like(a) for k in ("loose", "charge") if k in flt}
+        outs = [D.Output(g[k], out=extra[k], restore=1 if (k == "loose" or kind == "col") else 0,
+                         save_mode=0 if debug else 1) for k in extra]
+        jobs.append(D.make_job(pk, a, y, mask=m, rms=r, th_factor=th[ip], restore=1 if tight else 0,
+                               save_mode=0 if debug else 1, lf_tight=g["base"], lf_tighter=g.get("tighter"),
+                               bad=badt, outputs=outs))
+        keep += [pk, a, y, m, r, badt, g, extra]
+        res.append((y, m, r, extra))
     b = D.Batch(jobs, device=dev)
     b.run()
     torch.cuda.synchronize()
-    res = [(y.cpu().numpy(), m.cpu().numpy(), r.cpu().numpy()) for (y, m, r) in specs]
+    out = [(y.cpu().numpy(), None if m is None else m.cpu().numpy(), None if r is None else r.cpu().numpy(),
+            {k: v.cpu().numpy() for k, v in extra.items()}) for (y, m, r, extra) in res]
     b.close()
-    return res
+    return out
 
 
 def _frame(K, adc, bad):
@@ -218,15 +251,15 @@ def test_sigproc_matches_c_abi(K, tmp_path):
     info = out.info()
     assert info["ident"] == 17 and info["time"] == 3.0 and info["tick"] == 500.0
     assert out.frame_tags() == ["sigproc"]
-    assert out.trace_tags() == ["wiener"]
+    assert out.trace_tags() == ["tight_lf"]     # no refined wiener traces to publish
     tr = out.traces()
     assert len(tr) == sum(NW)
-    idx = out.tagged_traces("wiener")
+    idx = out.tagged_traces("tight_lf")
     assert list(idx) == list(range(sum(NW)))
-    summ = out.trace_summary("wiener")
+    summ = out.trace_summary("tight_lf")
     k = 0
     for ip in range(3):
-        y, m, r = exp[ip]
+        y, m, r, _ = exp[ip]
         for w in range(NW[ip]):
             ch, tb, q = tr[k]
             assert ch == CH0[ip] + w and tb == 0
@@ -241,7 +274,7 @@ def test_sigproc_matches_c_abi(K, tmp_path):
     # tight ROIs: the oracle's rules over the device mask
     rois = out.masks("tight_roi")
     for ip in range(3):
-        y, m, r = exp[ip]
+        y, m, r, _ = exp[ip]
         ref = O.roi_lists(m, pad=5, asy=0.1)
         for w in range(NW[ip]):
             assert rois.get(CH0[ip] + w, []) == [(a, b + 1) for (a, b) in ref[w]]
@@ -251,14 +284,79 @@ def test_sigproc_matches_c_abi(K, tmp_path):
     np.testing.assert_array_equal(out2.trace(3)[2], tr[3][2])
 
 
+def test_sigproc_tight_as_wiener(K, tmp_path):
+    """The wiener tag on the tight decon only when asked for."""
+    K.configure(_sigproc_config(tmp_path, {"tight_as_wiener": True}))
+    adc = _adc_planes(3)
+    out = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, {}))
+    assert sorted(out.trace_tags()) == ["tight_lf", "wiener"]
+    assert list(out.tagged_traces("wiener")) == list(out.tagged_traces("tight_lf"))
+    np.testing.assert_array_equal(out.trace_summary("wiener"), out.trace_summary("tight_lf"))
+
+
+@pytest.mark.parametrize("refine", [True, False])
+def test_sigproc_debug_outputs(K, tmp_path, refine):
+    """use_roi_debug_mode: tight_lf (the unfiltered decon_2D_init frame on
+    induction planes without refinement), loose_lf (the collection plane
+    repeats its tight decon) and decon_charge, each plane in order,
+    negatives kept and bad ranges zeroed, equal to the C ABI's outputs."""
+    print(f"sigproc debug outputs, refinement {refine}", flush=True)
+    K.configure(_sigproc_config(tmp_path, {"use_roi_debug_mode": True, "use_roi_refinement": refine}))
+    adc = _adc_planes(6)
+    bad = {(0, 5): [(100, 300)], (2, 8): [(500, 510)]}
+    out = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
+    exp = _expected(adc, bad, (3.0, 3.0, 5.0), debug=True, refine=refine)
+    assert sorted(out.trace_tags()) == ["decon_charge", "loose_lf", "tight_lf"]
+    tr = out.traces()
+    assert len(tr) == 3 * sum(NW)
+
+    def zero_bad(ip, frame):
+        f = frame.copy()
+        for (ipp, w), ranges in bad.items():
+            if ipp == ip:
+                for (b0, b1) in ranges:
+                    f[w, b0:b1] = 0
+        return f
+
+    for tag, key in (("tight_lf", "base"), ("loose_lf", "loose"), ("decon_charge", "charge")):
+        idx = list(out.tagged_traces(tag))
+        assert len(idx) == sum(NW)
+        k = 0
+        for ip in range(3):
+            y, m, r, extra = exp[ip]
+            want = y if (key == "base" or (key == "loose" and ip == 2)) else extra[key]
+            want = zero_bad(ip, want)
+            for w in range(NW[ip]):
+                ch, tb, q = tr[idx[k]]
+                assert ch == CH0[ip] + w
+                np.testing.assert_array_equal(q, want[w])
+                k += 1
+    assert any(np.any(q < 0) for _, _, q in tr)
+    if refine:
+        assert out.masks("tight_roi")
+    else:     # no tight ROIs on the induction planes without refinement
+        assert all(CH0[2] <= ch < CH0[2] + NW[2] for ch in out.masks("tight_roi"))
+
+
+def test_sigproc_without_refinement_saves_nothing(K, tmp_path):
+    K.configure(_sigproc_config(tmp_path, {"use_roi_refinement": False}))
+    out = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, _adc_planes(3), {(0, 1): [(0, 10)]}))
+    assert out.info()["ntraces"] == 0 and out.trace_tags() == []
+    assert out.frame_tags() == ["sigproc"]
+    assert out.masks("bad")[CH0[0] + 1] == [(0, 10)]
+
+
 def test_sigproc_debug_and_sparse(K, tmp_path):
     print("sigproc debug + sparse", flush=True)
-    K.configure(_sigproc_config(tmp_path, {"use_roi_debug_mode": True, "process_planes": [2]}))
+    # no decon_charge output: its filter would widen the kernel's kept
+    # bins, and the dense tight decon is compared bit for bit below
+    K.configure(_sigproc_config(tmp_path, {"use_roi_debug_mode": True, "process_planes": [2],
+                                           "decon_charge_tag": ""}))
     adc = _adc_planes(4)
     bad = {(2, 3): [(10, 20)]}
     out = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
-    assert sorted(out.trace_tags()) == ["tight_lf", "wiener"]
-    tr = out.traces()
+    assert sorted(out.trace_tags()) == ["loose_lf", "tight_lf"]
+    tr = [out.trace(i) for i in out.tagged_traces("tight_lf")]
     assert len(tr) == NW[2] and tr[0][0] == CH0[2]
     assert any(np.any(q < 0) for _, _, q in tr)         # debug keeps negatives
     assert not np.any(tr[3][2][10:20])
@@ -268,7 +366,7 @@ def test_sigproc_debug_and_sparse(K, tmp_path):
     K.configure(_sigproc_config(tmp_path, {"sparse": True, "process_planes": [2]}))
     out = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
     rebuilt = {}
-    summ = out.trace_summary("wiener")
+    summ = out.trace_summary("tight_lf")
     assert len(summ) == out.info()["ntraces"]
     for ch, tb, q in out.traces():
         assert q.size > 0 and np.all(q > 0)
@@ -280,7 +378,7 @@ def test_sigproc_debug_and_sparse(K, tmp_path):
 def test_sigproc_between_frame_files(K, tmp_path):
     """FrameFileSource -> HipOmnibusSigProc -> FrameFileSink: an ADC
     archive in the reference's layout (int16 frame, column by column, a
-    bad-channel mask) gives an output archive whose wiener frame, channels
+    bad-channel mask) gives an output archive whose tight_lf frame, channels
     and RMS summary equal the component's in-memory output."""
     import io
     import tarfile
@@ -304,7 +402,7 @@ def test_sigproc_between_frame_files(K, tmp_path):
             tf.addfile(ti, b)
     cfg = _sigproc_config(tmp_path) + [
         {"type": "FrameFileSource", "name": "in", "data": {"inname": str(src), "tags": ["orig"]}},
-        {"type": "FrameFileSink", "name": "out", "data": {"outname": str(dst), "tags": ["wiener"]}}]
+        {"type": "FrameFileSink", "name": "out", "data": {"outname": str(dst), "tags": ["tight_lf"]}}]
     K.configure(cfg)
     f = K.frame_source("FrameFileSource:in")
     out = K.frame_filter("HipOmnibusSigProc:osp", f)
@@ -318,13 +416,13 @@ def test_sigproc_between_frame_files(K, tmp_path):
     with tarfile.open(dst) as tf:
         for m in tf.getmembers():
             got[m.name] = np.load(io.BytesIO(tf.extractfile(m).read()))
-    assert sorted(got) == sorted(["channels_wiener_17.npy", "chanmask_bad_17.npy", "chanmask_tight_roi_17.npy",
-                                  "frame_wiener_17.npy", "summary_wiener_17.npy", "tickinfo_wiener_17.npy"])
-    np.testing.assert_array_equal(got["frame_wiener_17.npy"], want)
-    np.testing.assert_array_equal(got["channels_wiener_17.npy"], chans)
-    np.testing.assert_array_equal(got["summary_wiener_17.npy"], direct.trace_summary("wiener"))
-    assert list(got["tickinfo_wiener_17.npy"]) == [3.0, 500.0, 0.0]
-    assert not np.any(got["frame_wiener_17.npy"][NW[0] + 4, 200:260])
+    assert sorted(got) == sorted(["channels_tight_lf_17.npy", "chanmask_bad_17.npy", "chanmask_tight_roi_17.npy",
+                                  "frame_tight_lf_17.npy", "summary_tight_lf_17.npy", "tickinfo_tight_lf_17.npy"])
+    np.testing.assert_array_equal(got["frame_tight_lf_17.npy"], want)
+    np.testing.assert_array_equal(got["channels_tight_lf_17.npy"], chans)
+    np.testing.assert_array_equal(got["summary_tight_lf_17.npy"], direct.trace_summary("tight_lf"))
+    assert list(got["tickinfo_tight_lf_17.npy"]) == [3.0, 500.0, 0.0]
+    assert not np.any(got["frame_tight_lf_17.npy"][NW[0] + 4, 200:260])
     rois = direct.masks("tight_roi")
     flat = sorted((c, a, b) for c, rl in rois.items() for (a, b) in rl)
     assert sorted(map(tuple, got["chanmask_tight_roi_17.npy"].tolist())) == flat and flat

@@ -74,6 +74,10 @@ class LfFilter : public IFilterWaveform, public IConfigurable {
         auto d = lf_filter_waveform(nbins, m_max_freq, m_tau);
         return Waveform::realseq_t(d.begin(), d.end());
     }
+    std::vector<double> filter_waveform_f64(int nbins) const
+    {
+        return lf_filter_waveform(nbins, m_max_freq, m_tau);
+    }
 
   private:
     double m_max_freq{1 * units::megahertz};

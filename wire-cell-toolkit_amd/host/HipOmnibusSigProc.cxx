@@ -5,20 +5,28 @@
 // Per frame (operator(), :1382-1702): EOS and empty frames are forwarded
 // as in the reference; channel masks are re-indexed per plane (:1402-1416);
 // every processed plane is loaded (load_data, :304-352), deconvolved
-// (decon_2D_init, :944-1051), filtered with the tight time filter
-// (decon_2D_tightROI, :1077-1117, and decon_2D_tighterROI for the
-// induction planes, :1120-1160), baseline-restored (:867-906), and
-// thresholded (ROI_formation::find_ROI_by_decon_itself with cal_RMS,
-// sigproc/src/ROI_formation.cxx:364-531).  All planes of a frame run as
-// one batched launch per kernel stage.  The output follows save_data
-// (:371-472): traces tagged "wiener" carry the tight-stage decon
-// (negatives and bad ranges zeroed unless use_roi_debug_mode) with the
-// per-wire RMS as trace summary; use_roi_debug_mode adds the same traces
-// under "tight_lf"; the tight ROIs (extended and connected) are added to
-// the output channel-mask map under "tight_roi_mask" as half-open tick
-// ranges.  The later ROI refinement stages (loose ROIs, break/shrink/
-// extend, hits and charge decons under "wiener"/"gauss") are not part of
-// this component.
+// (decon_2D_init, :944-1051) and filtered into the outputs the
+// configuration asks for, all planes of a frame in one batched launch per
+// kernel stage:
+//  - the tight decon (decon_2D_tightROI, :1077-1117, and for the mask the
+//    tighter one, :1120-1160), baseline-restored (:867-906) and
+//    thresholded (find_ROI_by_decon_itself with cal_RMS,
+//    sigproc/src/ROI_formation.cxx:364-531), for the collection plane and,
+//    with use_roi_refinement, the induction planes (:1452-1467);
+//  - with use_roi_debug_mode, the reference's debug outputs: tight_lf
+//    (:1471-1474; an induction plane without refinement saves the
+//    unfiltered decon_2D_init frame), loose_lf (decon_2D_looseROI_
+//    debug_mode, :1478-1497) and decon_charge (decon_2D_charge, :1349-1380,
+//    :1508-1511,1590-1601).
+// The output follows save_data (:371-472).  The ROI refinement that makes
+// the reference's "wiener" and "gauss" traces (:1518-1620) is not part of
+// this component, so neither tag is produced by default: without the debug
+// mode the tight decon (negatives and bad ranges zeroed) is published
+// under tight_lf_tag with the per-wire RMS as summary, and the key
+// "tight_as_wiener" also tags it wiener_tag (with the RMS summary, as the
+// reference's wiener traces carry).  The tight ROIs (extended and
+// connected) are added to the output channel-mask map under
+// "tight_roi_mask" as half-open tick ranges.
 //
 // The response spectra are built once per (nticks, tick) on the device
 // in FP64 and reused for later frames of the same shape (the reference
@@ -80,12 +88,17 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
         cfg["gauss_tag"] = m_gauss_tag;
         cfg["frame_tag"] = m_frame_tag;
         cfg["tight_lf_tag"] = m_tight_lf_tag;
+        cfg["loose_lf_tag"] = m_loose_lf_tag;
+        cfg["decon_charge_tag"] = m_decon_charge_tag;
         cfg["use_roi_debug_mode"] = m_use_roi_debug_mode;
+        cfg["use_multi_plane_protection"] = false;
         cfg["use_roi_refinement"] = m_use_roi_refinement;
         cfg["sparse"] = m_sparse;
         // Keys of this component only.
         cfg["tight_roi_mask"] = m_tight_roi_mask;
+        cfg["tight_as_wiener"] = m_tight_as_wiener;
         cfg["spectral_cut"] = m_spectral_cut;
+        cfg["response_floor"] = m_response_floor;
         cfg["device"] = m_device;
         return cfg;
     }
@@ -122,8 +135,17 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
         m_gauss_tag = get(config, "gauss_tag", m_gauss_tag);
         m_frame_tag = get(config, "frame_tag", m_frame_tag);
         m_tight_lf_tag = get(config, "tight_lf_tag", m_tight_lf_tag);
+        m_loose_lf_tag = get(config, "loose_lf_tag", m_loose_lf_tag);
+        m_decon_charge_tag = get(config, "decon_charge_tag", m_decon_charge_tag);
         m_use_roi_debug_mode = get(config, "use_roi_debug_mode", m_use_roi_debug_mode);
         m_use_roi_refinement = get(config, "use_roi_refinement", m_use_roi_refinement);
+        if (get(config, "use_multi_plane_protection", false))
+            throw ConfigError("HipOmnibusSigProc: use_multi_plane_protection needs the ROI refinement, which "
+                              "this component does not run");
+        m_tight_as_wiener = get(config, "tight_as_wiener", m_tight_as_wiener);
+        m_response_floor = get(config, "response_floor", m_response_floor);
+        if (m_response_floor < 0 || m_response_floor >= 1)
+            throw ConfigError("HipOmnibusSigProc: response_floor must be in [0, 1)");
         m_sparse = get(config, "sparse", m_sparse);
         m_tight_roi_mask = get(config, "tight_roi_mask", m_tight_roi_mask);
         m_spectral_cut = get(config, "spectral_cut", m_spectral_cut);
@@ -168,6 +190,16 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
             throw SigProcError("HipOmnibusSigProc: nticks " + std::to_string(nticks) +
                                " has a prime factor above 13");
         prepare(nticks, in->tick());
+        if (m_adc_floats == 0) {
+            // without ROI refinement or the debug mode the reference saves
+            // no traces (:1452-1514): an empty frame with the input masks
+            auto sframe = std::make_shared<Aux::SimpleFrame>(in->ident(), in->time(),
+                                                             std::make_shared<ITrace::vector>(), in->tick(),
+                                                             in->masks());
+            sframe->tag_frame(m_frame_tag);
+            out = sframe;
+            return true;
+        }
 
         // Bad ranges per plane, indexed by wire (:1402-1416).
         std::vector<std::vector<Waveform::BinRange>> bad[3];
@@ -254,29 +286,31 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
         hcheck(wct_memcpy_dtoh_async(m_h_out.p, m_d_out.p, m_adc_floats * sizeof(float), m_stream), "download");
         hcheck(wct_memcpy_dtoh_async(m_h_mask.p, m_d_mask.p, m_adc_floats, m_stream), "download");
         hcheck(wct_memcpy_dtoh_async(m_h_rms.p, m_d_rms.p, m_rms_floats * sizeof(float), m_stream), "download");
+        for (int k = 0; k < kNExtra; ++k)
+            if (m_any_extra[k])
+                hcheck(wct_memcpy_dtoh_async(m_h_extra[k].p, m_d_extra[k].p, m_adc_floats * sizeof(float),
+                                             m_stream),
+                       "download");
         hcheck(wct_stream_sync(m_stream), "sync");
 
         // save_data (:371-472) per processed plane, in plane order.
         auto itraces = std::make_shared<ITrace::vector>();
-        IFrame::trace_list_t wiener_traces;
-        IFrame::trace_summary_t thresholds;
-        Waveform::ChannelMaskMap outmasks = inmasks;
-        const float* hout = (const float*)m_h_out.p;
         const float* hrms = (const float*)m_h_rms.p;
         const uint8_t* hmask = (const uint8_t*)m_h_mask.p;
-        for (int ip = 0; ip < 3; ++ip) {
+        // One plane of one decon output as traces (dense, or sparse
+        // positive runs); the device already applied save_data's zeroing
+        // in the nominal mode, the debug mode keeps negatives and zeroes
+        // only the bad ranges.
+        auto save = [&](const float* frame, int ip, IFrame::trace_list_t& idx, IFrame::trace_summary_t* thr) {
             const Plane& pl = m_planes[ip];
-            if (!pl.active) continue;
             for (int w = 0; w < pl.nw; ++w) {
-                const float* row = hout + pl.off + (size_t)w * nticks;
+                const float* row = frame + pl.off + (size_t)w * nticks;
                 ITrace::ChargeSequence charge(row, row + nticks);
                 if (m_use_roi_debug_mode) {
-                    // the device left negatives in; bad ranges are zeroed
-                    // in both modes
                     for (auto& r : bad[ip][w])
                         for (int k = std::max(0, r.first); k < std::min(nticks, r.second); ++k) charge[k] = 0;
                 }
-                const double thresh = hrms[pl.rms_off + w];
+                const double thresh = pl.tight ? hrms[pl.rms_off + w] : 0.0;
                 const int ident = m_channels[ip][w];
                 if (m_sparse) {
                     int i = 0;
@@ -285,19 +319,37 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
                         if (i >= nticks) break;
                         int e = i;
                         while (e < nticks && charge[e] != 0) ++e;
-                        wiener_traces.push_back(itraces->size());
+                        idx.push_back(itraces->size());
                         itraces->push_back(std::make_shared<Aux::SimpleTrace>(
                             ident, i, ITrace::ChargeSequence(charge.begin() + i, charge.begin() + e)));
-                        thresholds.push_back(thresh);
+                        if (thr) thr->push_back(thresh);
                         i = e;
                     }
                 } else {
-                    wiener_traces.push_back(itraces->size());
+                    idx.push_back(itraces->size());
                     itraces->push_back(std::make_shared<Aux::SimpleTrace>(ident, 0, std::move(charge)));
-                    thresholds.push_back(thresh);
+                    if (thr) thr->push_back(thresh);
                 }
             }
-            if (!m_tight_roi_mask.empty()) {
+        };
+        const bool debug = m_use_roi_debug_mode;
+        const bool publish = !debug && m_use_roi_refinement;   // the tight decon in place of wiener
+        IFrame::trace_list_t tight_traces, loose_traces, charge_traces;
+        IFrame::trace_summary_t thresholds;
+        Waveform::ChannelMaskMap outmasks = inmasks;
+        const float* hout = (const float*)m_h_out.p;
+        for (int ip = 0; ip < 3; ++ip) {
+            const Plane& pl = m_planes[ip];
+            if (!pl.active) continue;
+            if ((publish && pl.tight) || (debug && !m_tight_lf_tag.empty()))
+                save(hout, ip, tight_traces, publish ? &thresholds : nullptr);
+            if (debug && !m_loose_lf_tag.empty()) {
+                // the collection plane has no loose decon: the reference
+                // saves its tight one under loose_lf (:1503-1507)
+                const float* src = pl.has_extra[kLooseDbg] ? (const float*)m_h_extra[kLooseDbg].p : hout;
+                save(src, ip, loose_traces, nullptr);
+            }
+            if (pl.tight && !m_tight_roi_mask.empty()) {
                 auto rois = tight_rois(hmask + pl.off, pl.nw, nticks, nticks, m_pad, m_asy);
                 auto& cm = outmasks[m_tight_roi_mask];
                 for (int w = 0; w < pl.nw; ++w) {
@@ -307,23 +359,41 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
                 }
             }
         }
+        for (int ip = 0; ip < 3; ++ip) {
+            const Plane& pl = m_planes[ip];
+            if (pl.active && pl.has_extra[kCharge]) save((const float*)m_h_extra[kCharge].p, ip, charge_traces, nullptr);
+        }
 
         auto sframe = std::make_shared<Aux::SimpleFrame>(in->ident(), in->time(), itraces, in->tick(), outmasks);
         sframe->tag_frame(m_frame_tag);
-        if (!m_wiener_tag.empty()) sframe->tag_traces(m_wiener_tag, wiener_traces, thresholds);
-        if (m_use_roi_debug_mode && !m_tight_lf_tag.empty()) sframe->tag_traces(m_tight_lf_tag, wiener_traces);
+        if (publish) {
+            if (!m_tight_lf_tag.empty()) sframe->tag_traces(m_tight_lf_tag, tight_traces, thresholds);
+            if (m_tight_as_wiener && !m_wiener_tag.empty()) sframe->tag_traces(m_wiener_tag, tight_traces, thresholds);
+        }
+        if (debug) {
+            if (!m_tight_lf_tag.empty()) sframe->tag_traces(m_tight_lf_tag, tight_traces);
+            if (!m_loose_lf_tag.empty()) sframe->tag_traces(m_loose_lf_tag, loose_traces);
+            if (!m_decon_charge_tag.empty()) sframe->tag_traces(m_decon_charge_tag, charge_traces);
+        }
         out = sframe;
         return true;
     }
 
   private:
+    // Extra outputs of a plane's job (wct_decon_output), in job order.
+    enum Extra { kLooseDbg = 0, kCharge = 1, kNExtra = 2 };
+
     struct Plane {
         bool active{false};
         int nw{0};
         size_t off{0};       // first sample of this plane in the frame buffers
         size_t rms_off{0};
         wct_decon_kernel kernel{nullptr};
-        DevBuf lf_tight, lf_tighter;
+        bool tight{true};    // the job's own output is the tight decon (else the unfiltered frame)
+        DevBuf f_base, f_tighter, f_extra[kNExtra];
+        bool has_tighter{false};
+        bool has_extra[kNExtra]{false, false};
+        int extra_restore[kNExtra]{0, 0};
         float th_factor{3};
     };
 
@@ -334,12 +404,20 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
         for (auto& pl : m_planes) {
             if (pl.kernel) wct_decon_kernel_destroy(pl.kernel);
             pl.kernel = nullptr;
-            pl.lf_tight.release();
-            pl.lf_tighter.release();
+            pl.f_base.release();
+            pl.f_tighter.release();
+            for (auto& b : pl.f_extra) b.release();
+            pl.has_tighter = false;
+            for (auto& h : pl.has_extra) h = false;
             pl.active = false;
         }
         for (DevBuf* b : {&m_d_adc, &m_d_out, &m_d_mask, &m_d_rms, &m_d_bad, &m_d_ws}) b->release();
         for (HostBuf* b : {&m_h_adc, &m_h_out, &m_h_mask, &m_h_rms}) b->release();
+        for (int k = 0; k < kNExtra; ++k) {
+            m_d_extra[k].release();
+            m_h_extra[k].release();
+            m_any_extra[k] = false;
+        }
         if (m_stream) wct_stream_destroy(m_stream);
         m_stream = nullptr;
         m_built_nticks = -1;
@@ -379,12 +457,18 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
         const int wire_shift = (fine_nwires - 1) / 2;
 
         static const char* wiener_names[3] = {"Wiener_tight_U", "Wiener_tight_V", "Wiener_tight_W"};
+        const bool refine = m_use_roi_refinement, debug = m_use_roi_debug_mode;
         size_t off = 0, rms_off = 0;
         for (int ip : m_process_planes) {
             Plane& pl = m_planes[ip];
             if (pl.active) continue;   // listed twice
             pl.nw = (int)m_channels[ip].size();
             if (pl.nw == 0) continue;
+            const bool ind = ip < 2;
+            // the reference saves nothing for a plane unless it forms
+            // tight ROIs or runs the debug outputs (:1452-1514)
+            pl.tight = !ind || refine;
+            if (!pl.tight && !debug) continue;
             const auto& rows = avg[ip].rows;
             if ((int)rows.size() > fine_nwires)
                 throw SigProcError("HipOmnibusSigProc: plane " + std::to_string(ip) +
@@ -399,24 +483,58 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
                                         m_inter_gain, m_ADC_mV, m_fine_time_offset, nticks, resp.data(),
                                         m_stream),
                    "response");
-            auto wire_f = filter_f64("HfFilter", ip < 2 ? "Wire_ind" : "Wire_col", pl.nw);
-            auto time_f = filter_f64("HfFilter", wiener_names[ip], nticks);
-            hcheck(wct_decon_kernel_create(&pl.kernel, resp.data(), fine_nwires, pl.nw, nticks, wire_f.data(),
-                                           time_f.data(), wire_shift, time_shift, m_spectral_cut, m_stream),
+            // The time filters of this plane's outputs, in FP64: the
+            // products the reference forms in float (:1082-1102,
+            // :1166-1213, :1355-1364).
+            const auto wiener = filter_f64("HfFilter", wiener_names[ip], nticks);
+            auto lf = [&](const char* name) { return filter_f64("LfFilter", name, nticks); };
+            auto times = [](std::vector<double> a, const std::vector<double>& b) {
+                for (size_t i = 0; i < a.size(); ++i) a[i] *= b[i];
+                return a;
+            };
+            std::vector<double> f_base, f_tighter, f_extra[kNExtra];
+            if (pl.tight) {
+                f_base = ind ? times(wiener, lf("ROI_tight_lf")) : wiener;
+                if (ind) {
+                    f_tighter = times(wiener, lf("ROI_tighter_lf"));
+                    pl.has_tighter = true;
+                }
+            } else {
+                f_base.assign(nticks, 1.0);   // decon_2D_init's frame: no time filter
+            }
+            if (debug && ind) {
+                f_extra[kLooseDbg] = times(wiener, lf("ROI_loose_lf"));
+                pl.has_extra[kLooseDbg] = true;
+                pl.extra_restore[kLooseDbg] = 1;
+            }
+            if (debug && !m_decon_charge_tag.empty()) {
+                f_extra[kCharge] = filter_f64("HfFilter", "Gaus_wide", nticks);
+                pl.has_extra[kCharge] = true;
+                pl.extra_restore[kCharge] = ind ? 0 : 1;   // :1377-1379
+            }
+            std::vector<double> env(f_base);
+            for (const auto* f : {&f_tighter, &f_extra[0], &f_extra[1]})
+                for (size_t i = 0; i < f->size(); ++i) env[i] = std::max(env[i], std::fabs((*f)[i]));
+            auto wire_f = filter_f64("HfFilter", ind ? "Wire_ind" : "Wire_col", pl.nw);
+            hcheck(wct_decon_kernel_create_unfiltered(&pl.kernel, resp.data(), fine_nwires, pl.nw, nticks,
+                                                      wire_f.data(), env.data(), wire_shift, time_shift,
+                                                      m_spectral_cut, m_response_floor, m_stream),
                    "kernel");
             int knw = 0, knt = 0, kc = 0;
             hcheck(wct_decon_kernel_info(pl.kernel, &knw, &knt, &kc), "kernel");
-            if (ip < 2) {
-                // tight and tighter outputs of the induction planes differ
-                // by their low-frequency filter (:1083-1097, :1126-1140)
-                auto lt = Factory::find<IFilterWaveform>("LfFilter", "ROI_tight_lf")->filter_waveform(nticks);
-                auto ltt = Factory::find<IFilterWaveform>("LfFilter", "ROI_tighter_lf")->filter_waveform(nticks);
-                pl.lf_tight.ensure(sizeof(float) * kc);
-                pl.lf_tighter.ensure(sizeof(float) * kc);
-                hcheck(wct_memcpy_htod(pl.lf_tight.p, lt.data(), sizeof(float) * kc, m_stream), "upload");
-                hcheck(wct_memcpy_htod(pl.lf_tighter.p, ltt.data(), sizeof(float) * kc, m_stream), "upload");
-            }
-            pl.th_factor = ip < 2 ? m_th_factor_ind : m_th_factor_col;
+            auto upload = [&](DevBuf& dst, const std::vector<double>& f) {
+                std::vector<float> h(f.begin(), f.begin() + kc);
+                dst.ensure(sizeof(float) * kc);
+                hcheck(wct_memcpy_htod(dst.p, h.data(), sizeof(float) * kc, m_stream), "upload");
+            };
+            upload(pl.f_base, f_base);
+            if (pl.has_tighter) upload(pl.f_tighter, f_tighter);
+            for (int k = 0; k < kNExtra; ++k)
+                if (pl.has_extra[k]) {
+                    upload(pl.f_extra[k], f_extra[k]);
+                    m_any_extra[k] = true;
+                }
+            pl.th_factor = ind ? m_th_factor_ind : m_th_factor_col;
             pl.off = off;
             pl.rms_off = rms_off;
             off += (size_t)pl.nw * nticks;
@@ -434,6 +552,11 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
         m_h_out.ensure(sizeof(float) * std::max<size_t>(off, 1));
         m_h_mask.ensure(std::max<size_t>(off, 1));
         m_h_rms.ensure(sizeof(float) * std::max<size_t>(rms_off, 1));
+        for (int k = 0; k < kNExtra; ++k)
+            if (m_any_extra[k]) {
+                m_d_extra[k].ensure(sizeof(float) * std::max<size_t>(off, 1));
+                m_h_extra[k].ensure(sizeof(float) * std::max<size_t>(off, 1));
+            }
         m_built_nticks = nticks;
         m_built_period = period;
     }
@@ -442,6 +565,7 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
     {
         auto f = Factory::find<IFilterWaveform>(type, name);
         if (auto hf = std::dynamic_pointer_cast<HfFilter>(f)) return hf->filter_waveform_f64(n);
+        if (auto lf = std::dynamic_pointer_cast<LfFilter>(f)) return lf->filter_waveform_f64(n);
         auto v = f->filter_waveform(n);
         return std::vector<double>(v.begin(), v.end());
     }
@@ -469,11 +593,30 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
             j.bad_ptr = dbad + ptr_off[ip];
             j.bad_lo = dbad + lo_off[ip];
             j.bad_hi = dbad + hi_off[ip];
-            j.lf_tight = (const float*)pl.lf_tight.p;
-            j.lf_tighter = (const float*)pl.lf_tighter.p;
+            j.lf_tight = (const float*)pl.f_base.p;
+            j.lf_tighter = pl.has_tighter ? (const float*)pl.f_tighter.p : nullptr;
             j.th_factor = pl.th_factor;
-            j.restore_baseline = 1;
+            j.restore_baseline = pl.tight ? 1 : 0;
             j.save_mode = m_use_roi_debug_mode ? 0 : 1;
+            if (!pl.tight) {   // the unfiltered frame: no threshold, no ROIs
+                j.mask = nullptr;
+                j.rms = nullptr;
+            }
+            std::vector<wct_decon_output>& outs = m_outputs[ip];
+            outs.clear();
+            for (int k = 0; k < kNExtra; ++k) {
+                if (!pl.has_extra[k]) continue;
+                wct_decon_output o;
+                std::memset(&o, 0, sizeof(o));
+                o.filter = (const float*)pl.f_extra[k].p;
+                o.out = (float*)m_d_extra[k].p + pl.off;
+                o.ld_out = m_nticks;
+                o.restore_baseline = pl.extra_restore[k];
+                o.save_mode = m_use_roi_debug_mode ? 0 : 1;
+                outs.push_back(o);
+            }
+            j.outputs = outs.empty() ? nullptr : outs.data();
+            j.noutputs = (int)outs.size();
             jobs.push_back(j);
         }
         const size_t ws = wct_decon_workspace_size(jobs.data(), (int)jobs.size());
@@ -503,11 +646,15 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
     std::string m_gauss_tag{"gauss"};
     std::string m_frame_tag{"sigproc"};
     std::string m_tight_lf_tag{"tight_lf"};
+    std::string m_loose_lf_tag{"loose_lf"};
+    std::string m_decon_charge_tag{"decon_charge"};
     bool m_use_roi_debug_mode{false};
     bool m_use_roi_refinement{true};
     bool m_sparse{false};
     std::string m_tight_roi_mask{"tight_roi"};
+    bool m_tight_as_wiener{false};
     double m_spectral_cut{1e-7};
+    double m_response_floor{0.0};
     int m_device{0};
 
     std::vector<int> m_channels[3];
@@ -518,6 +665,10 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
     wct_decon_plan m_plan{nullptr};
     DevBuf m_d_adc, m_d_out, m_d_mask, m_d_rms, m_d_bad, m_d_ws;
     HostBuf m_h_adc, m_h_out, m_h_mask, m_h_rms;
+    DevBuf m_d_extra[kNExtra];
+    HostBuf m_h_extra[kNExtra];
+    bool m_any_extra[kNExtra]{false, false};
+    std::vector<wct_decon_output> m_outputs[3];
     size_t m_adc_floats{0}, m_rms_floats{0};
     int m_nticks{0};
     int m_built_nticks{-1};
