@@ -48,6 +48,9 @@ def parse():
                     help="skip the autotuner (use --chunk and --wire-tile, full row grid)")
     ap.add_argument("--wire-tile", type=int, default=0,
                     help="bins per wire-pass tile with --no-tune (0 = the plan's default)")
+    ap.add_argument("--row-plan", type=int, nargs=3, default=(0, 0, 0), metavar=("PLAN", "R2C", "C2R"),
+                    help="with --no-tune: row plan (0 runtime, 1 compile-time) and the r2c / c2r "
+                         "workgroups per CU of the compile-time plan")
     ap.add_argument("--eager", action="store_true", help="time plan runs without the hipGraph")
     ap.add_argument("--adjoint-frames", type=int, default=16,
                     help="frames of 800 x 6000 in the timed adjoint gradient step (0 = skip)")
@@ -251,19 +254,21 @@ def main():
         return [decon.make_job(kern[planes[i]], frames[i], outs[i], mask=masks[i], rms=rmss[i],
                                restore=1) for i in idx]
 
-    tuning = {"chunk_apas": args.chunk, "wire_tile": args.wire_tile, "row_grid_pct": 0, "tuned": False}
+    tuning = {"chunk_apas": args.chunk, "wire_tile": args.wire_tile, "row_grid_pct": 0,
+              "row_plan": list(args.row_plan), "tuned": False}
     if not args.no_tune:
         from wct_amd import tune
         t0 = time.perf_counter()
         best, res = tune.tune(make_jobs, len(planes), log=log)
-        tuning = {"chunk_apas": best[0], "wire_tile": best[1], "row_grid_pct": best[2], "tuned": True,
+        tuning = {"chunk_apas": best[0], "wire_tile": best[1], "row_grid_pct": best[2],
+                  "row_plan": list(best[3:]), "tuned": True,
                   "candidates": len(res), "tuning_s": round(time.perf_counter() - t0, 2),
                   "best_ms": round(res[best], 3),
                   "worst_ms": round(max(v for v in res.values() if isinstance(v, float)), 3)}
         log(f"bench: tuned {tuning}")
     from wct_amd import tune
     batches = tune.build_step(make_jobs, len(planes), tuning["chunk_apas"], tuning["wire_tile"],
-                              tuning["row_grid_pct"])
+                              tuning["row_grid_pct"], *tuning["row_plan"])
     graph = None if args.eager else decon.Graph(batches)
     ch_ticks = sum(nw * nt for _, nw, nt in planes)
     log(f"bench: {ch_ticks / 1e9:.3f} G channel-ticks per step, {len(batches)} chunks of "

@@ -245,10 +245,21 @@ int wct_decon_plan_destroy(wct_decon_plan plan);
  *                 need <= 1023 wires, 2 and 4 take up to 2047; default 4)
  *   row_grid_pct  persistent grid of the row passes as a percentage of
  *                 the most workgroups the device keeps resident
+ *   row_plan      1: rows of 3000 complex samples (6000 ticks) use
+ *                 transforms compiled for that length (radix 20, 10, 15
+ *                 with constant spans and twiddle offsets) instead of the
+ *                 runtime radix plan; 0: the runtime plan
+ *   r2c_blocks    with row_plan 1, workgroups per CU of the forward row
+ *                 pass: 2, 3 (default) or 4
+ *   c2r_blocks    with row_plan 1, workgroups per CU of the inverse row
+ *                 pass: 2 or 3 (default)
  * The autotuner (wct_amd.tune) times these per workload. */
 typedef struct {
     int wire_tile;
     int row_grid_pct;
+    int row_plan;
+    int r2c_blocks;
+    int c2r_blocks;
 } wct_decon_tuning;
 int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, int njobs,
                                 void* workspace, size_t ws_bytes, const wct_decon_tuning* tuning);

@@ -543,7 +543,7 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
         nw_all = std::max(nw_all, jobs[i].kernel->nw);
     }
     int Mmax = 0, tk = pick_tk(nw_all), nwmax = 0, twM = 0, twW = 0;
-    int row_pct = 100;
+    int row_pct = 100, row_plan = 0, r2c_blocks = 0, c2r_blocks = 0;
     if (tuning) {
         if (tuning->wire_tile) {
             WCT_REQUIRE(tuning->wire_tile == 2 || tuning->wire_tile == 4 || tuning->wire_tile == 8 ||
@@ -556,6 +556,19 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
                         "decon_plan: row grid percent %d not in [10, 100]", tuning->row_grid_pct);
             row_pct = tuning->row_grid_pct;
         }
+        WCT_REQUIRE(tuning->row_plan == 0 || tuning->row_plan == 1, WCT_ERR_ARG,
+                    "decon_plan: row plan %d not in {0, 1}", tuning->row_plan);
+        WCT_REQUIRE(tuning->r2c_blocks == 0 || (tuning->row_plan == 1 && tuning->r2c_blocks >= 2 &&
+                                                tuning->r2c_blocks <= 4),
+                    WCT_ERR_ARG, "decon_plan: r2c blocks %d not in {2, 3, 4} with the compile-time row plan",
+                    tuning->r2c_blocks);
+        WCT_REQUIRE(tuning->c2r_blocks == 0 || (tuning->row_plan == 1 && tuning->c2r_blocks >= 2 &&
+                                                tuning->c2r_blocks <= 3),
+                    WCT_ERR_ARG, "decon_plan: c2r blocks %d not in {2, 3} with the compile-time row plan",
+                    tuning->c2r_blocks);
+        row_plan = tuning->row_plan;
+        r2c_blocks = tuning->r2c_blocks;
+        c2r_blocks = tuning->c2r_blocks;
     }
     for (int i = 0; i < njobs; ++i) {
         const wct_decon_job& j = jobs[i];
@@ -681,6 +694,9 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
     p->nwmax = nwmax;
     p->tk = tk;
     p->row_pct = row_pct;
+    p->row_plan = row_plan;
+    p->r2c_blocks = r2c_blocks;
+    p->c2r_blocks = c2r_blocks;
     // two orderings of the same jobs: by row length for the row passes,
     // by wire count for the wire pass, each with its own cumulative row
     // and tile starts and its groups of equal length

@@ -40,6 +40,9 @@ struct wct_decon_plan_s {
     int twM = 0, twW = 0;        // largest twiddle tables (complex entries)
     int ncu = 0;                 // compute units of the device
     int row_pct = 100;           // persistent row grids: percent of the resident maximum
+    int row_plan = 0;            // 1: compile-time row plans where compiled (kStaticRowLen)
+    int r2c_blocks = 0;          // workgroups per CU of a compile-time-plan r2c pass (2, 3, 4)
+    int c2r_blocks = 0;          // workgroups per CU of a compile-time-plan c2r pass (2, 3)
     bool smooth5 = true;         // every length is 2^a 3^b 5^c
     bool spectrum = false;       // the third stage writes time spectra
 };
@@ -177,8 +180,11 @@ inline RowLds row_lds(int Mmax, int twmax, bool stats, int nkeys = 0)
 // resident rows than four); the larger instances keep every register.
 constexpr int r2c_min_blocks(int VH, bool S5) { return S5 && VH <= 12 ? 5 : 1; }
 
-template <int NT, int VH, bool S5>
-__global__ __launch_bounds__(NT, r2c_min_blocks(VH, S5)) void k_row_r2c(const DJob* __restrict__ jobs, int njobs,
+// MS > 0: a compile-time plan for rows of exactly MS complex samples
+// (fft_static: constant spans and twiddle offsets, no per-pass dispatch),
+// at BLK workgroups per CU.
+template <int NT, int VH, bool S5, int MS = 0, int BLK = r2c_min_blocks(VH, S5)>
+__global__ __launch_bounds__(NT, BLK) void k_row_r2c(const DJob* __restrict__ jobs, int njobs,
                                                 long rbeg, long rend, RowLds lay)
 {
     constexpr int MMAX = NT * VH;
@@ -219,7 +225,10 @@ __global__ __launch_bounds__(NT, r2c_min_blocks(VH, S5)) void k_row_r2c(const DJ
         }
         if (r + gridDim.x < rend) fetch(r + gridDim.x);
         lds_barrier();
-        fft_lds<float2, NT, MMAX, S5>(buf, J.planM, J.twM, -1, threadIdx.x);
+        if constexpr (MS > 0)
+            fft_static<float2, NT, MS>(buf, J.twM, -1, threadIdx.x);
+        else
+            fft_lds<float2, NT, MMAX, S5>(buf, J.planM, J.twM, -1, threadIdx.x);
         float2* dst = J.S + (long)w * J.ldk;
         for (int k = threadIdx.x; k < J.ldk; k += NT) {
             float2 X;
@@ -350,8 +359,8 @@ __device__ __forceinline__ void c2r_stage(const DJob& J, const float2* __restric
 // BAD: the group has bad-channel ranges (rows with excluded samples take
 // the general statistics path); without, that path is not compiled in,
 // which keeps the kernel within the registers of three workgroups per CU.
-template <int NT, int VH, bool S5, bool BAD>
-__global__ __launch_bounds__(NT, 3) void k_row_c2r_roi(const DJob* __restrict__ jobs, int njobs,
+template <int NT, int VH, bool S5, bool BAD, int MS = 0, int BLK = 3>
+__global__ __launch_bounds__(NT, BLK) void k_row_c2r_roi(const DJob* __restrict__ jobs, int njobs,
                                                     long rbeg, long rend, RowLds lay)
 {
     constexpr int MMAX = NT * VH;
@@ -369,7 +378,13 @@ __global__ __launch_bounds__(NT, 3) void k_row_c2r_roi(const DJob* __restrict__ 
 
         c2r_stage<NT, VH>(J, Y, J.lf1, buf);
         lds_barrier();
-        fft_lds<float2, NT, MMAX, S5>(buf, J.planM, twl, +1, threadIdx.x);
+        if constexpr (MS > 0)
+            fft_static<float2, NT, MS>(buf, twl, +1, threadIdx.x);
+        else
+            if constexpr (MS > 0)
+                fft_static<float2, NT, MS>(buf, twl, +1, threadIdx.x);
+            else
+                fft_lds<float2, NT, MMAX, S5>(buf, J.planM, twl, +1, threadIdx.x);
 
         float val[VPT];
         uint64_t valid = 0;
@@ -440,7 +455,13 @@ __global__ __launch_bounds__(NT, 3) void k_row_c2r_roi(const DJob* __restrict__ 
             lds_barrier();
             c2r_stage<NT, VH>(J, Y, J.lf2, buf);
             lds_barrier();
-            fft_lds<float2, NT, MMAX, S5>(buf, J.planM, twl, +1, threadIdx.x);
+            if constexpr (MS > 0)
+            fft_static<float2, NT, MS>(buf, twl, +1, threadIdx.x);
+        else
+            if constexpr (MS > 0)
+                fft_static<float2, NT, MS>(buf, twl, +1, threadIdx.x);
+            else
+                fft_lds<float2, NT, MMAX, S5>(buf, J.planM, twl, +1, threadIdx.x);
 #pragma unroll
             for (int i = 0; i < VH; ++i) {
                 const int n = threadIdx.x + i * NT;
@@ -601,18 +622,57 @@ static int dispatch_static_nw(int nw, int tk, F&& f)
     return f(std::integral_constant<int, 0>());
 }
 
-template <int VH, bool S5>
-static int launch_rows_r2c(wct_decon_plan p, const wct_decon_group& g, hipStream_t s)
+template <int VH, bool S5, int MS, int BLK>
+static int launch_r2c_inst(wct_decon_plan p, const wct_decon_group& g, hipStream_t s)
 {
     const RowLds lay = row_lds(g.len, 0, false);   // twiddles read through the caches
     const size_t lds = lay.bytes();
-    int rc = allow_lds(k_row_r2c<kNT, VH, S5>, lds);
+    int rc = allow_lds(k_row_r2c<kNT, VH, S5, MS, BLK>, lds);
     if (rc) return rc;
     int grid = 0;
-    rc = resident_grid(k_row_r2c<kNT, VH, S5>, kNT, lds, p->ncu, g.end - g.begin, p->row_pct, &grid);
+    rc = resident_grid(k_row_r2c<kNT, VH, S5, MS, BLK>, kNT, lds, p->ncu, g.end - g.begin, p->row_pct, &grid);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_row_r2c<kNT, VH, S5>), dim3(grid), dim3(kNT), lds, s, p->jobs, p->njobs, g.begin,
-                       g.end, lay);
+    hipLaunchKernelGGL((k_row_r2c<kNT, VH, S5, MS, BLK>), dim3(grid), dim3(kNT), lds, s, p->jobs, p->njobs,
+                       g.begin, g.end, lay);
+    WCT_HIP_CHECK(hipGetLastError());
+    return WCT_OK;
+}
+
+// Lengths with compile-time row plans (the 6000-tick rows), and whether
+// a group runs one under the plan's tuning.
+constexpr int kStaticRowLen = 3000;
+inline bool use_static_row(wct_decon_plan p, const wct_decon_group& g, int vh, bool s5)
+{
+    return p->row_plan == 1 && vh == 12 && s5 && g.len == kStaticRowLen;
+}
+
+template <int VH, bool S5>
+static int launch_rows_r2c(wct_decon_plan p, const wct_decon_group& g, hipStream_t s)
+{
+    if constexpr (VH == 12 && S5) {
+        if (use_static_row(p, g, VH, S5)) {
+            switch (p->r2c_blocks) {
+            case 2: return launch_r2c_inst<VH, S5, kStaticRowLen, 2>(p, g, s);
+            case 4: return launch_r2c_inst<VH, S5, kStaticRowLen, 4>(p, g, s);
+            default: return launch_r2c_inst<VH, S5, kStaticRowLen, 3>(p, g, s);
+            }
+        }
+    }
+    return launch_r2c_inst<VH, S5, 0, r2c_min_blocks(VH, S5)>(p, g, s);
+}
+
+template <int VH, bool S5, bool BAD, int MS, int BLK>
+static int launch_c2r_inst(wct_decon_plan p, const wct_decon_group& g, hipStream_t s, const RowLds& lay)
+{
+    const size_t lds = lay.bytes();
+    int rc = allow_lds(k_row_c2r_roi<kNT, VH, S5, BAD, MS, BLK>, lds);
+    if (rc) return rc;
+    int grid = 0;
+    rc = resident_grid(k_row_c2r_roi<kNT, VH, S5, BAD, MS, BLK>, kNT, lds, p->ncu, g.end - g.begin, p->row_pct,
+                       &grid);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_row_c2r_roi<kNT, VH, S5, BAD, MS, BLK>), dim3(grid), dim3(kNT), lds, s, p->jobs,
+                       p->njobs, g.begin, g.end, lay);
     WCT_HIP_CHECK(hipGetLastError());
     return WCT_OK;
 }
@@ -622,18 +682,22 @@ static int launch_rows_c2r(wct_decon_plan p, const wct_decon_group& g, hipStream
 {
     const RowLds lay = row_lds(g.len, g.twcount, true, 2 * kNT * VH);
     const size_t lds = lay.bytes();
-    int rc = allow_lds(k_row_c2r_roi<kNT, VH, S5, BAD>, lds);
+    int rc = WCT_OK;
+    bool done = false;
+    if constexpr (VH == 12 && S5) {
+        if (use_static_row(p, g, VH, S5)) {
+            rc = p->c2r_blocks == 2 ? launch_c2r_inst<VH, S5, BAD, kStaticRowLen, 2>(p, g, s, lay)
+                                    : launch_c2r_inst<VH, S5, BAD, kStaticRowLen, 3>(p, g, s, lay);
+            done = true;
+        }
+    }
+    if (!done) rc = launch_c2r_inst<VH, S5, BAD, 0, 3>(p, g, s, lay);
     if (rc) return rc;
-    int grid = 0;
-    rc = resident_grid(k_row_c2r_roi<kNT, VH, S5, BAD>, kNT, lds, p->ncu, g.end - g.begin, p->row_pct, &grid);
-    if (rc) return rc;
-    hipLaunchKernelGGL((k_row_c2r_roi<kNT, VH, S5, BAD>), dim3(grid), dim3(kNT), lds, s, p->jobs, p->njobs,
-                       g.begin, g.end, lay);
-    WCT_HIP_CHECK(hipGetLastError());
     if (!g.var) return WCT_OK;
     // the jobs' extra outputs, after the tight pass
     rc = allow_lds(k_row_c2r_var<kNT, VH, S5, BAD>, lds);
     if (rc) return rc;
+    int grid = 0;
     rc = resident_grid(k_row_c2r_var<kNT, VH, S5, BAD>, kNT, lds, p->ncu, g.end - g.begin, p->row_pct, &grid);
     if (rc) return rc;
     hipLaunchKernelGGL((k_row_c2r_var<kNT, VH, S5, BAD>), dim3(grid), dim3(kNT), lds, s, p->jobs, p->njobs,

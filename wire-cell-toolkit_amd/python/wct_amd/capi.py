@@ -12,7 +12,8 @@ class WctError(RuntimeError):
 
 
 class DeconTuning(C.Structure):
-    _fields_ = [("wire_tile", C.c_int), ("row_grid_pct", C.c_int)]
+    _fields_ = [("wire_tile", C.c_int), ("row_grid_pct", C.c_int), ("row_plan", C.c_int),
+                ("r2c_blocks", C.c_int), ("c2r_blocks", C.c_int)]
 
 
 class DeconOutput(C.Structure):

@@ -232,7 +232,8 @@ class Batch:
     """A fixed list of plane jobs and its workspace; run() launches the
     three stages once over all of them."""
 
-    def __init__(self, jobs, device=None, workspace=None, wire_tile=0, row_grid_pct=0):
+    def __init__(self, jobs, device=None, workspace=None, wire_tile=0, row_grid_pct=0, row_plan=0, r2c_blocks=0,
+                 c2r_blocks=0):
         import torch
         # the jobs' tensors must outlive the plan (and any graph of it)
         self._keep = [getattr(j, "_keep", None) for j in jobs]
@@ -247,7 +248,7 @@ class Batch:
             self.ws = workspace
 
         h = C.c_void_p()
-        tun = DeconTuning(int(wire_tile), int(row_grid_pct))
+        tun = DeconTuning(int(wire_tile), int(row_grid_pct), int(row_plan), int(r2c_blocks), int(c2r_blocks))
         check(lib().wct_decon_plan_create_tuned(C.byref(h), self.jobs, self.n, self.ws.data_ptr(),
                                                 self.ws.numel(), C.byref(tun)))
         self.plan = h
