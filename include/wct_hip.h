@@ -289,6 +289,24 @@ int wct_decon_run(const wct_decon_job* jobs, int njobs, void* workspace, size_t 
 int wct_spectrum_filter(const float* spectrum, int ld_spectrum, int nw, int nt, const float* filter,
                         int restore_baseline, float* out, int ld_out, void* stream);
 
+/* wct_spectrum_filter without any device allocation.  No counterpart in
+ * the reference: this is for callers that cannot allocate device memory
+ * (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The one-job table and the scaled filter that
+ * wct_spectrum_filter puts in stream-ordered scratch go into a fixed
+ * static device array of WCT_NOALLOC_SCRATCH_BYTES per device instead
+ * (held by each device the library runs on until the process ends);
+ * tick counts whose job and filter need more are refused.  The length's
+ * transform tables must exist first: wct_noalloc_prepare(nt), a set-up
+ * call that may allocate, creates them and resolves the scratch address
+ * on the current device; without it the entry point returns
+ * WCT_ERR_STATE.  Every call on a device shares that device's scratch:
+ * calls that overlap on one GPU (concurrent streams or threads) race. */
+#define WCT_NOALLOC_SCRATCH_BYTES 65536
+int wct_noalloc_prepare(int nt);
+int wct_spectrum_filter_noalloc(const float* spectrum, int ld_spectrum, int nw, int nt, const float* filter,
+                                int restore_baseline, float* out, int ld_out, void* stream);
+
 /* ---- the plain per-plane path ----
  * OmnibusSigProc's per-plane control flow (load_data, decon_2D_init,
  * decon_2D_tighterROI/tightROI, restore_baseline, cal_RMS + threshold,

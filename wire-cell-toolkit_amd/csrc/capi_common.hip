@@ -167,6 +167,28 @@ int get_twiddle_line(int N, int count, bool is_double, const void** out)
     return WCT_OK;
 }
 
+int find_plan(int n, bool is_double, const DevPlan** out)
+{
+    int dev = 0;
+    WCT_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_plan_mutex);
+    auto it = g_plans->find(PlanKey(dev, n, is_double));
+    WCT_REQUIRE(it != g_plans->end(), WCT_ERR_STATE, "no transform tables for length %d on device %d", n, dev);
+    *out = &it->second;
+    return WCT_OK;
+}
+
+int find_twiddle_line(int N, int count, bool is_double, const void** out)
+{
+    int dev = 0;
+    WCT_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_plan_mutex);
+    auto it = g_lines->find(LineKey(dev, N, count, is_double));
+    WCT_REQUIRE(it != g_lines->end(), WCT_ERR_STATE, "no twiddle line for length %d on device %d", N, dev);
+    *out = it->second;
+    return WCT_OK;
+}
+
 }  // namespace wct
 
 using namespace wct;

@@ -76,6 +76,11 @@ int get_plan(int n, bool is_double, const DevPlan** out);
 // Twiddles exp(-2 pi i k / N) for k in [0, count), as T2, cached.
 int get_twiddle_line(int N, int count, bool is_double, const void** out);
 
+// The cached plan / twiddle line when it exists on the current device,
+// WCT_ERR_STATE when it does not (these never allocate).
+int find_plan(int n, bool is_double, const DevPlan** out);
+int find_twiddle_line(int N, int count, bool is_double, const void** out);
+
 // Batched transforms used across translation units (dft.hip).
 template <typename T>
 int launch_rows(const void* in, void* out, int n, int nrows, long dist, int sign, T scale,

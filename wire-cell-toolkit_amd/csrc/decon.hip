@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "decon_kernels.hpp"
@@ -267,6 +268,62 @@ __global__ void k_scale_filter(const float* __restrict__ f, float* __restrict__ 
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) g[k] = f[k] * sc;
+}
+
+// Job table (one DJob) then the scaled filter (M + 1 floats).
+size_t spectrum_filter_scratch_bytes(int M)
+{
+    const size_t jb = (sizeof(DJob) + 255) & ~(size_t)255;
+    return jb + sizeof(float) * (size_t)(M + 1);
+}
+
+// Decon2DFilter's pass through K3's c2r stage over the caller's spectrum
+// rows, with the one-job table and the scaled filter in scratch (device
+// memory of spectrum_filter_scratch_bytes(nt / 2)).  Enqueues only.
+int spectrum_filter_run(const float* spectrum, int ld_spectrum, int nw, int nt, const float* filter,
+                        int restore_baseline, float* out, int ld_out, const DevPlan* pM, const float2* line,
+                        void* scratch, hipStream_t s)
+{
+    const int M = nt / 2;
+    WCT_REQUIRE(ld_spectrum >= M + 1 && ld_out >= nt && ld_out % 2 == 0 &&
+                    (reinterpret_cast<uintptr_t>(spectrum) & 7) == 0 && (reinterpret_cast<uintptr_t>(out) & 7) == 0,
+                WCT_ERR_ARG, "spectrum_filter: bad strides or misaligned buffers");
+    const size_t jb = (sizeof(DJob) + 255) & ~(size_t)255;
+    float* g = reinterpret_cast<float*>(static_cast<char*>(scratch) + jb);
+    DJob d;
+    memset(&d, 0, sizeof(d));
+    d.out = out;
+    d.ld_out = ld_out;
+    d.S = const_cast<float2*>(reinterpret_cast<const float2*>(spectrum));
+    d.ldk = ld_spectrum;
+    d.lf1 = g;
+    d.nw = nw;
+    d.nt = nt;
+    d.M = M;
+    d.kc = M + 1;
+    d.restore = restore_baseline ? 1 : 0;
+    d.planM = pM->plan;
+    d.twM = (const float2*)pM->tw;
+    d.wN = line;
+    // the job by value into the table, inv_c2r's 1/nt into the filter
+    hipLaunchKernelGGL(k_put_job, dim3(1), dim3(1), 0, s, d, static_cast<DJob*>(scratch));
+    hipLaunchKernelGGL(k_scale_filter, dim3((M + 1 + 255) / 256), dim3(256), 0, s, filter, g, M + 1,
+                       1.0f / (float)nt);
+    WCT_HIP_CHECK(hipGetLastError());
+    // a transient one-job plan on the stack
+    wct_decon_plan_s tmp;
+    tmp.jobs = static_cast<DJob*>(scratch);
+    tmp.njobs = 1;
+    tmp.rows = nw;
+    tmp.Mmax = M;
+    tmp.twM = d.planM.twcount;
+    tmp.rgroups.push_back({M, 0, (long)nw, d.planM.twcount, is_smooth5(M), false, false});
+    int dev = 0;
+    WCT_HIP_CHECK(hipGetDevice(&dev));
+    WCT_HIP_CHECK(hipDeviceGetAttribute(&tmp.ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int rc = run_rows(&tmp, s, true);
+    tmp.jobs = nullptr;
+    return rc;
 }
 
 }  // namespace wct
@@ -844,9 +901,6 @@ int wct_spectrum_filter(const float* spectrum, int ld_spectrum, int nw, int nt, 
                 "spectrum_filter: bad arguments (nw %d, nt %d)", nw, nt);
     const int M = nt / 2;
     WCT_REQUIRE(M <= 8192, WCT_ERR_UNSUPPORTED, "spectrum_filter: %d ticks above 16384", nt);
-    WCT_REQUIRE(ld_spectrum >= M + 1 && ld_out >= nt && ld_out % 2 == 0 &&
-                    (reinterpret_cast<uintptr_t>(spectrum) & 7) == 0 && (reinterpret_cast<uintptr_t>(out) & 7) == 0,
-                WCT_ERR_ARG, "spectrum_filter: bad strides or misaligned buffers");
     hipStream_t s = as_stream(stream);
     const DevPlan* pM = nullptr;
     int rc = get_plan(M, false, &pM);
@@ -854,56 +908,88 @@ int wct_spectrum_filter(const float* spectrum, int ld_spectrum, int nw, int nt, 
     const void* line = nullptr;
     rc = get_twiddle_line(nt, M + 1, false, &line);
     if (rc) return rc;
-    const size_t jb = (sizeof(DJob) + 255) & ~(size_t)255;
     void* scratch = nullptr;
-    WCT_HIP_CHECK(hipMallocAsync(&scratch, jb + sizeof(float) * (size_t)(M + 1), s));
-    float* g = reinterpret_cast<float*>(static_cast<char*>(scratch) + jb);
-    DJob d;
-    memset(&d, 0, sizeof(d));
-    d.out = out;
-    d.ld_out = ld_out;
-    d.S = const_cast<float2*>(reinterpret_cast<const float2*>(spectrum));
-    d.ldk = ld_spectrum;
-    d.lf1 = g;
-    d.nw = nw;
-    d.nt = nt;
-    d.M = M;
-    d.kc = M + 1;
-    d.restore = restore_baseline ? 1 : 0;
-    d.planM = pM->plan;
-    d.twM = (const float2*)pM->tw;
-    d.wN = (const float2*)line;
-    hipLaunchKernelGGL(k_put_job, dim3(1), dim3(1), 0, s, d, static_cast<DJob*>(scratch));
-    hipLaunchKernelGGL(k_scale_filter, dim3((M + 1 + 255) / 256), dim3(256), 0, s, filter, g, M + 1,
-                       1.0f / (float)nt);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) {
-        // K3's c2r stage over the caller's spectrum rows, through a
-        // transient one-job plan on the stack
-        wct_decon_plan_s tmp;
-        tmp.jobs = static_cast<DJob*>(scratch);
-        tmp.njobs = 1;
-        tmp.rows = nw;
-        tmp.Mmax = M;
-        tmp.twM = d.planM.twcount;
-        tmp.rgroups.push_back({M, 0, (long)nw, d.planM.twcount, is_smooth5(M), false, false});
-        int dev = 0;
-        e = hipGetDevice(&dev);
-        if (e == hipSuccess) e = hipDeviceGetAttribute(&tmp.ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        if (e == hipSuccess) rc = run_rows(&tmp, s, true);
-        tmp.jobs = nullptr;
-    }
+    WCT_HIP_CHECK(hipMallocAsync(&scratch, spectrum_filter_scratch_bytes(M), s));
+    rc = spectrum_filter_run(spectrum, ld_spectrum, nw, nt, filter, restore_baseline, out, ld_out, pM,
+                             (const float2*)line, scratch, s);
     const hipError_t ef = hipFreeAsync(scratch, s);
-    if (e != hipSuccess) {
-        set_error("spectrum_filter: %s", hipGetErrorString(e));
-        return WCT_ERR_HIP;
-    }
     if (rc) return rc;
     if (ef != hipSuccess) {
         set_error("spectrum_filter: %s", hipGetErrorString(ef));
         return WCT_ERR_HIP;
     }
     return WCT_OK;
+}
+
+// ---- the same without allocation ----------------------------------------
+// A fixed device scratch for the job table and the scaled filter of
+// wct_spectrum_filter_noalloc.  Host code never passes the array's name
+// to a kernel (that would be a host-side stand-in, not a device
+// address): its address on each device is looked up once with
+// hipGetSymbolAddress and used from then on.
+static __device__ __align__(256) unsigned char scratch[WCT_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mutex;
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch's device address on the current device (looked up on the
+// first use per device).
+int scratch_address(void** out)
+{
+    int dev = 0;
+    WCT_HIP_CHECK(hipGetDevice(&dev));
+    WCT_REQUIRE(dev >= 0 && dev < kMaxDevices, WCT_ERR_UNSUPPORTED, "device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        WCT_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        WCT_REQUIRE(p, WCT_ERR_HIP, "no device address for the static scratch");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return WCT_OK;
+}
+}  // namespace
+
+int wct_noalloc_prepare(int nt)
+{
+    WCT_REQUIRE(nt > 0 && nt % 2 == 0 && nt / 2 <= 8192, WCT_ERR_ARG, "noalloc_prepare: bad length %d", nt);
+    const int M = nt / 2;
+    WCT_REQUIRE(spectrum_filter_scratch_bytes(M) <= WCT_NOALLOC_SCRATCH_BYTES, WCT_ERR_UNSUPPORTED,
+                "noalloc_prepare: %d ticks need %zu bytes of scratch, %d available", nt,
+                spectrum_filter_scratch_bytes(M), WCT_NOALLOC_SCRATCH_BYTES);
+    void* p = nullptr;
+    int rc = scratch_address(&p);
+    if (rc) return rc;
+    const DevPlan* pM = nullptr;
+    rc = get_plan(M, false, &pM);
+    if (rc) return rc;
+    const void* line = nullptr;
+    return get_twiddle_line(nt, M + 1, false, &line);
+}
+
+int wct_spectrum_filter_noalloc(const float* spectrum, int ld_spectrum, int nw, int nt, const float* filter,
+                                int restore_baseline, float* out, int ld_out, void* stream)
+{
+    WCT_REQUIRE(spectrum && filter && out && nw > 0 && nt > 0 && nt % 2 == 0, WCT_ERR_ARG,
+                "spectrum_filter_noalloc: bad arguments (nw %d, nt %d)", nw, nt);
+    const int M = nt / 2;
+    WCT_REQUIRE(M <= 8192 && spectrum_filter_scratch_bytes(M) <= WCT_NOALLOC_SCRATCH_BYTES, WCT_ERR_UNSUPPORTED,
+                "spectrum_filter_noalloc: %d ticks need %zu bytes of scratch, %d available", nt,
+                spectrum_filter_scratch_bytes(M), WCT_NOALLOC_SCRATCH_BYTES);
+    const DevPlan* pM = nullptr;
+    int rc = find_plan(M, false, &pM);
+    if (rc) return rc;
+    const void* line = nullptr;
+    rc = find_twiddle_line(nt, M + 1, false, &line);
+    if (rc) return rc;
+    void* p = nullptr;
+    rc = scratch_address(&p);
+    if (rc) return rc;
+    return spectrum_filter_run(spectrum, ld_spectrum, nw, nt, filter, restore_baseline, out, ld_out, pM,
+                               (const float2*)line, p, as_stream(stream));
 }
 
 }  // extern "C"

@@ -89,6 +89,8 @@ _SIGS = {
     "wct_decon_workspace_size": (_Z, [C.POINTER(DeconJob), _I]),
     "wct_decon_run": (_I, [C.POINTER(DeconJob), _I, _V, _Z, _V]),
     "wct_spectrum_filter": (_I, [_V, _I, _I, _I, _V, _I, _V, _I, _V]),
+    "wct_noalloc_prepare": (_I, [_I]),
+    "wct_spectrum_filter_noalloc": (_I, [_V, _I, _I, _I, _V, _I, _V, _I, _V]),
     "wct_decon_plan_create": (_I, [C.POINTER(C.c_void_p), C.POINTER(DeconJob), _I, _V, _Z]),
     "wct_decon_plan_run": (_I, [_V, _V]),
     "wct_decon_plan_destroy": (_I, [_V]),
