@@ -41,7 +41,11 @@ def parse():
     ap.add_argument("--chunk", type=int, default=8, help="APAs per launch group")
     ap.add_argument("--cut", type=float, default=1e-7, help="spectral cut tolerance")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (0 = the CPUs this process may use, at most 16)")
+    ap.add_argument("--chargefit-apas", type=int, default=8,
+                    help="signal APAs in the timed 20-step charge fit (0 = skip)")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 accuracy check")
     ap.add_argument("--plain-apas", type=int, default=4,
                     help="APAs timed through the plain per-plane path (0 = skip)")
     ap.add_argument("--no-tune", action="store_true",
@@ -87,52 +91,89 @@ def make_data(planes, kernels_fwd, dev, seed=1234):
     return frames
 
 
+def cpu_info():
+    """CPU model (/proc/cpuinfo), logical CPU count (nproc) and the CPUs
+    this process may run on."""
+    model, family, number = "", "", ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and not model:
+                    model = v
+                elif k == "cpu family" and not family:
+                    family = v
+                elif k == "model" and not number:
+                    number = v
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"model": model, "family": family, "model_number": number, "nproc": os.cpu_count(),
+            "usable": usable}
+
+
 def cpu_baseline(planes, nthreads):
     """The oracle's FP32 restatement of the reference CPU path (decon_2D_init,
     tight filter, restore_baseline, cal_RMS/threshold), one plane per
-    thread, on a bounded sample of the workload's planes."""
+    thread as the reference runs one OmnibusSigProc per APA thread, on
+    bounded samples: the CPU-runnable collection plane (480 x 6000), one
+    APA (800/800/960 x 6000, three threads) and nthreads planes of the
+    workload (one per thread)."""
     import numpy as np
     from concurrent.futures import ThreadPoolExecutor
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
     from wct_amd import workload as W
-    sample = [planes[i % len(planes)] for i in range(nthreads)]
     specs = {}
-    for kind, nw, nt in set(sample):
-        sp = W.plane_spec(kind, nw, nt)
-        resp = O.overall_response(sp.fr_rows, sp.elec, W.FINE_PERIOD, W.TICK, W.INTER_GAIN,
-                                  W.ADC_MV, W.FTOFFSET, nt, dtype=np.float32)
-        specs[(kind, nw, nt)] = (sp, resp)
+
+    def spec(t):
+        if t not in specs:
+            sp = W.plane_spec(*t)
+            resp = O.overall_response(sp.fr_rows, sp.elec, W.FINE_PERIOD, W.TICK, W.INTER_GAIN,
+                                      W.ADC_MV, W.FTOFFSET, t[2], dtype=np.float32)
+            specs[t] = (sp, resp)
+        return specs[t]
+
     rng = np.random.default_rng(7)
-    inputs = [(rng.standard_normal((nw, nt)) * W.NOISE_ADC).astype(np.float32)
-              for (kind, nw, nt) in sample]
 
-    def one(i):
-        sp, resp = specs[sample[i]]
-        c = O.decon_init(inputs[i], resp, sp.wire_filter.astype(np.float32), sp.wire_shift,
-                         sp.time_shift, np.float32)
-        y = O.decon_apply_filter(c, sp.time_filter.astype(np.float32), np.float32)
-        yb, _ = O.restore_baseline(y)
-        O.roi_threshold(yb, W.TH_FACTOR)
-        return y.size
+    def run(sample):
+        inputs = [(rng.standard_normal((nw, nt)) * W.NOISE_ADC).astype(np.float32) for (_, nw, nt) in sample]
+        for t in set(sample):
+            spec(t)
 
-    t0 = time.perf_counter()
-    with ThreadPoolExecutor(nthreads) as ex:
-        tot = sum(ex.map(one, range(nthreads)))
-    dt = time.perf_counter() - t0
-    cpu = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-    return {"value": tot / dt, "unit": "channel-ticks/s", "cores": nthreads, "kind": "port",
-            "sample": f"{nthreads} planes ({'/'.join(f'{k}{nw}x{nt}' for k, nw, nt in sample[:3])}"
-                      f"...) one per thread, {dt:.1f} s wall; own mixed-radix FFT (no FFTW); "
-                      f"CPU: {cpu}"}
+        def one(i):
+            sp, resp = spec(sample[i])
+            c = O.decon_init(inputs[i], resp, sp.wire_filter.astype(np.float32), sp.wire_shift,
+                             sp.time_shift, np.float32)
+            y = O.decon_apply_filter(c, sp.time_filter.astype(np.float32), np.float32)
+            yb, _ = O.restore_baseline(y)
+            O.roi_threshold(yb, W.TH_FACTOR)
+            return y.size
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(len(sample)) as ex:
+            tot = sum(ex.map(one, range(len(sample))))
+        dt = time.perf_counter() - t0
+        return tot / dt, dt
+
+    info = cpu_info()
+    v0, d0 = run([("col", 480, 6000)])
+    v1, d1 = run(W.config_planes("apa"))
+    sample = [planes[i % len(planes)] for i in range(nthreads)]
+    v4, d4 = run(sample)
+    return {"value": v4, "unit": "channel-ticks/s", "cores": nthreads, "kind": "port",
+            "sample": f"{nthreads} planes of the 150-APA workload ({'/'.join(f'{k}{nw}x{nt}' for k, nw, nt in sample[:3])}"
+                      f"...) one per thread, {d4:.1f} s wall; own mixed-radix FFT (no FFTW); "
+                      f"CPU: {info['model']} (family {info['family']} model {info['model_number']}), "
+                      f"nproc {info['nproc']}, {info['usable']} usable",
+            "configs": {"col480x6000 (1 thread)": {"value": v0, "wall_s": round(d0, 2)},
+                        "apa 800/800/960x6000 (3 threads)": {"value": v1, "wall_s": round(d1, 2)},
+                        f"apa150 sample ({nthreads} threads)": {"value": v4, "wall_s": round(d4, 2)}},
+            "cpu": info}
 
 
 def calibration_copy(dev, mib=1024):
@@ -229,6 +270,122 @@ def plain_leg(planes, frames, outs, masks, rmss, kern, napas, reps=2):
     return ct / (ms / 1e3), ms, n
 
 
+def hbm_copy_peak(dev, gib=2, reps=5):
+    """Device-to-device copy of gib GiB (read + write) timed with events:
+    the HBM rate a plain stream copy reaches on this box, in GB/s."""
+    import torch
+    n = gib * (1 << 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * n * 4 / (ms / 1e3) / 1e9
+
+
+def fp64_decon(spec, resp, adc, dev):
+    """The fused decon's definition in FP64 torch.fft on the device:
+    Re IFFT2( FFT2(adc) Fw Ft conj(phase) / R ), 0 where R == 0."""
+    import numpy as np
+    import torch
+    nw, nt = spec.nw, spec.nt
+    Rr = np.zeros((nw, nt))
+    Rr[:resp.shape[0]] = resp
+    R = np.fft.fft2(Rr)
+    mm = np.arange(nw)[:, None]
+    kk = np.arange(nt)[None, :]
+    ts = spec.time_shift if spec.time_shift > 0 else 0
+    ph = np.exp(2j * np.pi * (mm * spec.wire_shift / nw + kk * ts / nt))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        G = np.where(R != 0, np.conj(ph) / R, 0) * spec.wire_filter[:, None] * spec.time_filter[None, :]
+    g = torch.from_numpy(G).to(dev)
+    x = torch.fft.fft2(adc.double())
+    return torch.fft.ifft2(x * g).real
+
+
+def accuracy_leg(planes, frames, kern, dev, apa=1):
+    """Per plane of one signal APA of the workload: the device decon before
+    restore_baseline (the same kernels and kept bins as the timed step)
+    against fp64_decon, as relative RMS and max error."""
+    import torch
+    from wct_amd import decon
+    out = {}
+    for i in range(3 * apa, 3 * apa + 3):
+        t = planes[i]
+        y = torch.empty_like(frames[i])
+        decon.Batch([decon.make_job(kern[t], frames[i], y, restore=0)]).run()
+        y64 = fp64_decon(kern[t].spec, kern[t].resp, frames[i], dev)
+        torch.cuda.synchronize()
+        d = y.double() - y64
+        rel = float(torch.linalg.norm(d) / torch.linalg.norm(y64))
+        mx = float(d.abs().max() / y64.abs().max())
+        out[f"plane{i - 3 * apa}_{t[0]}{t[1]}x{t[2]}"] = {"rel_rms": rel, "max_rel": mx}
+        del y, y64, d
+    return out
+
+
+def response_build_leg(types):
+    """The FP64 response-spectrum build per plane type, timed on its own:
+    init_overall_response on the device (wct_overall_response) and the
+    regularised inverse with the spectral cut (wct_decon_kernel_create)."""
+    import torch
+    from wct_amd import decon, workload as W
+    res = {}
+    for t in types:
+        spec = W.plane_spec(*t)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        resp = decon.overall_response(spec)
+        t1 = time.perf_counter()
+        k = decon.PlaneKernel(spec, resp=resp)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        k.close()
+        res[f"{t[0]}{t[1]}x{t[2]}"] = {"overall_response_ms": round((t1 - t0) * 1e3, 2),
+                                       "kernel_ms": round((t2 - t1) * 1e3, 2)}
+    return res
+
+
+def chargefit_leg(frames, planes, dev, napas=8, steps=20):
+    """20 steepest-descent steps of the charge fit (wct_amd.chargefit) on
+    the first napas signal APAs of the resident frames: 2 napas induction
+    frames and napas collection frames per step, six fused passes each."""
+    import torch
+    from wct_amd import autograd as A, chargefit as CF, workload as W
+    sel = [a for a in range(len(planes) // 3) if a % 2 == 1][:napas]
+    groups = {}
+    for a in sel:
+        for p in range(3):
+            groups.setdefault(planes[3 * a + p], []).append(frames[3 * a + p])
+    models = {t: A.Model(W.plane_spec(*t)) for t in groups}
+    adc = {t: torch.stack(v) for t, v in groups.items()}
+    st, sw = W.WIENER["sigma"], W.WIRE["sigma"]
+    for t in groups:     # warm the kernel caches
+        CF.fit(models[t], adc[t], 1, st, sw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = {}
+    for t in groups:
+        _, losses[t] = CF.fit(models[t], adc[t], steps, st, sw)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ct = sum(v.numel() for v in adc.values())
+    for m in models.values():
+        m.close()
+    return {"apas": len(sel), "steps": steps, "ms_per_step": round(dt * 1e3 / steps, 3),
+            "channel_ticks_per_step": ct, "channel_ticks_per_s": ct * steps / dt,
+            "loss_first_last": {f"{t[0]}{t[1]}x{t[2]}": [v[0], v[-1]] for t, v in losses.items()}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -240,6 +397,10 @@ def main():
     planes = W.config_planes(args.workload)
     types = sorted(set(planes))
     calibration_copy(dev)
+    copy_gbs = hbm_copy_peak(dev)
+    log(f"bench: device copy {copy_gbs:.0f} GB/s (read + write)")
+    resp_build = response_build_leg(types)
+    log(f"bench: FP64 response build {resp_build}")
     log(f"bench: {args.workload}: {len(planes)} planes, plane types {types}")
     kern, fwd = {}, {}
     for t in types:
@@ -311,14 +472,25 @@ def main():
                  "sample": f"{pn} planes ({pn // 3} APAs) of the same frames, one launch per "
                            f"reference step, {pms:.2f} ms per pass"}
 
+    accuracy, acc_ok = None, True
+    if not args.no_accuracy:
+        accuracy = accuracy_leg(planes, frames, kern, dev, apa=1 if len(planes) >= 6 else 0)
+        acc_ok = all(v["rel_rms"] <= 1e-4 for v in accuracy.values())
+        log(f"bench: FP64 accuracy {accuracy}")
+
     adjoint = None
     if args.adjoint_frames > 0:
         adjoint = adjoint_leg(dev, args.adjoint_frames)
         log(f"bench: adjoint step {adjoint}")
 
+    chargefit = None
+    if args.chargefit_apas > 0 and len(planes) >= 6:
+        chargefit = chargefit_leg(frames, planes, dev, napas=args.chargefit_apas)
+        log(f"bench: charge fit {chargefit}")
+
     cpu = None
     if not args.no_cpu:
-        n = args.cpu_threads or min(16, os.cpu_count() or 1)
+        n = args.cpu_threads or min(16, cpu_info()["usable"])
         log(f"bench: CPU baseline on {n} threads")
         cpu = cpu_baseline(planes, n)
 
@@ -360,13 +532,20 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "kernel": "fused decon pipeline (row r2c, wire pass, row c2r+ROI) per step",
+            "measured_copy_gbs": copy_gbs,
         },
+        "accuracy_fp64": accuracy,
+        "response_build_fp64": resp_build,
         "plain_gpu": plain,
         "adjoint_step": adjoint,
+        "chargefit": chargefit,
         "batched_vs_plain": None if plain is None else plain["ratio"],
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
+    if not acc_ok:
+        log("bench: FP64 accuracy above 1e-4 relative RMS")
+        sys.exit(3)
 
 
 if __name__ == "__main__":
