@@ -80,9 +80,14 @@ int wct_dft_transpose_complex(const float* in, float* out, int nrows, int ncols,
  * to build response spectra in FP64. */
 int wct_dft_fwd1b_f64(const double* in, double* out, int nrows, int ncols, int axis, void* stream);
 int wct_dft_inv1b_f64(const double* in, double* out, int nrows, int ncols, int axis, void* stream);
-/* Largest supported transform length and whether n is supported
- * (all prime factors in {2,3,5,7,11,13}). */
+/* Whether the IDFT functions above take length n (1 <= n <= 2^20; any
+ * length: those with a prime factor above 13 run Bluestein's chirp-z
+ * algorithm on 5-smooth transforms, FftwDFT's any-length plans,
+ * aux/src/FftwDFT.cxx:120-146), and whether n is native to the LDS
+ * kernels and the fused deconvolution (all prime factors in
+ * {2,3,5,7,11,13}). */
 int wct_dft_supported(int n);
+int wct_dft_native_length(int n);
 
 /* ---- response spectra, FP64 on the device ----
  * OmnibusSigProc::init_overall_response

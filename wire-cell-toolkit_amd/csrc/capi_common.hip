@@ -306,6 +306,8 @@ int wct_stream_sync(void* stream)
     return WCT_OK;
 }
 
-int wct_dft_supported(int n) { return fft_supported_length(n) ? 1 : 0; }
+int wct_dft_supported(int n) { return n >= 1 && n <= kMaxDftLength ? 1 : 0; }
+
+int wct_dft_native_length(int n) { return fft_supported_length(n) ? 1 : 0; }
 
 }  // extern "C"

@@ -44,6 +44,10 @@ struct Status {
 // Largest prime factor allowed by the LDS kernels.
 bool fft_supported_length(int n);
 
+// Largest transform length of the IDFT functions (other lengths than
+// 2^a 3^b 5^c 7^d 11^e 13^f go through Bluestein's algorithm).
+constexpr int kMaxDftLength = 1 << 20;
+
 // Device limit of LDS bytes per workgroup (160 KB on gfx950).
 int lds_limit_bytes();
 

@@ -3,6 +3,10 @@
 // LDS-tiled transposes.  These implement the IDFT contract of
 // iface/inc/WireCellIface/IDFT.h:9-127 with FftwDFT's conventions
 // (aux/src/FftwDFT.cxx:88-283).
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "common.hpp"
 
 namespace wct {
@@ -228,6 +232,204 @@ __global__ __launch_bounds__(kBlock) void k_fourstep_twiddle(typename C2<T>::typ
 // Largest row length the single-pass LDS kernel takes.
 template <typename T> constexpr int lds_row_limit() { return sizeof(T) == 8 ? 4096 : 8192; }
 
+// ---- Bluestein (chirp-z) transforms for lengths with a prime factor
+// above 13 ---------------------------------------------------------------
+// With nk = (n^2 + k^2 - (k - n)^2) / 2,
+//   X[k] = c_k sum_n (x_n c_n) conj(c_(k-n)),  c_n = exp(sign i pi n^2 / N),
+// a circular convolution of length L >= 2N - 1 (L with prime factors 2, 3
+// and 5), done with two length-L transforms and the transform of the
+// chirp kernel b (b_m = conj(c_m) for |m| < N, circularly), which is
+// built once per (N, L, sign) in FP64.  n^2 is reduced mod 2N before the
+// angle is formed, so the chirp is accurate to the last bit of FP64.
+
+__device__ __forceinline__ double2 chirp(long n, int N, int sign)
+{
+    const long q = (n * n) % (2L * N);
+    double sn, cs;
+    sincos((double)sign * M_PI * (double)q / (double)N, &sn, &cs);
+    double2 r;
+    r.x = cs;
+    r.y = sn;
+    return r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_blue_pre(const typename C2<T>::type* __restrict__ in, long dist, int N,
+                                                     int L, long total, int sign,
+                                                     typename C2<T>::type* __restrict__ work)
+{
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= total) return;
+    const long row = i / L;
+    const int n = (int)(i % L);
+    typename C2<T>::type v;
+    v.x = 0;
+    v.y = 0;
+    if (n < N) {
+        const auto x = in[row * dist + n];
+        const double2 c = chirp(n, N, sign);
+        v.x = (T)((double)x.x * c.x - (double)x.y * c.y);
+        v.y = (T)((double)x.x * c.y + (double)x.y * c.x);
+    }
+    work[i] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_blue_mul(typename C2<T>::type* __restrict__ work,
+                                                     const typename C2<T>::type* __restrict__ bhat, int L,
+                                                     long total)
+{
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= total) return;
+    work[i] = cmul(work[i], bhat[i % L]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_blue_post(const typename C2<T>::type* __restrict__ work, int N, int L,
+                                                      long total, int sign, T scale,
+                                                      typename C2<T>::type* __restrict__ out, long dist)
+{
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= total) return;
+    const long row = i / N;
+    const int k = (int)(i % N);
+    const auto w = work[row * L + k];
+    const double2 c = chirp(k, N, sign);
+    typename C2<T>::type v;
+    v.x = (T)(((double)w.x * c.x - (double)w.y * c.y) * (double)scale);
+    v.y = (T)(((double)w.x * c.y + (double)w.y * c.x) * (double)scale);
+    out[row * dist + k] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_blue_kernel(int N, int L, int sign, double2* __restrict__ b)
+{
+    const int m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= L) return;
+    double2 v;
+    v.x = 0;
+    v.y = 0;
+    const int d = m < N ? m : (L - m < N ? L - m : -1);
+    if (d >= 0) {
+        const double2 c = chirp(d, N, sign);
+        v.x = c.x;
+        v.y = -c.y;
+    }
+    b[m] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_d2f(const double2* __restrict__ in, float2* __restrict__ out, int n)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        float2 v;
+        v.x = (float)in[i].x;
+        v.y = (float)in[i].y;
+        out[i] = v;
+    }
+}
+
+// Smallest length >= n with prime factors 2, 3 and 5.
+static int smooth5_at_least(int n)
+{
+    for (int L = n;; ++L) {
+        int m = L;
+        for (int q : {2, 3, 5})
+            while (m % q == 0) m /= q;
+        if (m == 1) return L;
+    }
+}
+
+using BlueKey = std::tuple<int, int, int, bool>;   // device, N, sign, double
+static std::mutex g_blue_mutex;
+static std::map<BlueKey, void*>* g_blue = new std::map<BlueKey, void*>();
+
+template <typename T> int launch_rows(const void*, void*, int, int, long, int, T, hipStream_t);
+
+// FFT_L of the chirp kernel for (N, sign), in T, cached per device.
+template <typename T>
+static int blue_kernel(int N, int L, int sign, const void** out, hipStream_t s)
+{
+    constexpr bool dbl = sizeof(T) == 8;
+    int dev = 0;
+    WCT_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_blue_mutex);
+    const BlueKey key(dev, N, sign, dbl);
+    auto it = g_blue->find(key);
+    if (it != g_blue->end()) {
+        *out = it->second;
+        return WCT_OK;
+    }
+    double2* b = nullptr;
+    WCT_HIP_CHECK(hipMalloc((void**)&b, (size_t)L * sizeof(double2)));
+    hipLaunchKernelGGL(k_blue_kernel, dim3((L + kBlock - 1) / kBlock), dim3(kBlock), 0, s, N, L, sign, b);
+    WCT_HIP_CHECK(hipGetLastError());
+    int rc = launch_rows<double>(b, b, L, 1, L, -1, 1.0, s);
+    if (rc) return rc;
+    void* res = b;
+    if (!dbl) {
+        float2* f = nullptr;
+        WCT_HIP_CHECK(hipMalloc((void**)&f, (size_t)L * sizeof(float2)));
+        hipLaunchKernelGGL(k_d2f, dim3((L + kBlock - 1) / kBlock), dim3(kBlock), 0, s, b, f, L);
+        WCT_HIP_CHECK(hipGetLastError());
+        WCT_HIP_CHECK(hipStreamSynchronize(s));
+        WCT_HIP_CHECK(hipFree(b));
+        res = f;
+    }
+    WCT_HIP_CHECK(hipStreamSynchronize(s));
+    (*g_blue)[key] = res;
+    *out = res;
+    return WCT_OK;
+}
+
+// Batched rows of any length N by Bluestein's algorithm.
+template <typename T>
+static int blue_rows(const void* in, void* out, int N, int nrows, long dist, int sign, T scale, hipStream_t s)
+{
+    using T2 = typename C2<T>::type;
+    const int L = smooth5_at_least(2 * N - 1);
+    const void* bhat = nullptr;
+    int rc = blue_kernel<T>(N, L, sign, &bhat, s);
+    if (rc) return rc;
+    const long total = (long)nrows * L;
+    T2* work = nullptr;
+    WCT_HIP_CHECK(hipMallocAsync((void**)&work, total * sizeof(T2), s));
+    const dim3 g((total + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL((k_blue_pre<T>), g, dim3(kBlock), 0, s, (const T2*)in, dist, N, L, total, sign, work);
+    WCT_HIP_CHECK(hipGetLastError());
+    rc = launch_rows<T>(work, work, L, nrows, L, -1, T(1), s);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_blue_mul<T>), g, dim3(kBlock), 0, s, work, (const T2*)bhat, L, total);
+    WCT_HIP_CHECK(hipGetLastError());
+    rc = launch_rows<T>(work, work, L, nrows, L, +1, T(1) / T(L), s);
+    if (rc) return rc;
+    const long tout = (long)nrows * N;
+    hipLaunchKernelGGL((k_blue_post<T>), dim3((tout + kBlock - 1) / kBlock), dim3(kBlock), 0, s, work, N, L, tout,
+                       sign, scale, (T2*)out, dist);
+    WCT_HIP_CHECK(hipGetLastError());
+    WCT_HIP_CHECK(hipFreeAsync(work, s));
+    return WCT_OK;
+}
+
+// Columns of any length (nrows) by transposing to rows and back.
+template <typename T>
+static int blue_cols(const void* in, void* out, int nrows, int ncols, int sign, T scale, hipStream_t s)
+{
+    using T2 = typename C2<T>::type;
+    const size_t bytes = (size_t)nrows * ncols * sizeof(T2);
+    T2* t = nullptr;
+    WCT_HIP_CHECK(hipMallocAsync((void**)&t, bytes, s));
+    dim3 g1((ncols + 31) / 32, (nrows + 31) / 32);
+    hipLaunchKernelGGL((k_transpose<T2>), g1, dim3(kBlock), 0, s, (const T2*)in, t, nrows, ncols);
+    WCT_HIP_CHECK(hipGetLastError());
+    int rc = blue_rows<T>(t, t, nrows, ncols, nrows, sign, scale, s);
+    if (rc) return rc;
+    dim3 g2((nrows + 31) / 32, (ncols + 31) / 32);
+    hipLaunchKernelGGL((k_transpose<T2>), g2, dim3(kBlock), 0, s, (const T2*)t, (T2*)out, ncols, nrows);
+    WCT_HIP_CHECK(hipGetLastError());
+    WCT_HIP_CHECK(hipFreeAsync(t, s));
+    return WCT_OK;
+}
+
 // Batched row transforms of any supported length: one LDS pass when the
 // row fits, otherwise the four-step split n = n1 * n2 (column
 // transforms of length n1, twiddles, row transforms of length n2, then a
@@ -237,6 +439,9 @@ int launch_rows(const void* in, void* out, int n, int nrows, long dist, int sign
                 hipStream_t s)
 {
     using T2 = typename C2<T>::type;
+    WCT_REQUIRE(n >= 1 && n <= kMaxDftLength, WCT_ERR_UNSUPPORTED, "transform length %d not in [1, %d]", n,
+                kMaxDftLength);
+    if (!fft_supported_length(n)) return blue_rows<T>(in, out, n, nrows, dist, sign, scale, s);
     if (n <= lds_row_limit<T>()) return launch_rows_lds<T>(in, out, n, nrows, dist, sign, scale, s);
     WCT_REQUIRE(dist == n, WCT_ERR_UNSUPPORTED, "strided rows above the LDS limit");
     WCT_REQUIRE(fft_supported_length(n), WCT_ERR_UNSUPPORTED,
@@ -271,6 +476,9 @@ template int launch_rows<double>(const void*, void*, int, int, long, int, double
 template <typename T>
 int launch_cols_any(const void* in, void* out, int nrows, int ncols, int sign, T scale, hipStream_t s)
 {
+    WCT_REQUIRE(nrows >= 1 && nrows <= kMaxDftLength, WCT_ERR_UNSUPPORTED, "transform length %d not in [1, %d]",
+                nrows, kMaxDftLength);
+    if (!fft_supported_length(nrows)) return blue_cols<T>(in, out, nrows, ncols, sign, scale, s);
     return launch_cols<T>(in, out, nrows, ncols, sign, scale, s);
 }
 template int launch_cols_any<float>(const void*, void*, int, int, int, float, hipStream_t);
@@ -286,7 +494,7 @@ static int dft_1b(const T* in, T* out, int nrows, int ncols, int axis, int sign,
         return launch_rows<T>(in, out, ncols, nrows, ncols, sign, scale, s);
     }
     const T scale = sign > 0 ? T(1) / T(nrows) : T(1);
-    return launch_cols<T>(in, out, nrows, ncols, sign, scale, s);
+    return launch_cols_any<T>(in, out, nrows, ncols, sign, scale, s);
 }
 
 }  // namespace wct
@@ -336,7 +544,7 @@ int wct_dft_fwd2d(const float* in, float* out, int nrows, int ncols, void* strea
     hipStream_t s = as_stream(stream);
     int rc = launch_rows<float>(in, out, C, R, C, -1, 1.0f, s);
     if (rc) return rc;
-    return launch_cols<float>(out, out, R, C, -1, 1.0f, s);
+    return launch_cols_any<float>(out, out, R, C, -1, 1.0f, s);
 }
 
 int wct_dft_inv2d(const float* in, float* out, int nrows, int ncols, void* stream)
@@ -346,7 +554,7 @@ int wct_dft_inv2d(const float* in, float* out, int nrows, int ncols, void* strea
     hipStream_t s = as_stream(stream);
     int rc = launch_rows<float>(in, out, C, R, C, +1, 1.0f / (float)C, s);
     if (rc) return rc;
-    return launch_cols<float>(out, out, R, C, +1, 1.0f / (float)R, s);
+    return launch_cols_any<float>(out, out, R, C, +1, 1.0f / (float)R, s);
 }
 
 int wct_dft_transpose_real(const float* in, float* out, int nrows, int ncols, void* stream)

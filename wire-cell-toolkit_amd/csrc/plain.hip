@@ -317,8 +317,9 @@ int wct_plain_decon_plane(const float* adc, int ld_adc, int nw, int nt, const fl
     WCT_REQUIRE(nw > 0 && nt > 0 && nr > 0 && nr <= nw && ld_adc >= nt && ld_out >= nt &&
                     (!mask || ld_mask >= nt),
                 WCT_ERR_ARG, "plain decon: bad sizes (nw=%d nt=%d nr=%d)", nw, nt, nr);
-    WCT_REQUIRE(fft_supported_length(nw) && fft_supported_length(nt), WCT_ERR_UNSUPPORTED,
-                "plain decon: %d x %d has a prime factor above 13", nw, nt);
+    // any lengths: the IDFT functions take them (Bluestein above 13); the
+    // row statistics hold up to 64 x 256 samples per row
+    WCT_REQUIRE(nt <= 64 * 256, WCT_ERR_UNSUPPORTED, "plain decon: %d ticks above 16384", nt);
     WCT_REQUIRE(ws_bytes >= wct_plain_workspace_size(nw, nt), WCT_ERR_ARG, "plain decon: workspace too small");
     WCT_REQUIRE((bad_ptr == nullptr) == (bad_lo == nullptr) && (bad_lo == nullptr) == (bad_hi == nullptr),
                 WCT_ERR_ARG, "plain decon: bad ranges need ptr, lo and hi");

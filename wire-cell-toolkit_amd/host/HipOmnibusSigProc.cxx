@@ -186,9 +186,9 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
         }
         const int nticks = tbinmax - tbinmin;
         if (nticks <= 0) throw SigProcError("HipOmnibusSigProc: frame without samples");
-        if (!wct_dft_supported(nticks))
+        if (!wct_dft_native_length(nticks) || nticks % 2)
             throw SigProcError("HipOmnibusSigProc: nticks " + std::to_string(nticks) +
-                               " has a prime factor above 13");
+                               " is odd or has a prime factor above 13 (the fused decon's transforms)");
         prepare(nticks, in->tick());
         if (m_adc_floats == 0) {
             // without ROI refinement or the debug mode the reference saves

@@ -78,6 +78,7 @@ _SIGS = {
     "wct_dft_fwd1b_f64": (_I, [_V, _V, _I, _I, _I, _V]),
     "wct_dft_inv1b_f64": (_I, [_V, _V, _I, _I, _I, _V]),
     "wct_dft_supported": (_I, [_I]),
+    "wct_dft_native_length": (_I, [_I]),
     "wct_overall_response": (_I, [_V, _I, _I, _V, _I, _D, _D, _D, _D, _D, _I, _V, _V]),
     "wct_decon_kernel_create": (_I, [C.POINTER(C.c_void_p), _V, _I, _I, _I, _V, _V, _I, _I, _D, _V]),
     "wct_forward_kernel_create": (_I, [C.POINTER(C.c_void_p), _V, _I, _I, _I, _I, _I, _V]),
