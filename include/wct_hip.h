@@ -230,7 +230,30 @@ typedef struct {
      * plan is created; at most WCT_DECON_MAX_OUTPUTS). */
     const wct_decon_output* outputs;
     int noutputs;
+    /* Channel-by-channel electronics correction (decon_2D_init
+     * :949-982; Decon2DResponse.cxx:280-308): each wire's time spectrum
+     * is multiplied, bin by bin, by row chan_factor_row[w] of the
+     * chan_factors table before the wire transform.
+     *   chan_factors    : device table from wct_channel_factors, rows of
+     *                     ld_chan_factors complex64 values (>= kc)
+     *   chan_factor_row : device, nw ints; -1 leaves a wire uncorrected
+     * Both NULL: no correction.  Not with adjoint. */
+    const float* chan_factors;
+    int ld_chan_factors;
+    const int* chan_factor_row;
 } wct_decon_job;
+
+/* The correction factors of channel responses (OmnibusSigProc.cxx:
+ * 949-982): row r of out holds elec(k) / C_r(k) for k in [0, nt/2], or 0
+ * where C_r(k) == 0, with elec and C_r the length-nt forward DFTs of the
+ * nominal electronics response and of channel response r (cut or
+ * zero-padded to nt, :966).  Built in FP64 on the device, stored as
+ * complex64 rows of ld_out values (ld_out >= nt/2 + 1).
+ *   responses : host, nresp x nsamp floats (IChannelResponse samples)
+ *   elec      : host, nt doubles (the electronics response sampled on
+ *               the channel responses' binning over nt ticks) */
+int wct_channel_factors(float* out, int ld_out, const float* responses, int nresp, int nsamp,
+                        const double* elec, int nt, void* stream);
 
 /* Device scratch needed to run the jobs (spectrum buffers). */
 size_t wct_decon_workspace_size(const wct_decon_job* jobs, int njobs);

@@ -51,6 +51,12 @@ int wcth_dft(const char* tn, int op, const float* in, float* out, int nrows, int
 
 /* IFilterWaveform::filter_waveform(n) of component tn into out[n]. */
 int wcth_filter_waveform(const char* tn, int n, float* out);
+/* IChannelResponse "type:name" (iface/inc/WireCellIface/IChannelResponse.h:
+ * 17-27): the response of a channel (its first min(n, cap) samples into
+ * out, n its length) and the binning's start and bin size; a channel
+ * without a response fails with the component's KeyError text. */
+int wcth_channel_response(const char* tn, int channel, float* out, int cap, int* n, double* tmin,
+                          double* binsize);
 
 /* Frames: build an input frame, run an IFrameFilter ("type:name") on it
  * (in = NULL is end-of-stream; *out = NULL then), read the result. */

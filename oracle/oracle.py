@@ -48,6 +48,8 @@ def lib():
             "wo_overall_response_d": (I, [f64p, I, I, f64p, I, D, D, D, D, D, I, f64p]),
             "wo_decon_init_f": (I, [f32p, I, I, f32p, I, f32p, I, I, f32p]),
             "wo_decon_init_d": (I, [f64p, I, I, f64p, I, f64p, I, I, f64p]),
+            "wo_decon_init_cr_f": (I, [f32p, I, I, f32p, I, f32p, I, I, C.c_void_p, I, C.c_void_p, f32p]),
+            "wo_decon_init_cr_d": (I, [f64p, I, I, f64p, I, f64p, I, I, C.c_void_p, I, C.c_void_p, f64p]),
             "wo_decon_apply_filter_f": (I, [f32p, I, I, f32p, f32p]),
             "wo_decon_apply_filter_d": (I, [f64p, I, I, f64p, f64p]),
             "wo_decon_fused_d": (I, [f64p, I, I, f64p, I, f64p, f64p, I, I, f64p]),
@@ -257,14 +259,27 @@ def overall_response(avg, elec, fine_period, coarse_period, inter_gain, adc_mv,
 
 # ---- decon (OmnibusSigProc.cxx:944-1117) ----
 
-def decon_init(data, resp, wire_filter, wire_shift, time_shift, dtype=np.float32):
+def decon_init(data, resp, wire_filter, wire_shift, time_shift, dtype=np.float32,
+               ch_resp=None, ewave=None):
+    """decon_2D_init's m_c_data (before the time filters).  With ch_resp
+    (nw x nsamp channel responses) and ewave (nt electronics samples) the
+    per-channel correction (:949-982) runs after the time transform."""
     d = np.ascontiguousarray(data, dtype=dtype)
     r = np.ascontiguousarray(resp, dtype=dtype)
     wf = np.ascontiguousarray(wire_filter, dtype=dtype)
     nw, nt = d.shape
     c = np.zeros(2 * nw * nt, dtype=dtype)
-    rc = getattr(lib(), "wo_decon_init" + _sfx(dtype))(d, nw, nt, r, r.shape[0], wf,
-                                                       wire_shift, time_shift, c)
+    if ch_resp is None:
+        rc = getattr(lib(), "wo_decon_init" + _sfx(dtype))(d, nw, nt, r, r.shape[0], wf,
+                                                           wire_shift, time_shift, c)
+    else:
+        cr = np.ascontiguousarray(ch_resp, dtype=dtype)
+        ew = np.ascontiguousarray(ewave, dtype=dtype)
+        if cr.ndim != 2 or cr.shape[0] != nw or ew.shape != (nt,):
+            raise ValueError("ch_resp must be nw x nsamp and ewave nt long")
+        rc = getattr(lib(), "wo_decon_init_cr" + _sfx(dtype))(
+            d, nw, nt, r, r.shape[0], wf, wire_shift, time_shift,
+            cr.ctypes.data, cr.shape[1], ew.ctypes.data, c)
     if rc:
         raise RuntimeError("decon_init failed")
     return _uncplx(c).reshape(nw, nt)
@@ -280,10 +295,10 @@ def decon_apply_filter(c_data, time_filter, dtype=np.float32):
 
 
 def decon_stepwise(data, resp, wire_filter, time_filter, wire_shift, time_shift,
-                   dtype=np.float32):
+                   dtype=np.float32, ch_resp=None, ewave=None):
     """decon_2D_init followed by one decon_2D_*ROI variant, before
     restore_baseline, in the reference's step order and precision."""
-    c = decon_init(data, resp, wire_filter, wire_shift, time_shift, dtype)
+    c = decon_init(data, resp, wire_filter, wire_shift, time_shift, dtype, ch_resp, ewave)
     return decon_apply_filter(c, time_filter, dtype)
 
 

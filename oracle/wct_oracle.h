@@ -44,6 +44,12 @@ int wo_decon_init_f(const float* data, int nw, int nt, const float* resp, int nr
                     const float* wire_filter, int wire_shift, int time_shift, float* c_data);
 int wo_decon_init_d(const double* data, int nw, int nt, const double* resp, int nr,
                     const double* wire_filter, int wire_shift, int time_shift, double* c_data);
+int wo_decon_init_cr_f(const float* data, int nw, int nt, const float* resp, int nr,
+                       const float* wire_filter, int wire_shift, int time_shift,
+                       const float* ch_resp, int nsamp, const float* ewave, float* c_data);
+int wo_decon_init_cr_d(const double* data, int nw, int nt, const double* resp, int nr,
+                       const double* wire_filter, int wire_shift, int time_shift,
+                       const double* ch_resp, int nsamp, const double* ewave, double* c_data);
 int wo_decon_apply_filter_f(const float* c_data, int nw, int nt, const float* tf, float* out);
 int wo_decon_apply_filter_d(const double* c_data, int nw, int nt, const double* tf, double* out);
 int wo_decon_fused_d(const double* data, int nw, int nt, const double* resp, int nr,

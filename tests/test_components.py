@@ -140,8 +140,7 @@ def test_sigproc_eos_and_empty_frame():
     assert out.frame_tags() == []
 
 
-@pytest.mark.parametrize("extra,msg", [({"per_chan_resp": "PerChannelResponse"}, "per_chan_resp"),
-                                       ({"fft_flag": 1}, "fft_flag"),
+@pytest.mark.parametrize("extra,msg", [({"fft_flag": 1}, "fft_flag"),
                                        ({"process_planes": [0, 3]}, "process_planes"),
                                        ({"anode": "AnodeChannels:missing_planes"}, "plane"),
                                        ({"use_multi_plane_protection": True}, "refinement"),
@@ -241,8 +240,9 @@ def test_decon2d_configuration_errors(tmp_path):
     with pytest.raises(WctError, match="no anode"):
         K.configure(bad)
     K.reset()
-    bad[-2] = {"type": "Decon2DResponse", "name": "r", "data": {"anode": "AnodeChannels:apa0"}}
-    with pytest.raises(WctError, match="per-channel response"):
+    bad[-2] = {"type": "Decon2DResponse", "name": "r",
+               "data": {"anode": "AnodeChannels:apa0", "per_chan_resp": "NoSuchResponse:x"}}
+    with pytest.raises(WctError, match="m_cresp"):
         K.configure(bad)
     K.reset()
     bad[-2] = {"type": "Decon2DResponse", "name": "r",
@@ -281,4 +281,37 @@ def test_decon2d_eos_and_input_errors(tmp_path):
     s.add("i4", (4,), np.arange(4, dtype=np.int32), {"tag": "trace_tag", "type": "channels"})
     with pytest.raises(WctError, match="no filter"):
         K.tensorset_filter("Decon2DFilter:f", s)
+    K.reset()
+
+
+def test_per_channel_response_file(tmp_path):
+    """SigProc::PerChannelResponse (sigproc/src/PerChannelResponse.cxx:
+    28-78): single channels and channel groups, float samples, the
+    binning of the first entry, KeyError for a channel without a
+    response, and its configuration errors."""
+    import json
+    from wct_amd import components as K
+    from wct_amd.capi import WctError
+    r1 = [0.0, 1.5, 2.25, -0.5]
+    r2 = [1.0, 2.0, 3.0, 4.0, 0.1]
+    path = tmp_path / "chresp.json"
+    path.write_text(json.dumps({"tick": 500.0, "t0": 1000.0, "channels": [[7, r1], [[8, 9, 10], r2]]}))
+    K.reset()
+    K.configure([{"type": "PerChannelResponse", "name": "pcr", "data": {"filename": str(path)}}])
+    v, t0, bs = K.channel_response("PerChannelResponse:pcr", 7)
+    assert list(v) == r1 and t0 == 1000.0 and bs == 500.0
+    for ch in (8, 9, 10):
+        v, _, _ = K.channel_response("PerChannelResponse:pcr", ch)
+        np.testing.assert_array_equal(v, np.array(r2, np.float32))
+    with pytest.raises(WctError, match="no response for channel 11"):
+        K.channel_response("PerChannelResponse:pcr", 11)
+    bad = [({"tick": 500.0, "t0": 0.0}, "no channels given"),
+           ({"tick": 500.0, "t0": 0.0, "channels": [[1, []]]}, "zero length response")]
+    for i, (doc, msg) in enumerate(bad):
+        p = tmp_path / f"bad{i}.json"
+        p.write_text(json.dumps(doc))
+        with pytest.raises(WctError, match=msg):
+            K.configure([{"type": "PerChannelResponse", "name": f"b{i}", "data": {"filename": str(p)}}])
+    with pytest.raises(WctError, match="must supply a PerChannelResponse filename"):
+        K.configure([{"type": "PerChannelResponse", "name": "nofile", "data": {"filename": ""}}])
     K.reset()

@@ -178,11 +178,13 @@ def _filters(kind, nt, debug=False, tight=True):
     return f
 
 
-def _expected(adc, bad, th, debug=False, refine=True):
+def _expected(adc, bad, th, debug=False, refine=True, chan=None):
     """The same planes through the C ABI (workload's spec, padded to
     plane 0's 21 response regions as the reference sizes them), with the
-    component's unfiltered kernels and output filters.  Returns per
-    plane (base, mask, rms, {extra name: frame})."""
+    component's unfiltered kernels and output filters.  chan is
+    (responses, {channel: response row}, t0) for the per-channel
+    correction.  Returns per plane (base, mask, rms, {extra name:
+    frame})."""
     import torch
     from wct_amd import decon as D
     from wct_amd import workload as W
@@ -190,6 +192,12 @@ def _expected(adc, bad, th, debug=False, refine=True):
     kinds = ("ind", "ind", "col")
     jobs, keep, res = [], [], []
     nfine_rows = len(W.fr_paths("ind")[0])
+    table = None
+    if chan is not None:
+        from wct_amd import host
+        cresp, chmap, t0 = chan
+        ewave = host.coldelec_samples(NT, t0, t0 + NT * W.TICK, W.GAIN, W.SHAPING, 1.0)
+        table = D.channel_factors(cresp, ewave, NT, device=dev)
     for ip, kind in enumerate(kinds):
         sp = W.plane_spec(kind, NW[ip], NT, fold_lf=False)
         rows = np.zeros((nfine_rows, sp.fr_rows.shape[1]))
@@ -215,9 +223,14 @@ def _expected(adc, bad, th, debug=False, refine=True):
         extra = {k: torch.empty_like(a) for k in ("loose", "charge") if k in flt}
         outs = [D.Output(g[k], out=extra[k], restore=1 if (k == "loose" or kind == "col") else 0,
                          save_mode=0 if debug else 1) for k in extra]
+        cj = None
+        if table is not None:
+            rows = torch.tensor([chmap[CH0[ip] + w] for w in range(NW[ip])], dtype=torch.int32, device=dev)
+            cj = (table, rows)
+            keep.append(rows)
         jobs.append(D.make_job(pk, a, y, mask=m, rms=r, th_factor=th[ip], restore=1 if tight else 0,
                                save_mode=0 if debug else 1, lf_tight=g["base"], lf_tighter=g.get("tighter"),
-                               bad=badt, outputs=outs))
+                               bad=badt, outputs=outs, chan=cj))
         keep += [pk, a, y, m, r, badt, g, extra]
         res.append((y, m, r, extra))
     b = D.Batch(jobs, device=dev)
@@ -282,6 +295,54 @@ def test_sigproc_matches_c_abi(K, tmp_path):
     # a second frame of the same shape reuses the spectra and gives the same
     out2 = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
     np.testing.assert_array_equal(out2.trace(3)[2], tr[3][2])
+
+
+def _chan_file(tmp_path, t0, tick=500.0, nresp=4, nsamp=150):
+    """A PerChannelResponse file over the three planes' channels (groups
+    of channels sharing a response) and its contents."""
+    from test_gpu_chanresp import _channel_responses
+    cr = _channel_responses(nresp, nsamp, 21)
+    chmap = {CH0[ip] + w: (w + ip) % nresp for ip in range(3) for w in range(NW[ip])}
+    doc = {"tick": tick, "t0": t0,
+           "channels": [[[ch for ch, r in chmap.items() if r == k], cr[k].tolist()] for k in range(nresp)]}
+    path = os.path.join(str(tmp_path), "chresp.json")
+    with open(path, "w") as f:
+        json.dump(doc, f)
+    return path, cr, chmap
+
+
+def test_sigproc_per_channel_response(K, tmp_path):
+    """per_chan_resp: decon_2D_init's channel-by-channel correction
+    (:949-982) in the component gives what the C ABI gives with the same
+    factor table (the nominal response sampled from the responses' t0);
+    a response binned at another tick is refused at the first frame."""
+    print("sigproc per-channel correction", flush=True)
+    t0 = -1000.0
+    path, cr, chmap = _chan_file(tmp_path, t0)
+    cfg = _sigproc_config(tmp_path, {"per_chan_resp": "PerChannelResponse:pcr"})
+    cfg.insert(-1, {"type": "PerChannelResponse", "name": "pcr", "data": {"filename": path}})
+    K.configure(cfg)
+    adc = _adc_planes(8)
+    bad = {(0, 5): [(100, 300)]}
+    out = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
+    exp = _expected(adc, bad, (3.0, 3.0, 5.0), chan=(cr, chmap, t0))
+    plain = _expected(adc, bad, (3.0, 3.0, 5.0))
+    tr = out.traces()
+    k = 0
+    for ip in range(3):
+        y = exp[ip][0]
+        assert np.any(y != plain[ip][0])
+        for w in range(NW[ip]):
+            np.testing.assert_array_equal(tr[k][2], y[w])
+            k += 1
+    K.reset()
+    path2, _, _ = _chan_file(tmp_path, t0, tick=400.0)
+    cfg = _sigproc_config(tmp_path, {"per_chan_resp": "PerChannelResponse:pcr"})
+    cfg.insert(-1, {"type": "PerChannelResponse", "name": "pcr", "data": {"filename": path2}})
+    K.configure(cfg)
+    from wct_amd.capi import WctError
+    with pytest.raises(WctError, match="channel response size mismatch"):
+        K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
 
 
 def test_sigproc_tight_as_wiener(K, tmp_path):

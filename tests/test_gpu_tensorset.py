@@ -183,6 +183,62 @@ def test_tensorset_components_match_device_paths(dev, tmp_path):
     K.reset()
 
 
+def test_decon2d_response_per_channel(dev, tmp_path):
+    """per_chan_resp (Decon2DResponse.cxx:280-308): the component's
+    spectrum equals the device path's with the factor table of the
+    frame's channels (nominal response Response::ColdElec(gain,
+    shaping_time) from the responses' t0); a tick mismatch is refused."""
+    import json
+    import torch
+    from wct_amd import components as K
+    from wct_amd import decon, host
+    from wct_amd import workload as W
+    from wct_amd.capi import WctError
+    from test_components import _tensorset_config
+    from test_gpu_chanresp import _channel_responses
+    print("Decon2DResponse per-channel correction, 96 x 1000", flush=True)
+    nch, nt, t0 = 96, 1000, 2000.0
+    cr = _channel_responses(3, 120, 4)
+    chans = np.arange(100, 100 + nch, dtype=np.int32)
+    rows = (np.arange(nch) // 5) % 3
+    for tick, ok in ((W.TICK, True), (400.0, False)):
+        doc = {"tick": tick, "t0": t0,
+               "channels": [[[int(c) for c, r in zip(chans, rows) if r == k], cr[k].tolist()] for k in range(3)]}
+        path = tmp_path / "chresp.json"
+        path.write_text(json.dumps(doc))
+        K.reset()
+        cfg = _tensorset_config(tmp_path, resp_extra={
+            "gain": W.GAIN, "shaping_time": W.SHAPING, "inter_gain": W.INTER_GAIN, "ADC_mV": W.ADC_MV,
+            "ftoffset": W.FTOFFSET, "ctoffset": W.CTOFFSET, "iplane": 0,
+            "per_chan_resp": "PerChannelResponse:pcr"})
+        cfg.insert(0, {"type": "PerChannelResponse", "name": "pcr", "data": {"filename": str(path)}})
+        K.configure(cfg)
+        spec, resp, adc = _setup("ind", nch, nt, 6)
+        s = K.TensorSet(7, {"tick": W.TICK})
+        s.add_array(adc, "f4", {"tag": "trace_tag", "type": "waveform"})
+        s.add("i4", (nch,), chans, {"tag": "trace_tag", "type": "channels"})
+        if not ok:
+            with pytest.raises(WctError, match="channel response size mismatch"):
+                K.tensorset_filter("Decon2DResponse:r", s)
+            continue
+        r = K.tensorset_filter("Decon2DResponse:r", s)
+        sp = r.array(r.find("trace_tag", "waveform"))
+        ewave = host.coldelec_samples(nt, t0, t0 + nt * W.TICK, W.GAIN, W.SHAPING, 1.0)
+        table = decon.channel_factors(cr, ewave, nt, device=dev)
+        pk = decon.PlaneKernel(spec, cut_tol=0.0, time_filter=np.ones(nt), resp=decon.overall_response(spec))
+        a = torch.from_numpy(adc).to(dev)
+        want = torch.empty((nch, nt), dtype=torch.complex64, device=dev)
+        b = decon.Batch([decon.make_job(pk, a, None, spectrum=want,
+                                        chan=(table, torch.from_numpy(rows.astype(np.int32)).to(dev)))])
+        b.run()
+        torch.cuda.synchronize()
+        want = want.cpu().numpy()
+        assert np.abs(sp - want).max() <= 1e-5 * np.abs(want).max()
+        plain = _gpu_spectrum(spec, decon.overall_response(spec), adc, dev).cpu().numpy()
+        assert np.abs(plain - want).max() > 1e-3 * np.abs(want).max()
+    K.reset()
+
+
 def test_decon2d_filter_cmm_ranges_and_empty_rows(dev, tmp_path):
     from wct_amd import components as K
     from wct_amd.capi import WctError

@@ -397,6 +397,89 @@ int wct_overall_response(const double* fr_avg, int nr, int nin, const double* el
     return WCT_OK;
 }
 
+// Correction factors of one chunk of channel responses: spectra C (nc
+// rows of nt, row 0 the electronics spectrum E), out row r = E / C_{r+1}
+// over bins [0, nb), 0 where C is exactly 0 (OmnibusSigProc.cxx:972-978).
+__global__ void k_chan_factor(const double2* __restrict__ C, int nc, int nt, int nb, float2* __restrict__ out,
+                              int ld)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)nc * nb) return;
+    const int r = (int)(i / nb), k = (int)(i % nb);
+    const double2 e = C[k], c = C[(long)(r + 1) * nt + k];
+    float2 o;
+    o.x = o.y = 0.f;
+    if (c.x != 0.0 || c.y != 0.0) {
+        // e / c with Smith's scaling
+        double qr, qi;
+        if (fabs(c.x) >= fabs(c.y)) {
+            const double rat = c.y / c.x, den = c.x + c.y * rat;
+            qr = (e.x + e.y * rat) / den;
+            qi = (e.y - e.x * rat) / den;
+        } else {
+            const double rat = c.x / c.y, den = c.x * rat + c.y;
+            qr = (e.x * rat + e.y) / den;
+            qi = (e.y * rat - e.x) / den;
+        }
+        o.x = (float)qr;
+        o.y = (float)qi;
+    }
+    out[(long)r * ld + k] = o;
+}
+
+int wct_channel_factors(float* out, int ld_out, const float* responses, int nresp, int nsamp,
+                        const double* elec, int nt, void* stream)
+{
+    WCT_REQUIRE(out && responses && elec && nresp > 0 && nsamp > 0 && nt >= 2, WCT_ERR_ARG,
+                "channel_factors: bad arguments");
+    WCT_REQUIRE(nt <= kMaxDftLength, WCT_ERR_UNSUPPORTED, "channel_factors: %d ticks above %d", nt,
+                kMaxDftLength);
+    const int nb = nt / 2 + 1;
+    WCT_REQUIRE(ld_out >= nb && (reinterpret_cast<uintptr_t>(out) & 7) == 0, WCT_ERR_ARG,
+                "channel_factors: row stride %d below %d bins or misaligned table", ld_out, nb);
+    hipStream_t s = as_stream(stream);
+    // chunks of responses, each transformed with the electronics row
+    const int chunk = std::min(nresp, 256);
+    std::vector<double> h((size_t)(chunk + 1) * nt * 2);
+    double2* d = nullptr;
+    const size_t bytes = h.size() * sizeof(double);
+    WCT_HIP_CHECK(hipMallocAsync((void**)&d, bytes, s));
+    int rc = WCT_OK;
+    for (int r0 = 0; r0 < nresp && rc == WCT_OK; r0 += chunk) {
+        const int nc = std::min(chunk, nresp - r0);
+        std::fill(h.begin(), h.end(), 0.0);
+        for (int t = 0; t < nt; ++t) h[2 * (size_t)t] = elec[t];
+        const int ns = std::min(nsamp, nt);   // cut or zero-padded to nt (:966)
+        for (int r = 0; r < nc; ++r)
+            for (int t = 0; t < ns; ++t)
+                h[2 * ((size_t)(r + 1) * nt + t)] = responses[(size_t)(r0 + r) * nsamp + t];
+        const size_t used = (size_t)(nc + 1) * nt * 2 * sizeof(double);
+        rc = hipMemcpyAsync(d, h.data(), used, hipMemcpyHostToDevice, s) == hipSuccess ? WCT_OK : WCT_ERR_HIP;
+        if (rc) {
+            set_error("channel_factors: upload failed");
+            break;
+        }
+        rc = launch_rows<double>(d, d, nt, nc + 1, nt, -1, 1.0, s);
+        if (rc) break;
+        const long tot = (long)nc * nb;
+        hipLaunchKernelGGL(k_chan_factor, dim3((tot + 255) / 256), dim3(256), 0, s, d, nc, nt, nb,
+                           reinterpret_cast<float2*>(out) + (size_t)r0 * ld_out, ld_out);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("channel_factors: launch failed");
+            rc = WCT_ERR_HIP;
+            break;
+        }
+        // the host staging is refilled by the next chunk
+        if (hipStreamSynchronize(s) != hipSuccess) {
+            set_error("channel_factors: stream failed");
+            rc = WCT_ERR_HIP;
+        }
+    }
+    WCT_HIP_CHECK(hipFreeAsync(d, s));
+    WCT_HIP_CHECK(hipStreamSynchronize(s));
+    return rc;
+}
+
 // forward: the response convolution.  Otherwise the decon kernel, with
 // time_filter baked in, or (time_filter NULL) without a time filter and
 // envelope (or NULL) weighting the spectral cut.
@@ -662,6 +745,12 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
                     WCT_DECON_MAX_OUTPUTS);
         WCT_REQUIRE(!spec || j.noutputs == 0, WCT_ERR_ARG,
                     "decon_plan: spectrum job %d sets outputs of the decon mode", i);
+        WCT_REQUIRE(!j.chan_factors == !j.chan_factor_row, WCT_ERR_ARG,
+                    "decon_plan: job %d sets only one of chan_factors and chan_factor_row", i);
+        WCT_REQUIRE(!j.chan_factors || (!j.adjoint && j.ld_chan_factors >= K->kc &&
+                                        (reinterpret_cast<uintptr_t>(j.chan_factors) & 7) == 0),
+                    WCT_ERR_ARG, "decon_plan: job %d has a bad per-channel correction (adjoint, row stride %d "
+                    "below %d kept bins, or a misaligned table)", i, j.ld_chan_factors, K->kc);
         for (int q = 0; q < j.noutputs; ++q) {
             const wct_decon_output& o = j.outputs[q];
             WCT_REQUIRE(o.filter && (!o.alt_rows || o.filter_alt) &&
@@ -692,6 +781,9 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
             v.restore = o.restore_baseline;
             v.save_mode = o.save_mode;
         }
+        d.cf = reinterpret_cast<const float2*>(j.chan_factors);
+        d.cf_row = j.chan_factor_row;
+        d.ld_cf = j.ld_chan_factors;
         d.K = K->K;
         d.ld_adc = j.ld_adc;
         d.ld_out = j.ld_out;

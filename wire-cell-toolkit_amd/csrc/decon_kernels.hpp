@@ -97,6 +97,9 @@ struct DJob {
     int ld_spec;
     int nvar;         // extra outputs (K4)
     DVar var[kMaxVar];
+    const float2* cf;     // per-channel correction table (or null)
+    const int* cf_row;    // its row per wire, -1 for none
+    int ld_cf;
     FftPlan planM;
     FftPlan planW;
 };
@@ -274,6 +277,17 @@ __global__ __launch_bounds__(NT) void k_wire_pass(const DJob* __restrict__ jobs,
             const int idx = threadIdx.x + q * NT;
             if (idx < nw * TK) t[q] = S[(long)(idx / TK) * J.ldk + k0 + idx % TK];
         }
+        if (J.cf_row) {   // channel-by-channel electronics correction, before the wire transform
+            const int k = k0 + threadIdx.x % TK;
+#pragma unroll
+            for (int q = 0; q < NLD; ++q) {
+                const int idx = threadIdx.x + q * NT;
+                if (idx < nw * TK && k < J.kc) {
+                    const int ri = J.cf_row[idx / TK];
+                    if (ri >= 0) t[q] = cmul(t[q], J.cf[(long)ri * J.ld_cf + k]);
+                }
+            }
+        }
         // idx / TK = threadIdx.x / TK + q NT/TK and idx % TK = threadIdx.x % TK
         const int lbase = (threadIdx.x % TK) * ldc + pidx(threadIdx.x / TK);
 #pragma unroll
@@ -381,10 +395,7 @@ __global__ __launch_bounds__(NT, BLK) void k_row_c2r_roi(const DJob* __restrict_
         if constexpr (MS > 0)
             fft_static<float2, NT, MS>(buf, twl, +1, threadIdx.x);
         else
-            if constexpr (MS > 0)
-                fft_static<float2, NT, MS>(buf, twl, +1, threadIdx.x);
-            else
-                fft_lds<float2, NT, MMAX, S5>(buf, J.planM, twl, +1, threadIdx.x);
+            fft_lds<float2, NT, MMAX, S5>(buf, J.planM, twl, +1, threadIdx.x);
 
         float val[VPT];
         uint64_t valid = 0;
@@ -455,9 +466,6 @@ __global__ __launch_bounds__(NT, BLK) void k_row_c2r_roi(const DJob* __restrict_
             lds_barrier();
             c2r_stage<NT, VH>(J, Y, J.lf2, buf);
             lds_barrier();
-            if constexpr (MS > 0)
-            fft_static<float2, NT, MS>(buf, twl, +1, threadIdx.x);
-        else
             if constexpr (MS > 0)
                 fft_static<float2, NT, MS>(buf, twl, +1, threadIdx.x);
             else

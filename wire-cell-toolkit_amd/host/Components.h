@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <fstream>
+#include <map>
 #include <sstream>
 
 #include "Interface.h"
@@ -198,6 +199,68 @@ class FieldResponseFile : public IFieldResponse, public IConfigurable {
   private:
     std::string m_fname;
     Hip::FieldResponse m_fr;
+};
+
+struct KeyError : public std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// SigProc::PerChannelResponse (sigproc/src/PerChannelResponse.cxx:28-78):
+// a JSON file {"tick": T, "t0": T0, "channels": [[ch, [samples]], [[ch,
+// ch, ...], [samples]], ...]}.  tick, t0 and the samples are read as
+// floats, and the binning is that of the first entry: nsamp bins from t0
+// at tick.  Plain .json only (no bzip2 in this build), resolved along
+// WIRECELL_PATH.
+class PerChannelResponse : public IChannelResponse, public IConfigurable {
+  public:
+    Configuration default_configuration() const override
+    {
+        Configuration cfg;
+        cfg["filename"] = m_fname;
+        return cfg;
+    }
+    void configure(const Configuration& cfg) override
+    {
+        m_fname = get(cfg, "filename", m_fname);
+        if (m_fname.empty()) throw ConfigError("must supply a PerChannelResponse filename");
+        const std::string path = FieldResponseFile::resolve(m_fname);
+        std::ifstream in(path);
+        if (!in) throw ConfigError("PerChannelResponse: cannot open " + path);
+        std::stringstream buf;
+        buf << in.rdbuf();
+        const Configuration top = Configuration::parse(buf.str());
+        const double tick = (float)top["tick"].asDouble();
+        const double t0 = (float)top["t0"].asDouble();
+        const Configuration& chans = top["channels"];
+        if (chans.isNull()) throw ConfigError("no channels given in file " + m_fname);
+        m_cr.clear();
+        m_bins = Binning(0, 0, 0);
+        for (const auto& entry : chans) {
+            const Configuration& samples = entry[1];
+            const int n = (int)samples.size();
+            if (n == 0) throw ConfigError("zero length response in file " + m_fname);
+            realseq_t resp(n);
+            for (int i = 0; i < n; ++i) resp[i] = (float)samples[i].asDouble();
+            if (entry[0].isArray()) {
+                for (const auto& ch : entry[0]) m_cr[ch.asInt()] = resp;
+            } else {
+                m_cr[entry[0].asInt()] = resp;
+            }
+            if (!m_bins.nbins) m_bins = Binning(n, t0, t0 + n * tick);
+        }
+    }
+    const realseq_t& channel_response(int channel_ident) const override
+    {
+        auto it = m_cr.find(channel_ident);
+        if (it == m_cr.end()) throw KeyError("no response for channel " + std::to_string(channel_ident));
+        return it->second;
+    }
+    Binning channel_response_binning() const override { return m_bins; }
+
+  private:
+    std::string m_fname;
+    std::map<int, realseq_t> m_cr;
+    Binning m_bins{0, 0, 0};
 };
 
 // The channel planes of one anode.  Configuration:

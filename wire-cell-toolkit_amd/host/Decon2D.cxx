@@ -3,7 +3,9 @@
 //   Decon2DResponse  sig/src/Decon2DResponse.cxx:28-415
 //     waveform tensor {nchans, nticks} f4  ->  its 2D deconvolution as a
 //     complex time spectrum {nchans, nticks} c8 (no time filter), from
-//     the spectrum mode of the fused plan (wct_decon_job.spectrum)
+//     the spectrum mode of the fused plan (wct_decon_job.spectrum), with
+//     the channel-by-channel electronics correction when per_chan_resp
+//     names an IChannelResponse
 //   Decon2DFilter    sig/src/Decon2DFilter.cxx:22-172
 //     that spectrum times a product of IFilterWaveform time filters,
 //     inverse real transform, Sig::restore_baseline, bad:cmm_range
@@ -20,6 +22,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -114,8 +117,8 @@ class Decon2DResponse : public ITensorSetFilter, public IConfigurable {
         return cfg;
     }
 
-    // Decon2DResponse.cxx:42-73 (the anode and the field response must
-    // exist; per-channel response correction is not supported here).
+    // Decon2DResponse.cxx:42-73 (the anode, the field response and, unless
+    // per_chan_resp is "", the channel response must exist).
     void configure(const Configuration& cfg) override
     {
         m_cfg = cfg;
@@ -123,9 +126,11 @@ class Decon2DResponse : public ITensorSetFilter, public IConfigurable {
         if (!find_or_null<IAnodeChannels>(anode))
             throw ValueError("Decon2DResponse::configure: no anode \"" + anode + "\"");
         const auto cresp = get<std::string>(cfg, "per_chan_resp", "PerChannelResponse");
-        if (!cresp.empty())
-            throw ConfigError("Decon2DResponse: per-channel response correction (per_chan_resp = \"" + cresp +
-                              "\") is not supported; set it to \"\"");
+        m_cresp = nullptr;
+        if (!cresp.empty()) {
+            m_cresp = find_or_null<IChannelResponse>(cresp);
+            if (!m_cresp) throw ValueError("Sig::Decon2DResponse::configure !m_cresp");
+        }
         m_fresp_tn = get<std::string>(cfg, "field_response", "FieldResponse");
         if (!find_or_null<IFieldResponse>(m_fresp_tn))
             throw ValueError("Decon2DResponse::configure: no field response \"" + m_fresp_tn + "\"");
@@ -147,6 +152,11 @@ class Decon2DResponse : public ITensorSetFilter, public IConfigurable {
         auto summary = Aux::get_tens(in, tag, "summary");
         const double period = get<double>(in->metadata(), "tick", 0.5 * units::microsecond);
         prepare((int)nchans, (int)nticks, period);
+        if (m_cresp) {
+            if (ch->dtype() != "i4" || ch->size() < nchans * sizeof(int))
+                throw ValueError("Decon2DResponse: channels tensor is not {nchans} i4");
+            channel_rows((const int*)ch->data(), (int)nchans, (int)nticks, period);
+        }
 
         const size_t n = nchans * nticks;
         std::memcpy(m_h_io.p, iwf->data(), n * sizeof(float));
@@ -228,19 +238,79 @@ class Decon2DResponse : public ITensorSetFilter, public IConfigurable {
         m_d_b.ensure(n * sizeof(float));
         m_d_spec.ensure(n * 2 * sizeof(float));
         m_h_io.ensure(n * 2 * sizeof(float));
+        m_nchans = nchans;
+        m_nticks = nticks;
+        m_period = period;
+        m_gain = gain;
+        m_shaping = shaping;
+        make_plan();
+    }
+
+    void make_plan()
+    {
+        if (m_plan) wct_decon_plan_destroy(m_plan);
+        m_plan = nullptr;
         wct_decon_job j;
         std::memset(&j, 0, sizeof(j));
         j.kernel = m_kernel;
         j.adc = (const float*)m_d_b.p;
-        j.ld_adc = nticks;
+        j.ld_adc = m_nticks;
         j.spectrum = (float*)m_d_spec.p;
-        j.ld_spectrum = nticks;
+        j.ld_spectrum = m_nticks;
+        if (m_cf_ld) {
+            j.chan_factors = (const float*)m_d_cf.p;
+            j.ld_chan_factors = m_cf_ld;
+            j.chan_factor_row = (const int*)m_d_cfrows.p;
+        }
         const size_t ws = wct_decon_workspace_size(&j, 1);
         m_d_ws.ensure(std::max<size_t>(ws, 1));
-        device_check(wct_decon_plan_create(&m_plan, &j, 1, m_d_ws.p, ws), "plan");
-        m_nchans = nchans;
-        m_nticks = nticks;
-        m_period = period;
+        device_check(wct_decon_plan_create(&m_plan, &j, 1, m_d_ws.p, m_d_ws.bytes), "plan");
+    }
+
+    // The channel-by-channel electronics correction (Decon2DResponse.cxx:
+    // 280-308) for this frame's channels: a table row of factors per
+    // distinct channel response, built in FP64 on the device the first
+    // time the response is seen (the nominal response is
+    // Response::ColdElec(gain, shaping_time) at the bin centres of nticks
+    // bins from the channel responses' start), and the table row of each
+    // channel.  A channel without a response throws the response's
+    // KeyError.
+    void channel_rows(const int* chans, int nchans, int nticks, double period)
+    {
+        const Binning crb = m_cresp->channel_response_binning();
+        if (crb.binsize() != period)
+            throw ValueError("Decon2DResponse::decon_2D_init: channel response size mismatch");
+        std::vector<int> rows(nchans);
+        bool grown = false;
+        for (int i = 0; i < nchans; ++i) {
+            const auto& r = m_cresp->channel_response(chans[i]);
+            auto it = m_cf_index.find(r);
+            if (it == m_cf_index.end()) {
+                it = m_cf_index.emplace(r, (int)m_cf_list.size()).first;
+                m_cf_list.push_back(&it->first);
+                grown = true;
+            }
+            rows[i] = it->second;
+        }
+        if (grown || crb.minval != m_cf_tmin) {
+            const Binning tb(nticks, crb.minval, crb.minval + nticks * period);
+            const auto ewave = coldelec_samples(nticks, tb.minval, tb.maxval, m_gain, m_shaping, 1.0);
+            size_t nsamp = 1;
+            for (const auto* r : m_cf_list) nsamp = std::max(nsamp, r->size());
+            std::vector<float> h(m_cf_list.size() * nsamp, 0.0f);
+            for (size_t i = 0; i < m_cf_list.size(); ++i)
+                std::copy(m_cf_list[i]->begin(), m_cf_list[i]->end(), h.begin() + i * nsamp);
+            const int ld = nticks / 2 + 1;
+            m_d_cf.ensure(sizeof(float) * 2 * (size_t)ld * m_cf_list.size());
+            device_check(wct_channel_factors((float*)m_d_cf.p, ld, h.data(), (int)m_cf_list.size(), (int)nsamp,
+                                             ewave.data(), nticks, m_stream),
+                         "channel factors");
+            m_d_cfrows.ensure(sizeof(int) * (size_t)nchans);
+            m_cf_ld = ld;
+            m_cf_tmin = crb.minval;
+            make_plan();   // the table may have moved
+        }
+        device_check(wct_memcpy_htod(m_d_cfrows.p, rows.data(), sizeof(int) * (size_t)nchans, m_stream), "upload");
     }
 
     void release()
@@ -254,13 +324,24 @@ class Decon2DResponse : public ITensorSetFilter, public IConfigurable {
         m_plan = nullptr;
         m_kernel = nullptr;
         m_stream = nullptr;
-        for (DevBuf* b : {&m_d_a, &m_d_b, &m_d_spec, &m_d_ws}) b->release();
+        for (DevBuf* b : {&m_d_a, &m_d_b, &m_d_spec, &m_d_ws, &m_d_cf, &m_d_cfrows}) b->release();
         m_h_io.release();
         m_nchans = m_nticks = -1;
         m_period = 0;
+        m_cf_index.clear();
+        m_cf_list.clear();
+        m_cf_ld = 0;
+        m_cf_tmin = 0;
     }
 
     Configuration m_cfg;
+    std::shared_ptr<IChannelResponse> m_cresp;
+    std::map<std::vector<float>, int> m_cf_index;      // distinct channel responses -> table row
+    std::vector<const std::vector<float>*> m_cf_list;  // in table order
+    DevBuf m_d_cf, m_d_cfrows;
+    int m_cf_ld{0};
+    double m_cf_tmin{0};
+    double m_gain{0}, m_shaping{0};
     std::string m_fresp_tn{"FieldResponse"};
     int m_device{0};
     int m_nchans{-1}, m_nticks{-1};

@@ -110,9 +110,6 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
         m_field_response = get(config, "field_response", m_field_response);
         m_elecresponse_tn = get(config, "elecresponse", m_elecresponse_tn);
         m_per_chan_resp = get(config, "per_chan_resp", m_per_chan_resp);
-        if (!m_per_chan_resp.empty())
-            throw ConfigError("HipOmnibusSigProc: per-channel response correction (per_chan_resp = \"" +
-                              m_per_chan_resp + "\") is not supported; set it to \"\"");
         if (get(config, "fft_flag", 0) != 0)
             throw ConfigError("HipOmnibusSigProc: fft_flag != 0 (padded transforms) is not supported");
         m_fine_time_offset = get(config, "ftoffset", m_fine_time_offset);
@@ -391,6 +388,7 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
         wct_decon_kernel kernel{nullptr};
         bool tight{true};    // the job's own output is the tight decon (else the unfiltered frame)
         DevBuf f_base, f_tighter, f_extra[kNExtra];
+        DevBuf cf_rows;      // per-channel correction: table row of each wire
         bool has_tighter{false};
         bool has_extra[kNExtra]{false, false};
         int extra_restore[kNExtra]{0, 0};
@@ -407,11 +405,13 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
             pl.f_base.release();
             pl.f_tighter.release();
             for (auto& b : pl.f_extra) b.release();
+            pl.cf_rows.release();
             pl.has_tighter = false;
             for (auto& h : pl.has_extra) h = false;
             pl.active = false;
         }
-        for (DevBuf* b : {&m_d_adc, &m_d_out, &m_d_mask, &m_d_rms, &m_d_bad, &m_d_ws}) b->release();
+        for (DevBuf* b : {&m_d_adc, &m_d_out, &m_d_mask, &m_d_rms, &m_d_bad, &m_d_ws, &m_d_cf}) b->release();
+        m_cf_ld = 0;
         for (HostBuf* b : {&m_h_adc, &m_h_out, &m_h_mask, &m_h_rms}) b->release();
         for (int k = 0; k < kNExtra; ++k) {
             m_d_extra[k].release();
@@ -541,6 +541,7 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
             rms_off += pl.nw;
             pl.active = true;
         }
+        if (!m_per_chan_resp.empty()) build_channel_factors(nticks, period);
         m_adc_floats = off;
         m_rms_floats = rms_off;
         m_nticks = nticks;
@@ -559,6 +560,58 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
             }
         m_built_nticks = nticks;
         m_built_period = period;
+    }
+
+    // The channel-by-channel electronics correction of decon_2D_init
+    // (:949-982): the nominal electronics response sampled over nticks
+    // from the channel responses' start, one table row of factors per
+    // distinct channel response (built in FP64 on the device), and per
+    // plane the table row of each wire.  A channel without a response
+    // throws the response's KeyError, as in the reference.
+    void build_channel_factors(int nticks, double period)
+    {
+        auto cr = Factory::find_tn<IChannelResponse>(m_per_chan_resp);
+        const Binning crb = cr->channel_response_binning();
+        if (crb.binsize() != period)
+            throw SigProcError("OmnibusSigProc::decon_2D_init: channel response size mismatch");
+        const Binning tb(nticks, crb.minval, crb.minval + nticks * period);
+        std::vector<double> ewave;
+        auto iwf = Factory::find_tn<IWaveform>(m_elecresponse_tn);
+        if (auto ce = std::dynamic_pointer_cast<ColdElecResponse>(iwf))
+            ewave = ce->samples_f64(tb);
+        else {
+            auto f = iwf->waveform_samples(tb);
+            ewave.assign(f.begin(), f.end());
+        }
+        std::map<std::vector<float>, int> index;
+        std::vector<const std::vector<float>*> uniq;
+        size_t nsamp = 1;
+        for (int ip = 0; ip < 3; ++ip) {
+            Plane& pl = m_planes[ip];
+            if (!pl.active) continue;
+            std::vector<int> rows(pl.nw);
+            for (int w = 0; w < pl.nw; ++w) {
+                const auto& r = cr->channel_response(m_channels[ip][w]);
+                auto it = index.find(r);
+                if (it == index.end()) {
+                    it = index.emplace(r, (int)uniq.size()).first;
+                    uniq.push_back(&it->first);
+                    nsamp = std::max(nsamp, r.size());
+                }
+                rows[w] = it->second;
+            }
+            pl.cf_rows.ensure(sizeof(int) * rows.size());
+            hcheck(wct_memcpy_htod(pl.cf_rows.p, rows.data(), sizeof(int) * rows.size(), m_stream), "upload");
+        }
+        if (uniq.empty()) return;
+        std::vector<float> h(uniq.size() * nsamp, 0.0f);
+        for (size_t i = 0; i < uniq.size(); ++i) std::copy(uniq[i]->begin(), uniq[i]->end(), h.begin() + i * nsamp);
+        const int ld = nticks / 2 + 1;
+        m_d_cf.ensure(sizeof(float) * 2 * (size_t)ld * uniq.size());
+        hcheck(wct_channel_factors((float*)m_d_cf.p, ld, h.data(), (int)uniq.size(), (int)nsamp, ewave.data(),
+                                   nticks, m_stream),
+               "channel factors");
+        m_cf_ld = ld;
     }
 
     std::vector<double> filter_f64(const char* type, const char* name, int n) const
@@ -595,6 +648,11 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
             j.bad_hi = dbad + hi_off[ip];
             j.lf_tight = (const float*)pl.f_base.p;
             j.lf_tighter = pl.has_tighter ? (const float*)pl.f_tighter.p : nullptr;
+            if (m_cf_ld) {
+                j.chan_factors = (const float*)m_d_cf.p;
+                j.ld_chan_factors = m_cf_ld;
+                j.chan_factor_row = (const int*)pl.cf_rows.p;
+            }
             j.th_factor = pl.th_factor;
             j.restore_baseline = pl.tight ? 1 : 0;
             j.save_mode = m_use_roi_debug_mode ? 0 : 1;
@@ -664,6 +722,8 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
     void* m_stream{nullptr};
     wct_decon_plan m_plan{nullptr};
     DevBuf m_d_adc, m_d_out, m_d_mask, m_d_rms, m_d_bad, m_d_ws;
+    DevBuf m_d_cf;       // per-channel correction table
+    int m_cf_ld{0};      // its row stride (complex values), 0 without the correction
     HostBuf m_h_adc, m_h_out, m_h_mask, m_h_rms;
     DevBuf m_d_extra[kNExtra];
     HostBuf m_h_extra[kNExtra];

@@ -11,6 +11,7 @@
 //   IFilterWaveform iface/inc/WireCellIface/IFilterWaveform.h:12-18
 //   IWaveform       iface/inc/WireCellIface/IWaveform.h:16-29
 //   IFieldResponse  iface/inc/WireCellIface/IFieldResponse.h
+//   IChannelResponse iface/inc/WireCellIface/IChannelResponse.h:17-27
 //   IFrameSink, IFrameSource  iface/inc/WireCellIface/IFrameSink.h:11-21,
 //                   IFrameSource.h:12-20
 // IAnodeChannels stands in for the part of IAnodePlane that
@@ -164,6 +165,16 @@ class IWaveform : public IComponent<IWaveform> {
     virtual double waveform_period() const = 0;
     virtual const sequence_type& waveform_samples() const = 0;
     virtual sequence_type waveform_samples(const Binning& tbins) const = 0;
+};
+
+// A per-channel (electronics) response: samples on the binning that
+// channel_response_binning() returns.
+class IChannelResponse : public IComponent<IChannelResponse> {
+  public:
+    typedef std::vector<float> realseq_t;
+    virtual ~IChannelResponse() = default;
+    virtual const realseq_t& channel_response(int channel_ident) const = 0;
+    virtual Binning channel_response_binning() const = 0;
 };
 
 class IFieldResponse : public IComponent<IFieldResponse> {

@@ -148,6 +148,23 @@ int wcth_filter_waveform(const char* tn, int n, float* out)
     }
 }
 
+int wcth_channel_response(const char* tn, int channel, float* out, int cap, int* n, double* tmin,
+                          double* binsize)
+{
+    try {
+        auto cr = Factory::find_tn<IChannelResponse>(tn);
+        const auto& v = cr->channel_response(channel);
+        const Binning b = cr->channel_response_binning();
+        if (n) *n = (int)v.size();
+        if (tmin) *tmin = b.minval;
+        if (binsize) *binsize = b.nbins ? b.binsize() : 0.0;
+        if (out) std::copy(v.begin(), v.begin() + std::min<size_t>(v.size(), cap > 0 ? cap : 0), out);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
 void* wcth_frame_new(int ident, double time, double tick)
 {
     auto h = new FrameHandle;

@@ -44,6 +44,8 @@ class DeconJob(C.Structure):
         ("adjoint", C.c_int),
         ("spectrum", C.c_void_p), ("ld_spectrum", C.c_int),
         ("outputs", C.POINTER(DeconOutput)), ("noutputs", C.c_int),
+        ("chan_factors", C.c_void_p), ("ld_chan_factors", C.c_int),
+        ("chan_factor_row", C.c_void_p),
     ]
 
 
@@ -52,6 +54,7 @@ _V, _I, _D, _F, _Z = C.c_void_p, C.c_int, C.c_double, C.c_float, C.c_size_t
 _SIGS = {
     "wct_last_error": (C.c_char_p, []),
     "wct_abi_version": (_I, []),
+    "wct_channel_factors": (_I, [_V, _I, _V, _I, _I, _V, _I, _V]),
     "wct_device_count": (_I, [C.POINTER(C.c_int)]),
     "wct_set_device": (_I, [_I]),
     "wct_malloc": (_I, [C.POINTER(C.c_void_p), _Z]),

@@ -65,6 +65,16 @@ def filter_waveform(tn, n):
     return out
 
 
+def channel_response(tn, channel):
+    """(samples, tmin, binsize) of an IChannelResponse's channel."""
+    import ctypes as C
+    n, t0, bs = C.c_int(), C.c_double(), C.c_double()
+    _check(lib().wcth_channel_response(tn.encode(), channel, None, 0, C.byref(n), C.byref(t0), C.byref(bs)))
+    out = np.zeros(n.value, np.float32)
+    _check(lib().wcth_channel_response(tn.encode(), channel, out.ctypes.data, n.value, None, None, None))
+    return out, t0.value, bs.value
+
+
 class Frame:
     """An IFrame handle (input under construction, or a filter result)."""
 

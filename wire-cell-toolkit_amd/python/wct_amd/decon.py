@@ -126,11 +126,14 @@ def _check_gpu(dev):
 
 def make_job(kernel, adc, out, mask=None, rms=None, th_factor=W.TH_FACTOR, restore=1,
              save_mode=0, lf_tight=None, lf_tighter=None, bad=None, adjoint=False, spectrum=None,
-             outputs=()):
+             outputs=(), chan=None):
     """One wct_decon_job.  adc/out/mask/rms are torch tensors (2D for the
     frames); bad is (ptr, lo, hi) int32 device tensors or None.  With
     spectrum (a complex64 (nw, nt) tensor) the job writes the time
-    spectrum of the deconvolved frame instead (out must be None).
+    spectrum of the deconvolved frame instead (out must be None).  chan
+    is (table, rows) for the per-channel electronics correction: table a
+    complex64 (nresp, >= kc) tensor from channel_factors(), rows an
+    int32 (nw,) tensor of table rows (-1: no correction).
 
     Every tensor is checked against the kernel's (nw, nt) before any
     launch (ValueError on a mismatch), and the job keeps references to
@@ -144,6 +147,16 @@ def make_job(kernel, adc, out, mask=None, rms=None, th_factor=W.TH_FACTOR, resto
     j.adc = adc.data_ptr()
     j.ld_adc = adc.stride(0)
     keep = [kernel, adc]
+    if chan is not None:
+        table, rows = chan
+        _check_tensor("chan table", table, (None, None), (torch.complex64,), dev)
+        if table.shape[1] < kc or table.stride(1) != 1:
+            raise ValueError(f"chan table: rows of {table.shape[1]} bins, the kernel keeps {kc}")
+        _check_tensor("chan rows", rows, (nw,), (torch.int32,), dev)
+        j.chan_factors = table.data_ptr()
+        j.ld_chan_factors = table.stride(0)
+        j.chan_factor_row = rows.data_ptr()
+        keep += [table, rows]
     if spectrum is not None:
         if out is not None:
             raise ValueError("a spectrum-mode job has no out frame")
@@ -226,6 +239,23 @@ def make_job(kernel, adc, out, mask=None, rms=None, th_factor=W.TH_FACTOR, resto
     _check_gpu(dev)
     j._keep = keep
     return j
+
+
+def channel_factors(responses, elec, nt, device="cuda:0", stream=None):
+    """Per-channel correction table (wct_channel_factors): responses is
+    (nresp, nsamp) float32 channel responses, elec nt float64 samples of
+    the nominal electronics response; returns complex64 (nresp, nt/2+1)
+    rows of elec(k) / C_r(k) on the device (0 where C_r(k) == 0)."""
+    import numpy as np
+    import torch
+    r = np.ascontiguousarray(responses, np.float32)
+    e = np.ascontiguousarray(elec, np.float64)
+    if r.ndim != 2 or e.shape != (nt,):
+        raise ValueError("responses must be (nresp, nsamp) and elec nt long")
+    out = torch.empty((r.shape[0], nt // 2 + 1), dtype=torch.complex64, device=device)
+    check(lib().wct_channel_factors(out.data_ptr(), out.stride(0), r.ctypes.data, r.shape[0], r.shape[1],
+                                    e.ctypes.data, nt, _stream() if stream is None else stream))
+    return out
 
 
 class Batch:

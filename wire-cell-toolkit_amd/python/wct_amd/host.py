@@ -24,6 +24,8 @@ _SIGS = {
     "wcth_known_types": (C.c_int, [C.c_char_p, C.c_size_t]),
     "wcth_dft": (C.c_int, [C.c_char_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "wcth_filter_waveform": (C.c_int, [C.c_char_p, C.c_int, C.c_void_p]),
+    "wcth_channel_response": (C.c_int, [C.c_char_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                        C.c_void_p]),
     "wcth_frame_new": (C.c_void_p, [C.c_int, C.c_double, C.c_double]),
     "wcth_frame_free": (None, [C.c_void_p]),
     "wcth_frame_add_trace": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]),
