@@ -120,6 +120,27 @@ int wcth_tensorset_tensor_md(void* set, int i, char* metadata_json, size_t cap);
  * (first, last) pairs; returns the pair count, -2 if cap was exceeded. */
 int wcth_roi_lists(const unsigned char* mask, int nw, int nt, long ld, int pad, float asy,
                    int* roi_count, int* roi_pairs, int cap);
+/* ROI_formation::find_ROI_loose + extend_ROI_loose
+ * (sigproc/src/ROI_formation.cxx:538-937) over a plane's loose decon r
+ * (nw rows of nt floats at row stride ld).  bad_ptr[nw+1], bad_lo,
+ * bad_hi: per-row inclusive bad tick ranges (CSR) or NULL.  rms: per-row
+ * RMS (the row's cal_RMS without its bad ticks, as the decon's extra
+ * output computes it) or NULL to compute it here.  tight_count[nw] +
+ * tight_pairs: the plane's tight ROIs (inclusive) to extend by, or NULL
+ * for none (the collection plane).  rebin, l_factor, l_max_th,
+ * l_factor1, l_short_length, l_jump_one_bin: OmnibusSigProc's keys of
+ * those names.  Output as wcth_roi_lists. */
+int wcth_loose_rois(const float* r, int nw, int nt, long ld, const int* bad_ptr, const int* bad_lo,
+                    const int* bad_hi, const float* rms, const int* tight_count, const int* tight_pairs,
+                    int rebin, double l_factor, double l_max_th, double l_factor1, int l_short_length,
+                    int l_jump_one_bin, int* roi_count, int* roi_pairs, int cap);
+/* ROI_refinement::apply_roi (sigproc/src/ROI_refinement.cxx:118-218) in
+ * place on r (nw rows of nt at row stride ld) with roi_count[nw] +
+ * roi_pairs inclusive ROIs per row; linear = 1 for the induction planes
+ * (a straight baseline between the ROI's end samples is subtracted),
+ * 0 for the collection plane.  Results are the reference's int
+ * truncations, stored as floats. */
+int wcth_apply_roi(float* r, int nw, int nt, long ld, const int* roi_count, const int* roi_pairs, int linear);
 
 #ifdef __cplusplus
 }

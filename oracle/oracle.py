@@ -59,6 +59,9 @@ def lib():
             "wo_roi_threshold_f": (None, [f32p, C.c_void_p, I, I, F, i32p, i32p, i32p, I, f64p, f64p, u8p]),
             "wo_roi_threshold_d": (None, [f64p, C.c_void_p, I, I, F, i32p, i32p, i32p, I, f64p, f64p, u8p]),
             "wo_roi_lists": (I, [u8p, I, I, I, F, i32p, i32p, I]),
+            "wo_loose_rois": (I, [f32p, I, I, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, I, D, D, D, I, I, i32p, i32p, I]),
+            "wo_apply_roi": (None, [f32p, I, I, i32p, i32p, I]),
             "wo_hf_filter": (None, [I, D, D, D, I, f64p]),
             "wo_lf_filter": (None, [I, D, D, f64p]),
             "wo_coldelec": (D, [D, D, D]),
@@ -200,6 +203,68 @@ def roi_lists(mask, pad=5, asy=0.1, cap=1 << 22):
         out.append([(int(pairs[2 * (k + i)]), int(pairs[2 * (k + i) + 1])) for i in range(cnt[w])])
         k += cnt[w]
     return out
+
+
+def _csr(lists, nw):
+    """Per-row lists of (a, b) pairs as CSR int32 arrays (ptr, flat pairs)."""
+    ptr = np.zeros(nw + 1, np.int32)
+    flat = []
+    for w in range(nw):
+        rows = lists[w] if w < len(lists) else []
+        ptr[w + 1] = ptr[w] + len(rows)
+        for a, b in rows:
+            flat += [a, b]
+    return ptr, np.ascontiguousarray(flat if flat else [0, 0], np.int32)
+
+
+def _unflat(cnt, pairs):
+    out, k = [], 0
+    for c in cnt:
+        out.append([(int(pairs[2 * (k + i)]), int(pairs[2 * (k + i) + 1])) for i in range(c)])
+        k += c
+    return out
+
+
+def loose_rois(r_data, bad=None, rms=None, tight=None, rebin=6, l_factor=3.5, l_max_th=10000.0,
+               l_factor1=0.7, l_short_length=3, l_jump_one_bin=0, cap=1 << 22):
+    """find_ROI_loose (+ extend_ROI_loose with the tight lists) of one
+    plane.  bad: per-row lists of inclusive (lo, hi) tick ranges; rms:
+    per-row RMS replacing cal_RMS; tight: per-row inclusive tight ROIs or
+    None (no extension, the collection plane)."""
+    r = np.ascontiguousarray(r_data, np.float32)
+    nw, nt = r.shape
+    keep = []
+
+    def ptr(a):
+        if a is None:
+            return None
+        keep.append(a)
+        return a.ctypes.data
+    bp = blo = bhi = None
+    if bad is not None:
+        p, flat = _csr(bad, nw)
+        bp, blo, bhi = ptr(p), ptr(np.ascontiguousarray(flat[0::2])), ptr(np.ascontiguousarray(flat[1::2]))
+    rp = ptr(None if rms is None else np.ascontiguousarray(rms, np.float64))
+    tp = tpairs = None
+    if tight is not None:
+        p, flat = _csr(tight, nw)
+        tp, tpairs = ptr(p), ptr(flat)
+    cnt = np.zeros(nw, np.int32)
+    pairs = np.zeros(2 * cap, np.int32)
+    tot = lib().wo_loose_rois(r, nw, nt, bp, blo, bhi, rp, tp, tpairs, rebin, l_factor, l_max_th, l_factor1,
+                              l_short_length, l_jump_one_bin, cnt, pairs, cap)
+    if tot < 0:
+        raise RuntimeError("ROI capacity exceeded")
+    return _unflat(cnt, pairs)
+
+
+def apply_roi(r_data, rois, linear):
+    """ROI_refinement::apply_roi on a copy of r_data (float32)."""
+    r = np.array(r_data, np.float32, copy=True, order="C")
+    nw, nt = r.shape
+    p, flat = _csr(rois, nw)
+    lib().wo_apply_roi(r, nw, nt, p, flat, int(linear))
+    return r
 
 
 # ---- response (util/src/Response.cxx, OmnibusSigProc.cxx:707-865) ----

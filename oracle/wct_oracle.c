@@ -18,6 +18,9 @@
  *   - a fused FP64 decon (mathematically identical; the FP64 truth)
  *   - cal_RMS + tight-ROI threshold (ROI_formation.cxx:364-531)
  *   - ROI run list, pad/merge and connect (ROI_formation.cxx:167-360)
+ *   - loose ROIs and their extension by the tight ones
+ *     (ROI_formation.cxx:538-937), ROI_refinement::apply_roi
+ *     (ROI_refinement.cxx:118-218)
  *   - Response::coldelec / ColdElec sampling (util/src/Response.cxx:277-335,
  *     gen/src/ColdElecResponse.cxx:44-49), HfFilter/LfFilter
  *     (sigproc/src/HfFilter.cxx:38-52, LfFilter.cxx:31-45,
@@ -445,6 +448,210 @@ int wo_roi_lists(const unsigned char* mask, int nw, int nt, int pad, float asy,
     }
     free(rows);
     return over ? -1 : tot;
+}
+
+/* ------------------------------------------------------------------ */
+/* Loose ROIs (ROI_formation::find_ROI_loose :687-937 with local_ave   */
+/* :598-618, find_ROI_end :620-653, find_ROI_begin :654-685 and        */
+/* extend_ROI_loose :538-596) and ROI_refinement::apply_roi            */
+/* (ROI_refinement.cxx:118-218).                                       */
+/* ------------------------------------------------------------------ */
+static double wo_local_ave(const float* s, int n, int bin, int width)
+{
+    double sum1 = 0, sum2 = 0;
+    for (int i = -width; i < width + 1; i++) {
+        int c = bin + i;
+        while (c < 0) c += n;
+        while (c >= n) c -= n;
+        sum1 += s[c];
+        sum2++;
+    }
+    return sum2 > 0 ? sum1 / sum2 : 0;
+}
+
+static int wo_roi_end(const float* s, int n, int bin, double th, int jump)
+{
+    int end = bin;
+    double content = s[end];
+    while (content > th) {
+        end++;
+        content = end >= n ? s[end - n] : s[end];
+        if (end >= n - 1) { end = n - 1; break; }
+    }
+    while (wo_local_ave(s, n, end + 1, 1) < wo_local_ave(s, n, end, 1) + 25 ||
+           (jump && wo_local_ave(s, n, end + 2, 1) < wo_local_ave(s, n, end, 1) + 25)) {
+        end++;
+        if (end >= n - 1) { end = n - 1; break; }
+    }
+    return end;
+}
+
+static int wo_roi_begin(const float* s, int n, int bin, double th, int jump)
+{
+    int begin = bin;
+    double content = s[begin];
+    while (content > th) {
+        begin--;
+        content = begin < 0 ? s[begin + n] : s[begin];
+        if (begin <= 0) { begin = 0; break; }
+    }
+    while (wo_local_ave(s, n, begin - 1, 1) < wo_local_ave(s, n, begin, 1) + 25 ||
+           (jump && wo_local_ave(s, n, begin - 2, 1) < wo_local_ave(s, n, begin, 1) + 25)) {
+        begin--;
+        if (begin <= 0) { begin = 0; break; }
+    }
+    return begin;
+}
+
+static int wo_in_ranges(const int* lo, const int* hi, int q0, int q1, int t)
+{
+    for (int q = q0; q < q1; ++q)
+        if (t >= lo[q] && t <= hi[q]) return 1;
+    return 0;
+}
+
+/* r_data: nw x nt (the loose decon).  bad_ptr[nw+1], bad_lo, bad_hi:
+ * per-row inclusive tick ranges, or NULL.  rms: per-row RMS to use in
+ * place of cal_RMS of the row without its bad ticks, or NULL.
+ * tight_ptr[nw+1], tight_pairs: the plane's tight ROIs (inclusive) for
+ * extend_ROI_loose, or NULL for none (the collection plane).  Output
+ * flattened as wo_roi_lists.  The merge loop of :861-900 is not run (its
+ * flag starts at 0). */
+int wo_loose_rois(const float* r_data, int nw, int nt, const int* bad_ptr, const int* bad_lo,
+                  const int* bad_hi, const double* rms, const int* tight_ptr, const int* tight_pairs,
+                  int rebin, double l_factor, double l_max_th, double l_factor1, int l_short_length,
+                  int l_jump_one_bin, int* roi_count, int* roi_pairs, int cap)
+{
+    float* sig = (float*)malloc(sizeof(float) * (nt > 0 ? nt : 1));
+    float* sig1 = (float*)malloc(sizeof(float) * (nt > 0 ? nt : 1));
+    const int n2 = rebin > 0 ? nt / rebin : 0;
+    float* s2 = (float*)malloc(sizeof(float) * (n2 > 0 ? n2 : 1));
+    int tot = 0, over = 0;
+    for (int w = 0; w < nw; ++w) {
+        const float* row = r_data + (size_t)w * nt;
+        const int q0 = bad_ptr ? bad_ptr[w] : 0, q1 = bad_ptr ? bad_ptr[w + 1] : 0;
+        int nc = 0;
+        for (int t = 0; t < nt; ++t) {
+            if (q1 > q0 && wo_in_ranges(bad_lo, bad_hi, q0, q1, t)) {
+                sig1[t] = 0;
+            } else {
+                sig[nc++] = row[t];
+                sig1[t] = row[t];
+            }
+        }
+        for (int i = 0; i < n2; ++i) {
+            double temp = 0;
+            for (int j = 0; j < rebin; ++j) temp += sig1[rebin * i + j];
+            s2[i] = (float)temp;
+        }
+        const double r = rms ? rms[w] : wo_cal_rms_f(sig, (size_t)nc);
+        float th = (float)(r * rebin * l_factor);
+        if (th > l_max_th) th = (float)l_max_th;
+        wo_pairs rois = {0, 0, NULL};
+        for (int j = 1; j < n2 - 1; j++) {
+            const double content = s2[j], prev = s2[j - 1], next = s2[j + 1];
+            int flag = 0, begin = 0, end = 0, max_bin = 0;
+            if (content > th) {
+                begin = wo_roi_begin(s2, n2, j, th * l_factor1, l_jump_one_bin);
+                end = wo_roi_end(s2, n2, j, th * l_factor1, l_jump_one_bin);
+                max_bin = begin;
+                for (int k = begin; k <= end; k++)
+                    if (s2[k] > s2[max_bin]) max_bin = k;
+                flag = 1;
+            } else if (content > prev && content > next) {
+                begin = wo_roi_begin(s2, n2, j, prev, l_jump_one_bin);
+                end = wo_roi_end(s2, n2, j, next, l_jump_one_bin);
+                max_bin = begin;
+                for (int k = begin; k <= end; k++)
+                    if (s2[k] > s2[max_bin]) max_bin = k;
+                const float rise = s2[max_bin] - s2[begin] + s2[max_bin] - s2[end];
+                if (rise > th * 2) flag = 1;
+                int tb = max_bin - l_short_length;
+                if (tb < begin) tb = begin;
+                int te = max_bin + l_short_length;
+                if (te > end) te = end;
+                const float d0 = s2[max_bin] - s2[tb], d1 = s2[max_bin] - s2[te];
+                if (d0 > th * l_factor1 && d1 > th * l_factor1) flag = 1;
+            }
+            if (flag) {
+                if (rois.n > 0 && begin <= rois.v[2 * (rois.n - 1) + 1]) rois.v[2 * (rois.n - 1) + 1] = end;
+                else wo_push(&rois, begin, end);
+                j = end < n2 ? end : n2;
+            }
+        }
+        for (int i = 0; i < rois.n; ++i) {
+            rois.v[2 * i] *= rebin;
+            rois.v[2 * i + 1] = rois.v[2 * i + 1] * rebin + (rebin - 1);
+        }
+        if (tight_ptr) {
+            wo_pairs ext = {0, 0, NULL};
+            for (int i = 0; i < rois.n; ++i) {
+                int start = rois.v[2 * i], stop = rois.v[2 * i + 1];
+                for (int k = tight_ptr[w]; k < tight_ptr[w + 1]; ++k) {
+                    const int ts = tight_pairs[2 * k], te = tight_pairs[2 * k + 1];
+                    if (start > ts && start < te) start = ts;
+                    if (stop > ts && stop < te) stop = te;
+                }
+                if (ext.n == 0) wo_push(&ext, start, stop);
+                else if (start < ext.v[2 * (ext.n - 1) + 1]) ext.v[2 * (ext.n - 1) + 1] = stop;
+                else wo_push(&ext, start, stop);
+            }
+            free(rois.v);
+            rois = ext;
+        }
+        roi_count[w] = rois.n;
+        for (int i = 0; i < rois.n; ++i) {
+            if (tot < cap) { roi_pairs[2 * tot] = rois.v[2 * i]; roi_pairs[2 * tot + 1] = rois.v[2 * i + 1]; }
+            else over = 1;
+            ++tot;
+        }
+        free(rois.v);
+    }
+    free(sig);
+    free(sig1);
+    free(s2);
+    return over ? -1 : tot;
+}
+
+/* The int conversion of a float as x86-64 performs it (cvttss2si):
+ * truncation toward zero, NaN and values outside the int range give
+ * INT_MIN. */
+static int wo_x86_trunc(float v)
+{
+    if (v >= -2147483648.0f && v < 2147483648.0f) return (int)v;
+    return (int)0x80000000u;
+}
+
+/* apply_roi on one plane (r_data nw x nt, in place): samples outside
+ * every ROI become 0; ROIs (CSR roi_ptr[nw+1], inclusive pairs) are
+ * applied in order, so a later one overwrites an overlap.  linear = 1
+ * (induction, :120-166): sample i of ROI [s, e] becomes the int of
+ * r(i) - ((r(e) - r(s)) * (i - s) / (e - s) + r(s)) in float arithmetic
+ * (a one-tick ROI computes 0 * 0 / 0); linear = 0 (collection,
+ * :167-202): the int of r(i). */
+void wo_apply_roi(float* r_data, int nw, int nt, const int* roi_ptr, const int* roi_pairs, int linear)
+{
+    float* out = (float*)malloc(sizeof(float) * (nt > 0 ? nt : 1));
+    for (int w = 0; w < nw; ++w) {
+        float* row = r_data + (size_t)w * nt;
+        for (int t = 0; t < nt; ++t) out[t] = 0;
+        for (int k = roi_ptr[w]; k < roi_ptr[w + 1]; ++k) {
+            const int sb = roi_pairs[2 * k], eb = roi_pairs[2 * k + 1];
+            const float sc = row[sb], ec = row[eb];
+            for (int i = sb; i < eb + 1; i++) {
+                float v;
+                if (linear) {
+                    const float slope_part = (ec - sc) * (float)(i - sb) / (float)(eb - sb);
+                    v = row[i] - (slope_part + sc);
+                } else {
+                    v = row[i];
+                }
+                out[i] = (float)wo_x86_trunc(v);
+            }
+        }
+        for (int t = 0; t < nt; ++t) row[t] = out[t];
+    }
+    free(out);
 }
 
 /* ------------------------------------------------------------------ */

@@ -71,6 +71,12 @@ void wo_roi_threshold_d(const double* r_data, const double* r_tight, int nw, int
 int wo_roi_lists(const unsigned char* mask, int nw, int nt, int pad, float asy,
                  int* roi_count, int* roi_pairs, int cap);
 
+int wo_loose_rois(const float* r_data, int nw, int nt, const int* bad_ptr, const int* bad_lo,
+                  const int* bad_hi, const double* rms, const int* tight_ptr, const int* tight_pairs,
+                  int rebin, double l_factor, double l_max_th, double l_factor1, int l_short_length,
+                  int l_jump_one_bin, int* roi_count, int* roi_pairs, int cap);
+void wo_apply_roi(float* r_data, int nw, int nt, const int* roi_ptr, const int* roi_pairs, int linear);
+
 void wo_hf_filter(int n, double max_freq, double sigma, double power, int flag, double* out);
 void wo_lf_filter(int n, double max_freq, double tau, double* out);
 double wo_coldelec(double time, double gain, double shaping);

@@ -327,6 +327,74 @@ int wcth_roi_lists(const unsigned char* mask, int nw, int nt, long ld, int pad, 
 }
 
 
+static std::vector<Hip::RoiList> unflatten(const int* count, const int* pairs, int nw)
+{
+    std::vector<Hip::RoiList> rows(nw);
+    int k = 0;
+    for (int w = 0; w < nw; ++w)
+        for (int i = 0; i < count[w]; ++i, ++k) rows[w].emplace_back(pairs[2 * k], pairs[2 * k + 1]);
+    return rows;
+}
+
+int wcth_loose_rois(const float* r, int nw, int nt, long ld, const int* bad_ptr, const int* bad_lo,
+                    const int* bad_hi, const float* rms, const int* tight_count, const int* tight_pairs,
+                    int rebin, double l_factor, double l_max_th, double l_factor1, int l_short_length,
+                    int l_jump_one_bin, int* roi_count, int* roi_pairs, int cap)
+{
+    if (!r || nw < 0 || nt < 0 || ld < nt || !roi_count || rebin < 1 || (bad_ptr && (!bad_lo || !bad_hi)) ||
+        (tight_count && !tight_pairs)) {
+        g_err = "loose_rois: bad arguments";
+        return -1;
+    }
+    try {
+        Hip::LooseParams lp;
+        lp.rebin = rebin;
+        lp.factor = l_factor;
+        lp.max_th = l_max_th;
+        lp.factor1 = l_factor1;
+        lp.short_length = l_short_length;
+        lp.jump_one_bin = l_jump_one_bin;
+        std::vector<Hip::RoiList> tight;
+        if (tight_count) tight = unflatten(tight_count, tight_pairs, nw);
+        auto rows = Hip::loose_rois(r, nw, nt, ld, bad_ptr, bad_lo, bad_hi, rms, tight_count ? &tight : nullptr, lp);
+        int tot = 0;
+        bool over = false;
+        for (int w = 0; w < nw; ++w) {
+            roi_count[w] = (int)rows[w].size();
+            for (const auto& q : rows[w]) {
+                if (tot < cap) {
+                    roi_pairs[2 * tot] = q.first;
+                    roi_pairs[2 * tot + 1] = q.second;
+                } else {
+                    over = true;
+                }
+                ++tot;
+            }
+        }
+        return over ? -2 : tot;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int wcth_apply_roi(float* r, int nw, int nt, long ld, const int* roi_count, const int* roi_pairs, int linear)
+{
+    if (!r || nw < 0 || nt < 0 || ld < nt || !roi_count || !roi_pairs) {
+        g_err = "apply_roi: bad arguments";
+        return -1;
+    }
+    auto rows = unflatten(roi_count, roi_pairs, nw);
+    for (int w = 0; w < nw; ++w)
+        for (const auto& q : rows[w])
+            if (q.first < 0 || q.second >= nt || q.first > q.second) {
+                g_err = "apply_roi: ROI [" + std::to_string(q.first) + ", " + std::to_string(q.second) +
+                        "] of row " + std::to_string(w) + " outside [0, " + std::to_string(nt) + ")";
+                return -1;
+            }
+    Hip::apply_roi(r, nw, nt, ld, rows, linear != 0);
+    return 0;
+}
+
 void* wcth_tensorset_new(int ident, const char* metadata_json)
 {
     try {

@@ -293,6 +293,66 @@ def roi_lists(mask, pad=5, asy=0.1, cap=1 << 22):
     return out
 
 
+def _csr(lists, nw):
+    cnt = np.array([len(lists[w]) if w < len(lists) else 0 for w in range(nw)], np.int32)
+    flat = [v for w in range(nw) for pair in (lists[w] if w < len(lists) else []) for v in pair]
+    return cnt, np.ascontiguousarray(flat if flat else [0, 0], np.int32)
+
+
+def _rows(cnt, pairs):
+    out, k = [], 0
+    for c in cnt:
+        out.append([(int(pairs[2 * (k + i)]), int(pairs[2 * (k + i) + 1])) for i in range(c)])
+        k += c
+    return out
+
+
+def loose_rois(r, bad=None, rms=None, tight=None, rebin=6, l_factor=3.5, l_max_th=10000.0, l_factor1=0.7,
+               l_short_length=3, l_jump_one_bin=0, cap=1 << 22):
+    """find_ROI_loose (+ extend_ROI_loose by the tight lists) of a plane's
+    loose decon r (nw x nt float32).  bad: per-row lists of inclusive
+    (lo, hi) tick ranges; rms: per-row RMS; tight: per-row inclusive ROIs
+    or None."""
+    a = np.ascontiguousarray(r, np.float32)
+    nw, nt = a.shape
+    keep = []
+    bp = blo = bhi = None
+    if bad is not None:
+        ptr = np.zeros(nw + 1, np.int32)
+        lo, hi = [], []
+        for w in range(nw):
+            for (x, y) in (bad[w] if w < len(bad) else []):
+                lo.append(x)
+                hi.append(y)
+            ptr[w + 1] = len(lo)
+        keep += [ptr, np.ascontiguousarray(lo or [0], np.int32), np.ascontiguousarray(hi or [0], np.int32)]
+        bp, blo, bhi = (k.ctypes.data for k in keep[-3:])
+    rp = None
+    if rms is not None:
+        keep.append(np.ascontiguousarray(rms, np.float32))
+        rp = keep[-1].ctypes.data
+    tc = tpairs = None
+    if tight is not None:
+        keep += list(_csr(tight, nw))
+        tc, tpairs = keep[-2].ctypes.data, keep[-1].ctypes.data
+    cnt = np.zeros(nw, np.int32)
+    pairs = np.zeros(2 * cap, np.int32)
+    tot = lib().wcth_loose_rois(a.ctypes.data, nw, nt, nt, bp, blo, bhi, rp, tc, tpairs, rebin, l_factor, l_max_th,
+                                l_factor1, l_short_length, l_jump_one_bin, cnt.ctypes.data, pairs.ctypes.data, cap)
+    if tot < 0:
+        raise WctError(lib().wcth_last_error().decode() if tot == -1 else "ROI capacity exceeded")
+    return _rows(cnt, pairs)
+
+
+def apply_roi(r, rois, linear):
+    """ROI_refinement::apply_roi on a copy of r (float32)."""
+    a = np.array(r, np.float32, copy=True, order="C")
+    nw, nt = a.shape
+    cnt, flat = _csr(rois, nw)
+    _check(lib().wcth_apply_roi(a.ctypes.data, nw, nt, nt, cnt.ctypes.data, flat.ctypes.data, int(linear)))
+    return a
+
+
 def field_response_json(planes, origin, tstart, period, speed):
     """A field response in the reference's JSON schema
     (util/src/Response.cxx:32-72).  planes: list of (planeid, location,

@@ -42,6 +42,10 @@ _SIGS = {
                                      C.POINTER(C.c_int)]),
     "wcth_frame_masks": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_int]),
+    "wcth_loose_rois": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
+                                  C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
+    "wcth_apply_roi": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_long, C.c_void_p, C.c_void_p, C.c_int]),
     "wcth_roi_lists": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_long, C.c_int, C.c_float,
                                  C.c_void_p, C.c_void_p, C.c_int]),
     "wcth_frame_tag_frame": (C.c_int, [C.c_void_p, C.c_char_p]),
