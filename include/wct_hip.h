@@ -80,6 +80,20 @@ int wct_dft_transpose_complex(const float* in, float* out, int nrows, int ncols,
  * to build response spectra in FP64. */
 int wct_dft_fwd1b_f64(const double* in, double* out, int nrows, int ncols, int axis, void* stream);
 int wct_dft_inv1b_f64(const double* in, double* out, int nrows, int ncols, int axis, void* stream);
+/* ---- failure-path hooks (tests) ----
+ * The reference has none (its errors are exceptions); these force the
+ * error paths of a caller from outside.
+ * wct_debug_inject: the entry point named entry (e.g. "wct_decon_plan_run")
+ * fails with status (< 0) on its (after + 1)-th call from now, once, with
+ * the error text "<entry>: injected failure (status N)"; a NULL or empty
+ * entry, or status 0, disarms it.  Entry points doing device work check
+ * it first, before their arguments.
+ * wct_debug_alloc_cap: every later device allocation the library makes
+ * above bytes (0: no cap) fails with WCT_ERR_NOMEM, as when the device is
+ * out of memory.  Both are process-wide. */
+int wct_debug_inject(const char* entry, int after, int status);
+int wct_debug_alloc_cap(size_t bytes);
+
 /* Whether the IDFT functions above take length n (1 <= n <= 2^20; any
  * length: those with a prime factor above 13 run Bluestein's chirp-z
  * algorithm on 5-smooth transforms, FftwDFT's any-length plans,

@@ -1,7 +1,9 @@
 // C ABI plumbing: error string, device/memory/stream helpers and the
 // per-device FFT plan cache.
+#include <atomic>
 #include <cstdarg>
 #include <cstring>
+#include <string>
 
 #include "common.hpp"
 
@@ -27,6 +29,65 @@ static bool small_primes_only(int n)
 }
 
 bool fft_supported_length(int n) { return small_primes_only(n); }
+
+// Failure-path hooks: one armed entry point (by name) fails with a given
+// status on its (after + 1)-th call from now; device allocations above
+// the cap fail as out of memory.
+static std::mutex g_hook_mutex;
+static std::string g_inject_entry;
+static int g_inject_after = 0, g_inject_status = 0;
+static std::atomic<bool> g_inject_armed{false};
+static std::atomic<size_t> g_alloc_cap{0};
+
+int inject_check(const char* entry)
+{
+    if (!g_inject_armed.load(std::memory_order_relaxed)) return 0;
+    std::lock_guard<std::mutex> lk(g_hook_mutex);
+    if (!g_inject_armed.load() || g_inject_entry != entry) return 0;
+    if (g_inject_after > 0) {
+        --g_inject_after;
+        return 0;
+    }
+    g_inject_armed = false;
+    set_error("%s: injected failure (status %d)", entry, g_inject_status);
+    return g_inject_status;
+}
+
+static int alloc_status(hipError_t e, size_t bytes)
+{
+    if (e == hipSuccess) return WCT_OK;
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        set_error("device allocation of %zu bytes: out of memory", bytes);
+        return WCT_ERR_NOMEM;
+    }
+    set_error("device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    return WCT_ERR_HIP;
+}
+
+static bool above_cap(size_t bytes)
+{
+    const size_t cap = g_alloc_cap.load(std::memory_order_relaxed);
+    if (cap && bytes > cap) {
+        set_error("device allocation of %zu bytes above the %zu-byte cap", bytes, cap);
+        return true;
+    }
+    return false;
+}
+
+int dev_alloc(void** p, size_t bytes)
+{
+    *p = nullptr;
+    if (above_cap(bytes)) return WCT_ERR_NOMEM;
+    return alloc_status(hipMalloc(p, bytes ? bytes : 1), bytes);
+}
+
+int dev_alloc_async(void** p, size_t bytes, hipStream_t s)
+{
+    *p = nullptr;
+    if (above_cap(bytes)) return WCT_ERR_NOMEM;
+    return alloc_status(hipMallocAsync(p, bytes ? bytes : 1, s), bytes);
+}
 
 int lds_limit_bytes()
 {
@@ -117,13 +178,13 @@ int get_plan(int n, bool is_double, const DevPlan** out)
         std::vector<double> tab;
         fill_twiddles<double>(dp.plan, tab);
         dp.tw_count = tab.size() / 2;
-        WCT_HIP_CHECK(hipMalloc(&dp.tw, tab.size() * sizeof(double)));
+        WCT_TRY(dev_alloc(&dp.tw, tab.size() * sizeof(double)));
         WCT_HIP_CHECK(hipMemcpy(dp.tw, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
     } else {
         std::vector<float> tab;
         fill_twiddles<float>(dp.plan, tab);
         dp.tw_count = tab.size() / 2;
-        WCT_HIP_CHECK(hipMalloc(&dp.tw, tab.size() * sizeof(float)));
+        WCT_TRY(dev_alloc(&dp.tw, tab.size() * sizeof(float)));
         WCT_HIP_CHECK(hipMemcpy(dp.tw, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     auto res = g_plans->emplace(key, dp);
@@ -150,7 +211,7 @@ int get_twiddle_line(int N, int count, bool is_double, const void** out)
             tab[2 * k] = cos(ang);
             tab[2 * k + 1] = sin(ang);
         }
-        WCT_HIP_CHECK(hipMalloc(&d, tab.size() * sizeof(double)));
+        WCT_TRY(dev_alloc(&d, tab.size() * sizeof(double)));
         WCT_HIP_CHECK(hipMemcpy(d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
     } else {
         std::vector<float> tab(2 * (size_t)count);
@@ -159,7 +220,7 @@ int get_twiddle_line(int N, int count, bool is_double, const void** out)
             tab[2 * k] = (float)cos(ang);
             tab[2 * k + 1] = (float)sin(ang);
         }
-        WCT_HIP_CHECK(hipMalloc(&d, tab.size() * sizeof(float)));
+        WCT_TRY(dev_alloc(&d, tab.size() * sizeof(float)));
         WCT_HIP_CHECK(hipMemcpy(d, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     (*g_lines)[key] = d;
@@ -213,14 +274,9 @@ int wct_set_device(int device)
 
 int wct_malloc(void** dptr, size_t bytes)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(dptr, WCT_ERR_ARG, "null pointer");
-    hipError_t e = hipMalloc(dptr, bytes ? bytes : 1);
-    if (e == hipErrorOutOfMemory) {
-        set_error("hipMalloc(%zu) out of memory", bytes);
-        return WCT_ERR_NOMEM;
-    }
-    WCT_HIP_CHECK(e);
-    return WCT_OK;
+    return dev_alloc(dptr, bytes);
 }
 
 int wct_free(void* dptr)
@@ -287,6 +343,7 @@ int wct_memcpy_dtoh_async(void* dst, const void* src, size_t bytes, void* stream
 
 int wct_stream_create(void** stream)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(stream, WCT_ERR_ARG, "null pointer");
     hipStream_t s;
     WCT_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -303,6 +360,29 @@ int wct_stream_destroy(void* stream)
 int wct_stream_sync(void* stream)
 {
     WCT_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
+    return WCT_OK;
+}
+
+int wct_debug_inject(const char* entry, int after, int status)
+{
+    std::lock_guard<std::mutex> lk(g_hook_mutex);
+    if (!entry || !*entry || status == 0) {
+        g_inject_armed = false;
+        g_inject_entry.clear();
+        return WCT_OK;
+    }
+    WCT_REQUIRE(after >= 0 && status < 0, WCT_ERR_ARG, "debug_inject: after %d must be >= 0 and status %d < 0",
+                after, status);
+    g_inject_entry = entry;
+    g_inject_after = after;
+    g_inject_status = status;
+    g_inject_armed = true;
+    return WCT_OK;
+}
+
+int wct_debug_alloc_cap(size_t bytes)
+{
+    g_alloc_cap = bytes;
     return WCT_OK;
 }
 

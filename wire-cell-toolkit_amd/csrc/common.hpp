@@ -41,6 +41,43 @@ struct Status {
         }                                                                                 \
     } while (0)
 
+// ---- failure-path hooks ----------------------------------------------
+// The status wct_debug_inject armed for this entry point's call, or 0.
+int inject_check(const char* entry);
+// First statement of every C-ABI entry point that does device work.
+#define WCT_ENTRY()                                                                       \
+    do {                                                                                  \
+        if (const int rc_ = ::wct::inject_check(__func__)) return rc_;                    \
+    } while (0)
+
+// Device allocations of the library: WCT_ERR_NOMEM (with the error text)
+// when HIP is out of memory or the request is above the cap that
+// wct_debug_alloc_cap set; other HIP failures give WCT_ERR_HIP.
+int dev_alloc(void** p, size_t bytes);
+int dev_alloc_async(void** p, size_t bytes, hipStream_t s);
+
+// A stream-ordered device temporary, released (hipFreeAsync on its
+// stream) when it goes out of scope, on early returns included.
+template <typename T>
+struct DevTemp {
+    T* p = nullptr;
+    hipStream_t s;
+    explicit DevTemp(hipStream_t stream) : s(stream) {}
+    DevTemp(const DevTemp&) = delete;
+    DevTemp& operator=(const DevTemp&) = delete;
+    ~DevTemp()
+    {
+        if (p) (void)hipFreeAsync(p, s);
+    }
+    int alloc(size_t count) { return dev_alloc_async(reinterpret_cast<void**>(&p), count * sizeof(T), s); }
+    operator T*() const { return p; }
+};
+
+#define WCT_TRY(expr)                                                                     \
+    do {                                                                                  \
+        if (const int rc_ = (expr)) return rc_;                                           \
+    } while (0)
+
 // Largest prime factor allowed by the LDS kernels.
 bool fft_supported_length(int n);
 

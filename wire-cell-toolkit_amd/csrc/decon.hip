@@ -337,6 +337,7 @@ int wct_overall_response(const double* fr_avg, int nr, int nin, const double* el
                          double adc_mv, double fine_time_offset, int nt, double* resp_out,
                          void* stream)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(fr_avg && elec && resp_out && nr > 0 && nin > 0 && nfine >= nin && nt > 0 &&
                     fine_period > 0 && coarse_period > 0,
                 WCT_ERR_ARG, "overall_response: bad arguments");
@@ -350,9 +351,9 @@ int wct_overall_response(const double* fr_avg, int nr, int nin, const double* el
         for (int i = 0; i < nin; ++i) h[2 * ((size_t)r * nfine + i)] = fr_avg[(size_t)r * nin + i];
     const double esc = inter_gain * adc_mv * (-1);
     for (int i = 0; i < nfine; ++i) h[2 * ((size_t)nr * nfine + i)] = elec[i] * esc;
-    double2* d = nullptr;
+    DevTemp<double2> d(s);
     const size_t bytes = h.size() * sizeof(double);
-    WCT_HIP_CHECK(hipMallocAsync((void**)&d, bytes, s));
+    WCT_TRY(d.alloc(h.size() / 2));
     WCT_HIP_CHECK(hipMemcpyAsync(d, h.data(), bytes, hipMemcpyHostToDevice, s));
     int rc = launch_rows<double>(d, d, nfine, nr + 1, nfine, -1, 1.0, s);
     if (rc) return rc;
@@ -364,7 +365,6 @@ int wct_overall_response(const double* fr_avg, int nr, int nin, const double* el
     if (rc) return rc;
     WCT_HIP_CHECK(hipMemcpyAsync(h.data(), d, (size_t)nr * nfine * 2 * sizeof(double),
                                  hipMemcpyDeviceToHost, s));
-    WCT_HIP_CHECK(hipFreeAsync(d, s));
     WCT_HIP_CHECK(hipStreamSynchronize(s));
     // fine shift (:816-831) and redigitise (:834-859) on the host: a
     // short serial interpolation per row, kept in the reference's form
@@ -430,6 +430,7 @@ __global__ void k_chan_factor(const double2* __restrict__ C, int nc, int nt, int
 int wct_channel_factors(float* out, int ld_out, const float* responses, int nresp, int nsamp,
                         const double* elec, int nt, void* stream)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(out && responses && elec && nresp > 0 && nsamp > 0 && nt >= 2, WCT_ERR_ARG,
                 "channel_factors: bad arguments");
     WCT_REQUIRE(nt <= kMaxDftLength, WCT_ERR_UNSUPPORTED, "channel_factors: %d ticks above %d", nt,
@@ -441,9 +442,8 @@ int wct_channel_factors(float* out, int ld_out, const float* responses, int nres
     // chunks of responses, each transformed with the electronics row
     const int chunk = std::min(nresp, 256);
     std::vector<double> h((size_t)(chunk + 1) * nt * 2);
-    double2* d = nullptr;
-    const size_t bytes = h.size() * sizeof(double);
-    WCT_HIP_CHECK(hipMallocAsync((void**)&d, bytes, s));
+    DevTemp<double2> d(s);
+    WCT_TRY(d.alloc(h.size() / 2));
     int rc = WCT_OK;
     for (int r0 = 0; r0 < nresp && rc == WCT_OK; r0 += chunk) {
         const int nc = std::min(chunk, nresp - r0);
@@ -475,8 +475,6 @@ int wct_channel_factors(float* out, int ld_out, const float* responses, int nres
             rc = WCT_ERR_HIP;
         }
     }
-    WCT_HIP_CHECK(hipFreeAsync(d, s));
-    WCT_HIP_CHECK(hipStreamSynchronize(s));
     return rc;
 }
 
@@ -499,12 +497,12 @@ static int kernel_create(wct_decon_kernel* out, const double* resp, int nr, int 
     hipStream_t s = as_stream(stream);
     const int kh = nt / 2 + 1;
     const size_t N = (size_t)nw * nt;
-    double2* R = nullptr;
-    double* tmp = nullptr;
-    WCT_HIP_CHECK(hipMallocAsync((void**)&R, N * sizeof(double2), s));
+    DevTemp<double2> R(s);
+    DevTemp<double> tmp(s);
+    WCT_TRY(R.alloc(N));
     WCT_HIP_CHECK(hipMemsetAsync(R, 0, N * sizeof(double2), s));
     const int nrr = std::min(nr, nw);
-    WCT_HIP_CHECK(hipMallocAsync((void**)&tmp, (size_t)nrr * nt * sizeof(double), s));
+    WCT_TRY(tmp.alloc((size_t)nrr * nt));
     WCT_HIP_CHECK(hipMemcpyAsync(tmp, resp, (size_t)nrr * nt * sizeof(double), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_real_to_cplx, dim3(((long)nrr * nt + 255) / 256), dim3(256), 0, s, tmp, R,
                        (long)nrr * nt);
@@ -518,26 +516,25 @@ static int kernel_create(wct_decon_kernel* out, const double* resp, int nr, int 
     double den_floor = 0.0;
     if (r_floor > 0 && !forward) {
         constexpr int nb = 512;
-        double* bmax = nullptr;
-        WCT_HIP_CHECK(hipMallocAsync((void**)&bmax, nb * sizeof(double), s));
+        DevTemp<double> bmax(s);
+        WCT_TRY(bmax.alloc(nb));
         hipLaunchKernelGGL(k_den_max, dim3(nb), dim3(kNT), 0, s, R, (long)N, bmax);
         WCT_HIP_CHECK(hipGetLastError());
         std::vector<double> hb(nb);
         WCT_HIP_CHECK(hipMemcpyAsync(hb.data(), bmax, nb * sizeof(double), hipMemcpyDeviceToHost, s));
-        WCT_HIP_CHECK(hipFreeAsync(bmax, s));
         WCT_HIP_CHECK(hipStreamSynchronize(s));
         double dmax = 0;
         for (double v : hb) dmax = std::max(dmax, v);
         den_floor = r_floor * r_floor * dmax;
     }
-    double *Fw = nullptr, *Ft = nullptr, *Fe = nullptr, *Kabs = nullptr, *cmax = nullptr;
-    double2* Kd = nullptr;
-    WCT_HIP_CHECK(hipMallocAsync((void**)&Fw, nw * sizeof(double), s));
-    WCT_HIP_CHECK(hipMallocAsync((void**)&Ft, nt * sizeof(double), s));
-    WCT_HIP_CHECK(hipMallocAsync((void**)&Fe, nt * sizeof(double), s));
-    WCT_HIP_CHECK(hipMallocAsync((void**)&Kd, (size_t)kh * nw * sizeof(double2), s));
-    WCT_HIP_CHECK(hipMallocAsync((void**)&Kabs, (size_t)kh * nw * sizeof(double), s));
-    WCT_HIP_CHECK(hipMallocAsync((void**)&cmax, kh * sizeof(double), s));
+    DevTemp<double> Fw(s), Ft(s), Fe(s), Kabs(s), cmax(s);
+    DevTemp<double2> Kd(s);
+    WCT_TRY(Fw.alloc(nw));
+    WCT_TRY(Ft.alloc(nt));
+    WCT_TRY(Fe.alloc(nt));
+    WCT_TRY(Kd.alloc((size_t)kh * nw));
+    WCT_TRY(Kabs.alloc((size_t)kh * nw));
+    WCT_TRY(cmax.alloc(kh));
     const long nk = (long)kh * nw;
     if (forward) {
         hipLaunchKernelGGL(k_build_fwd, dim3((nk + 255) / 256), dim3(256), 0, s, R, nw, nt, kh,
@@ -570,25 +567,23 @@ static int kernel_create(wct_decon_kernel* out, const double* resp, int nr, int 
     kk->nw = nw;
     kk->nt = nt;
     kk->kc = kc;
-    WCT_HIP_CHECK(hipGetDevice(&kk->device));
-    hipError_t e = hipMalloc((void**)&kk->K, (size_t)kc * nw * sizeof(float2));
-    if (e != hipSuccess) {
+    if (hipGetDevice(&kk->device) != hipSuccess) {
         delete kk;
-        set_error("kernel_create: hipMalloc failed: %s", hipGetErrorString(e));
-        return WCT_ERR_NOMEM;
+        set_error("kernel_create: no current device");
+        return WCT_ERR_HIP;
+    }
+    if (const int rc_a = dev_alloc((void**)&kk->K, (size_t)kc * nw * sizeof(float2))) {
+        delete kk;
+        return rc_a;
     }
     const long nkc = (long)kc * nw;
     hipLaunchKernelGGL(k_to_f32, dim3((nkc + 255) / 256), dim3(256), 0, s, Kd, kk->K, nkc);
-    WCT_HIP_CHECK(hipGetLastError());
-    WCT_HIP_CHECK(hipFreeAsync(R, s));
-    WCT_HIP_CHECK(hipFreeAsync(tmp, s));
-    WCT_HIP_CHECK(hipFreeAsync(Fw, s));
-    WCT_HIP_CHECK(hipFreeAsync(Ft, s));
-    WCT_HIP_CHECK(hipFreeAsync(Fe, s));
-    WCT_HIP_CHECK(hipFreeAsync(Kd, s));
-    WCT_HIP_CHECK(hipFreeAsync(Kabs, s));
-    WCT_HIP_CHECK(hipFreeAsync(cmax, s));
-    WCT_HIP_CHECK(hipStreamSynchronize(s));
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipFree(kk->K);
+        delete kk;
+        set_error("kernel_create: conversion kernel failed");
+        return WCT_ERR_HIP;
+    }
     *out = kk;
     return WCT_OK;
 }
@@ -597,6 +592,7 @@ int wct_decon_kernel_create(wct_decon_kernel* out, const double* resp, int nr, i
                             const double* wire_filter, const double* time_filter, int wire_shift,
                             int time_shift, double cut_tol, void* stream)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(time_filter, WCT_ERR_ARG, "kernel_create: no time filter");
     return kernel_create(out, resp, nr, nw, nt, wire_filter, time_filter, nullptr, wire_shift, time_shift,
                          cut_tol, 0.0, false, stream);
@@ -606,6 +602,7 @@ int wct_decon_kernel_create_unfiltered(wct_decon_kernel* out, const double* resp
                                        const double* wire_filter, const double* envelope, int wire_shift,
                                        int time_shift, double cut_tol, double r_floor, void* stream)
 {
+    WCT_ENTRY();
     return kernel_create(out, resp, nr, nw, nt, wire_filter, nullptr, envelope, wire_shift, time_shift,
                          cut_tol, r_floor, false, stream);
 }
@@ -613,6 +610,7 @@ int wct_decon_kernel_create_unfiltered(wct_decon_kernel* out, const double* resp
 int wct_forward_kernel_create(wct_decon_kernel* out, const double* resp, int nr, int nw, int nt,
                               int wire_shift, int time_shift, void* stream)
 {
+    WCT_ENTRY();
     return kernel_create(out, resp, nr, nw, nt, nullptr, nullptr, nullptr, wire_shift, time_shift, 0.0, 0.0,
                          true, stream);
 }
@@ -662,12 +660,14 @@ size_t wct_decon_workspace_size(const wct_decon_job* jobs, int njobs)
 int wct_decon_plan_create(wct_decon_plan* out, const wct_decon_job* jobs, int njobs,
                           void* workspace, size_t ws_bytes)
 {
+    WCT_ENTRY();
     return wct_decon_plan_create_tuned(out, jobs, njobs, workspace, ws_bytes, nullptr);
 }
 
 int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, int njobs,
                                 void* workspace, size_t ws_bytes, const wct_decon_tuning* tuning)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(out && jobs && njobs > 0, WCT_ERR_ARG, "decon_plan: no jobs");
     *out = nullptr;
     const size_t need = wct_decon_workspace_size(jobs, njobs);
@@ -877,13 +877,18 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
         p->tgroups.back().end = acc;
         if (!is_smooth5(ht[i].nw)) p->tgroups.back().smooth5 = false;
     }
-    hipError_t e = hipMalloc((void**)&p->jobs, njobs * sizeof(DJob));
-    if (e == hipSuccess) e = hipMalloc((void**)&p->jobs_t, njobs * sizeof(DJob));
-    if (e == hipSuccess) e = hipMemcpy(p->jobs, hr.data(), njobs * sizeof(DJob), hipMemcpyHostToDevice);
+    int ra = dev_alloc((void**)&p->jobs, njobs * sizeof(DJob));
+    if (!ra) ra = dev_alloc((void**)&p->jobs_t, njobs * sizeof(DJob));
+    if (ra) {
+        if (p->jobs) (void)hipFree(p->jobs);
+        delete p;
+        return ra;
+    }
+    hipError_t e = hipMemcpy(p->jobs, hr.data(), njobs * sizeof(DJob), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->jobs_t, ht.data(), njobs * sizeof(DJob), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        if (p->jobs) (void)hipFree(p->jobs);
-        if (p->jobs_t) (void)hipFree(p->jobs_t);
+        (void)hipFree(p->jobs);
+        (void)hipFree(p->jobs_t);
         delete p;
         set_error("decon_plan: job table upload failed: %s", hipGetErrorString(e));
         return WCT_ERR_HIP;
@@ -894,6 +899,7 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
 
 int wct_decon_plan_run(wct_decon_plan p, void* stream)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(p && p->jobs, WCT_ERR_ARG, "decon_plan_run: null plan");
     return plan_run(p, as_stream(stream));
 }
@@ -909,6 +915,7 @@ int wct_decon_plan_destroy(wct_decon_plan p)
 
 int wct_decon_graph_create(wct_decon_graph* out, const wct_decon_plan* plans, int nplans, void* stream)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(out && plans && nplans > 0, WCT_ERR_ARG, "decon_graph: no plans");
     *out = nullptr;
     for (int i = 0; i < nplans; ++i) WCT_REQUIRE(plans[i] && plans[i]->jobs, WCT_ERR_ARG, "decon_graph: null plan");
@@ -954,6 +961,7 @@ int wct_decon_graph_create(wct_decon_graph* out, const wct_decon_plan* plans, in
 
 int wct_decon_graph_launch(wct_decon_graph g, void* stream)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(g && g->exec, WCT_ERR_ARG, "decon_graph_launch: null graph");
     WCT_HIP_CHECK(hipGraphLaunch(g->exec, as_stream(stream)));
     return WCT_OK;
@@ -971,6 +979,7 @@ int wct_decon_graph_destroy(wct_decon_graph g)
 int wct_decon_run(const wct_decon_job* jobs, int njobs, void* workspace, size_t ws_bytes,
                   void* stream)
 {
+    WCT_ENTRY();
     wct_decon_plan p = nullptr;
     int rc = wct_decon_plan_create(&p, jobs, njobs, workspace, ws_bytes);
     if (rc) return rc;
@@ -989,6 +998,7 @@ int wct_decon_run(const wct_decon_job* jobs, int njobs, void* workspace, size_t 
 int wct_spectrum_filter(const float* spectrum, int ld_spectrum, int nw, int nt, const float* filter,
                         int restore_baseline, float* out, int ld_out, void* stream)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(spectrum && filter && out && nw > 0 && nt > 0 && nt % 2 == 0, WCT_ERR_ARG,
                 "spectrum_filter: bad arguments (nw %d, nt %d)", nw, nt);
     const int M = nt / 2;
@@ -1000,17 +1010,10 @@ int wct_spectrum_filter(const float* spectrum, int ld_spectrum, int nw, int nt, 
     const void* line = nullptr;
     rc = get_twiddle_line(nt, M + 1, false, &line);
     if (rc) return rc;
-    void* scratch = nullptr;
-    WCT_HIP_CHECK(hipMallocAsync(&scratch, spectrum_filter_scratch_bytes(M), s));
-    rc = spectrum_filter_run(spectrum, ld_spectrum, nw, nt, filter, restore_baseline, out, ld_out, pM,
-                             (const float2*)line, scratch, s);
-    const hipError_t ef = hipFreeAsync(scratch, s);
-    if (rc) return rc;
-    if (ef != hipSuccess) {
-        set_error("spectrum_filter: %s", hipGetErrorString(ef));
-        return WCT_ERR_HIP;
-    }
-    return WCT_OK;
+    DevTemp<unsigned char> scratch(s);
+    WCT_TRY(scratch.alloc(spectrum_filter_scratch_bytes(M)));
+    return spectrum_filter_run(spectrum, ld_spectrum, nw, nt, filter, restore_baseline, out, ld_out, pM,
+                               (const float2*)line, scratch, s);
 }
 
 // ---- the same without allocation ----------------------------------------
@@ -1047,6 +1050,7 @@ int scratch_address(void** out)
 
 int wct_noalloc_prepare(int nt)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(nt > 0 && nt % 2 == 0 && nt / 2 <= 8192, WCT_ERR_ARG, "noalloc_prepare: bad length %d", nt);
     const int M = nt / 2;
     WCT_REQUIRE(spectrum_filter_scratch_bytes(M) <= WCT_NOALLOC_SCRATCH_BYTES, WCT_ERR_UNSUPPORTED,
@@ -1065,6 +1069,7 @@ int wct_noalloc_prepare(int nt)
 int wct_spectrum_filter_noalloc(const float* spectrum, int ld_spectrum, int nw, int nt, const float* filter,
                                 int restore_baseline, float* out, int ld_out, void* stream)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(spectrum && filter && out && nw > 0 && nt > 0 && nt % 2 == 0, WCT_ERR_ARG,
                 "spectrum_filter_noalloc: bad arguments (nw %d, nt %d)", nw, nt);
     const int M = nt / 2;

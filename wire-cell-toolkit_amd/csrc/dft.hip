@@ -195,18 +195,15 @@ static int launch_transpose(const V* in, V* out, int nrows, int ncols, hipStream
 {
     const size_t bytes = (size_t)nrows * ncols * sizeof(V);
     V* dst = out;
-    V* tmp = nullptr;
+    DevTemp<V> tmp(s);
     if (in == out) {
-        WCT_HIP_CHECK(hipMallocAsync((void**)&tmp, bytes ? bytes : 1, s));
+        WCT_TRY(tmp.alloc((size_t)nrows * ncols));
         dst = tmp;
     }
     dim3 grid((ncols + 31) / 32, (nrows + 31) / 32);
     hipLaunchKernelGGL((k_transpose<V>), grid, dim3(kBlock), 0, s, in, dst, nrows, ncols);
     WCT_HIP_CHECK(hipGetLastError());
-    if (tmp) {
-        WCT_HIP_CHECK(hipMemcpyAsync(out, tmp, bytes, hipMemcpyDeviceToDevice, s));
-        WCT_HIP_CHECK(hipFreeAsync(tmp, s));
-    }
+    if (tmp) WCT_HIP_CHECK(hipMemcpyAsync(out, tmp, bytes, hipMemcpyDeviceToDevice, s));
     return WCT_OK;
 }
 
@@ -359,20 +356,28 @@ static int blue_kernel(int N, int L, int sign, const void** out, hipStream_t s)
         *out = it->second;
         return WCT_OK;
     }
+    // b is stream-ordered scratch when the table is kept in float
+    DevTemp<double2> scratch(s);
     double2* b = nullptr;
-    WCT_HIP_CHECK(hipMalloc((void**)&b, (size_t)L * sizeof(double2)));
+    if (dbl) {
+        WCT_TRY(dev_alloc((void**)&b, (size_t)L * sizeof(double2)));
+    } else {
+        WCT_TRY(scratch.alloc((size_t)L));
+        b = scratch;
+    }
     hipLaunchKernelGGL(k_blue_kernel, dim3((L + kBlock - 1) / kBlock), dim3(kBlock), 0, s, N, L, sign, b);
     WCT_HIP_CHECK(hipGetLastError());
     int rc = launch_rows<double>(b, b, L, 1, L, -1, 1.0, s);
-    if (rc) return rc;
+    if (rc) {
+        if (dbl) (void)hipFree(b);
+        return rc;
+    }
     void* res = b;
     if (!dbl) {
         float2* f = nullptr;
-        WCT_HIP_CHECK(hipMalloc((void**)&f, (size_t)L * sizeof(float2)));
+        WCT_TRY(dev_alloc((void**)&f, (size_t)L * sizeof(float2)));
         hipLaunchKernelGGL(k_d2f, dim3((L + kBlock - 1) / kBlock), dim3(kBlock), 0, s, b, f, L);
         WCT_HIP_CHECK(hipGetLastError());
-        WCT_HIP_CHECK(hipStreamSynchronize(s));
-        WCT_HIP_CHECK(hipFree(b));
         res = f;
     }
     WCT_HIP_CHECK(hipStreamSynchronize(s));
@@ -391,8 +396,8 @@ static int blue_rows(const void* in, void* out, int N, int nrows, long dist, int
     int rc = blue_kernel<T>(N, L, sign, &bhat, s);
     if (rc) return rc;
     const long total = (long)nrows * L;
-    T2* work = nullptr;
-    WCT_HIP_CHECK(hipMallocAsync((void**)&work, total * sizeof(T2), s));
+    DevTemp<T2> work(s);
+    WCT_TRY(work.alloc(total));
     const dim3 g((total + kBlock - 1) / kBlock);
     hipLaunchKernelGGL((k_blue_pre<T>), g, dim3(kBlock), 0, s, (const T2*)in, dist, N, L, total, sign, work);
     WCT_HIP_CHECK(hipGetLastError());
@@ -406,7 +411,6 @@ static int blue_rows(const void* in, void* out, int N, int nrows, long dist, int
     hipLaunchKernelGGL((k_blue_post<T>), dim3((tout + kBlock - 1) / kBlock), dim3(kBlock), 0, s, work, N, L, tout,
                        sign, scale, (T2*)out, dist);
     WCT_HIP_CHECK(hipGetLastError());
-    WCT_HIP_CHECK(hipFreeAsync(work, s));
     return WCT_OK;
 }
 
@@ -416,8 +420,8 @@ static int blue_cols(const void* in, void* out, int nrows, int ncols, int sign, 
 {
     using T2 = typename C2<T>::type;
     const size_t bytes = (size_t)nrows * ncols * sizeof(T2);
-    T2* t = nullptr;
-    WCT_HIP_CHECK(hipMallocAsync((void**)&t, bytes, s));
+    DevTemp<T2> t(s);
+    WCT_TRY(t.alloc(bytes / sizeof(T2)));
     dim3 g1((ncols + 31) / 32, (nrows + 31) / 32);
     hipLaunchKernelGGL((k_transpose<T2>), g1, dim3(kBlock), 0, s, (const T2*)in, t, nrows, ncols);
     WCT_HIP_CHECK(hipGetLastError());
@@ -426,7 +430,6 @@ static int blue_cols(const void* in, void* out, int nrows, int ncols, int sign, 
     dim3 g2((nrows + 31) / 32, (ncols + 31) / 32);
     hipLaunchKernelGGL((k_transpose<T2>), g2, dim3(kBlock), 0, s, (const T2*)t, (T2*)out, ncols, nrows);
     WCT_HIP_CHECK(hipGetLastError());
-    WCT_HIP_CHECK(hipFreeAsync(t, s));
     return WCT_OK;
 }
 
@@ -461,13 +464,12 @@ int launch_rows(const void* in, void* out, int n, int nrows, long dist, int sign
     WCT_HIP_CHECK(hipGetLastError());
     rc = launch_rows_lds<T>(out, out, n2, nrows * n1, n2, sign, scale, s);
     if (rc) return rc;
-    T2* tmp = nullptr;
-    WCT_HIP_CHECK(hipMallocAsync((void**)&tmp, total * sizeof(T2), s));
+    DevTemp<T2> tmp(s);
+    WCT_TRY(tmp.alloc(total));
     dim3 grid((n2 + 31) / 32, (n1 + 31) / 32, nrows);
-    hipLaunchKernelGGL((k_transpose<T2>), grid, dim3(kBlock), 0, s, (const T2*)o, tmp, n1, n2);
+    hipLaunchKernelGGL((k_transpose<T2>), grid, dim3(kBlock), 0, s, (const T2*)o, (T2*)tmp, n1, n2);
     WCT_HIP_CHECK(hipGetLastError());
     WCT_HIP_CHECK(hipMemcpyAsync(o, tmp, total * sizeof(T2), hipMemcpyDeviceToDevice, s));
-    WCT_HIP_CHECK(hipFreeAsync(tmp, s));
     return WCT_OK;
 }
 template int launch_rows<float>(const void*, void*, int, int, long, int, float, hipStream_t);
@@ -505,33 +507,39 @@ extern "C" {
 
 int wct_dft_fwd1d(const float* in, float* out, int size, void* stream)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(in && out && size > 0, WCT_ERR_ARG, "fwd1d: bad arguments");
     return launch_rows<float>(in, out, size, 1, size, -1, 1.0f, as_stream(stream));
 }
 
 int wct_dft_inv1d(const float* in, float* out, int size, void* stream)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(in && out && size > 0, WCT_ERR_ARG, "inv1d: bad arguments");
     return launch_rows<float>(in, out, size, 1, size, +1, 1.0f / (float)size, as_stream(stream));
 }
 
 int wct_dft_fwd1b(const float* in, float* out, int nrows, int ncols, int axis, void* stream)
 {
+    WCT_ENTRY();
     return dft_1b<float>(in, out, nrows, ncols, axis, -1, as_stream(stream));
 }
 
 int wct_dft_inv1b(const float* in, float* out, int nrows, int ncols, int axis, void* stream)
 {
+    WCT_ENTRY();
     return dft_1b<float>(in, out, nrows, ncols, axis, +1, as_stream(stream));
 }
 
 int wct_dft_fwd1b_f64(const double* in, double* out, int nrows, int ncols, int axis, void* stream)
 {
+    WCT_ENTRY();
     return dft_1b<double>(in, out, nrows, ncols, axis, -1, as_stream(stream));
 }
 
 int wct_dft_inv1b_f64(const double* in, double* out, int nrows, int ncols, int axis, void* stream)
 {
+    WCT_ENTRY();
     return dft_1b<double>(in, out, nrows, ncols, axis, +1, as_stream(stream));
 }
 
@@ -539,6 +547,7 @@ int wct_dft_inv1b_f64(const double* in, double* out, int nrows, int ncols, int a
 // (R = ncols) x (C = nrows) row-major array for the transform.
 int wct_dft_fwd2d(const float* in, float* out, int nrows, int ncols, void* stream)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(in && out && nrows > 0 && ncols > 0, WCT_ERR_ARG, "fwd2d: bad arguments");
     const int R = ncols, C = nrows;
     hipStream_t s = as_stream(stream);
@@ -549,6 +558,7 @@ int wct_dft_fwd2d(const float* in, float* out, int nrows, int ncols, void* strea
 
 int wct_dft_inv2d(const float* in, float* out, int nrows, int ncols, void* stream)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(in && out && nrows > 0 && ncols > 0, WCT_ERR_ARG, "inv2d: bad arguments");
     const int R = ncols, C = nrows;
     hipStream_t s = as_stream(stream);
@@ -559,12 +569,14 @@ int wct_dft_inv2d(const float* in, float* out, int nrows, int ncols, void* strea
 
 int wct_dft_transpose_real(const float* in, float* out, int nrows, int ncols, void* stream)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(in && out && nrows > 0 && ncols > 0, WCT_ERR_ARG, "transpose: bad arguments");
     return launch_transpose<float>(in, out, nrows, ncols, as_stream(stream));
 }
 
 int wct_dft_transpose_complex(const float* in, float* out, int nrows, int ncols, void* stream)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(in && out && nrows > 0 && ncols > 0, WCT_ERR_ARG, "transpose: bad arguments");
     return launch_transpose<float2>((const float2*)in, (float2*)out, nrows, ncols,
                                     as_stream(stream));

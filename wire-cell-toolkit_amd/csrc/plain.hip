@@ -312,6 +312,7 @@ int wct_plain_decon_plane(const float* adc, int ld_adc, int nw, int nt, const fl
                           const int* bad_hi, float* out, int ld_out, unsigned char* mask, int ld_mask, float* rms,
                           int save_mode, void* workspace, size_t ws_bytes, void* stream)
 {
+    WCT_ENTRY();
     WCT_REQUIRE(adc && resp && wire_filter && tight_filter && out && workspace, WCT_ERR_ARG,
                 "plain decon: null pointer");
     WCT_REQUIRE(nw > 0 && nt > 0 && nr > 0 && nr <= nw && ld_adc >= nt && ld_out >= nt &&

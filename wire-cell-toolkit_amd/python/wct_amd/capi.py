@@ -55,6 +55,8 @@ _SIGS = {
     "wct_last_error": (C.c_char_p, []),
     "wct_abi_version": (_I, []),
     "wct_channel_factors": (_I, [_V, _I, _V, _I, _I, _V, _I, _V]),
+    "wct_debug_inject": (_I, [C.c_char_p, _I, _I]),
+    "wct_debug_alloc_cap": (_I, [_Z]),
     "wct_device_count": (_I, [C.POINTER(C.c_int)]),
     "wct_set_device": (_I, [_I]),
     "wct_malloc": (_I, [C.POINTER(C.c_void_p), _Z]),
