@@ -53,8 +53,8 @@ def parse():
     ap.add_argument("--wire-tile", type=int, default=0,
                     help="bins per wire-pass tile with --no-tune (0 = the plan's default)")
     ap.add_argument("--row-plan", type=int, nargs=3, default=(0, 0, 0), metavar=("PLAN", "R2C", "C2R"),
-                    help="with --no-tune: row plan (0 runtime, 1 compile-time) and the r2c / c2r "
-                         "workgroups per CU of the compile-time plan")
+                    help="with --no-tune: row plan (-1 runtime, 0 default = 1 compile-time) and the r2c / "
+                         "c2r workgroups per CU of the compile-time plan (0 = its default)")
     ap.add_argument("--eager", action="store_true", help="time plan runs without the hipGraph")
     ap.add_argument("--adjoint-frames", type=int, default=16,
                     help="frames of 800 x 6000 in the timed adjoint gradient step (0 = skip)")

@@ -287,14 +287,15 @@ int wct_decon_plan_destroy(wct_decon_plan plan);
  *                 need <= 1023 wires, 2 and 4 take up to 2047; default 4)
  *   row_grid_pct  persistent grid of the row passes as a percentage of
  *                 the most workgroups the device keeps resident
- *   row_plan      1: rows of 3000 complex samples (6000 ticks) use
- *                 transforms compiled for that length (radix 20, 10, 15
- *                 with constant spans and twiddle offsets) instead of the
- *                 runtime radix plan; 0: the runtime plan
- *   r2c_blocks    with row_plan 1, workgroups per CU of the forward row
- *                 pass: 2, 3 (default) or 4
- *   c2r_blocks    with row_plan 1, workgroups per CU of the inverse row
- *                 pass: 2 or 3 (default)
+ *   row_plan      0 (default) or 1: rows of 3000 complex samples (6000
+ *                 ticks) use transforms compiled for that length (radix
+ *                 15, 20, 10 on an unpadded row: every LDS address a base
+ *                 plus an immediate); -1: the runtime radix plan for
+ *                 every length
+ *   r2c_blocks    with the compiled plan, workgroups per CU of the
+ *                 forward row pass: 3, 4 (default) or 5
+ *   c2r_blocks    with the compiled plan, workgroups per CU of the
+ *                 inverse row pass: 2, 3 (default) or 4
  * The autotuner (wct_amd.tune) times these per workload. */
 typedef struct {
     int wire_tile;

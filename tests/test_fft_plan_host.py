@@ -4,7 +4,8 @@ wire-cell-toolkit_amd/csrc/fft_lds.hpp is compiled for the host (its
 device functions made host-callable) into a small program that
   - runs every register butterfly Dft<R> (R = 2, 3, 4, 5, 8, 10, 15, 20 and
     the generic 7, 11, 13) against a direct DFT in both directions;
-  - runs full transforms through the radix schedule sf_radix with the
+  - runs full transforms through the radix schedule sf_radix (both its
+    orders: the default and the odd-first one of the unpadded row plans) with the
     Stockham pass formula of stockham_pass (y[(j - k) R + k + r Ns] =
     DFT_R(x[j + r n/R] w^(r k))) and the host twiddle-table layout, for
     lengths whose plans use the prime-factor radices 10, 15 and 20;
@@ -80,7 +81,7 @@ template <int R> void pass(std::vector<double2>& buf, int n, int Ns, const doubl
     buf = out;
 }
 
-double check_fft(int n, int sign)
+double check_fft(int n, int sign, bool of = false)
 {
     std::vector<double2> buf(n), tw;
     std::vector<cd> x(n);
@@ -91,8 +92,8 @@ double check_fft(int n, int sign)
     }
     // per pass: w^(r k) for k < Ns, r in [1, R), w = exp(-2 pi i/(Ns R))
     int Ns = 1;
-    for (int p = 0; sf_radix(n, p); ++p) {
-        const int R = sf_radix(n, p);
+    for (int p = 0; sf_radix(n, p, of); ++p) {
+        const int R = sf_radix(n, p, of);
         for (int k = 0; k < Ns; ++k)
             for (int r = 1; r < R; ++r) {
                 const double ang = -2.0 * M_PI * (double)((long)r * k % ((long)Ns * R)) / (double)(Ns * R);
@@ -104,9 +105,9 @@ double check_fft(int n, int sign)
         Ns *= R;
     }
     Ns = 1;
-    for (int p = 0; sf_radix(n, p); ++p) {
-        const int R = sf_radix(n, p);
-        const double2* t = tw.data() + sf_twoff(n, p);
+    for (int p = 0; sf_radix(n, p, of); ++p) {
+        const int R = sf_radix(n, p, of);
+        const double2* t = tw.data() + sf_twoff(n, p, of);
         switch (R) {
         case 2: pass<2>(buf, n, Ns, t, sign); break;
         case 3: pass<3>(buf, n, Ns, t, sign); break;
@@ -150,27 +151,33 @@ int main()
     const int ns[] = {10, 20, 40, 100, 250, 480, 800, 960, 1000, 1200, 3000, 4000, 5000, 1001, 8008, 2 * 15 * 20 * 7};
     bool first = true;
     for (int n : ns) {
-        printf("%s\"%d\": [%.3e, %.3e]", first ? "" : ", ", n, check_fft(n, -1), check_fft(n, 1));
+        printf("%s\"%d\": [%.3e, %.3e, %.3e, %.3e]", first ? "" : ", ", n, check_fft(n, -1), check_fft(n, 1),
+               check_fft(n, -1, true), check_fft(n, 1, true));
         first = false;
     }
-    printf("}, \"plans\": {");
-    first = true;
-    for (int n : {3000, 800, 960, 4000, 5000, 2048}) {
-        printf("%s\"%d\": [", first ? "" : ", ", n);
-        for (int p = 0; sf_radix(n, p); ++p) printf("%s%d", p ? ", " : "", sf_radix(n, p));
-        printf("]");
-        first = false;
+    for (int of = 0; of < 2; ++of) {
+        printf(of ? "}, \"plans_odd_first\": {" : "}, \"plans\": {");
+        first = true;
+        for (int n : {3000, 800, 960, 4000, 5000, 2048}) {
+            printf("%s\"%d\": [", first ? "" : ", ", n);
+            for (int p = 0; sf_radix(n, p, of); ++p) printf("%s%d", p ? ", " : "", sf_radix(n, p, of));
+            printf("]");
+            first = false;
+        }
     }
     int bad = 0, maxp = 0;
     for (int n = 2; n <= 20000; ++n) {
         int m = n;
         for (int q : {2, 3, 5, 7, 11, 13})
             while (m % q == 0) m /= q;
-        long prod = 1;
-        int p = 0;
-        for (; sf_radix(n, p); ++p) prod *= sf_radix(n, p);
-        maxp = p > maxp ? p : maxp;
-        if (m == 1 ? prod != n : p != 0) ++bad;
+        for (int of = 0; of < 2; ++of) {
+            long prod = 1;
+            int p = 0;
+            for (; sf_radix(n, p, of); ++p) prod *= sf_radix(n, p, of);
+            maxp = p > maxp ? p : maxp;
+            if (m == 1 ? prod != n : p != 0) ++bad;
+            if (of && m == 1 && sf_npass(n, true) != sf_npass(n)) ++bad;
+        }
     }
     printf("}, \"bad_lengths\": %d, \"max_passes\": %d}\n", bad, maxp);
     return 0;
@@ -198,8 +205,8 @@ def test_butterflies_match_direct_dft(report):
 
 
 def test_stockham_plans_match_direct_dft(report):
-    for n, (fwd, inv) in report["fft"].items():
-        assert fwd < 1e-13 and inv < 1e-13, (n, fwd, inv)
+    for n, errs in report["fft"].items():
+        assert max(errs) < 1e-13, (n, errs)      # both schedules, both directions
 
 
 def test_radix_schedule(report):
@@ -212,3 +219,8 @@ def test_radix_schedule(report):
     assert plans["4000"] == [20, 20, 10]
     assert plans["5000"] == [10, 10, 10, 5]
     assert plans["2048"] == [8, 8, 8, 4]
+    odd = report["plans_odd_first"]     # the same radices, odd ones first
+    assert odd["3000"] == [15, 20, 10]
+    assert odd["960"] == [15, 8, 8]
+    assert odd["5000"] == [5, 10, 10, 10]
+    assert odd["800"] == plans["800"] and odd["2048"] == plans["2048"]

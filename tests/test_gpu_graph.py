@@ -135,20 +135,20 @@ def test_static_wire_lengths_bit_identical_across_tiles(dev):
 def test_compile_time_row_plan_agrees(dev):
     """Rows of 6000 ticks on the transforms compiled for 3000 points
     (row_plan 1, at each occupancy choice) against the runtime radix
-    plan: the same passes with the multiply-adds ordered differently, so
-    FP32 rounding apart (relative RMS <= 1e-6; a mask bit may flip where
-    a sample sits on the threshold)."""
+    plan (row_plan -1): other pass orders and multiply-adds, so FP32
+    rounding apart (relative RMS <= 1e-6; a mask bit may flip where a
+    sample sits on the threshold)."""
     import torch
     from wct_amd import decon as D
     planes = [("ind", 800, 6000), ("col", 960, 6000), ("ind", 800, 8000)]
     print("compile-time row plan vs runtime plan", planes, flush=True)
     kern, adc, out, mask, rms, make_jobs = _setup(dev, planes)
-    b = D.Batch(make_jobs(range(3)))
+    b = D.Batch(make_jobs(range(3)), row_plan=-1)
     b.run()
     torch.cuda.synchronize()
     ref = _snapshot(out, mask, rms)
     b.close()
-    for r2c, c2r in ((2, 2), (3, 3), (4, 2), (0, 0)):
+    for r2c, c2r in ((3, 2), (4, 3), (5, 4), (0, 0)):
         for o in out:
             o.zero_()
         b = D.Batch(make_jobs(range(3)), row_plan=1, r2c_blocks=r2c, c2r_blocks=c2r)
@@ -174,7 +174,9 @@ def test_bad_tuning_is_refused(dev):
     with pytest.raises(WctError, match="row grid"):
         D.Batch(make_jobs(range(2)), row_grid_pct=5)
     with pytest.raises(WctError, match="r2c blocks"):
-        D.Batch(make_jobs(range(2)), r2c_blocks=3)
+        D.Batch(make_jobs(range(2)), row_plan=-1, r2c_blocks=3)
+    with pytest.raises(WctError, match="r2c blocks"):
+        D.Batch(make_jobs(range(2)), r2c_blocks=2)
     with pytest.raises(WctError, match="c2r blocks"):
         D.Batch(make_jobs(range(2)), row_plan=1, c2r_blocks=5)
     with pytest.raises(WctError, match="row plan"):

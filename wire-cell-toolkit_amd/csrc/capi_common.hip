@@ -105,11 +105,11 @@ int lds_limit_bytes()
 
 // The radix schedule is sf_radix's (fft_lds.hpp), shared with the
 // compile-time plans.
-static void factorize(int n, FftPlan& pl)
+static void factorize(int n, FftPlan& pl, bool odd_first)
 {
     int np = 0;
-    while (np < kMaxPasses && sf_radix(n, np)) {
-        pl.rad[np] = sf_radix(n, np);
+    while (np < kMaxPasses && sf_radix(n, np, odd_first)) {
+        pl.rad[np] = sf_radix(n, np, odd_first);
         ++np;
     }
     pl.npass = np;
@@ -143,20 +143,20 @@ static void fill_twiddles(const FftPlan& pl, std::vector<T>& tab)
         }
 }
 
-using PlanKey = std::tuple<int, int, bool>;   // device, n, double
+using PlanKey = std::tuple<int, int, bool, bool>;   // device, n, double, odd-first schedule
 static std::mutex g_plan_mutex;
 static std::map<PlanKey, DevPlan>* g_plans = new std::map<PlanKey, DevPlan>();
 using LineKey = std::tuple<int, int, int, bool>;
 static std::map<LineKey, void*>* g_lines = new std::map<LineKey, void*>();
 
-int get_plan(int n, bool is_double, const DevPlan** out)
+int get_plan(int n, bool is_double, const DevPlan** out, bool odd_first)
 {
     WCT_REQUIRE(fft_supported_length(n), WCT_ERR_UNSUPPORTED,
                 "FFT length %d has a prime factor above 13", n);
     int dev = 0;
     WCT_HIP_CHECK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_plan_mutex);
-    auto key = PlanKey(dev, n, is_double);
+    auto key = PlanKey(dev, n, is_double, odd_first);
     auto it = g_plans->find(key);
     if (it != g_plans->end()) {
         *out = &it->second;
@@ -164,7 +164,7 @@ int get_plan(int n, bool is_double, const DevPlan** out)
     }
     DevPlan dp;
     memset(&dp.plan, 0, sizeof(dp.plan));
-    factorize(n, dp.plan);
+    factorize(n, dp.plan, odd_first);
     int Ns = 1;
     size_t off = 0;
     for (int p = 0; p < dp.plan.npass; ++p) {
@@ -233,7 +233,7 @@ int find_plan(int n, bool is_double, const DevPlan** out)
     int dev = 0;
     WCT_HIP_CHECK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_plan_mutex);
-    auto it = g_plans->find(PlanKey(dev, n, is_double));
+    auto it = g_plans->find(PlanKey(dev, n, is_double, false));
     WCT_REQUIRE(it != g_plans->end(), WCT_ERR_STATE, "no transform tables for length %d on device %d", n, dev);
     *out = &it->second;
     return WCT_OK;

@@ -112,7 +112,7 @@ struct DevPlan {
 
 // Returns a cached plan (created on first use) for length n on the
 // current device.  is_double selects double2 twiddles.
-int get_plan(int n, bool is_double, const DevPlan** out);
+int get_plan(int n, bool is_double, const DevPlan** out, bool odd_first = false);
 
 // Twiddles exp(-2 pi i k / N) for k in [0, count), as T2, cached.
 int get_twiddle_line(int N, int count, bool is_double, const void** out);

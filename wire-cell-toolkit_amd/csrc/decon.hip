@@ -310,8 +310,10 @@ int spectrum_filter_run(const float* spectrum, int ld_spectrum, int nw, int nt, 
     hipLaunchKernelGGL(k_scale_filter, dim3((M + 1 + 255) / 256), dim3(256), 0, s, filter, g, M + 1,
                        1.0f / (float)nt);
     WCT_HIP_CHECK(hipGetLastError());
-    // a transient one-job plan on the stack
+    // a transient one-job plan on the stack, on the runtime row plan (the
+    // job carries no twiddles of the compile-time schedule)
     wct_decon_plan_s tmp;
+    tmp.row_plan = 0;
     tmp.jobs = static_cast<DJob*>(scratch);
     tmp.njobs = 1;
     tmp.rows = nw;
@@ -683,7 +685,7 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
         nw_all = std::max(nw_all, jobs[i].kernel->nw);
     }
     int Mmax = 0, tk = pick_tk(nw_all), nwmax = 0, twM = 0, twW = 0;
-    int row_pct = 100, row_plan = 0, r2c_blocks = 0, c2r_blocks = 0;
+    int row_pct = 100, row_plan = 1, r2c_blocks = 0, c2r_blocks = 0;
     if (tuning) {
         if (tuning->wire_tile) {
             WCT_REQUIRE(tuning->wire_tile == 2 || tuning->wire_tile == 4 || tuning->wire_tile == 8 ||
@@ -696,17 +698,17 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
                         "decon_plan: row grid percent %d not in [10, 100]", tuning->row_grid_pct);
             row_pct = tuning->row_grid_pct;
         }
-        WCT_REQUIRE(tuning->row_plan == 0 || tuning->row_plan == 1, WCT_ERR_ARG,
-                    "decon_plan: row plan %d not in {0, 1}", tuning->row_plan);
-        WCT_REQUIRE(tuning->r2c_blocks == 0 || (tuning->row_plan == 1 && tuning->r2c_blocks >= 2 &&
-                                                tuning->r2c_blocks <= 4),
-                    WCT_ERR_ARG, "decon_plan: r2c blocks %d not in {2, 3, 4} with the compile-time row plan",
+        WCT_REQUIRE(tuning->row_plan >= -1 && tuning->row_plan <= 1, WCT_ERR_ARG,
+                    "decon_plan: row plan %d not in {-1, 0, 1}", tuning->row_plan);
+        WCT_REQUIRE(tuning->r2c_blocks == 0 || (tuning->row_plan >= 0 && tuning->r2c_blocks >= 3 &&
+                                                tuning->r2c_blocks <= 5),
+                    WCT_ERR_ARG, "decon_plan: r2c blocks %d not in {3, 4, 5} with the compile-time row plan",
                     tuning->r2c_blocks);
-        WCT_REQUIRE(tuning->c2r_blocks == 0 || (tuning->row_plan == 1 && tuning->c2r_blocks >= 2 &&
-                                                tuning->c2r_blocks <= 3),
-                    WCT_ERR_ARG, "decon_plan: c2r blocks %d not in {2, 3} with the compile-time row plan",
+        WCT_REQUIRE(tuning->c2r_blocks == 0 || (tuning->row_plan >= 0 && tuning->c2r_blocks >= 2 &&
+                                                tuning->c2r_blocks <= 4),
+                    WCT_ERR_ARG, "decon_plan: c2r blocks %d not in {2, 3, 4} with the compile-time row plan",
                     tuning->c2r_blocks);
-        row_plan = tuning->row_plan;
+        row_plan = tuning->row_plan < 0 ? 0 : 1;
         r2c_blocks = tuning->r2c_blocks;
         c2r_blocks = tuning->c2r_blocks;
     }
@@ -812,6 +814,12 @@ int wct_decon_plan_create_tuned(wct_decon_plan* out, const wct_decon_job* jobs, 
         if (rc) return rc;
         d.planM = pM->plan;
         d.twM = (const float2*)pM->tw;
+        if (d.M == kStaticRowLen) {   // the compile-time row plan's schedule
+            const DevPlan* pL = nullptr;
+            rc = get_plan(d.M, false, &pL, true);
+            if (rc) return rc;
+            d.twL = (const float2*)pL->tw;
+        }
         d.planW = pW->plan;
         d.twW = (const float2*)pW->tw;
         const void* line = nullptr;

@@ -40,9 +40,10 @@ struct wct_decon_plan_s {
     int twM = 0, twW = 0;        // largest twiddle tables (complex entries)
     int ncu = 0;                 // compute units of the device
     int row_pct = 100;           // persistent row grids: percent of the resident maximum
-    int row_plan = 0;            // 1: compile-time row plans where compiled (kStaticRowLen)
-    int r2c_blocks = 0;          // workgroups per CU of a compile-time-plan r2c pass (2, 3, 4)
-    int c2r_blocks = 0;          // workgroups per CU of a compile-time-plan c2r pass (2, 3)
+    int row_plan = 0;            // 1: compile-time row plans where compiled (kStaticRowLen, jobs carry
+                                 // twL); 0: runtime plans (transient plans built outside plan_create)
+    int r2c_blocks = 0;          // workgroups per CU of a compile-time-plan r2c pass (3, 4, 5)
+    int c2r_blocks = 0;          // workgroups per CU of a compile-time-plan c2r pass (2, 3, 4)
     bool smooth5 = true;         // every length is 2^a 3^b 5^c
     bool spectrum = false;       // the third stage writes time spectra
 };
@@ -85,6 +86,7 @@ struct DJob {
     const float2* K;
     float2* S;
     const float2* twM;
+    const float2* twL;    // twiddles of the odd-first schedule (compile-time row plans)
     const float2* wN;
     const float2* twW;
     long row0;
@@ -174,6 +176,26 @@ inline RowLds row_lds(int Mmax, int twmax, bool stats, int nkeys = 0)
     return l;
 }
 
+// Row buffer layouts: the runtime plans keep a padded row (pidx); the
+// compile-time row plans (MS > 0) an unpadded one (fft_lin).  Slot i of
+// this thread is sample threadIdx.x + i NT.
+template <int NT, int MS>
+__device__ __forceinline__ int row_slot(int i)
+{
+    if constexpr (MS > 0)
+        return (int)threadIdx.x + i * NT;
+    else
+        return pidx_slot<NT>(pidx(threadIdx.x), i);
+}
+template <int MS>
+__device__ __forceinline__ int row_at(int k)
+{
+    if constexpr (MS > 0)
+        return k;
+    else
+        return pidx(k);
+}
+
 // ---- K1: real FFT along ticks --------------------------------------
 // Persistent: each block walks rows blockIdx.x, +gridDim.x, ...; the
 // twiddles are read through the caches (LDS holds only the row, which
@@ -223,13 +245,13 @@ __global__ __launch_bounds__(NT, BLK) void k_row_r2c(const DJob* __restrict__ jo
                     if (in_bad(J, w, 2 * n, false)) v.x = 0.f;
                     if (in_bad(J, w, 2 * n + 1, false)) v.y = 0.f;
                 }
-                buf[pidx_slot<NT>(pidx(threadIdx.x), i)] = v;
+                buf[row_slot<NT, MS>(i)] = v;
             }
         }
         if (r + gridDim.x < rend) fetch(r + gridDim.x);
         lds_barrier();
         if constexpr (MS > 0)
-            fft_static<float2, NT, MS>(buf, J.twM, -1, threadIdx.x);
+            fft_lin<float2, NT, MS>(buf, J.twL, -1, threadIdx.x);
         else
             fft_lds<float2, NT, MMAX, S5>(buf, J.planM, J.twM, -1, threadIdx.x);
         float2* dst = J.S + (long)w * J.ldk;
@@ -238,8 +260,8 @@ __global__ __launch_bounds__(NT, BLK) void k_row_r2c(const DJob* __restrict__ jo
             X.x = 0.f;
             X.y = 0.f;
             if (k < J.kc) {
-                const float2 Zk = buf[pidx(k == M ? 0 : k)];
-                const float2 Zm = buf[pidx(k == 0 ? 0 : M - k)];
+                const float2 Zk = buf[row_at<MS>(k == M ? 0 : k)];
+                const float2 Zm = buf[row_at<MS>(k == 0 ? 0 : M - k)];
                 // X = (Zk + conj Zm)/2 - i W^k (Zk - conj Zm)/2
                 const float sr = Zk.x + Zm.x, si = Zk.y - Zm.y;
                 const float dr = Zk.x - Zm.x, di = Zk.y + Zm.y;
@@ -330,7 +352,7 @@ __global__ __launch_bounds__(NT) void k_wire_pass(const DJob* __restrict__ jobs,
 
 // All threads load their Y[k], Y[M-k] pairs first (every load in
 // flight), then build Z' in LDS.
-template <int NT, int VH>
+template <int NT, int VH, int MS = 0>
 __device__ __forceinline__ void c2r_stage(const DJob& J, const float2* __restrict__ Y,
                                           const float* __restrict__ g, float2* buf)
 {
@@ -364,7 +386,7 @@ __device__ __forceinline__ void c2r_stage(const DJob& J, const float2* __restric
             float2 zz;
             zz.x = sr - pi;
             zz.y = si + pr;
-            buf[pidx_slot<NT>(pidx(threadIdx.x), i)] = zz;
+            buf[row_slot<NT, MS>(i)] = zz;
         }
     }
 }
@@ -390,10 +412,10 @@ __global__ __launch_bounds__(NT, BLK) void k_row_c2r_roi(const DJob* __restrict_
         const float2* Y = J.S + (long)w * J.ldk;
         const bool has_bad = BAD && J.bad_ptr && J.bad_ptr[w + 1] > J.bad_ptr[w];
 
-        c2r_stage<NT, VH>(J, Y, J.lf1, buf);
+        c2r_stage<NT, VH, MS>(J, Y, J.lf1, buf);
         lds_barrier();
         if constexpr (MS > 0)
-            fft_static<float2, NT, MS>(buf, twl, +1, threadIdx.x);
+            fft_lin<float2, NT, MS>(buf, J.twL, +1, threadIdx.x);
         else
             fft_lds<float2, NT, MMAX, S5>(buf, J.planM, twl, +1, threadIdx.x);
 
@@ -405,7 +427,7 @@ __global__ __launch_bounds__(NT, BLK) void k_row_c2r_roi(const DJob* __restrict_
             float2 z;
             z.x = z.y = 0.f;
             if (n < M) {
-                z = buf[pidx_slot<NT>(pidx(threadIdx.x), i)];
+                z = buf[row_slot<NT, MS>(i)];
                 valid |= 3ull << (2 * i);
             }
             val[2 * i] = z.x;
@@ -464,10 +486,10 @@ __global__ __launch_bounds__(NT, BLK) void k_row_c2r_roi(const DJob* __restrict_
 
         if (J.lf2) {   // the "tighter" variant for the mask (decon_2D_tighterROI, :1120-1160)
             lds_barrier();
-            c2r_stage<NT, VH>(J, Y, J.lf2, buf);
+            c2r_stage<NT, VH, MS>(J, Y, J.lf2, buf);
             lds_barrier();
             if constexpr (MS > 0)
-                fft_static<float2, NT, MS>(buf, twl, +1, threadIdx.x);
+                fft_lin<float2, NT, MS>(buf, J.twL, +1, threadIdx.x);
             else
                 fft_lds<float2, NT, MMAX, S5>(buf, J.planM, twl, +1, threadIdx.x);
 #pragma unroll
@@ -475,7 +497,7 @@ __global__ __launch_bounds__(NT, BLK) void k_row_c2r_roi(const DJob* __restrict_
                 const int n = threadIdx.x + i * NT;
                 float2 z;
                 z.x = z.y = 0.f;
-                if (n < M) z = buf[pidx_slot<NT>(pidx(threadIdx.x), i)];
+                if (n < M) z = buf[row_slot<NT, MS>(i)];
                 val[2 * i] = z.x;
                 val[2 * i + 1] = z.y;
             }
@@ -660,9 +682,9 @@ static int launch_rows_r2c(wct_decon_plan p, const wct_decon_group& g, hipStream
     if constexpr (VH == 12 && S5) {
         if (use_static_row(p, g, VH, S5)) {
             switch (p->r2c_blocks) {
-            case 2: return launch_r2c_inst<VH, S5, kStaticRowLen, 2>(p, g, s);
-            case 4: return launch_r2c_inst<VH, S5, kStaticRowLen, 4>(p, g, s);
-            default: return launch_r2c_inst<VH, S5, kStaticRowLen, 3>(p, g, s);
+            case 3: return launch_r2c_inst<VH, S5, kStaticRowLen, 3>(p, g, s);
+            case 5: return launch_r2c_inst<VH, S5, kStaticRowLen, 5>(p, g, s);
+            default: return launch_r2c_inst<VH, S5, kStaticRowLen, 4>(p, g, s);
             }
         }
     }
@@ -694,8 +716,11 @@ static int launch_rows_c2r(wct_decon_plan p, const wct_decon_group& g, hipStream
     bool done = false;
     if constexpr (VH == 12 && S5) {
         if (use_static_row(p, g, VH, S5)) {
-            rc = p->c2r_blocks == 2 ? launch_c2r_inst<VH, S5, BAD, kStaticRowLen, 2>(p, g, s, lay)
-                                    : launch_c2r_inst<VH, S5, BAD, kStaticRowLen, 3>(p, g, s, lay);
+            switch (p->c2r_blocks) {
+            case 2: rc = launch_c2r_inst<VH, S5, BAD, kStaticRowLen, 2>(p, g, s, lay); break;
+            case 4: rc = launch_c2r_inst<VH, S5, BAD, kStaticRowLen, 4>(p, g, s, lay); break;
+            default: rc = launch_c2r_inst<VH, S5, BAD, kStaticRowLen, 3>(p, g, s, lay); break;
+            }
             done = true;
         }
     }

@@ -351,9 +351,12 @@ __device__ __forceinline__ void stage_twiddles(T2* dst, const T2* __restrict__ s
 // one or two of the 2s (radix 10 or 20), the rest of the 2s in 8s plus a
 // 4 or a 2.  The 5s take only as many 2s as save a pass, preferring
 // radix 10 over 20 (fewer registers for the same count).  Order: 8s, the
-// 4 or 2, 20s, 10s, 15s, 5s, 3s, 7s, 11s, 13s.  Returns 0 past the last
-// pass (and for a length with a prime factor above 13).
-constexpr int sf_radix(int n, int p)
+// 4 or 2, 20s, 10s, 15s, 5s, 3s, 7s, 11s, 13s; with odd_first (the
+// unpadded compile-time row plans) 15s, 5s and 3s come first, so that the
+// first pass's stores, strided by an odd radix, hit distinct LDS banks.
+// Returns 0 past the last pass (and for a length with a prime factor
+// above 13).
+constexpr int sf_radix(int n, int p, bool odd_first = false)
 {
     if (n < 2) return 0;
     int a = 0, b = 0, c = 0, m = n;
@@ -376,34 +379,37 @@ constexpr int sf_radix(int n, int p)
     const int l2 = a - x;
     const int cnt[11] = {l2 / 3, l2 % 3 == 2 ? 1 : 0, l2 % 3 == 1 ? 1 : 0, n20, n10, n15, n5, b - n15, e7, e11, e13};
     const int rad[11] = {8, 4, 2, 20, 10, 15, 5, 3, 7, 11, 13};
+    const int ord_even[11] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10};
+    const int ord_odd[11] = {5, 6, 7, 0, 1, 2, 3, 4, 8, 9, 10};
     int idx = p;
-    for (int q = 0; q < 11; ++q) {
+    for (int i = 0; i < 11; ++i) {
+        const int q = odd_first ? ord_odd[i] : ord_even[i];
         if (idx < cnt[q]) return rad[q];
         idx -= cnt[q];
     }
     return 0;
 }
 
-constexpr int sf_npass(int n)
+constexpr int sf_npass(int n, bool odd_first = false)
 {
     int p = 0;
-    while (sf_radix(n, p)) ++p;
+    while (sf_radix(n, p, odd_first)) ++p;
     return p;
 }
 
-constexpr int sf_span(int n, int p)
+constexpr int sf_span(int n, int p, bool odd_first = false)
 {
     int ns = 1;
-    for (int q = 0; q < p; ++q) ns *= sf_radix(n, q);
+    for (int q = 0; q < p; ++q) ns *= sf_radix(n, q, odd_first);
     return ns;
 }
 
-constexpr int sf_twoff(int n, int p)
+constexpr int sf_twoff(int n, int p, bool odd_first = false)
 {
     int off = 0, ns = 1;
     for (int q = 0; q < p; ++q) {
-        off += ns * (sf_radix(n, q) - 1);
-        ns *= sf_radix(n, q);
+        off += ns * (sf_radix(n, q, odd_first) - 1);
+        ns *= sf_radix(n, q, odd_first);
     }
     return off;
 }
@@ -456,6 +462,77 @@ __device__ __forceinline__ void fft_static_from(T2* buf, const T2* __restrict__ 
                                                      sign, tid);
         fft_static_from<T2, NT, N, P + 1>(buf, tw, sign, tid);
     }
+}
+
+// One Stockham pass of a compile-time plan on an unpadded row: input r
+// of butterfly j is buf[j + r m], output r goes to buf[(j - k) R + k +
+// r NS] (k = j mod NS).  Every address is one base per butterfly plus a
+// constant, so the LDS instructions carry the offsets as immediates and
+// no per-sample index arithmetic is left.  Reads are contiguous across
+// the lanes; the stores of the first pass are strided by R, free of bank
+// conflicts when R is odd (the odd-first schedule puts 15s, 5s and 3s
+// first).
+template <typename T2, int R, int NT, int N, int NS>
+__device__ __forceinline__ void stockham_pass_lin(T2* buf, const T2* __restrict__ tw,
+                                                  const T2* __restrict__ gtab, int sign, int tid)
+{
+    constexpr int m = N / R;
+    constexpr int MAXB = (m + NT - 1) / NT;
+    static_assert((MAXB - 1) * NT + (R - 1) * m < 8192 && (R - 1) * NS < 8192, "LDS immediate offsets");
+    T2 v[MAXB][R];
+    const T2* src = buf + tid;
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int j = tid + b * NT;
+        if (m % NT == 0 || j < m) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = src[b * NT + r * m];
+        }
+    }
+    lds_barrier();
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int j = tid + b * NT;
+        if (m % NT == 0 || j < m) {
+            const int k = NS == 1 ? 0 : j % NS;
+            if (NS > 1) {
+                const T2* t = tw + k * (R - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    T2 w = t[r - 1];
+                    if (sign > 0) w.y = -w.y;
+                    v[b][r] = cmul(v[b][r], w);
+                }
+            }
+            Dft<T2, R>::run(v[b], sign, gtab + generic_tab_offset(R));
+            T2* dst = buf + (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[r * NS] = v[b][r];
+        }
+    }
+    lds_barrier();
+}
+
+template <typename T2, int NT, int N, int P>
+__device__ __forceinline__ void fft_lin_from(T2* buf, const T2* __restrict__ tw, int sign, int tid)
+{
+    if constexpr (P < sf_npass(N, true)) {
+        constexpr int R = sf_radix(N, P, true);
+        stockham_pass_lin<T2, R, NT, N, sf_span(N, P, true)>(buf, tw + sf_twoff(N, P, true),
+                                                             tw + sf_twoff(N, sf_npass(N, true), true), sign, tid);
+        fft_lin_from<T2, NT, N, P + 1>(buf, tw, sign, tid);
+    }
+}
+
+// Full transform of length N (a compile-time constant) of an unpadded
+// LDS row (entry i at buf[i], in and out), on the odd-first radix
+// schedule with that schedule's twiddle table (get_plan with odd_first).
+// Every thread of the group calls it.
+template <typename T2, int NT, int N>
+__device__ __forceinline__ void fft_lin(T2* buf, const T2* __restrict__ tw, int sign, int tid)
+{
+    static_assert(sf_npass(N, true) > 0 && sf_npass(N, true) <= kMaxPasses, "length with prime factors above 13");
+    fft_lin_from<T2, NT, N, 0>(buf, tw, sign, tid);
 }
 
 // Full transform of length N (a compile-time constant) in LDS; same

@@ -5,9 +5,9 @@ candidates on the device, replaying the whole step as one hipGraph, and
 returns the fastest with every measurement.  A candidate is
 (APAs per launch group, wire-pass tile width, persistent grid fraction
 of the row passes, row plan, r2c workgroups per CU, c2r workgroups per
-CU): the row plan chooses between the runtime radix plan and the
-transforms compiled for 3000-point rows (radix 20, 10, 15 with constant
-spans), each at its occupancy choices.  Launch shapes give bit-identical
+CU): the row plan chooses between the runtime radix plan (-1) and the
+transforms compiled for 3000-point rows (radix 15, 20, 10 on an unpadded
+row, 1), the latter at its occupancy choices.  Launch shapes give bit-identical
 outputs; the two row plans order their multiply-adds differently, so
 they agree to FP32 rounding.
 
@@ -22,7 +22,7 @@ from . import decon as D
 # A half-size persistent row grid measured 50-60 % slower than the full one
 # on the 150-APA workload, so the default space keeps the full grid and
 # spends the candidates on plan size, wire tile and row plan instead.
-ROW_PLANS = ((0, 0, 0), (1, 2, 2), (1, 2, 3), (1, 3, 2), (1, 3, 3), (1, 4, 2), (1, 4, 3))
+ROW_PLANS = ((-1, 0, 0), (1, 4, 3), (1, 3, 3), (1, 5, 3), (1, 4, 2))
 DEFAULT_SPACE = {"chunk_apas": (8, 16, 32, 64, 150), "wire_tile": (2, 4, 8, 16), "row_grid_pct": (100,),
                  "row_plan": ROW_PLANS}
 
@@ -98,10 +98,10 @@ def tune(make_jobs, nplanes, space=None, reps=2, log=None, planes_per_apa=3):
         return ms
 
     # stage 1: row plans
-    for rp in space.get("row_plan", ((0, 0, 0),)):
+    for rp in space.get("row_plan", ((1, 0, 0),)):
         run((whole, tile0, grid0) + tuple(rp))
     ok = {c: v for c, v in results.items() if isinstance(v, float)}
-    best_rp = min(ok, key=ok.get)[3:] if ok else (0, 0, 0)
+    best_rp = min(ok, key=ok.get)[3:] if ok else (1, 0, 0)
     # stage 2: group size, wire tile and grid with that row plan
     for chunk, tile, grid in itertools.product(space["chunk_apas"], space["wire_tile"], space["row_grid_pct"]):
         run((chunk, tile, grid) + tuple(best_rp))
