@@ -11,8 +11,11 @@ are processed in chunks of --chunk APAs whose spectra share one
 workspace, three launches per chunk.
 
 Prints one JSON line (the driver's contract) with the roofline of the
-fused pipeline and the CPU baseline (the oracle's FP32 step-by-step
-restatement of the reference path, one plane per thread) beside it.
+dominant kernel (the stage with the longest launch, timed on its own
+with HIP events after the timed steps; the whole pipeline's figures sit
+under roofline.pipeline) and the CPU baseline (the oracle's FP32
+step-by-step restatement of the reference path, one plane per thread)
+beside it.
 """
 import argparse
 import json
@@ -176,6 +179,45 @@ def cpu_baseline(planes, nthreads):
             "cpu": info}
 
 
+STAGES = ("k_row_r2c (forward row transforms)", "k_wire_pass (wire transforms x K)",
+          "k_row_c2r_roi (inverse rows, baseline, RMS, ROI mask, outputs)")
+
+
+def stage_times(batches, reps):
+    """ms per step of each of the three launches of a step, timed with HIP
+    events on the stream the launches run on (each chunk's stages in
+    order, as a step runs them)."""
+    import torch
+    stream = torch.cuda.current_stream()
+    acc = [0.0, 0.0, 0.0]
+    for _ in range(reps):
+        for b in batches:
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            ev[0].record(stream)
+            for st in range(3):
+                b.run_stage(st)
+                ev[st + 1].record(stream)
+            torch.cuda.synchronize()
+            for st in range(3):
+                acc[st] += ev[st].elapsed_time(ev[st + 1])
+    return [a / reps for a in acc]
+
+
+def stage_alg_bytes(planes, kern):
+    """Algorithmic bytes of each stage per step: r2c reads the frame (4 B
+    per sample) and writes the kept bins (8 B each); the wire pass reads
+    and writes the kept bins and reads K; c2r reads the kept bins and
+    writes the output (4 B) and the mask (1 B) per sample."""
+    b = [0.0, 0.0, 0.0]
+    for t in planes:
+        kind, nw, nt = t
+        kc = kern[t].kc
+        b[0] += 4.0 * nw * nt + 8.0 * kc * nw
+        b[1] += 24.0 * kc * nw
+        b[2] += 8.0 * kc * nw + 5.0 * nw * nt
+    return b
+
+
 def calibration_copy(dev, mib=1024):
     """One elementwise copy of known size (mib MiB read, mib MiB written):
     its FETCH_SIZE / WRITE_SIZE counters scale the decon kernels' to
@@ -190,10 +232,10 @@ def calibration_copy(dev, mib=1024):
 
 
 def hbm_traffic():
-    """HBM bytes per channel-tick of the fused pipeline from the
-    committed counter summary (FETCH_SIZE + WRITE_SIZE of the launches of
-    one 150-APA step run as one plan, scaled by the calibration copy), as
-    (bytes, channel-ticks), or None when absent."""
+    """HBM bytes of the fused pipeline from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE of the launches of one 150-APA step run as
+    one plan, scaled by the calibration copy), as (bytes of the step,
+    [bytes of each stage], channel-ticks), or None when absent."""
     path = os.path.join(REPO, "profiles", "decon_fused_apa150_hbm_traffic.md")
     try:
         vals = {}
@@ -201,8 +243,9 @@ def hbm_traffic():
             for line in f:
                 if line.startswith("- ") and ":" in line:
                     k, v = line[2:].split(":", 1)
-                    vals[k.strip()] = v.strip().split()[0]
-        return float(vals["bytes_per_chunk"]), int(vals["channel_ticks_per_chunk"])
+                    vals[k.strip()] = v.strip()
+        stages = [float(x) for x in vals["stage_bytes_per_chunk"].split(",")]
+        return float(vals["bytes_per_chunk"]), stages, int(vals["channel_ticks_per_chunk"])
     except (OSError, KeyError, ValueError):
         return None
 
@@ -464,6 +507,9 @@ def main():
     achieved = B_ALG * ch_ticks / (gpu_ms / 1e3) / 1e9
     log(f"bench: {ms:.3f} ms/step wall, {gpu_ms:.3f} ms/step on the stream")
 
+    stage_ms = stage_times(batches, reps=max(2, min(args.steps, 5)))
+    log("bench: stages " + ", ".join(f"{m:.3f} ms" for m in stage_ms))
+
     plain = None
     if args.plain_apas > 0:
         pv, pms, pn = plain_leg(planes, frames, outs, masks, rmss, kern, args.plain_apas)
@@ -494,10 +540,15 @@ def main():
         log(f"bench: CPU baseline on {n} threads")
         cpu = cpu_baseline(planes, n)
 
-    traffic = None
+    traffic = stage_traffic = None
     tr = hbm_traffic()
     if tr is not None:
-        traffic = tr[0] * ch_ticks / tr[1]     # bytes per step
+        traffic = tr[0] * ch_ticks / tr[2]     # bytes per step
+        stage_traffic = [b * ch_ticks / tr[2] for b in tr[1]]
+    # the dominant kernel: the stage with the longest launch time
+    alg = stage_alg_bytes(planes, kern)
+    dom = max(range(3), key=lambda i: stage_ms[i])
+    dom_gbs = alg[dom] / (stage_ms[dom] / 1e3) / 1e9
 
     res = {
         "metric": "deconvolved channel-ticks/s (batched vs plain, 1 GPU) at <=1e-4 rel RMS err",
@@ -526,12 +577,19 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": dom_gbs,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "kernel": "fused decon pipeline (row r2c, wire pass, row c2r+ROI) per step",
+            "frac": dom_gbs / HBM_PEAK_GBS,
+            "traffic": None if stage_traffic is None else stage_traffic[dom],
+            "kernel": STAGES[dom],
+            "launch_ms": stage_ms[dom],
+            "alg_bytes": alg[dom],
+            "stages_ms": stage_ms,
+            "stages_alg_bytes": alg,
+            "stages_traffic": stage_traffic,
+            "pipeline": {"achieved": achieved, "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                         "alg_bytes_per_channel_tick": B_ALG},
             "measured_copy_gbs": copy_gbs,
         },
         "accuracy_fp64": accuracy,

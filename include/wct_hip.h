@@ -281,6 +281,11 @@ typedef struct wct_decon_plan_s* wct_decon_plan;
 int wct_decon_plan_create(wct_decon_plan* out, const wct_decon_job* jobs, int njobs,
                           void* workspace, size_t ws_bytes);
 int wct_decon_plan_run(wct_decon_plan plan, void* stream);
+/* One stage of wct_decon_plan_run (0: the forward row transforms, 1: the
+ * wire pass, 2: the inverse row pass with the statistics and outputs),
+ * for timing a stage on its own.  A run is the three stages in order on
+ * one stream; other orders leave the outputs undefined. */
+int wct_decon_plan_run_stage(wct_decon_plan plan, int stage, void* stream);
 int wct_decon_plan_destroy(wct_decon_plan plan);
 /* Launch-shape choices of a plan (0 = the default for that field):
  *   wire_tile     bins per wire-pass workgroup: 2, 4, 8 or 16 (16 and 8

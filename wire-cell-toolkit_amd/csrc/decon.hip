@@ -912,6 +912,16 @@ int wct_decon_plan_run(wct_decon_plan p, void* stream)
     return plan_run(p, as_stream(stream));
 }
 
+int wct_decon_plan_run_stage(wct_decon_plan p, int stage, void* stream)
+{
+    WCT_ENTRY();
+    WCT_REQUIRE(p && p->jobs, WCT_ERR_ARG, "decon_plan_run_stage: null plan");
+    WCT_REQUIRE(stage >= 0 && stage <= 2, WCT_ERR_ARG, "decon_plan_run_stage: stage %d not in [0, 2]", stage);
+    hipStream_t s = as_stream(stream);
+    if (stage == 1) return run_wire(p, s);
+    return run_rows(p, s, stage == 2);
+}
+
 int wct_decon_plan_destroy(wct_decon_plan p)
 {
     if (!p) return WCT_OK;

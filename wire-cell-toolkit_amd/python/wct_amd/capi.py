@@ -99,6 +99,7 @@ _SIGS = {
     "wct_spectrum_filter_noalloc": (_I, [_V, _I, _I, _I, _V, _I, _V, _I, _V]),
     "wct_decon_plan_create": (_I, [C.POINTER(C.c_void_p), C.POINTER(DeconJob), _I, _V, _Z]),
     "wct_decon_plan_run": (_I, [_V, _V]),
+    "wct_decon_plan_run_stage": (_I, [_V, _I, _V]),
     "wct_decon_plan_destroy": (_I, [_V]),
     "wct_decon_plan_create_tuned": (_I, [C.POINTER(C.c_void_p), C.POINTER(DeconJob), _I, _V, _Z,
                                          C.POINTER(DeconTuning)]),

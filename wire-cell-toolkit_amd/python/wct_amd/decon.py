@@ -286,6 +286,11 @@ class Batch:
     def run(self, stream=None):
         check(lib().wct_decon_plan_run(self.plan, _stream() if stream is None else stream))
 
+    def run_stage(self, stage, stream=None):
+        """One of the three launches of run() (0 r2c, 1 wire pass, 2 c2r);
+        run them in order."""
+        check(lib().wct_decon_plan_run_stage(self.plan, int(stage), _stream() if stream is None else stream))
+
     def close(self):
         if getattr(self, "plan", None):
             import torch
