@@ -35,8 +35,8 @@ __global__ __launch_bounds__(256) void k_row_spectrum(const DJob* __restrict__ j
     for (long r = rbeg + blockIdx.x; r < rend; r += gridDim.x) {
         const DJob& J = jobs[find_job_rows(jobs, njobs, r)];
         const int w = (int)(r - J.row0);
-        const float2* Y = J.S + (long)w * J.ldk;
-        float2* o = J.spec + (long)w * J.ld_spec;
+        const auto Y = G(J.S + (long)w * J.ldk);
+        const auto o = G(J.spec + (long)w * J.ld_spec);
         const float sc = (float)J.nt;
         for (int k = threadIdx.x; k < J.nt; k += blockDim.x) {
             const int kk = k <= J.M ? k : J.nt - k;

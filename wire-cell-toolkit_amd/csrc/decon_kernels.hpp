@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "gmem.hpp"
 #include "rowstats.hpp"
 
 struct wct_decon_kernel_s {
@@ -129,9 +130,10 @@ __device__ __forceinline__ int find_job_tiles(const DJob* jobs, int njobs, long 
 __device__ __forceinline__ bool in_bad(const DJob& J, int w, int t, bool inclusive)
 {
     if (!J.bad_ptr) return false;
-    const int q0 = J.bad_ptr[w], q1 = J.bad_ptr[w + 1];
+    const auto bp = G(J.bad_ptr), blo = G(J.bad_lo), bhi = G(J.bad_hi);
+    const int q0 = bp[w], q1 = bp[w + 1];
     for (int q = q0; q < q1; ++q) {
-        const int lo = J.bad_lo[q], hi = J.bad_hi[q];
+        const int lo = blo[q], hi = bhi[q];
         if (t >= lo && (inclusive ? t <= hi : t < hi)) return true;
     }
     return false;
@@ -222,7 +224,7 @@ __global__ __launch_bounds__(NT, BLK) void k_row_r2c(const DJob* __restrict__ jo
     auto fetch = [&](long rr) {
         const DJob& Jn = jobs[find_job_rows(jobs, njobs, rr)];
         const int wn = (int)(rr - Jn.row0);
-        const float2* src = reinterpret_cast<const float2*>(Jn.adc + (long)wn * Jn.ld_adc);
+        const auto src = G(reinterpret_cast<const float2*>(Jn.adc + (long)wn * Jn.ld_adc));
 #pragma unroll
         for (int i = 0; i < VH; ++i) {
             const int n = threadIdx.x + i * NT;
@@ -235,7 +237,7 @@ __global__ __launch_bounds__(NT, BLK) void k_row_r2c(const DJob* __restrict__ jo
         const int w = (int)(r - J.row0);
         const int M = J.M;
 
-        const bool has_bad = J.bad_ptr && J.bad_ptr[w + 1] > J.bad_ptr[w];
+        const bool has_bad = J.bad_ptr && G(J.bad_ptr)[w + 1] > G(J.bad_ptr)[w];
 #pragma unroll
         for (int i = 0; i < VH; ++i) {
             const int n = threadIdx.x + i * NT;
@@ -251,10 +253,10 @@ __global__ __launch_bounds__(NT, BLK) void k_row_r2c(const DJob* __restrict__ jo
         if (r + gridDim.x < rend) fetch(r + gridDim.x);
         lds_barrier();
         if constexpr (MS > 0)
-            fft_lin<float2, NT, MS>(buf, J.twL, -1, threadIdx.x);
+            fft_lin<float2, NT, MS>(buf, G(J.twL), -1, threadIdx.x);
         else
-            fft_lds<float2, NT, MMAX, S5>(buf, J.planM, J.twM, -1, threadIdx.x);
-        float2* dst = J.S + (long)w * J.ldk;
+            fft_lds<float2, NT, MMAX, S5>(buf, J.planM, G(J.twM), -1, threadIdx.x);
+        const auto dst = G(J.S + (long)w * J.ldk);
         for (int k = threadIdx.x; k < J.ldk; k += NT) {
             float2 X;
             X.x = 0.f;
@@ -265,13 +267,119 @@ __global__ __launch_bounds__(NT, BLK) void k_row_r2c(const DJob* __restrict__ jo
                 // X = (Zk + conj Zm)/2 - i W^k (Zk - conj Zm)/2
                 const float sr = Zk.x + Zm.x, si = Zk.y - Zm.y;
                 const float dr = Zk.x - Zm.x, di = Zk.y + Zm.y;
-                const float2 W = J.wN[k];
+                const float2 W = G(J.wN)[k];
                 const float wr = W.x * dr - W.y * di, wi = W.x * di + W.y * dr;
                 X.x = 0.5f * (sr + wi);
                 X.y = 0.5f * (si - wr);
             }
             dst[k] = X;
         }
+        lds_barrier();
+    }
+}
+
+// K1 on a compile-time row plan of MS complex samples (6000-tick rows).
+// Vector-memory counters retire in issue order and count stores too, so
+// a global load issued after the next row's prefetch cannot be waited
+// for without waiting for the prefetch.  Nothing in the row's transform
+// therefore reads global memory: the plan's twiddle table (one table per
+// length, shared by every job of the launch) is staged into LDS once per
+// workgroup, and the r2c post-process twiddles W^k of the first kWReg NT
+// bins are held in registers for the whole launch (bins past them, only
+// when more than kWReg NT are kept, read the table in global memory).
+// The row's job is found once, when its samples are fetched.
+constexpr int kWReg = 4;
+
+template <int MS>
+constexpr int lin_tw_entries()
+{
+    return sf_twoff(MS, sf_npass(MS, true), true) + kGenericTabSize;
+}
+
+template <int NT, int MS, int BLK>
+__global__ __launch_bounds__(NT, BLK) void k_row_r2c_lin(const DJob* __restrict__ jobs, int njobs,
+                                                         long rbeg, long rend)
+{
+    constexpr int VH = (MS + NT - 1) / NT;
+    constexpr int NTW = lin_tw_entries<MS>();
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* buf = reinterpret_cast<float2*>(smem_raw);
+    float2* tws = buf + MS;
+    long r = rbeg + blockIdx.x;
+    if (r >= rend) return;
+    int jn = find_job_rows(jobs, njobs, r);
+    {
+        const auto src = G(jobs[jn].twL);
+        for (int i = threadIdx.x; i < NTW; i += NT) tws[i] = src[i];
+    }
+    float2 wreg[kWReg];
+    {
+        const auto wN = G(jobs[jn].wN);   // M + 1 entries
+#pragma unroll
+        for (int i = 0; i < kWReg; ++i) {
+            const int k = threadIdx.x + i * NT;
+            wreg[i] = wN[k <= MS ? k : MS];
+        }
+    }
+    float2 z[VH];
+    auto fetch = [&](long rr, int j) {
+        const DJob& Jn = jobs[j];
+        const auto src = G(reinterpret_cast<const float2*>(Jn.adc + (rr - Jn.row0) * (long)Jn.ld_adc));
+#pragma unroll
+        for (int i = 0; i < VH; ++i) {
+            const int n = threadIdx.x + i * NT;
+            z[i] = src[n < MS ? n : MS - 1];   // in-row address; slots past the row are not stored
+        }
+    };
+    fetch(r, jn);
+    for (; r < rend; r += gridDim.x) {
+        const DJob& J = jobs[jn];
+        const int w = (int)(r - J.row0);
+        const bool has_bad = J.bad_ptr && G(J.bad_ptr)[w + 1] > G(J.bad_ptr)[w];
+#pragma unroll
+        for (int i = 0; i < VH; ++i) {
+            const int n = threadIdx.x + i * NT;
+            if (n < MS) {
+                float2 v = z[i];
+                if (has_bad) {   // load_data zeroes [lo, hi) of bad channels (:337-346)
+                    if (in_bad(J, w, 2 * n, false)) v.x = 0.f;
+                    if (in_bad(J, w, 2 * n + 1, false)) v.y = 0.f;
+                }
+                buf[n] = v;
+            }
+        }
+        const auto dst = G(J.S + (long)w * J.ldk);
+        const int kc = J.kc, ldk = J.ldk;
+        const auto wN = G(J.wN);
+        const long rn = r + gridDim.x;
+        if (rn < rend) {
+            jn = find_job_rows(jobs, njobs, rn);
+            fetch(rn, jn);
+        }
+        lds_barrier();
+        fft_lin<float2, NT, MS>(buf, tws, -1, threadIdx.x);
+        auto bin = [&](int k, float2 W) {
+            float2 X;
+            X.x = 0.f;
+            X.y = 0.f;
+            if (k < kc) {
+                const float2 Zk = buf[k == MS ? 0 : k];
+                const float2 Zm = buf[k == 0 ? 0 : MS - k];
+                // X = (Zk + conj Zm)/2 - i W^k (Zk - conj Zm)/2
+                const float sr = Zk.x + Zm.x, si = Zk.y - Zm.y;
+                const float dr = Zk.x - Zm.x, di = Zk.y + Zm.y;
+                const float wr = W.x * dr - W.y * di, wi = W.x * di + W.y * dr;
+                X.x = 0.5f * (sr + wi);
+                X.y = 0.5f * (si - wr);
+            }
+            dst[k] = X;
+        };
+#pragma unroll
+        for (int i = 0; i < kWReg; ++i) {
+            const int k = threadIdx.x + i * NT;
+            if (k < ldk) bin(k, wreg[i]);
+        }
+        for (int k = threadIdx.x + kWReg * NT; k < ldk; k += NT) bin(k, wN[k < kc ? k : 0]);
         lds_barrier();
     }
 }
@@ -288,8 +396,8 @@ __global__ __launch_bounds__(NT) void k_wire_pass(const DJob* __restrict__ jobs,
     const DJob& J = jobs[find_job_tiles(jobs, njobs, t)];
     const int k0 = (int)(t - J.tile0) * TK;
     const int nw = NWS ? NWS : J.nw;
-    float2* S = J.S;
-    stage_twiddles(twl, J.twW, J.planW.twcount, threadIdx.x, NT);
+    const auto S = G(J.S);
+    stage_twiddles(twl, G(J.twW), J.planW.twcount, threadIdx.x, NT);
     constexpr int NLD = ((NWS ? NWS : NWMAX) * TK + NT - 1) / NT;   // elements per thread, all loads in flight
     static_assert(NT % TK == 0 && (NT / TK) % 16 == 0, "wire tile layout");
     {
@@ -305,8 +413,8 @@ __global__ __launch_bounds__(NT) void k_wire_pass(const DJob* __restrict__ jobs,
             for (int q = 0; q < NLD; ++q) {
                 const int idx = threadIdx.x + q * NT;
                 if (idx < nw * TK && k < J.kc) {
-                    const int ri = J.cf_row[idx / TK];
-                    if (ri >= 0) t[q] = cmul(t[q], J.cf[(long)ri * J.ld_cf + k]);
+                    const int ri = G(J.cf_row)[idx / TK];
+                    if (ri >= 0) t[q] = cmul(t[q], G(J.cf)[(long)ri * J.ld_cf + k]);
                 }
             }
         }
@@ -327,7 +435,7 @@ __global__ __launch_bounds__(NT) void k_wire_pass(const DJob* __restrict__ jobs,
     else
         fft_lds<float2, SL, NWMAX, S5>(col, J.planW, twl, -1, st);
     const int k = k0 + slice;
-    const float2* Kk = J.K + (long)k * nw;
+    const auto Kk = G(J.K + (long)k * nw);
     for (int m = st; m < nw; m += SL) {
         float2 o;
         o.x = 0.f;
@@ -352,9 +460,8 @@ __global__ __launch_bounds__(NT) void k_wire_pass(const DJob* __restrict__ jobs,
 
 // All threads load their Y[k], Y[M-k] pairs first (every load in
 // flight), then build Z' in LDS.
-template <int NT, int VH, int MS = 0>
-__device__ __forceinline__ void c2r_stage(const DJob& J, const float2* __restrict__ Y,
-                                          const float* __restrict__ g, float2* buf)
+template <int NT, int VH, int MS = 0, typename YP, typename GP>
+__device__ __forceinline__ void c2r_stage(const DJob& J, YP Y, GP g, float2* buf)
 {
     const int M = J.M, kc = J.kc;
     float2 a[VH], b[VH];
@@ -381,7 +488,7 @@ __device__ __forceinline__ void c2r_stage(const DJob& J, const float2* __restric
             if (M - k == M) bv.y = 0.f;
             const float sr = av.x + bv.x, si = av.y - bv.y;
             const float dr = av.x - bv.x, di = av.y + bv.y;
-            const float2 W = J.wN[k];
+            const float2 W = G(J.wN)[k];
             const float pr = W.x * dr + W.y * di, pi = W.x * di - W.y * dr;
             float2 zz;
             zz.x = sr - pi;
@@ -408,14 +515,14 @@ __global__ __launch_bounds__(NT, BLK) void k_row_c2r_roi(const DJob* __restrict_
         const DJob& J = jobs[find_job_rows(jobs, njobs, r)];
         const int w = (int)(r - J.row0);
         const int M = J.M;
-        const float2* twl = J.twM;
-        const float2* Y = J.S + (long)w * J.ldk;
-        const bool has_bad = BAD && J.bad_ptr && J.bad_ptr[w + 1] > J.bad_ptr[w];
+        const auto twl = G(J.twM);
+        const auto Y = G(J.S + (long)w * J.ldk);
+        const bool has_bad = BAD && J.bad_ptr && G(J.bad_ptr)[w + 1] > G(J.bad_ptr)[w];
 
-        c2r_stage<NT, VH, MS>(J, Y, J.lf1, buf);
+        c2r_stage<NT, VH, MS>(J, Y, G(J.lf1), buf);
         lds_barrier();
         if constexpr (MS > 0)
-            fft_lin<float2, NT, MS>(buf, J.twL, +1, threadIdx.x);
+            fft_lin<float2, NT, MS>(buf, G(J.twL), +1, threadIdx.x);
         else
             fft_lds<float2, NT, MMAX, S5>(buf, J.planM, twl, +1, threadIdx.x);
 
@@ -463,7 +570,7 @@ __global__ __launch_bounds__(NT, BLK) void k_row_c2r_roi(const DJob* __restrict_
         for (int i = 0; i < VPT; ++i)   // slots past the row hold 0 < th
             if ((!has_bad || ((pm >> i) & 1ull)) && (double)val[i] > th) m1 |= 1ull << i;
 
-        float* o = J.out + (long)w * J.ld_out;
+        const auto o = G(J.out + (long)w * J.ld_out);
 #pragma unroll
         for (int i = 0; i < VH; ++i) {
             const int n = threadIdx.x + i * NT;
@@ -480,16 +587,16 @@ __global__ __launch_bounds__(NT, BLK) void k_row_c2r_roi(const DJob* __restrict_
                 float2 ab;
                 ab.x = a;
                 ab.y = b;
-                *reinterpret_cast<float2*>(o + 2 * n) = ab;
+                *reinterpret_cast<gptr<float2>>(o + 2 * n) = ab;
             }
         }
 
         if (J.lf2) {   // the "tighter" variant for the mask (decon_2D_tighterROI, :1120-1160)
             lds_barrier();
-            c2r_stage<NT, VH, MS>(J, Y, J.lf2, buf);
+            c2r_stage<NT, VH, MS>(J, Y, G(J.lf2), buf);
             lds_barrier();
             if constexpr (MS > 0)
-                fft_lin<float2, NT, MS>(buf, J.twL, +1, threadIdx.x);
+                fft_lin<float2, NT, MS>(buf, G(J.twL), +1, threadIdx.x);
             else
                 fft_lds<float2, NT, MMAX, S5>(buf, J.planM, twl, +1, threadIdx.x);
 #pragma unroll
@@ -511,18 +618,18 @@ __global__ __launch_bounds__(NT, BLK) void k_row_c2r_roi(const DJob* __restrict_
         }
 
         if (J.mask) {
-            unsigned char* mk = J.mask + (long)w * J.ld_mask;
+            const auto mk = G(J.mask + (long)w * J.ld_mask);
 #pragma unroll
             for (int i = 0; i < VH; ++i) {
                 const int n = threadIdx.x + i * NT;
                 if (n < M) {
                     const unsigned a0 = (unsigned)((m1 >> (2 * i)) & 1ull) | ((unsigned)((m2 >> (2 * i)) & 1ull) << 1);
                     const unsigned a1 = (unsigned)((m1 >> (2 * i + 1)) & 1ull) | ((unsigned)((m2 >> (2 * i + 1)) & 1ull) << 1);
-                    *reinterpret_cast<unsigned short*>(mk + 2 * n) = (unsigned short)(a0 | (a1 << 8));
+                    *reinterpret_cast<gptr<unsigned short>>(mk + 2 * n) = (unsigned short)(a0 | (a1 << 8));
                 }
             }
         }
-        if (J.rms && threadIdx.x == 0) J.rms[w] = rms;
+        if (J.rms && threadIdx.x == 0) G(J.rms)[w] = rms;
         lds_barrier();
     }
 }
@@ -548,14 +655,14 @@ __global__ __launch_bounds__(NT, 2) void k_row_c2r_var(const DJob* __restrict__ 
         if (J.nvar == 0) continue;   // uniform over the block
         const int w = (int)(r - J.row0);
         const int M = J.M;
-        const float2* Y = J.S + (long)w * J.ldk;
-        const bool has_bad = BAD && J.bad_ptr && J.bad_ptr[w + 1] > J.bad_ptr[w];
+        const auto Y = G(J.S + (long)w * J.ldk);
+        const bool has_bad = BAD && J.bad_ptr && G(J.bad_ptr)[w + 1] > G(J.bad_ptr)[w];
         for (int v = 0; v < J.nvar; ++v) {
             const DVar& V = J.var[v];
-            const float* g = (V.alt_rows && V.alt_rows[w]) ? V.g_alt : V.g;
+            const auto g = G((V.alt_rows && G(V.alt_rows)[w]) ? V.g_alt : V.g);
             c2r_stage<NT, VH>(J, Y, g, buf);
             lds_barrier();
-            fft_lds<float2, NT, MMAX, S5>(buf, J.planM, J.twM, +1, threadIdx.x);
+            fft_lds<float2, NT, MMAX, S5>(buf, J.planM, G(J.twM), +1, threadIdx.x);
             float val[VPT];
             uint64_t valid = 0;
 #pragma unroll
@@ -592,7 +699,7 @@ __global__ __launch_bounds__(NT, 2) void k_row_c2r_var(const DJob* __restrict__ 
                 }
             }
             if (V.out) {
-                float* o = V.out + (long)w * V.ld_out;
+                const auto o = G(V.out + (long)w * V.ld_out);
 #pragma unroll
                 for (int i = 0; i < VH; ++i) {
                     const int n = threadIdx.x + i * NT;
@@ -609,11 +716,11 @@ __global__ __launch_bounds__(NT, 2) void k_row_c2r_var(const DJob* __restrict__ 
                         float2 ab;
                         ab.x = a;
                         ab.y = b;
-                        *reinterpret_cast<float2*>(o + 2 * n) = ab;
+                        *reinterpret_cast<gptr<float2>>(o + 2 * n) = ab;
                     }
                 }
             }
-            if (V.rms && threadIdx.x == 0) V.rms[w] = rms;
+            if (V.rms && threadIdx.x == 0) G(V.rms)[w] = rms;
             lds_barrier();
         }
     }
@@ -652,6 +759,21 @@ static int dispatch_static_nw(int nw, int tk, F&& f)
     return f(std::integral_constant<int, 0>());
 }
 
+template <int MS, int BLK>
+static int launch_r2c_lin(wct_decon_plan p, const wct_decon_group& g, hipStream_t s)
+{
+    const size_t lds = (size_t)(MS + lin_tw_entries<MS>()) * sizeof(float2);
+    int rc = allow_lds(k_row_r2c_lin<kNT, MS, BLK>, lds);
+    if (rc) return rc;
+    int grid = 0;
+    rc = resident_grid(k_row_r2c_lin<kNT, MS, BLK>, kNT, lds, p->ncu, g.end - g.begin, p->row_pct, &grid);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_row_r2c_lin<kNT, MS, BLK>), dim3(grid), dim3(kNT), lds, s, p->jobs, p->njobs, g.begin,
+                       g.end);
+    WCT_HIP_CHECK(hipGetLastError());
+    return WCT_OK;
+}
+
 template <int VH, bool S5, int MS, int BLK>
 static int launch_r2c_inst(wct_decon_plan p, const wct_decon_group& g, hipStream_t s)
 {
@@ -682,7 +804,7 @@ static int launch_rows_r2c(wct_decon_plan p, const wct_decon_group& g, hipStream
     if constexpr (VH == 12 && S5) {
         if (use_static_row(p, g, VH, S5)) {
             switch (p->r2c_blocks) {
-            case 3: return launch_r2c_inst<VH, S5, kStaticRowLen, 3>(p, g, s);
+            case 3: return launch_r2c_lin<kStaticRowLen, 3>(p, g, s);
             case 5: return launch_r2c_inst<VH, S5, kStaticRowLen, 5>(p, g, s);
             default: return launch_r2c_inst<VH, S5, kStaticRowLen, 4>(p, g, s);
             }

@@ -80,7 +80,8 @@ __device__ __forceinline__ T2 cisgn(T2 a, int sign)
 template <typename T2, int R> struct Dft;
 
 template <typename T2> struct Dft<T2, 2> {
-    static __device__ __forceinline__ void run(T2* v, int, const T2*)
+    template <typename TB>
+    static __device__ __forceinline__ void run(T2* v, int, TB)
     {
         T2 a = v[0], b = v[1];
         v[0] = cadd(a, b);
@@ -89,7 +90,8 @@ template <typename T2> struct Dft<T2, 2> {
 };
 
 template <typename T2> struct Dft<T2, 3> {
-    static __device__ __forceinline__ void run(T2* v, int sign, const T2*)
+    template <typename TB>
+    static __device__ __forceinline__ void run(T2* v, int sign, TB)
     {
         using T = decltype(v[0].x);
         const T h = T(0.86602540378443864676);
@@ -106,7 +108,8 @@ template <typename T2> struct Dft<T2, 3> {
 };
 
 template <typename T2> struct Dft<T2, 4> {
-    static __device__ __forceinline__ void run(T2* v, int sign, const T2*)
+    template <typename TB>
+    static __device__ __forceinline__ void run(T2* v, int sign, TB)
     {
         T2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
         T2 t2 = cadd(v[1], v[3]), t3 = cisgn(csub(v[1], v[3]), sign);
@@ -118,7 +121,8 @@ template <typename T2> struct Dft<T2, 4> {
 };
 
 template <typename T2> struct Dft<T2, 5> {
-    static __device__ __forceinline__ void run(T2* v, int sign, const T2*)
+    template <typename TB>
+    static __device__ __forceinline__ void run(T2* v, int sign, TB)
     {
         using T = decltype(v[0].x);
         const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
@@ -140,7 +144,8 @@ template <typename T2> struct Dft<T2, 5> {
 };
 
 template <typename T2> struct Dft<T2, 8> {
-    static __device__ __forceinline__ void run(T2* v, int sign, const T2*)
+    template <typename TB>
+    static __device__ __forceinline__ void run(T2* v, int sign, TB)
     {
         using T = decltype(v[0].x);
         const T r2 = T(0.70710678118654752440);
@@ -179,7 +184,8 @@ constexpr int inv_mod(int a, int m)
 }
 
 template <typename T2, int N1, int N2> struct DftPfa {
-    static __device__ __forceinline__ void run(T2* v, int sign, const T2*)
+    template <typename TB>
+    static __device__ __forceinline__ void run(T2* v, int sign, TB)
     {
         constexpr int N = N1 * N2;
         constexpr int o1 = N2 * inv_mod(N2 % N1, N1), o2 = N1 * inv_mod(N1 % N2, N2);
@@ -208,7 +214,8 @@ template <typename T2> struct Dft<T2, 20> : DftPfa<T2, 4, 5> {};
 // Generic odd prime radix (7, 11, 13) from a small table of
 // exp(-2 pi i q / R), q < R (forward sign).
 template <typename T2, int R> struct DftGeneric {
-    static __device__ __forceinline__ void run(T2* v, int sign, const T2* tab)
+    template <typename TB>
+    static __device__ __forceinline__ void run(T2* v, int sign, TB tab)
     {
         T2 out[R];
 #pragma unroll
@@ -253,9 +260,8 @@ __device__ __forceinline__ int pass_read_idx(int tid, int b, int rm)
 // buf: LDS sequence of length n.  The group has NT threads, this thread
 // is tid.  MAXB bounds the butterflies one thread owns in this pass.
 
-template <typename T2, int R, int NT, int MAXB>
-__device__ __forceinline__ void stockham_pass(T2* buf, int n, int Ns, const T2* __restrict__ tw,
-                                              const T2* __restrict__ gtab, int sign, int tid)
+template <typename T2, int R, int NT, int MAXB, typename TW>
+__device__ __forceinline__ void stockham_pass(T2* buf, int n, int Ns, TW tw, TW gtab, int sign, int tid)
 {
     const int m = n / R;
     T2 v[MAXB][R];
@@ -303,14 +309,14 @@ __device__ __forceinline__ void stockham_pass(T2* buf, int n, int Ns, const T2* 
 // staged in LDS by the caller (stage_twiddles).  Every thread of the
 // workgroup must call this (it contains barriers); when the workgroup is
 // split into slices each slice passes its own buf and slice-local tid.
-template <typename T2, int NT, int NMAX, bool SMOOTH5 = false>
-__device__ __forceinline__ void fft_lds(T2* buf, const FftPlan& pl, const T2* __restrict__ tw, int sign, int tid)
+template <typename T2, int NT, int NMAX, bool SMOOTH5 = false, typename TW>
+__device__ __forceinline__ void fft_lds(T2* buf, const FftPlan& pl, TW tw, int sign, int tid)
 {
-    const T2* gtab = tw + pl.dftoff;
+    const TW gtab = tw + pl.dftoff;
     int Ns = 1;
     for (int p = 0; p < pl.npass; ++p) {
         const int R = pl.rad[p];
-        const T2* t = tw + pl.twoff[p];
+        const TW t = tw + pl.twoff[p];
         switch (R) {
         case 2: stockham_pass<T2, 2, NT, (NMAX / 2 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
         case 3: stockham_pass<T2, 3, NT, (NMAX / 3 + NT - 1) / NT>(buf, pl.n, Ns, t, gtab, sign, tid); break;
@@ -336,8 +342,8 @@ __device__ __forceinline__ void fft_lds(T2* buf, const FftPlan& pl, const T2* __
 }
 
 // Copies a plan's twiddle table to LDS (all threads of the block).
-template <typename T2>
-__device__ __forceinline__ void stage_twiddles(T2* dst, const T2* __restrict__ src, int count, int tid,
+template <typename T2, typename TS>
+__device__ __forceinline__ void stage_twiddles(T2* dst, TS src, int count, int tid,
                                                int nthreads)
 {
     for (int i = tid; i < count; i += nthreads) dst[i] = src[i];
@@ -415,9 +421,8 @@ constexpr int sf_twoff(int n, int p, bool odd_first = false)
 }
 
 // One Stockham pass with every size a constant.
-template <typename T2, int R, int NT, int N, int NS>
-__device__ __forceinline__ void stockham_pass_c(T2* buf, const T2* __restrict__ tw, const T2* __restrict__ gtab,
-                                                int sign, int tid)
+template <typename T2, int R, int NT, int N, int NS, typename TW>
+__device__ __forceinline__ void stockham_pass_c(T2* buf, TW tw, TW gtab, int sign, int tid)
 {
     constexpr int m = N / R;
     constexpr int MAXB = (m + NT - 1) / NT;
@@ -453,8 +458,8 @@ __device__ __forceinline__ void stockham_pass_c(T2* buf, const T2* __restrict__ 
     lds_barrier();
 }
 
-template <typename T2, int NT, int N, int P>
-__device__ __forceinline__ void fft_static_from(T2* buf, const T2* __restrict__ tw, int sign, int tid)
+template <typename T2, int NT, int N, int P, typename TW>
+__device__ __forceinline__ void fft_static_from(T2* buf, TW tw, int sign, int tid)
 {
     if constexpr (P < sf_npass(N)) {
         constexpr int R = sf_radix(N, P);
@@ -472,9 +477,8 @@ __device__ __forceinline__ void fft_static_from(T2* buf, const T2* __restrict__ 
 // the lanes; the stores of the first pass are strided by R, free of bank
 // conflicts when R is odd (the odd-first schedule puts 15s, 5s and 3s
 // first).
-template <typename T2, int R, int NT, int N, int NS>
-__device__ __forceinline__ void stockham_pass_lin(T2* buf, const T2* __restrict__ tw,
-                                                  const T2* __restrict__ gtab, int sign, int tid)
+template <typename T2, int R, int NT, int N, int NS, typename TW>
+__device__ __forceinline__ void stockham_pass_lin(T2* buf, TW tw, TW gtab, int sign, int tid)
 {
     constexpr int m = N / R;
     constexpr int MAXB = (m + NT - 1) / NT;
@@ -496,7 +500,7 @@ __device__ __forceinline__ void stockham_pass_lin(T2* buf, const T2* __restrict_
         if (m % NT == 0 || j < m) {
             const int k = NS == 1 ? 0 : j % NS;
             if (NS > 1) {
-                const T2* t = tw + k * (R - 1);
+                const TW t = tw + k * (R - 1);
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
                     T2 w = t[r - 1];
@@ -513,8 +517,8 @@ __device__ __forceinline__ void stockham_pass_lin(T2* buf, const T2* __restrict_
     lds_barrier();
 }
 
-template <typename T2, int NT, int N, int P>
-__device__ __forceinline__ void fft_lin_from(T2* buf, const T2* __restrict__ tw, int sign, int tid)
+template <typename T2, int NT, int N, int P, typename TW>
+__device__ __forceinline__ void fft_lin_from(T2* buf, TW tw, int sign, int tid)
 {
     if constexpr (P < sf_npass(N, true)) {
         constexpr int R = sf_radix(N, P, true);
@@ -528,8 +532,8 @@ __device__ __forceinline__ void fft_lin_from(T2* buf, const T2* __restrict__ tw,
 // LDS row (entry i at buf[i], in and out), on the odd-first radix
 // schedule with that schedule's twiddle table (get_plan with odd_first).
 // Every thread of the group calls it.
-template <typename T2, int NT, int N>
-__device__ __forceinline__ void fft_lin(T2* buf, const T2* __restrict__ tw, int sign, int tid)
+template <typename T2, int NT, int N, typename TW>
+__device__ __forceinline__ void fft_lin(T2* buf, TW tw, int sign, int tid)
 {
     static_assert(sf_npass(N, true) > 0 && sf_npass(N, true) <= kMaxPasses, "length with prime factors above 13");
     fft_lin_from<T2, NT, N, 0>(buf, tw, sign, tid);
@@ -537,8 +541,8 @@ __device__ __forceinline__ void fft_lin(T2* buf, const T2* __restrict__ tw, int 
 
 // Full transform of length N (a compile-time constant) in LDS; same
 // contract as fft_lds, with the twiddle table of the runtime plan of N.
-template <typename T2, int NT, int N>
-__device__ __forceinline__ void fft_static(T2* buf, const T2* __restrict__ tw, int sign, int tid)
+template <typename T2, int NT, int N, typename TW>
+__device__ __forceinline__ void fft_static(T2* buf, TW tw, int sign, int tid)
 {
     static_assert(sf_npass(N) > 0 && sf_npass(N) <= kMaxPasses, "length with prime factors above 13");
     fft_static_from<T2, NT, N, 0>(buf, tw, sign, tid);
