@@ -142,6 +142,43 @@ int wcth_loose_rois(const float* r, int nw, int nt, long ld, const int* bad_ptr,
  * truncations, stored as floats. */
 int wcth_apply_roi(float* r, int nw, int nt, long ld, const int* roi_count, const int* roi_pairs, int linear);
 
+/* PeakFinding::find_peak (sigproc/src/PeakFinding.cxx:34-474) on n
+ * samples: up to cap positions x and heights y (the source at the
+ * rounded position); returns the peak count, -2 if cap was exceeded. */
+int wcth_find_peaks(const float* signal, int n, int max_peaks, double sigma, double threshold, double* x,
+                    double* y, int cap);
+
+/* ROI_refinement (sigproc/src/ROI_refinement.cxx) of one plane, as a
+ * handle.  plane 0/1 (induction: the loose ROIs are refined) or 2
+ * (collection: the tight ones).  params: the 13 values r_th_factor,
+ * r_fake_signal_low_th, r_fake_signal_high_th,
+ * r_fake_signal_low_th_ind_factor, r_fake_signal_high_th_ind_factor,
+ * r_pad, r_break_roi_loop, r_th_peak, r_sep_peak,
+ * r_low_peak_sep_threshold_pre, r_max_npeaks, r_sigma, r_th_percent
+ * (OmnibusSigProc's keys), or NULL for their defaults; wrapped: the
+ * isWrapped key.  NULL on a bad argument (wcth_last_error). */
+void* wcth_refine_new(int plane, int nw, int nt, const double* params, int wrapped);
+void wcth_refine_free(void* h);
+/* ROI_refinement::load_data (:269-699): the plane's decon r (nw rows of
+ * nt at row stride ld), per-row inclusive bad tick ranges (CSR or NULL),
+ * the tight stage's per-wire RMS, and the tight and loose ROI lists
+ * (count per wire + inclusive pairs; loose NULL on the collection
+ * plane). */
+int wcth_refine_load(void* h, const float* r, long ld, const int* bad_ptr, const int* bad_lo, const int* bad_hi,
+                     const float* rms, const int* tight_count, const int* tight_pairs, const int* loose_count,
+                     const int* loose_pairs);
+/* One step by name: "cleanup" (CleanUpROIs), "merge"
+ * (generate_merge_ROIs), "check" (CheckROIs), "break" (BreakROIs),
+ * "shrink" (ShrinkROIs), "cleanup_collection", "cleanup_induction",
+ * "extend" (ExtendROIs), or "all" for OmnibusSigProc's sequence. */
+int wcth_refine_step(void* h, const char* step);
+/* The plane's ROIs per wire in list order: extended ranges (after
+ * "extend") or the ROI ranges; output as wcth_roi_lists. */
+int wcth_refine_rois(void* h, int extended, int* roi_count, int* roi_pairs, int cap);
+/* The contents of wire w's ROIs, concatenated in list order; returns
+ * the value count, -2 if cap was exceeded. */
+int wcth_refine_contents(void* h, int w, float* out, int cap);
+
 #ifdef __cplusplus
 }
 #endif

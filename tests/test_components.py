@@ -143,7 +143,7 @@ def test_sigproc_eos_and_empty_frame():
 @pytest.mark.parametrize("extra,msg", [({"fft_flag": 1}, "fft_flag"),
                                        ({"process_planes": [0, 3]}, "process_planes"),
                                        ({"anode": "AnodeChannels:missing_planes"}, "plane"),
-                                       ({"use_multi_plane_protection": True}, "refinement"),
+                                       ({"use_multi_plane_protection": True}, "wire geometry"),
                                        ({"response_floor": 1.5}, "response_floor")])
 def test_sigproc_configuration_errors(extra, msg):
     from wct_amd import components as K

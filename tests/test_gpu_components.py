@@ -6,15 +6,17 @@ those exact cases) plus random data against the FP64 oracle DFT, and
 calls from several threads at once.
 
 HipOmnibusSigProc (IFrameFilter): a three-plane anode frame pushed
-through the component must give exactly what the C ABI gives on the
-same data (dense tight_lf traces with the per-wire RMS summary, and the
-debug mode's tight_lf / loose_lf / decon_charge outputs), save_data's
-post-processing (negatives and bad ranges zeroed; debug mode keeps
-negatives; sparse runs), tight ROI lists equal to the oracle's
-run/extend/connect rules applied to the device mask, the wiener tag only
-on request (the ROI refinement that makes the reference's wiener traces
-is not run), and no traces without ROI refinement or the debug mode, as
-the reference saves none then.
+through the component must give exactly what the C ABI and the oracle
+give on the same data: the wiener and gauss traces (the hits and charge
+decons of the C ABI cut by the oracle's apply_roi to the ROIs of the
+oracle's ROI refinement, which starts from the oracle's tight and loose
+ROI lists on the C ABI's mask, loose and refine decons), with the
+per-wire RMS summary on wiener, and the debug mode's tight_lf / loose_lf
+/ decon_charge outputs; save_data's post-processing (negatives and bad
+ranges zeroed; debug mode keeps negatives; sparse runs); tight ROI lists
+equal to the oracle's run/extend/connect rules applied to the device
+mask; and no traces without ROI refinement or the debug mode, as the
+reference saves none then.
 """
 import json
 import os
@@ -141,6 +143,9 @@ def _sigproc_config(tmpdir, extra=None):
         {"type": "LfFilter", "name": "ROI_tighter_lf", "data": W.LF_TIGHTER},
         {"type": "LfFilter", "name": "ROI_loose_lf", "data": W.LF_LOOSE},
         {"type": "HfFilter", "name": "Gaus_wide", "data": W.GAUS_WIDE},
+        {"type": "HfFilter", "name": "Wiener_wide_U", "data": W.WIENER_WIDE},
+        {"type": "HfFilter", "name": "Wiener_wide_V", "data": W.WIENER_WIDE},
+        {"type": "HfFilter", "name": "Wiener_wide_W", "data": W.WIENER_WIDE},
         {"type": "HipDFT", "name": ""},
         {"type": "HipOmnibusSigProc", "name": "osp", "data": data},
     ]
@@ -158,33 +163,59 @@ def _adc_planes(seed):
     return out
 
 
-def _filters(kind, nt, debug=False, tight=True):
+def _filters(kind, nt, debug=False, tight=True, refine=True, charge=True):
     """The component's output filters of one plane, in FP64: the base
     (tight, or all ones for an induction plane without ROI refinement),
-    the tighter one, and in debug mode the loose and charge ones."""
+    the tighter one; in debug mode the loose one (loose_dbg, no tight-LF
+    rows) and the charge one; with refinement the loose one with its
+    tight-LF alternative (alt), the refine, hits and charge ones."""
     from wct_amd import host, workload as W
     wiener = host.hf_filter(nt, **W.WIENER)
+    ind = kind == "ind"
     f = {}
     if tight:
-        f["base"] = wiener * host.lf_filter(nt, **W.LF_TIGHT) if kind == "ind" else wiener
-        if kind == "ind":
+        f["base"] = wiener * host.lf_filter(nt, **W.LF_TIGHT) if ind else wiener
+        if ind:
             f["tighter"] = wiener * host.lf_filter(nt, **W.LF_TIGHTER)
     else:
         f["base"] = np.ones(nt)
-    if debug:
-        if kind == "ind":
-            f["loose"] = wiener * host.lf_filter(nt, **W.LF_LOOSE)
+    if debug and ind:
+        f["loose_dbg"] = wiener * host.lf_filter(nt, **W.LF_LOOSE)
+    if refine or (debug and charge):
         f["charge"] = host.hf_filter(nt, **W.GAUS_WIDE)
+    if refine:
+        if ind:
+            f["loose"] = wiener * host.lf_filter(nt, **W.LF_LOOSE)
+            f["alt"] = wiener * host.lf_filter(nt, **W.LF_TIGHT)
+            f["refine"] = wiener
+        f["hits"] = host.hf_filter(nt, **W.WIENER_WIDE)
     return f
 
 
-def _expected(adc, bad, th, debug=False, refine=True, chan=None):
+# the component's extra outputs, in its order; baseline restored on
+# induction planes for the loose and refine decons, on the collection
+# plane for every one (decon_2D_hits/charge, OmnibusSigProc.cxx:1344-1379)
+_EXTRAS = ("loose_dbg", "charge", "loose", "refine", "hits")
+
+
+def _alt_rows(bad, lfn, ip):
+    """decon_2D_looseROI's tight-LF rows: a bad channel within 2 wires (1
+    on the V plane) or an lf_noisy one within 2, in the same plane."""
+    nw = NW[ip]
+    isbad = [bool(bad.get((ip, w))) for w in range(nw)]
+    isl = [(ip, w) in lfn for w in range(nw)]
+    nb = 2 if ip == 0 else 1
+    return np.array([int(any(isbad[max(0, w - nb):w + nb + 1]) or any(isl[max(0, w - 2):w + 3]))
+                     for w in range(nw)], np.uint8)
+
+
+def _expected(adc, bad, th, debug=False, refine=True, chan=None, charge=True, lfn=()):
     """The same planes through the C ABI (workload's spec, padded to
     plane 0's 21 response regions as the reference sizes them), with the
     component's unfiltered kernels and output filters.  chan is
     (responses, {channel: response row}, t0) for the per-channel
-    correction.  Returns per plane (base, mask, rms, {extra name:
-    frame})."""
+    correction; lfn the (plane, wire)s of lf_noisy channels.  Returns
+    per plane (base, mask, rms, {extra name: frame})."""
     import torch
     from wct_amd import decon as D
     from wct_amd import workload as W
@@ -205,7 +236,7 @@ def _expected(adc, bad, th, debug=False, refine=True, chan=None):
         sp.fr_rows = rows
         sp.wire_shift = (nfine_rows - 1) // 2
         tight = kind == "col" or refine
-        flt = _filters(kind, NT, debug, tight)
+        flt = _filters(kind, NT, debug, tight, refine, charge)
         env = np.max(np.abs(np.stack(list(flt.values()))), axis=0)
         pk = D.PlaneKernel(sp, unfiltered=True, envelope=env)
         g = {k: torch.from_numpy(v[:pk.kc].astype(np.float32)).to(dev) for k, v in flt.items()}
@@ -220,16 +251,19 @@ def _expected(adc, bad, th, debug=False, refine=True, chan=None):
                 hi.append(b1)
             ptr.append(len(lo))
         badt = tuple(torch.tensor(v if v else [0], dtype=torch.int32, device=dev) for v in (ptr, lo, hi))
-        extra = {k: torch.empty_like(a) for k in ("loose", "charge") if k in flt}
-        outs = [D.Output(g[k], out=extra[k], restore=1 if (k == "loose" or kind == "col") else 0,
-                         save_mode=0 if debug else 1) for k in extra]
+        extra = {k: torch.empty_like(a) for k in _EXTRAS if k in flt}
+        alt = torch.from_numpy(_alt_rows(bad, lfn, ip)).to(dev) if "loose" in flt else None
+        outs = [D.Output(g[k], out=extra[k], restore=1 if (k in ("loose_dbg", "loose", "refine") or kind == "col")
+                         else 0, save_mode=0, filter_alt=g["alt"] if k == "loose" else None,
+                         alt_rows=alt if k == "loose" else None) for k in extra]
+        keep.append(alt)
         cj = None
         if table is not None:
             rows = torch.tensor([chmap[CH0[ip] + w] for w in range(NW[ip])], dtype=torch.int32, device=dev)
             cj = (table, rows)
             keep.append(rows)
         jobs.append(D.make_job(pk, a, y, mask=m, rms=r, th_factor=th[ip], restore=1 if tight else 0,
-                               save_mode=0 if debug else 1, lf_tight=g["base"], lf_tighter=g.get("tighter"),
+                               save_mode=0, lf_tight=g["base"], lf_tighter=g.get("tighter"),
                                bad=badt, outputs=outs, chan=cj))
         keep += [pk, a, y, m, r, badt, g, extra]
         res.append((y, m, r, extra))
@@ -254,35 +288,79 @@ def _frame(K, adc, bad):
     return f
 
 
+def _refined(exp, bad, nt=NT):
+    """Per plane the oracle's refined ROIs (ROI_refinement restated in
+    oracle/roi_refine.py, from the oracle's tight ROI lists on the C ABI's
+    mask and, on the induction planes, its loose ROI lists on the C ABI's
+    loose decon, with the refine decon as the ROIs' data) and the wiener
+    and gauss frames: the C ABI's hits and charge decons cut by the
+    oracle's apply_roi, then save_data's zeroing of negatives and
+    [lo, hi) bad ranges."""
+    import roi_refine as RR
+    res = []
+    for ip in range(3):
+        y, m, r, extra = exp[ip]
+        nw = NW[ip]
+        badl = [list(bad.get((ip, w), [])) for w in range(nw)]
+        tight = O.roi_lists(m, pad=5, asy=0.1)
+        if ip < 2:
+            loose = O.loose_rois(extra["loose"], bad=badl, tight=tight)
+            base = extra["refine"]
+        else:
+            loose, base = None, y
+        R = RR.RoiRefinement(ip, nw)
+        R.load_data(base, r, tight, loose, bad=badl)
+        R.refine()
+        rois = R.rois(True)
+        frames = []
+        for key in ("hits", "charge"):
+            cut = O.apply_roi(extra[key], rois, ip < 2)
+            cut = np.where(cut > 0, cut, 0).astype(np.float32)
+            for w in range(nw):
+                for (b0, b1) in badl[w]:
+                    cut[w, max(0, b0):min(nt, b1)] = 0
+            frames.append(cut)
+        res.append((rois, frames[0], frames[1]))
+    return res
+
+
 def test_sigproc_matches_c_abi(K, tmp_path):
-    print("sigproc vs C ABI, 3 planes", NW, "x", NT, flush=True)
+    print("sigproc vs C ABI + oracle ROI refinement, 3 planes", NW, "x", NT, flush=True)
     K.configure(_sigproc_config(tmp_path))
     adc = _adc_planes(3)
     bad = {(0, 5): [(100, 300)], (2, 7): [(0, NT)], (2, 8): [(500, 510), (900, 950)]}
     out = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
     exp = _expected(adc, bad, (3.0, 3.0, 5.0))
+    ref = _refined(exp, bad)
     info = out.info()
     assert info["ident"] == 17 and info["time"] == 3.0 and info["tick"] == 500.0
     assert out.frame_tags() == ["sigproc"]
-    assert out.trace_tags() == ["tight_lf"]     # no refined wiener traces to publish
+    assert sorted(out.trace_tags()) == ["gauss", "wiener"]
     tr = out.traces()
-    assert len(tr) == sum(NW)
-    idx = out.tagged_traces("tight_lf")
-    assert list(idx) == list(range(sum(NW)))
-    summ = out.trace_summary("tight_lf")
+    assert len(tr) == 2 * sum(NW)
+    widx, gidx = list(out.tagged_traces("wiener")), list(out.tagged_traces("gauss"))
+    assert len(widx) == len(gidx) == sum(NW)
+    summ = out.trace_summary("wiener")
     k = 0
+    nroi = 0
     for ip in range(3):
-        y, m, r, _ = exp[ip]
+        rois, wf, gf = ref[ip]
+        r = exp[ip][2]
+        nroi += sum(len(x) for x in rois)
         for w in range(NW[ip]):
-            ch, tb, q = tr[k]
+            ch, tb, q = tr[widx[k]]
             assert ch == CH0[ip] + w and tb == 0
-            np.testing.assert_array_equal(q, y[w])
+            np.testing.assert_array_equal(q, wf[w])
             assert summ[k] == r[w]
+            ch, tb, q = tr[gidx[k]]
+            assert ch == CH0[ip] + w and tb == 0
+            np.testing.assert_array_equal(q, gf[w])
             k += 1
-        assert np.all(y >= 0)
+    print("refined ROIs", nroi, flush=True)
+    assert nroi > 0 and any(np.any(tr[i][2] > 0) for i in widx)
     # bad ranges zeroed, input "bad" map passed through
-    assert not np.any(tr[5][2][100:300])
-    assert not np.any(tr[NW[0] + NW[1] + 7][2])
+    assert not np.any(tr[widx[5]][2][100:300])
+    assert not np.any(tr[widx[NW[0] + NW[1] + 7]][2])
     assert out.masks("bad")[CH0[0] + 5] == [(100, 300)]
     # tight ROIs: the oracle's rules over the device mask
     rois = out.masks("tight_roi")
@@ -294,7 +372,77 @@ def test_sigproc_matches_c_abi(K, tmp_path):
     assert any(rois.get(CH0[2] + w) for w in range(NW[2]))
     # a second frame of the same shape reuses the spectra and gives the same
     out2 = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
-    np.testing.assert_array_equal(out2.trace(3)[2], tr[3][2])
+    for i in (widx[3], gidx[NW[0] + 2]):
+        np.testing.assert_array_equal(out2.trace(i)[2], tr[i][2])
+
+
+def _track_planes(seed, ntracks=(8, 8, 10)):
+    """ADC planes with straight tracks through the forward response
+    (workload's induction and collection models) plus noise."""
+    import torch
+    from wct_amd import decon as D
+    from wct_amd import workload as W
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda:0")
+    out = []
+    for ip, kind in enumerate(("ind", "ind", "col")):
+        fk = D.ForwardKernel(W.plane_spec(kind, NW[ip], NT))
+        q = torch.from_numpy(W.tracks_raster(rng, ntracks[ip], NW[ip], NT)).to(dev)
+        sig = torch.empty_like(q)
+        D.convolve(fk, q, sig)
+        torch.cuda.synchronize()
+        out.append((sig.cpu().numpy() + rng.normal(0.0, W.NOISE_ADC, (NW[ip], NT))).astype(np.float32))
+    return out
+
+
+def test_sigproc_refinement_on_tracks(K, tmp_path):
+    """wiener and gauss on frames with straight tracks (many ROIs, linked
+    across wires, broken at valleys and shrunk) equal the C ABI's decons cut
+    by the oracle's refinement, with bad channels on every plane."""
+    print("sigproc refinement on track frames", flush=True)
+    K.configure(_sigproc_config(tmp_path))
+    for seed in (21, 22):
+        adc = _track_planes(seed)
+        bad = {(0, 9): [(0, NT)], (1, 30): [(400, 700)], (2, 17): [(100, 120)]}
+        out = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
+        exp = _expected(adc, bad, (3.0, 3.0, 5.0))
+        ref = _refined(exp, bad)
+        tr = out.traces()
+        widx, gidx = list(out.tagged_traces("wiener")), list(out.tagged_traces("gauss"))
+        k = 0
+        for ip in range(3):
+            for w in range(NW[ip]):
+                np.testing.assert_array_equal(tr[widx[k]][2], ref[ip][1][w])
+                np.testing.assert_array_equal(tr[gidx[k]][2], ref[ip][2][w])
+                k += 1
+        nroi = [sum(len(x) for x in ref[ip][0]) for ip in range(3)]
+        print("seed", seed, "refined ROIs per plane", nroi, flush=True)
+        assert min(nroi) >= 5
+
+
+def test_sigproc_loose_fallback_rows(K, tmp_path):
+    """decon_2D_looseROI's tight-LF rows next to bad and lf_noisy channels
+    (:1215-1231) in the component's refinement: an lf_noisy channel and a
+    bad one give what the C ABI gives with those rows flagged."""
+    print("sigproc loose decon fallback rows", flush=True)
+    K.configure(_sigproc_config(tmp_path))
+    adc = _adc_planes(11)
+    bad = {(1, 20): [(0, 40)]}
+    lfn = {(0, 10), (1, 33)}
+    f = _frame(K, adc, bad)
+    for (ip, w) in sorted(lfn):
+        f.add_mask("lf_noisy", CH0[ip] + w, 0, NT)
+    out = K.frame_filter("HipOmnibusSigProc:osp", f)
+    exp = _expected(adc, bad, (3.0, 3.0, 5.0), lfn=lfn)
+    assert _alt_rows(bad, lfn, 0)[8:13].all() and _alt_rows(bad, lfn, 1)[19:22].all()
+    ref = _refined(exp, bad)
+    tr = out.traces()
+    widx = list(out.tagged_traces("wiener"))
+    k = 0
+    for ip in range(3):
+        for w in range(NW[ip]):
+            np.testing.assert_array_equal(tr[widx[k]][2], ref[ip][1][w])
+            k += 1
 
 
 def _chan_file(tmp_path, t0, tick=500.0, nresp=4, nsamp=150):
@@ -319,21 +467,26 @@ def test_sigproc_per_channel_response(K, tmp_path):
     print("sigproc per-channel correction", flush=True)
     t0 = -1000.0
     path, cr, chmap = _chan_file(tmp_path, t0)
-    cfg = _sigproc_config(tmp_path, {"per_chan_resp": "PerChannelResponse:pcr"})
+    cfg = _sigproc_config(tmp_path, {"per_chan_resp": "PerChannelResponse:pcr", "use_roi_debug_mode": True})
     cfg.insert(-1, {"type": "PerChannelResponse", "name": "pcr", "data": {"filename": path}})
     K.configure(cfg)
     adc = _adc_planes(8)
     bad = {(0, 5): [(100, 300)]}
     out = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
-    exp = _expected(adc, bad, (3.0, 3.0, 5.0), chan=(cr, chmap, t0))
-    plain = _expected(adc, bad, (3.0, 3.0, 5.0))
+    exp = _expected(adc, bad, (3.0, 3.0, 5.0), debug=True, chan=(cr, chmap, t0))
+    plain = _expected(adc, bad, (3.0, 3.0, 5.0), debug=True)
     tr = out.traces()
+    idx = list(out.tagged_traces("tight_lf"))
     k = 0
     for ip in range(3):
-        y = exp[ip][0]
+        y = exp[ip][0].copy()
         assert np.any(y != plain[ip][0])
+        for (ipp, w), ranges in bad.items():
+            if ipp == ip:
+                for (b0, b1) in ranges:
+                    y[w, b0:b1] = 0
         for w in range(NW[ip]):
-            np.testing.assert_array_equal(tr[k][2], y[w])
+            np.testing.assert_array_equal(tr[idx[k]][2], y[w])
             k += 1
     K.reset()
     path2, _, _ = _chan_file(tmp_path, t0, tick=400.0)
@@ -343,16 +496,6 @@ def test_sigproc_per_channel_response(K, tmp_path):
     from wct_amd.capi import WctError
     with pytest.raises(WctError, match="channel response size mismatch"):
         K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
-
-
-def test_sigproc_tight_as_wiener(K, tmp_path):
-    """The wiener tag on the tight decon only when asked for."""
-    K.configure(_sigproc_config(tmp_path, {"tight_as_wiener": True}))
-    adc = _adc_planes(3)
-    out = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, {}))
-    assert sorted(out.trace_tags()) == ["tight_lf", "wiener"]
-    assert list(out.tagged_traces("wiener")) == list(out.tagged_traces("tight_lf"))
-    np.testing.assert_array_equal(out.trace_summary("wiener"), out.trace_summary("tight_lf"))
 
 
 @pytest.mark.parametrize("refine", [True, False])
@@ -367,9 +510,10 @@ def test_sigproc_debug_outputs(K, tmp_path, refine):
     bad = {(0, 5): [(100, 300)], (2, 8): [(500, 510)]}
     out = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
     exp = _expected(adc, bad, (3.0, 3.0, 5.0), debug=True, refine=refine)
-    assert sorted(out.trace_tags()) == ["decon_charge", "loose_lf", "tight_lf"]
+    tags = ["decon_charge", "loose_lf", "tight_lf"] + (["gauss", "wiener"] if refine else [])
+    assert sorted(out.trace_tags()) == sorted(tags)
     tr = out.traces()
-    assert len(tr) == 3 * sum(NW)
+    assert len(tr) == len(tags) * sum(NW)
 
     def zero_bad(ip, frame):
         f = frame.copy()
@@ -379,13 +523,13 @@ def test_sigproc_debug_outputs(K, tmp_path, refine):
                     f[w, b0:b1] = 0
         return f
 
-    for tag, key in (("tight_lf", "base"), ("loose_lf", "loose"), ("decon_charge", "charge")):
+    for tag, key in (("tight_lf", "base"), ("loose_lf", "loose_dbg"), ("decon_charge", "charge")):
         idx = list(out.tagged_traces(tag))
         assert len(idx) == sum(NW)
         k = 0
         for ip in range(3):
             y, m, r, extra = exp[ip]
-            want = y if (key == "base" or (key == "loose" and ip == 2)) else extra[key]
+            want = y if (key == "base" or (key == "loose_dbg" and ip == 2)) else extra[key]
             want = zero_bad(ip, want)
             for w in range(NW[ip]):
                 ch, tb, q = tr[idx[k]]
@@ -416,30 +560,36 @@ def test_sigproc_debug_and_sparse(K, tmp_path):
     adc = _adc_planes(4)
     bad = {(2, 3): [(10, 20)]}
     out = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
-    assert sorted(out.trace_tags()) == ["loose_lf", "tight_lf"]
+    assert sorted(out.trace_tags()) == ["gauss", "loose_lf", "tight_lf", "wiener"]
     tr = [out.trace(i) for i in out.tagged_traces("tight_lf")]
     assert len(tr) == NW[2] and tr[0][0] == CH0[2]
     assert any(np.any(q < 0) for _, _, q in tr)         # debug keeps negatives
     assert not np.any(tr[3][2][10:20])
-    dense = {ch: q for ch, _, q in tr}
 
+    K.reset()
+    K.configure(_sigproc_config(tmp_path, {"process_planes": [2]}))
+    out = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
+    dense = {ch: q for ch, _, q in (out.trace(i) for i in out.tagged_traces("wiener"))}
+    assert len(dense) == NW[2] and any(np.any(q > 0) for q in dense.values())
     K.reset()
     K.configure(_sigproc_config(tmp_path, {"sparse": True, "process_planes": [2]}))
     out = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, adc, bad))
     rebuilt = {}
-    summ = out.trace_summary("tight_lf")
-    assert len(summ) == out.info()["ntraces"]
-    for ch, tb, q in out.traces():
+    widx = list(out.tagged_traces("wiener"))
+    summ = out.trace_summary("wiener")
+    assert len(summ) == len(widx)
+    for i in widx:
+        ch, tb, q = out.trace(i)
         assert q.size > 0 and np.all(q > 0)
         rebuilt.setdefault(ch, np.zeros(NT, np.float32))[tb:tb + q.size] = q
     for ch, q in dense.items():
-        np.testing.assert_array_equal(rebuilt.get(ch, np.zeros(NT, np.float32)), np.maximum(q, 0))
+        np.testing.assert_array_equal(rebuilt.get(ch, np.zeros(NT, np.float32)), q)
 
 
 def test_sigproc_between_frame_files(K, tmp_path):
     """FrameFileSource -> HipOmnibusSigProc -> FrameFileSink: an ADC
     archive in the reference's layout (int16 frame, column by column, a
-    bad-channel mask) gives an output archive whose tight_lf frame, channels
+    bad-channel mask) gives an output archive whose wiener frame, channels
     and RMS summary equal the component's in-memory output."""
     import io
     import tarfile
@@ -463,7 +613,7 @@ def test_sigproc_between_frame_files(K, tmp_path):
             tf.addfile(ti, b)
     cfg = _sigproc_config(tmp_path) + [
         {"type": "FrameFileSource", "name": "in", "data": {"inname": str(src), "tags": ["orig"]}},
-        {"type": "FrameFileSink", "name": "out", "data": {"outname": str(dst), "tags": ["tight_lf"]}}]
+        {"type": "FrameFileSink", "name": "out", "data": {"outname": str(dst), "tags": ["wiener"]}}]
     K.configure(cfg)
     f = K.frame_source("FrameFileSource:in")
     out = K.frame_filter("HipOmnibusSigProc:osp", f)
@@ -472,18 +622,18 @@ def test_sigproc_between_frame_files(K, tmp_path):
     K.frame_sink("FrameFileSink:out", None)
     # the same frame built in memory
     direct = K.frame_filter("HipOmnibusSigProc:osp", _frame(K, [a.astype(np.float32) for a in adc16], bad))
-    want = np.stack([q for _, _, q in direct.traces()])
+    want = np.stack([direct.trace(i)[2] for i in direct.tagged_traces("wiener")])
     got = {}
     with tarfile.open(dst) as tf:
         for m in tf.getmembers():
             got[m.name] = np.load(io.BytesIO(tf.extractfile(m).read()))
-    assert sorted(got) == sorted(["channels_tight_lf_17.npy", "chanmask_bad_17.npy", "chanmask_tight_roi_17.npy",
-                                  "frame_tight_lf_17.npy", "summary_tight_lf_17.npy", "tickinfo_tight_lf_17.npy"])
-    np.testing.assert_array_equal(got["frame_tight_lf_17.npy"], want)
-    np.testing.assert_array_equal(got["channels_tight_lf_17.npy"], chans)
-    np.testing.assert_array_equal(got["summary_tight_lf_17.npy"], direct.trace_summary("tight_lf"))
-    assert list(got["tickinfo_tight_lf_17.npy"]) == [3.0, 500.0, 0.0]
-    assert not np.any(got["frame_tight_lf_17.npy"][NW[0] + 4, 200:260])
+    assert sorted(got) == sorted(["channels_wiener_17.npy", "chanmask_bad_17.npy", "chanmask_tight_roi_17.npy",
+                                  "frame_wiener_17.npy", "summary_wiener_17.npy", "tickinfo_wiener_17.npy"])
+    np.testing.assert_array_equal(got["frame_wiener_17.npy"], want)
+    np.testing.assert_array_equal(got["channels_wiener_17.npy"], chans)
+    np.testing.assert_array_equal(got["summary_wiener_17.npy"], direct.trace_summary("wiener"))
+    assert list(got["tickinfo_wiener_17.npy"]) == [3.0, 500.0, 0.0]
+    assert not np.any(got["frame_wiener_17.npy"][NW[0] + 4, 200:260])
     rois = direct.masks("tight_roi")
     flat = sorted((c, a, b) for c, rl in rois.items() for (a, b) in rl)
     assert sorted(map(tuple, got["chanmask_tight_roi_17.npy"].tolist())) == flat and flat

@@ -13,20 +13,25 @@
 //    thresholded (find_ROI_by_decon_itself with cal_RMS,
 //    sigproc/src/ROI_formation.cxx:364-531), for the collection plane and,
 //    with use_roi_refinement, the induction planes (:1452-1467);
+//  - with use_roi_refinement, the ROI refinement (:1467-1620): for the
+//    induction planes the loose decon (decon_2D_looseROI, :1162-1238,
+//    with its tight-LF rows next to bad or lf_noisy channels) and its
+//    loose ROIs (ROI_formation::find_ROI_loose, extend_ROI_loose), the
+//    refine decon (decon_2D_ROI_refine, :1053-1075); per plane the
+//    ROI_refinement steps on the host (RoiRefinement.h, planes in
+//    parallel threads); then the hits and charge decons (decon_2D_hits,
+//    decon_2D_charge, :1316-1380) cut to the refined ROIs by apply_roi and
+//    saved as the "wiener" traces (with the per-wire RMS as summary) and
+//    the "gauss" traces;
 //  - with use_roi_debug_mode, the reference's debug outputs: tight_lf
 //    (:1471-1474; an induction plane without refinement saves the
 //    unfiltered decon_2D_init frame), loose_lf (decon_2D_looseROI_
 //    debug_mode, :1478-1497) and decon_charge (decon_2D_charge, :1349-1380,
 //    :1508-1511,1590-1601).
-// The output follows save_data (:371-472).  The ROI refinement that makes
-// the reference's "wiener" and "gauss" traces (:1518-1620) is not part of
-// this component, so neither tag is produced by default: without the debug
-// mode the tight decon (negatives and bad ranges zeroed) is published
-// under tight_lf_tag with the per-wire RMS as summary, and the key
-// "tight_as_wiener" also tags it wiener_tag (with the RMS summary, as the
-// reference's wiener traces carry).  The tight ROIs (extended and
-// connected) are added to the output channel-mask map under
-// "tight_roi_mask" as half-open tick ranges.
+// The output follows save_data (:371-472).  The multi-plane protection
+// (MP2ROI/MP3ROI) needs the anode's wire geometry and is refused.  The
+// tight ROIs (extended and connected) are added to the output channel-mask
+// map under "tight_roi_mask" as half-open tick ranges.
 //
 // The response spectra are built once per (nticks, tick) on the device
 // in FP64 and reused for later frames of the same shape (the reference
@@ -34,6 +39,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -47,6 +53,7 @@
 #include "Response.h"
 #include "Buffers.h"
 #include "RoiFormation.h"
+#include "RoiRefinement.h"
 
 namespace WireCell {
 namespace Hip {
@@ -94,9 +101,31 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
         cfg["use_multi_plane_protection"] = false;
         cfg["use_roi_refinement"] = m_use_roi_refinement;
         cfg["sparse"] = m_sparse;
+        cfg["lroi_rebin"] = m_loose.rebin;
+        cfg["lroi_th_factor"] = m_loose.factor;
+        cfg["lroi_max_th"] = m_loose.max_th;
+        cfg["lroi_th_factor1"] = m_loose.factor1;
+        cfg["lroi_short_length"] = m_loose.short_length;
+        cfg["lroi_jump_one_bin"] = m_loose.jump_one_bin;
+        cfg["r_th_factor"] = (double)m_refine.th_factor;
+        cfg["r_fake_signal_low_th"] = (double)m_refine.fake_signal_low_th;
+        cfg["r_fake_signal_high_th"] = (double)m_refine.fake_signal_high_th;
+        cfg["r_fake_signal_low_th_ind_factor"] = (double)m_refine.fake_signal_low_th_ind_factor;
+        cfg["r_fake_signal_high_th_ind_factor"] = (double)m_refine.fake_signal_high_th_ind_factor;
+        cfg["r_pad"] = m_refine.pad;
+        cfg["r_break_roi_loop"] = m_refine.break_roi_loop;
+        cfg["r_th_peak"] = (double)m_refine.th_peak;
+        cfg["r_sep_peak"] = (double)m_refine.sep_peak;
+        cfg["r_low_peak_sep_threshold_pre"] = (double)m_refine.low_peak_sep_threshold_pre;
+        cfg["r_max_npeaks"] = m_refine.max_npeaks;
+        cfg["r_sigma"] = (double)m_refine.sigma;
+        // the reference's defaults carry these two keys misspelt
+        // (OmnibusSigProc.cxx:265,297); configure reads r_th_percent and
+        // isWrapped, as the reference's does (:114,160)
+        cfg["r_th_precent"] = (double)m_refine.th_percent;
+        cfg["isWarped"] = m_refine.wrapped;
         // Keys of this component only.
         cfg["tight_roi_mask"] = m_tight_roi_mask;
-        cfg["tight_as_wiener"] = m_tight_as_wiener;
         cfg["spectral_cut"] = m_spectral_cut;
         cfg["response_floor"] = m_response_floor;
         cfg["device"] = m_device;
@@ -137,9 +166,30 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
         m_use_roi_debug_mode = get(config, "use_roi_debug_mode", m_use_roi_debug_mode);
         m_use_roi_refinement = get(config, "use_roi_refinement", m_use_roi_refinement);
         if (get(config, "use_multi_plane_protection", false))
-            throw ConfigError("HipOmnibusSigProc: use_multi_plane_protection needs the ROI refinement, which "
-                              "this component does not run");
-        m_tight_as_wiener = get(config, "tight_as_wiener", m_tight_as_wiener);
+            throw ConfigError("HipOmnibusSigProc: use_multi_plane_protection (MP2ROI/MP3ROI) needs the anode's "
+                              "wire geometry, which this component does not read");
+        m_loose.rebin = get(config, "lroi_rebin", m_loose.rebin);
+        m_loose.factor = get(config, "lroi_th_factor", m_loose.factor);
+        m_loose.max_th = get(config, "lroi_max_th", m_loose.max_th);
+        m_loose.factor1 = get(config, "lroi_th_factor1", m_loose.factor1);
+        m_loose.short_length = get(config, "lroi_short_length", m_loose.short_length);
+        m_loose.jump_one_bin = get(config, "lroi_jump_one_bin", m_loose.jump_one_bin);
+        if (m_loose.rebin < 1) throw ConfigError("HipOmnibusSigProc: lroi_rebin must be at least 1");
+        auto getf = [&](const char* key, float& v) { v = (float)get(config, key, (double)v); };
+        getf("r_th_factor", m_refine.th_factor);
+        getf("r_fake_signal_low_th", m_refine.fake_signal_low_th);
+        getf("r_fake_signal_high_th", m_refine.fake_signal_high_th);
+        getf("r_fake_signal_low_th_ind_factor", m_refine.fake_signal_low_th_ind_factor);
+        getf("r_fake_signal_high_th_ind_factor", m_refine.fake_signal_high_th_ind_factor);
+        m_refine.pad = get(config, "r_pad", m_refine.pad);
+        m_refine.break_roi_loop = get(config, "r_break_roi_loop", m_refine.break_roi_loop);
+        getf("r_th_peak", m_refine.th_peak);
+        getf("r_sep_peak", m_refine.sep_peak);
+        getf("r_low_peak_sep_threshold_pre", m_refine.low_peak_sep_threshold_pre);
+        m_refine.max_npeaks = get(config, "r_max_npeaks", m_refine.max_npeaks);
+        getf("r_sigma", m_refine.sigma);
+        getf("r_th_percent", m_refine.th_percent);
+        m_refine.wrapped = get(config, "isWrapped", m_refine.wrapped);
         m_response_floor = get(config, "response_floor", m_response_floor);
         if (m_response_floor < 0 || m_response_floor >= 1)
             throw ConfigError("HipOmnibusSigProc: response_floor must be in [0, 1)");
@@ -268,6 +318,30 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
         const bool plan_stale = csr.size() * sizeof(int) > m_d_bad.bytes || !m_plan;
         m_d_bad.ensure(std::max<size_t>(csr.size(), 1) * sizeof(int));
 
+        // decon_2D_looseROI's rows that take the tight LF (:1215-1231): a
+        // bad channel within 2 wires (1 on the V plane) or an lf_noisy one
+        // within 2 wires, in the same plane
+        if (m_any_extra[kLoose]) {
+            auto lfn = inmasks.find("lf_noisy");
+            for (int ip = 0; ip < 2; ++ip) {
+                Plane& pl = m_planes[ip];
+                if (!pl.active || !pl.has_extra[kLoose]) continue;
+                std::vector<char> isbad(pl.nw, 0), islfn(pl.nw, 0);
+                for (int w = 0; w < pl.nw; ++w) isbad[w] = !bad[ip][w].empty() || (badit != inmasks.end() &&
+                                                          badit->second.count(m_channels[ip][w]));
+                if (lfn != inmasks.end())
+                    for (int w = 0; w < pl.nw; ++w) islfn[w] = lfn->second.count(m_channels[ip][w]) ? 1 : 0;
+                const int nb = ip ? 1 : 2, nl = 2;
+                unsigned char* alt = (unsigned char*)m_h_alt.p + pl.alt_off;
+                for (int w = 0; w < pl.nw; ++w) {
+                    bool a = false;
+                    for (int q = std::max(0, w - nb); q <= std::min(pl.nw - 1, w + nb) && !a; ++q) a = isbad[q];
+                    for (int q = std::max(0, w - nl); q <= std::min(pl.nw - 1, w + nl) && !a; ++q) a = islfn[q];
+                    alt[w] = a ? 1 : 0;
+                }
+            }
+            hcheck(wct_memcpy_htod_async(m_d_alt.p, m_h_alt.p, m_alt_bytes, m_stream), "upload");
+        }
         hcheck(wct_memcpy_htod_async(m_d_adc.p, m_h_adc.p, m_adc_floats * sizeof(float), m_stream), "upload");
         if (!csr.empty())
             hcheck(wct_memcpy_htod_async(m_d_bad.p, csr.data(), csr.size() * sizeof(int), m_stream), "upload");
@@ -298,12 +372,18 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
         // positive runs); the device already applied save_data's zeroing
         // in the nominal mode, the debug mode keeps negatives and zeroes
         // only the bad ranges.
-        auto save = [&](const float* frame, int ip, IFrame::trace_list_t& idx, IFrame::trace_summary_t* thr) {
+        // host_zero: the frame is not zeroed on the device (the ROI-cut
+        // wiener and gauss decons); save_data's nominal zeroing of
+        // negatives and bad ranges is done here.
+        auto save = [&](const float* frame, int ip, IFrame::trace_list_t& idx, IFrame::trace_summary_t* thr,
+                        bool host_zero = false) {
             const Plane& pl = m_planes[ip];
             for (int w = 0; w < pl.nw; ++w) {
                 const float* row = frame + pl.off + (size_t)w * nticks;
                 ITrace::ChargeSequence charge(row, row + nticks);
-                if (m_use_roi_debug_mode) {
+                if (host_zero && !m_use_roi_debug_mode)
+                    for (auto& q : charge) q = q > 0.0f ? q : 0.0f;
+                if (m_use_roi_debug_mode || host_zero) {
                     for (auto& r : bad[ip][w])
                         for (int k = std::max(0, r.first); k < std::min(nticks, r.second); ++k) charge[k] = 0;
                 }
@@ -329,17 +409,28 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
                 }
             }
         };
-        const bool debug = m_use_roi_debug_mode;
-        const bool publish = !debug && m_use_roi_refinement;   // the tight decon in place of wiener
-        IFrame::trace_list_t tight_traces, loose_traces, charge_traces;
+        const bool debug = m_use_roi_debug_mode, refine = m_use_roi_refinement;
+        IFrame::trace_list_t tight_traces, loose_traces, charge_traces, wiener_traces, gauss_traces;
         IFrame::trace_summary_t thresholds;
         Waveform::ChannelMaskMap outmasks = inmasks;
         const float* hout = (const float*)m_h_out.p;
+
+        // ROI refinement of every plane, planes in parallel
+        std::vector<RoiList> refined[3];
+        if (refine) {
+            std::vector<std::future<void>> work;
+            for (int ip = 0; ip < 3; ++ip)
+                if (m_planes[ip].active)
+                    work.push_back(std::async(std::launch::async, [&, ip] {
+                        refined[ip] = refine_plane(ip, nticks, bad[ip], hmask, hrms);
+                    }));
+            for (auto& f : work) f.get();
+        }
+
         for (int ip = 0; ip < 3; ++ip) {
             const Plane& pl = m_planes[ip];
             if (!pl.active) continue;
-            if ((publish && pl.tight) || (debug && !m_tight_lf_tag.empty()))
-                save(hout, ip, tight_traces, publish ? &thresholds : nullptr);
+            if (debug && !m_tight_lf_tag.empty()) save(hout, ip, tight_traces, nullptr);
             if (debug && !m_loose_lf_tag.empty()) {
                 // the collection plane has no loose decon: the reference
                 // saves its tight one under loose_lf (:1503-1507)
@@ -356,16 +447,30 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
                 }
             }
         }
+        // per plane: wiener (hits cut to the ROIs), the debug mode's
+        // decon_charge (before the cut), gauss (charge cut to the ROIs)
+        std::vector<float> cut;
+        auto cut_plane = [&](int ip, int extra) {
+            const Plane& pl = m_planes[ip];
+            const float* src = (const float*)m_h_extra[extra].p + pl.off;
+            cut.assign(src, src + (size_t)pl.nw * nticks);
+            apply_roi(cut.data(), pl.nw, nticks, nticks, refined[ip], ip < 2);
+            return cut.data() - pl.off;
+        };
         for (int ip = 0; ip < 3; ++ip) {
             const Plane& pl = m_planes[ip];
-            if (pl.active && pl.has_extra[kCharge]) save((const float*)m_h_extra[kCharge].p, ip, charge_traces, nullptr);
+            if (!pl.active) continue;
+            if (refine && !m_wiener_tag.empty()) save(cut_plane(ip, kHits), ip, wiener_traces, &thresholds, true);
+            if (debug && pl.has_extra[kCharge] && !m_decon_charge_tag.empty())
+                save((const float*)m_h_extra[kCharge].p, ip, charge_traces, nullptr);
+            if (refine && !m_gauss_tag.empty()) save(cut_plane(ip, kCharge), ip, gauss_traces, nullptr, true);
         }
 
         auto sframe = std::make_shared<Aux::SimpleFrame>(in->ident(), in->time(), itraces, in->tick(), outmasks);
         sframe->tag_frame(m_frame_tag);
-        if (publish) {
-            if (!m_tight_lf_tag.empty()) sframe->tag_traces(m_tight_lf_tag, tight_traces, thresholds);
-            if (m_tight_as_wiener && !m_wiener_tag.empty()) sframe->tag_traces(m_wiener_tag, tight_traces, thresholds);
+        if (refine) {
+            if (!m_wiener_tag.empty()) sframe->tag_traces(m_wiener_tag, wiener_traces, thresholds);
+            if (!m_gauss_tag.empty()) sframe->tag_traces(m_gauss_tag, gauss_traces);
         }
         if (debug) {
             if (!m_tight_lf_tag.empty()) sframe->tag_traces(m_tight_lf_tag, tight_traces);
@@ -378,7 +483,41 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
 
   private:
     // Extra outputs of a plane's job (wct_decon_output), in job order.
-    enum Extra { kLooseDbg = 0, kCharge = 1, kNExtra = 2 };
+    enum Extra { kLooseDbg = 0, kCharge = 1, kLoose = 2, kRefine = 3, kHits = 4, kNExtra = 5 };
+
+    // The refinement of one plane: its tight ROIs from the device mask,
+    // on induction planes the loose ROIs of the loose decon (RMS without
+    // the bad ticks, extended by the tight ROIs) and the refine decon as
+    // the ROIs' data, on the collection plane the tight decon; then the
+    // ROI_refinement steps.  Returns the refined (extended) ROIs per wire.
+    std::vector<RoiList> refine_plane(int ip, int nticks, const std::vector<std::vector<Waveform::BinRange>>& badw,
+                                      const uint8_t* hmask, const float* hrms) const
+    {
+        const Plane& pl = m_planes[ip];
+        const bool ind = ip < 2;
+        const auto tight = tight_rois(hmask + pl.off, pl.nw, nticks, nticks, m_pad, m_asy);
+        std::vector<RoiList> badl(pl.nw);
+        std::vector<int> bptr(pl.nw + 1, 0), blo, bhi;
+        for (int w = 0; w < pl.nw; ++w) {
+            for (const auto& r : badw[w]) {
+                badl[w].emplace_back(r.first, r.second);
+                blo.push_back(r.first);
+                bhi.push_back(r.second);
+            }
+            bptr[w + 1] = (int)blo.size();
+        }
+        const float* base = (const float*)m_h_out.p + pl.off;
+        std::vector<RoiList> loose;
+        if (ind) {
+            loose = loose_rois((const float*)m_h_extra[kLoose].p + pl.off, pl.nw, nticks, nticks, bptr.data(),
+                               blo.data(), bhi.data(), nullptr, &tight, m_loose);
+            base = (const float*)m_h_extra[kRefine].p + pl.off;
+        }
+        RoiRefinement R(ip, pl.nw, nticks, m_refine);
+        R.load(base, nticks, badl, std::vector<float>(hrms + pl.rms_off, hrms + pl.rms_off + pl.nw), tight, loose);
+        R.refine();
+        return R.rois(true);
+    }
 
     struct Plane {
         bool active{false};
@@ -388,10 +527,12 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
         wct_decon_kernel kernel{nullptr};
         bool tight{true};    // the job's own output is the tight decon (else the unfiltered frame)
         DevBuf f_base, f_tighter, f_extra[kNExtra];
+        DevBuf f_loose_alt;  // the loose decon's filter on its tight-LF rows
+        size_t alt_off{0};   // this plane's rows in the loose fallback-row flags
         DevBuf cf_rows;      // per-channel correction: table row of each wire
         bool has_tighter{false};
-        bool has_extra[kNExtra]{false, false};
-        int extra_restore[kNExtra]{0, 0};
+        bool has_extra[kNExtra]{};
+        int extra_restore[kNExtra]{};
         float th_factor{3};
     };
 
@@ -404,15 +545,17 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
             pl.kernel = nullptr;
             pl.f_base.release();
             pl.f_tighter.release();
+            pl.f_loose_alt.release();
             for (auto& b : pl.f_extra) b.release();
             pl.cf_rows.release();
             pl.has_tighter = false;
             for (auto& h : pl.has_extra) h = false;
             pl.active = false;
         }
-        for (DevBuf* b : {&m_d_adc, &m_d_out, &m_d_mask, &m_d_rms, &m_d_bad, &m_d_ws, &m_d_cf}) b->release();
+        for (DevBuf* b : {&m_d_adc, &m_d_out, &m_d_mask, &m_d_rms, &m_d_bad, &m_d_ws, &m_d_cf, &m_d_alt}) b->release();
         m_cf_ld = 0;
-        for (HostBuf* b : {&m_h_adc, &m_h_out, &m_h_mask, &m_h_rms}) b->release();
+        m_alt_bytes = 0;
+        for (HostBuf* b : {&m_h_adc, &m_h_out, &m_h_mask, &m_h_rms, &m_h_alt}) b->release();
         for (int k = 0; k < kNExtra; ++k) {
             m_d_extra[k].release();
             m_h_extra[k].release();
@@ -465,10 +608,10 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
             pl.nw = (int)m_channels[ip].size();
             if (pl.nw == 0) continue;
             const bool ind = ip < 2;
-            // the reference saves nothing for a plane unless it forms
-            // tight ROIs or runs the debug outputs (:1452-1514)
+            // the reference saves nothing unless it refines ROIs or runs
+            // the debug outputs (:1452-1620)
             pl.tight = !ind || refine;
-            if (!pl.tight && !debug) continue;
+            if (!refine && !debug) continue;
             const auto& rows = avg[ip].rows;
             if ((int)rows.size() > fine_nwires)
                 throw SigProcError("HipOmnibusSigProc: plane " + std::to_string(ip) +
@@ -507,14 +650,32 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
                 pl.has_extra[kLooseDbg] = true;
                 pl.extra_restore[kLooseDbg] = 1;
             }
-            if (debug && !m_decon_charge_tag.empty()) {
+            if (refine || (debug && !m_decon_charge_tag.empty())) {
                 f_extra[kCharge] = filter_f64("HfFilter", "Gaus_wide", nticks);
                 pl.has_extra[kCharge] = true;
                 pl.extra_restore[kCharge] = ind ? 0 : 1;   // :1377-1379
             }
+            std::vector<double> f_alt;
+            if (refine) {
+                static const char* wide_names[3] = {"Wiener_wide_U", "Wiener_wide_V", "Wiener_wide_W"};
+                if (ind) {   // decon_2D_looseROI (:1162-1238), decon_2D_ROI_refine (:1053-1075)
+                    f_extra[kLoose] = times(wiener, lf("ROI_loose_lf"));
+                    f_alt = times(wiener, lf("ROI_tight_lf"));
+                    pl.has_extra[kLoose] = true;
+                    pl.extra_restore[kLoose] = 1;
+                    f_extra[kRefine] = wiener;
+                    pl.has_extra[kRefine] = true;
+                    pl.extra_restore[kRefine] = 1;
+                }
+                f_extra[kHits] = filter_f64("HfFilter", wide_names[ip], nticks);   // decon_2D_hits (:1316-1347)
+                pl.has_extra[kHits] = true;
+                pl.extra_restore[kHits] = ind ? 0 : 1;
+            }
             std::vector<double> env(f_base);
-            for (const auto* f : {&f_tighter, &f_extra[0], &f_extra[1]})
+            for (const auto* f : {&f_tighter, &f_alt})
                 for (size_t i = 0; i < f->size(); ++i) env[i] = std::max(env[i], std::fabs((*f)[i]));
+            for (const auto& f : f_extra)
+                for (size_t i = 0; i < f.size(); ++i) env[i] = std::max(env[i], std::fabs(f[i]));
             auto wire_f = filter_f64("HfFilter", ind ? "Wire_ind" : "Wire_col", pl.nw);
             hcheck(wct_decon_kernel_create_unfiltered(&pl.kernel, resp.data(), fine_nwires, pl.nw, nticks,
                                                       wire_f.data(), env.data(), wire_shift, time_shift,
@@ -534,6 +695,11 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
                     upload(pl.f_extra[k], f_extra[k]);
                     m_any_extra[k] = true;
                 }
+            if (pl.has_extra[kLoose]) {
+                upload(pl.f_loose_alt, f_alt);
+                pl.alt_off = m_alt_bytes;
+                m_alt_bytes += pl.nw;
+            }
             pl.th_factor = ind ? m_th_factor_ind : m_th_factor_col;
             pl.off = off;
             pl.rms_off = rms_off;
@@ -553,6 +719,10 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
         m_h_out.ensure(sizeof(float) * std::max<size_t>(off, 1));
         m_h_mask.ensure(std::max<size_t>(off, 1));
         m_h_rms.ensure(sizeof(float) * std::max<size_t>(rms_off, 1));
+        if (m_alt_bytes) {
+            m_d_alt.ensure(m_alt_bytes);
+            m_h_alt.ensure(m_alt_bytes);
+        }
         for (int k = 0; k < kNExtra; ++k)
             if (m_any_extra[k]) {
                 m_d_extra[k].ensure(sizeof(float) * std::max<size_t>(off, 1));
@@ -655,7 +825,9 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
             }
             j.th_factor = pl.th_factor;
             j.restore_baseline = pl.tight ? 1 : 0;
-            j.save_mode = m_use_roi_debug_mode ? 0 : 1;
+            // the tight decon is saved only in the debug mode (negatives
+            // kept) and is the collection plane's refinement input
+            j.save_mode = 0;
             if (!pl.tight) {   // the unfiltered frame: no threshold, no ROIs
                 j.mask = nullptr;
                 j.rms = nullptr;
@@ -667,10 +839,14 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
                 wct_decon_output o;
                 std::memset(&o, 0, sizeof(o));
                 o.filter = (const float*)pl.f_extra[k].p;
+                if (k == kLoose) {
+                    o.filter_alt = (const float*)pl.f_loose_alt.p;
+                    o.alt_rows = (const unsigned char*)m_d_alt.p + pl.alt_off;
+                }
                 o.out = (float*)m_d_extra[k].p + pl.off;
                 o.ld_out = m_nticks;
                 o.restore_baseline = pl.extra_restore[k];
-                o.save_mode = m_use_roi_debug_mode ? 0 : 1;
+                o.save_mode = 0;   // saved (and zeroed) on the host, after apply_roi
                 outs.push_back(o);
             }
             j.outputs = outs.empty() ? nullptr : outs.data();
@@ -710,7 +886,8 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
     bool m_use_roi_refinement{true};
     bool m_sparse{false};
     std::string m_tight_roi_mask{"tight_roi"};
-    bool m_tight_as_wiener{false};
+    LooseParams m_loose;
+    RefineParams m_refine;
     double m_spectral_cut{1e-7};
     double m_response_floor{0.0};
     int m_device{0};
@@ -727,7 +904,10 @@ class HipOmnibusSigProc : public IFrameFilter, public IConfigurable {
     HostBuf m_h_adc, m_h_out, m_h_mask, m_h_rms;
     DevBuf m_d_extra[kNExtra];
     HostBuf m_h_extra[kNExtra];
-    bool m_any_extra[kNExtra]{false, false};
+    bool m_any_extra[kNExtra]{};
+    DevBuf m_d_alt;      // loose decon: 1 on rows that take the tight LF (all planes)
+    HostBuf m_h_alt;
+    size_t m_alt_bytes{0};
     std::vector<wct_decon_output> m_outputs[3];
     size_t m_adc_floats{0}, m_rms_floats{0};
     int m_nticks{0};

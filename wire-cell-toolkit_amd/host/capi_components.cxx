@@ -11,7 +11,9 @@
 #include "Factory.h"
 #include "Frame.h"
 #include "Interface.h"
+#include "PeakFinding.h"
 #include "RoiFormation.h"
+#include "RoiRefinement.h"
 #include "Tensor.h"
 
 using namespace WireCell;
@@ -393,6 +395,149 @@ int wcth_apply_roi(float* r, int nw, int nt, long ld, const int* roi_count, cons
             }
     Hip::apply_roi(r, nw, nt, ld, rows, linear != 0);
     return 0;
+}
+
+int wcth_find_peaks(const float* signal, int n, int max_peaks, double sigma, double threshold, double* x,
+                    double* y, int cap)
+{
+    if (!signal || n < 0 || cap < 0 || (cap > 0 && (!x || !y))) {
+        g_err = "find_peaks: bad arguments";
+        return -1;
+    }
+    try {
+        Hip::PeakSearch ps;
+        ps.max_peaks = max_peaks;
+        ps.sigma = sigma;
+        ps.threshold = threshold;
+        const auto pk = Hip::find_peaks(std::vector<float>(signal, signal + n), ps);
+        const int np = (int)pk.x.size();
+        for (int i = 0; i < np && i < cap; ++i) {
+            x[i] = pk.x[i];
+            y[i] = pk.y[i];
+        }
+        return np > cap ? -2 : np;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+void* wcth_refine_new(int plane, int nw, int nt, const double* params, int wrapped)
+{
+    try {
+        Hip::RefineParams p;
+        if (params) {
+            p.th_factor = (float)params[0];
+            p.fake_signal_low_th = (float)params[1];
+            p.fake_signal_high_th = (float)params[2];
+            p.fake_signal_low_th_ind_factor = (float)params[3];
+            p.fake_signal_high_th_ind_factor = (float)params[4];
+            p.pad = (int)params[5];
+            p.break_roi_loop = (int)params[6];
+            p.th_peak = (float)params[7];
+            p.sep_peak = (float)params[8];
+            p.low_peak_sep_threshold_pre = (float)params[9];
+            p.max_npeaks = (int)params[10];
+            p.sigma = (float)params[11];
+            p.th_percent = (float)params[12];
+        }
+        p.wrapped = wrapped != 0;
+        return new Hip::RoiRefinement(plane, nw, nt, p);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void wcth_refine_free(void* h) { delete static_cast<Hip::RoiRefinement*>(h); }
+
+int wcth_refine_load(void* h, const float* r, long ld, const int* bad_ptr, const int* bad_lo, const int* bad_hi,
+                     const float* rms, const int* tight_count, const int* tight_pairs, const int* loose_count,
+                     const int* loose_pairs)
+{
+    auto* R = static_cast<Hip::RoiRefinement*>(h);
+    if (!R || !r || !rms || !tight_count || (bad_ptr && (!bad_lo || !bad_hi))) {
+        g_err = "refine_load: bad arguments";
+        return -1;
+    }
+    try {
+        const int nw = R->nwires(), nt = R->nticks();
+        if (ld < nt) throw std::invalid_argument("refine_load: row stride below the tick count");
+        std::vector<Hip::RoiList> bad;
+        if (bad_ptr) {
+            bad.resize(nw);
+            for (int w = 0; w < nw; ++w)
+                for (int q = bad_ptr[w]; q < bad_ptr[w + 1]; ++q) bad[w].emplace_back(bad_lo[q], bad_hi[q]);
+        }
+        std::vector<Hip::RoiList> loose;
+        if (loose_count) loose = unflatten(loose_count, loose_pairs, nw);
+        R->load(r, ld, bad, std::vector<float>(rms, rms + nw), unflatten(tight_count, tight_pairs, nw), loose);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int wcth_refine_step(void* h, const char* step)
+{
+    auto* R = static_cast<Hip::RoiRefinement*>(h);
+    if (!R || !step) {
+        g_err = "refine_step: bad arguments";
+        return -1;
+    }
+    try {
+        const std::string s(step);
+        if (s == "cleanup") R->cleanup_rois();
+        else if (s == "merge") R->generate_merge_rois();
+        else if (s == "check") R->check_rois();
+        else if (s == "break") R->break_rois();
+        else if (s == "shrink") R->shrink_rois();
+        else if (s == "cleanup_collection") R->cleanup_collection_rois();
+        else if (s == "cleanup_induction") R->cleanup_induction_rois();
+        else if (s == "extend") R->extend_rois();
+        else if (s == "all") R->refine();
+        else throw std::invalid_argument("refine_step: unknown step '" + s + "'");
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int wcth_refine_rois(void* h, int extended, int* roi_count, int* roi_pairs, int cap)
+{
+    auto* R = static_cast<Hip::RoiRefinement*>(h);
+    if (!R || !roi_count || cap < 0 || (cap > 0 && !roi_pairs)) {
+        g_err = "refine_rois: bad arguments";
+        return -1;
+    }
+    const auto rows = R->rois(extended != 0);
+    int tot = 0;
+    for (size_t w = 0; w < rows.size(); ++w) {
+        roi_count[w] = (int)rows[w].size();
+        for (const auto& q : rows[w]) {
+            if (tot < cap) {
+                roi_pairs[2 * tot] = q.first;
+                roi_pairs[2 * tot + 1] = q.second;
+            }
+            ++tot;
+        }
+    }
+    return tot > cap ? -2 : tot;
+}
+
+int wcth_refine_contents(void* h, int w, float* out, int cap)
+{
+    auto* R = static_cast<Hip::RoiRefinement*>(h);
+    if (!R || w < 0 || w >= R->nwires() || cap < 0 || (cap > 0 && !out)) {
+        g_err = "refine_contents: bad arguments";
+        return -1;
+    }
+    int tot = 0;
+    for (const auto& c : R->contents(w))
+        for (float v : c) {
+            if (tot < cap) out[tot] = v;
+            ++tot;
+        }
+    return tot > cap ? -2 : tot;
 }
 
 void* wcth_tensorset_new(int ident, const char* metadata_json)

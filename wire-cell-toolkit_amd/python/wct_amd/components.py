@@ -353,6 +353,95 @@ def apply_roi(r, rois, linear):
     return a
 
 
+def find_peaks(signal, max_peaks=200, sigma=1.0, threshold=0.05):
+    """PeakFinding::find_peak: (positions, heights) as float64 arrays."""
+    a = np.ascontiguousarray(signal, np.float32)
+    cap = max(1, a.size)
+    x = np.zeros(cap)
+    y = np.zeros(cap)
+    n = lib().wcth_find_peaks(a.ctypes.data, a.size, max_peaks, sigma, threshold, x.ctypes.data, y.ctypes.data, cap)
+    if n < 0:
+        raise WctError(lib().wcth_last_error().decode() if n == -1 else "peak capacity exceeded")
+    return x[:n].copy(), y[:n].copy()
+
+
+REFINE_KEYS = ("r_th_factor", "r_fake_signal_low_th", "r_fake_signal_high_th", "r_fake_signal_low_th_ind_factor",
+               "r_fake_signal_high_th_ind_factor", "r_pad", "r_break_roi_loop", "r_th_peak", "r_sep_peak",
+               "r_low_peak_sep_threshold_pre", "r_max_npeaks", "r_sigma", "r_th_percent")
+REFINE_DEFAULTS = (3.0, 500.0, 1000.0, 1.0, 1.0, 5, 2, 3.0, 6.0, 1200.0, 200, 2.0, 0.1)
+
+
+class RoiRefinement:
+    """ROI_refinement of one plane through the C front (wcth_refine_*).
+    plane 0/1 induction, 2 collection; params: dict of OmnibusSigProc's
+    r_* keys (defaults for the rest)."""
+
+    def __init__(self, plane, nw, nt, params=None, wrapped=False):
+        vals = dict(zip(REFINE_KEYS, REFINE_DEFAULTS))
+        vals.update(params or {})
+        self._par = np.array([vals[k] for k in REFINE_KEYS], np.float64)
+        self.nw, self.nt = nw, nt
+        h = lib().wcth_refine_new(plane, nw, nt, self._par.ctypes.data, int(bool(wrapped)))
+        if not h:
+            raise WctError(lib().wcth_last_error().decode())
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().wcth_refine_free(h)
+            self._h = None
+
+    def load(self, r, rms, tight, loose=None, bad=None):
+        a = np.ascontiguousarray(r, np.float32)
+        assert a.shape == (self.nw, self.nt)
+        rm = np.ascontiguousarray(rms, np.float32)
+        tc, tp = _csr(tight, self.nw)
+        lc = lp = None
+        keep = []
+        if loose is not None:
+            lc, lp = _csr(loose, self.nw)
+            keep += [lc, lp]
+        bp = blo = bhi = None
+        if bad is not None:
+            ptr = np.zeros(self.nw + 1, np.int32)
+            lo, hi = [], []
+            for w in range(self.nw):
+                for (x, y) in bad[w]:
+                    lo.append(x)
+                    hi.append(y)
+                ptr[w + 1] = len(lo)
+            keep += [ptr, np.ascontiguousarray(lo or [0], np.int32), np.ascontiguousarray(hi or [0], np.int32)]
+            bp, blo, bhi = (k.ctypes.data for k in keep[-3:])
+        rc = lib().wcth_refine_load(self._h, a.ctypes.data, self.nt, bp, blo, bhi, rm.ctypes.data, tc.ctypes.data,
+                                    tp.ctypes.data, None if lc is None else lc.ctypes.data,
+                                    None if lp is None else lp.ctypes.data)
+        _check(rc)
+
+    def step(self, name):
+        _check(lib().wcth_refine_step(self._h, name.encode()))
+
+    def rois(self, extended=True, cap=1 << 22):
+        cnt = np.zeros(self.nw, np.int32)
+        pairs = np.zeros(2 * cap, np.int32)
+        tot = lib().wcth_refine_rois(self._h, int(bool(extended)), cnt.ctypes.data, pairs.ctypes.data, cap)
+        if tot < 0:
+            raise WctError(lib().wcth_last_error().decode() if tot == -1 else "ROI capacity exceeded")
+        return _rows(cnt, pairs)
+
+    def contents(self, wire):
+        """The contents of the wire's ROIs, concatenated in list order."""
+        cap = 4 * self.nt
+        while True:
+            out = np.zeros(cap, np.float32)
+            n = lib().wcth_refine_contents(self._h, wire, out.ctypes.data, cap)
+            if n == -1:
+                raise WctError(lib().wcth_last_error().decode())
+            if n >= 0:
+                return out[:n].copy()
+            cap *= 4
+
+
 def field_response_json(planes, origin, tstart, period, speed):
     """A field response in the reference's JSON schema
     (util/src/Response.cxx:32-72).  planes: list of (planeid, location,

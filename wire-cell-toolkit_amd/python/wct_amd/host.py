@@ -48,6 +48,15 @@ _SIGS = {
     "wcth_apply_roi": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_long, C.c_void_p, C.c_void_p, C.c_int]),
     "wcth_roi_lists": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_long, C.c_int, C.c_float,
                                  C.c_void_p, C.c_void_p, C.c_int]),
+    "wcth_find_peaks": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p,
+                                  C.c_void_p, C.c_int]),
+    "wcth_refine_new": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int]),
+    "wcth_refine_free": (None, [C.c_void_p]),
+    "wcth_refine_load": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "wcth_refine_step": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "wcth_refine_rois": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
+    "wcth_refine_contents": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
     "wcth_frame_tag_frame": (C.c_int, [C.c_void_p, C.c_char_p]),
     "wcth_frame_tag_traces": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
     "wcth_frame_sink": (C.c_int, [C.c_char_p, C.c_void_p]),
