@@ -377,6 +377,66 @@ int wct_plain_decon_plane(const float* adc, int ld_adc, int nw, int nt, const fl
                           unsigned char* mask, int ld_mask, float* rms, int save_mode, void* workspace,
                           size_t ws_bytes, void* stream);
 
+/* ---- forward simulation convolution ----
+ * Gen::ImpactTransform (gen/src/ImpactTransform.cxx:19-347), the forward
+ * model of the deconvolution: the charges BinnedDiffusion_transform::
+ * get_charge_vec sorts into impact groups (gen/src/
+ * BinnedDiffusion_transform.cxx:227-443) convolved with the plane's impact
+ * responses over wires and ticks.
+ *
+ * wct_fft_best_length: WireCell::fft_best_length
+ * (util/src/FFTBestLength.cxx:12-62). */
+long wct_fft_best_length(long n, int keep_odd_even);
+
+/* The frame of one transform (ImpactTransform.cxx:84-115): channels
+ * [start_ch, end_ch) padded by npad_wire each side to the nw wires
+ * [frame_start_ch, frame_end_ch) (m_start_ch, m_end_ch), ticks
+ * [frame_start_tick, frame_end_tick) (m_start_tick, m_end_tick), nt of
+ * them. */
+typedef struct wct_impact_frame {
+    int start_ch, end_ch, npad_wire;
+    int frame_start_ch, frame_end_ch;
+    int frame_start_tick, frame_end_tick;
+    int nw, nt;
+} wct_impact_frame;
+/* From BinnedDiffusion_transform's impact_bin_range and time_bin_range
+ * (impact_lo/hi, time_lo/hi), the response's waveform_pad and the
+ * response geometry: num_group = round(pitch / impact) + 1 impact groups,
+ * nwires_resp the response's wire count (:28-29). */
+int wct_impact_geometry(int num_group, int nwires_resp, int impact_lo, int impact_hi, int time_lo, int time_hi,
+                        int waveform_pad, wct_impact_frame* out);
+
+/* The response side (ImpactTransform.cxx:34-74): resp holds
+ * num_group x nwires_resp rows of nresp floats (device or host memory,
+ * copied), row (i, j) the waveform() of the impact response that
+ * m_vec_map_resp[i][j - P] holds, P = (nwires_resp - 1) / 2.  The
+ * reference re-derives that waveform from spectrum() with inv_c2r
+ * (:158-161); both are the same samples.  nwires_resp must be odd (an
+ * even count leaves the reference's map without the rows it reads). */
+typedef struct wct_impact_s* wct_impact;
+int wct_impact_create(wct_impact* out, int num_group, int nwires_resp, const float* resp, int nresp,
+                      void* stream);
+int wct_impact_destroy(wct_impact h);
+/* The constructor's convolution (:117-306) into out (nw x nt floats,
+ * device, m_decon_data row-major by wire).  The charges of group g are
+ * entries [group_offsets[g], group_offsets[g + 1]) of ch, tick and q
+ * (device arrays; group_offsets is a host array of num_group + 1), the
+ * tuples of m_vec_vec_charge[g].  Entries outside the frame are skipped
+ * (the reference's Eigen access would be out of bounds).  Charges add up
+ * with float atomics, so sums over one cell may round differently from
+ * run to run.  The handle keeps frame-sized work buffers (grown on
+ * demand); calls on one handle must not overlap. */
+int wct_impact_run(wct_impact h, const wct_impact_frame* frame, const long* group_offsets, const int* ch,
+                   const int* tick, const double* q, float* out, void* stream);
+/* ImpactTransform::waveform(iwire) (:312-347) for every wire of the
+ * frame: out has nw rows of nsamples floats (row r: channel
+ * frame_start_ch + r); decon is wct_impact_run's output.  With nlong > 0
+ * each row is convolved with long_aux (device, nlong floats; the
+ * response's long_aux_waveform) over fft_best_length(nsamples +
+ * long_pad) samples. */
+int wct_impact_waveforms(wct_impact h, const wct_impact_frame* frame, const float* decon, int nsamples,
+                         const float* long_aux, int nlong, int long_pad, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

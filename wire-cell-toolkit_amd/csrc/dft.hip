@@ -8,14 +8,17 @@
 #include <tuple>
 
 #include "common.hpp"
+#include "gmem.hpp"
 
 namespace wct {
 
 constexpr int kBlock = 256;
 
 // ---- rows: each slice of SLICE threads transforms one row in LDS ----
-// LDS: rows of padded_len(n) entries, then the plan's twiddle table.
-template <typename T, int SLICE, int NMAX>
+// LDS: rows of padded_len(n) entries, then the plan's twiddle table
+// (TWG: no table in LDS, the passes read it through the caches, which
+// for long rows leaves room for more workgroups per CU).
+template <typename T, int SLICE, int NMAX, bool TWG = false>
 __global__ __launch_bounds__(kBlock) void k_fft_rows(const typename C2<T>::type* __restrict__ in,
                                                      typename C2<T>::type* __restrict__ out,
                                                      int nrows, long dist, FftPlan pl,
@@ -33,16 +36,19 @@ __global__ __launch_bounds__(kBlock) void k_fft_rows(const typename C2<T>::type*
     const int ld = padded_len(n);
     T2* buf = smem + (size_t)slice * ld;
     T2* twl = smem + (size_t)RPB * ld;
-    stage_twiddles(twl, tw, pl.twcount, threadIdx.x, kBlock);
+    if (!TWG) stage_twiddles(twl, tw, pl.twcount, threadIdx.x, kBlock);
     const bool active = row < nrows;
     if (active) {
-        const T2* src = in + row * dist;
+        const auto src = G(in + row * dist);
         for (int i = tid; i < n; i += SLICE) buf[pidx(i)] = src[i];
     }
     __syncthreads();
-    fft_lds<T2, SLICE, NMAX>(buf, pl, twl, sign, tid);
+    if (TWG)
+        fft_lds<T2, SLICE, NMAX>(buf, pl, G(tw), sign, tid);
+    else
+        fft_lds<T2, SLICE, NMAX>(buf, pl, (const T2*)twl, sign, tid);
     if (active) {
-        T2* dst = out + row * dist;
+        const auto dst = G(out + row * dist);
         for (int i = tid; i < n; i += SLICE) {
             T2 v = buf[pidx(i)];
             v.x *= scale;
@@ -129,21 +135,23 @@ static int launch_rows_lds(const void* in, void* out, int n, int nrows, long dis
     const T2* src = (const T2*)in;
     T2* dst = (T2*)out;
     constexpr bool dbl = sizeof(T) == 8;
-#define WCT_ROWS(SL, NM)                                                                         \
-    do {                                                                                         \
-        const int rpb = kBlock / SL;                                                             \
-        dim3 grid((nrows + rpb - 1) / rpb);                                                      \
-        size_t lds = ((size_t)rpb * padded_len(n) + dp->plan.twcount) * sizeof(T2);              \
-        int rc_ = allow_lds(k_fft_rows<T, SL, NM>, lds);                                         \
-        if (rc_) return rc_;                                                                     \
-        hipLaunchKernelGGL((k_fft_rows<T, SL, NM>), grid, dim3(kBlock), lds, s, src, dst, nrows, \
-                           dist, dp->plan, tw, sign, scale);                                     \
+#define WCT_ROWS(SL, NM, TWG)                                                                        \
+    do {                                                                                             \
+        const int rpb = kBlock / SL;                                                                 \
+        dim3 grid((nrows + rpb - 1) / rpb);                                                          \
+        size_t lds = ((size_t)rpb * padded_len(n) + (TWG ? 0 : dp->plan.twcount)) * sizeof(T2);      \
+        int rc_ = allow_lds(k_fft_rows<T, SL, NM, TWG>, lds);                                        \
+        if (rc_) return rc_;                                                                         \
+        hipLaunchKernelGGL((k_fft_rows<T, SL, NM, TWG>), grid, dim3(kBlock), lds, s, src, dst, nrows, \
+                           dist, dp->plan, tw, sign, scale);                                         \
     } while (0)
-    if (n <= 64) WCT_ROWS(16, 64);
-    else if (n <= 256) WCT_ROWS(64, 256);
-    else if (n <= 1024) WCT_ROWS(128, 1024);
-    else if (n <= 4096) WCT_ROWS(256, 4096);
-    else if (!dbl && n <= 8192) WCT_ROWS(256, 8192);
+    // Rows above 1024 entries read the twiddles through the caches: their
+    // tables (about n entries) would otherwise double the LDS a row takes.
+    if (n <= 64) WCT_ROWS(16, 64, false);
+    else if (n <= 256) WCT_ROWS(64, 256, false);
+    else if (n <= 1024) WCT_ROWS(128, 1024, false);
+    else if (n <= 4096) WCT_ROWS(256, 4096, true);
+    else if (!dbl && n <= 8192) WCT_ROWS(256, 8192, true);
     else {
         set_error("transform length %d is above the LDS limit", n);
         return WCT_ERR_UNSUPPORTED;

@@ -49,6 +49,11 @@ class DeconJob(C.Structure):
     ]
 
 
+class ImpactFrame(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("start_ch", "end_ch", "npad_wire", "frame_start_ch", "frame_end_ch",
+                                        "frame_start_tick", "frame_end_tick", "nw", "nt")]
+
+
 # name -> (restype, argtypes)
 _V, _I, _D, _F, _Z = C.c_void_p, C.c_int, C.c_double, C.c_float, C.c_size_t
 _SIGS = {
@@ -109,6 +114,12 @@ _SIGS = {
     "wct_plain_workspace_size": (_Z, [_I, _I]),
     "wct_plain_decon_plane": (_I, [_V, _I, _I, _I, _V, _I, _V, _V, _V, _I, _I, _F, _V, _V, _V, _V, _I, _V,
                                    _I, _V, _I, _V, _Z, _V]),
+    "wct_fft_best_length": (C.c_long, [C.c_long, _I]),
+    "wct_impact_geometry": (_I, [_I, _I, _I, _I, _I, _I, _I, C.POINTER(ImpactFrame)]),
+    "wct_impact_create": (_I, [C.POINTER(C.c_void_p), _I, _I, _V, _I, _V]),
+    "wct_impact_destroy": (_I, [_V]),
+    "wct_impact_run": (_I, [_V, C.POINTER(ImpactFrame), C.POINTER(C.c_long), _V, _V, _V, _V, _V]),
+    "wct_impact_waveforms": (_I, [_V, C.POINTER(ImpactFrame), _V, _I, _V, _I, _I, _V, _V]),
 }
 
 # Every symbol include/wct_hip.h declares (checked by the CPU tests).
