@@ -59,6 +59,10 @@ def parse():
                     help="with --no-tune: row plan (-1 runtime, 0 default = 1 compile-time) and the r2c / "
                          "c2r workgroups per CU of the compile-time plan (0 = its default)")
     ap.add_argument("--eager", action="store_true", help="time plan runs without the hipGraph")
+    ap.add_argument("--impact", type=int, default=1,
+                    help="1: time the forward simulation convolution (one 896 x 6272 plane frame) as a side leg")
+    ap.add_argument("--impact-only", action="store_true",
+                    help="run only the forward simulation leg and print its own JSON line")
     ap.add_argument("--adjoint-frames", type=int, default=16,
                     help="frames of 800 x 6000 in the timed adjoint gradient step (0 = skip)")
     return ap.parse_args()
@@ -285,6 +289,84 @@ def adjoint_leg(dev, nframes, reps=3):
             "passes": 6, "channel_ticks_per_s": ct / (ms / 1e3)}
 
 
+def impact_leg(dev, reps=20, cpu=True):
+    """Gen::ImpactTransform on one plane: 20000 depositions (1.44 M charge
+    entries in 11 impact groups) convolved with 21-wire responses into an
+    896 x 6272 frame, charges and responses resident on the device.  The
+    dominant kernel, the forward time transform of the six packed frames
+    (6 x 896 rows of 6272), is timed on its own through wct_dft_fwd1b,
+    which launches the same kernel on the same shape.  The CPU baseline is
+    the oracle's FP64 numpy restatement (single-threaded transforms) on
+    the same inputs, which also gives the relative error."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from wct_amd import workload as W
+    from wct_amd.capi import check, lib
+    from wct_amd.impact import ImpactTransform
+    P = W.IMPACT
+    G = P["num_group"]
+    resp = W.impact_responses(G, P["nwires_resp"], P["nresp"]).astype(np.float32)
+    it = ImpactTransform(resp)
+    fr = it.geometry(P["impact_range"], P["time_range"], P["waveform_pad"])
+    offs, ch, tk, q = W.impact_charges(fr, G, P["ndepo"], P["time_range"])
+    dch = torch.as_tensor(ch, device=dev)
+    dtk = torch.as_tensor(tk, device=dev)
+    dq = torch.as_tensor(q, device=dev)
+    nw, nt = fr["nw"], fr["nt"]
+    out = torch.empty((nw, nt), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, n):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(n):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    run = lambda: it.run(fr, offs, dch, dtk, dq, out=out)
+    for _ in range(3):
+        run()
+    ms = timed(run, reps)
+    nframes = (G - 1) // 2 + 1
+    rows = torch.empty((nframes * nw, nt), dtype=torch.complex64, device=dev)
+    rows.normal_()
+    fwd = lambda: check(lib().wct_dft_fwd1b(C.c_void_p(rows.data_ptr()), C.c_void_p(rows.data_ptr()),
+                                            nframes * nw, nt, 1, stream.cuda_stream))
+    fwd()
+    row_ms = timed(fwd, reps)
+    row_bytes = 2 * 8 * nframes * nw * nt
+    res = {
+        "workload": f"ImpactTransform, {G} impact groups x {P['nwires_resp']}-wire responses, "
+                    f"{P['ndepo']} depositions, frame {nw} x {nt}",
+        "charges": int(offs[-1]), "frame": [nw, nt], "ms_per_plane": round(ms, 4),
+        "samples_per_s": nw * nt / (ms / 1e3),
+        "dominant_kernel": {"kernel": "k_fft_rows (forward time transform of the packed frames)",
+                            "launch_ms": round(row_ms, 4), "alg_bytes": row_bytes,
+                            "achieved_gbs": row_bytes / (row_ms / 1e3) / 1e9,
+                            "frac": row_bytes / (row_ms / 1e3) / 1e9 / HBM_PEAK_GBS},
+        "dtype": "f32",
+    }
+    if cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import impact_transform as IT
+        geo = IT.geometry(G, P["nwires_resp"], P["impact_range"], P["time_range"], P["waveform_pad"])
+        groups = [np.stack([ch[a:b], tk[a:b], q[a:b]], axis=1) for a, b in zip(offs[:-1], offs[1:])]
+        t0 = time.perf_counter()
+        want = IT.transform(groups, resp.astype(np.float64), geo)
+        cpu_s = time.perf_counter() - t0
+        got = out.cpu().numpy().astype(np.float64)
+        d = got - want
+        res["rel_rms_vs_fp64"] = float(np.sqrt((d * d).sum() / (want * want).sum()))
+        res["cpu_baseline"] = {"value": nw * nt / cpu_s, "unit": "samples/s", "cores": 1, "kind": "port",
+                               "sample": "the whole plane through the oracle's numpy FP64 restatement",
+                               "seconds": round(cpu_s, 3)}
+    it.close()
+    return res
+
+
 def plain_leg(planes, frames, outs, masks, rmss, kern, napas, reps=2):
     """The plain per-plane device path (one launch per reference step,
     FP32, response spectrum rebuilt per call) over the first napas APAs
@@ -437,6 +519,10 @@ def main():
     import torch
     from wct_amd import decon, workload as W
     dev = torch.device("cuda:0")
+    if args.impact_only:
+        print(json.dumps({"metric": "forward simulation samples/s (ImpactTransform, 1 GPU)",
+                          **impact_leg(dev, cpu=not args.no_cpu)}), flush=True)
+        return
     planes = W.config_planes(args.workload)
     types = sorted(set(planes))
     calibration_copy(dev)
@@ -524,6 +610,10 @@ def main():
         acc_ok = all(v["rel_rms"] <= 1e-4 for v in accuracy.values())
         log(f"bench: FP64 accuracy {accuracy}")
 
+    impact = None
+    if args.impact:
+        log("bench: forward simulation (ImpactTransform) leg")
+        impact = impact_leg(dev, cpu=not args.no_cpu)
     adjoint = None
     if args.adjoint_frames > 0:
         adjoint = adjoint_leg(dev, args.adjoint_frames)
@@ -596,6 +686,7 @@ def main():
         "response_build_fp64": resp_build,
         "plain_gpu": plain,
         "adjoint_step": adjoint,
+        "forward_sim": impact,
         "chargefit": chargefit,
         "batched_vs_plain": None if plain is None else plain["ratio"],
         "cpu_baseline": cpu,

@@ -57,6 +57,13 @@ def _response_frame(resp_g, P, nw, nt):
     return np.fft.fft(np.fft.fft(frame, axis=1), axis=0)
 
 
+def _cols(group):
+    """(channel, tick, charge) columns of one group's entries (a list of
+    tuples or an (n, 3) array)."""
+    a = np.asarray(group, dtype=np.float64).reshape(-1, 3)
+    return a[:, 0].astype(np.int64), a[:, 1].astype(np.int64), a[:, 2]
+
+
 def transform(charges, resp, geo):
     """m_decon_data (nw x nt) of the constructor (:117-306)."""
     G, P = geo["num_group"], geo["num_pad_wire"]
@@ -67,18 +74,18 @@ def transform(charges, resp, geo):
     acc = np.zeros((nw, nt), dtype=np.complex128)
     for i in range(num_double):
         c = np.zeros((nw, nt), dtype=np.complex128)
-        for ch, tick, q in charges[i]:
-            c[ch + npad - start_ch, tick - t0] += q
+        ch, tick, q = _cols(charges[i])
+        np.add.at(c, (ch + npad - start_ch, tick - t0), q)
         ii = 2 * num_double - i
-        for ch, tick, q in charges[ii]:
-            c[end_ch + npad - 1 - ch, tick - t0] += 1j * q
+        ch, tick, q = _cols(charges[ii])
+        np.add.at(c, (end_ch + npad - 1 - ch, tick - t0), 1j * q)
         c = np.fft.fft2(c)
         c = c * _response_frame(resp[i], P, nw, nt)
         acc += np.fft.ifft(c, axis=0)
     i = num_double
     d = np.zeros((nw, nt), dtype=np.float64)
-    for ch, tick, q in charges[i]:
-        d[ch + npad - start_ch, tick - t0] += q
+    ch, tick, q = _cols(charges[i])
+    np.add.at(d, (ch + npad - start_ch, tick - t0), q)
     c = np.fft.fft(np.fft.fft(d, axis=1), axis=0) * _response_frame(resp[i], P, nw, nt)
     acc += np.fft.ifft(c, axis=0)
     acc = np.fft.ifft(acc, axis=1)

@@ -157,3 +157,54 @@ def charge_raster(rng, nw, nt, occupancy=0.02, ntracks=None, blip_mean=20000.0):
 def tracks_per_plane(nw, per_480=20):
     """Track count at the density of 20 tracks per 480 wires."""
     return max(1, int(round(per_480 * nw / 480.0)))
+
+
+# ---- forward simulation convolution (Gen::ImpactTransform) ----
+# One induction plane of 800 wires read out over 6000 ticks: 11 impact
+# groups (10 impact positions per pitch), responses over 21 wires, the
+# frame of a transform whose charge spans the whole plane (896 x 6272).
+IMPACT = dict(num_group=11, nwires_resp=21, nresp=6000, impact_range=(0, 8000), time_range=(0, 6000),
+              waveform_pad=200, ndepo=20000)
+
+
+def impact_responses(num_group, nwires_resp, nresp, seed=7):
+    """Response rows (num_group, nwires_resp, nresp): a bipolar induction
+    shape on the offset wires, unipolar on the central one, shifted with
+    the impact position and falling off with the offset; zero after 400
+    ticks as the reference's short-padded responses are."""
+    rng = np.random.default_rng(seed)
+    P = (nwires_resp - 1) // 2
+    t = np.arange(min(nresp, 400), dtype=np.float64)
+    r = np.zeros((num_group, nwires_resp, nresp))
+    for g in range(num_group):
+        for j in range(nwires_resp):
+            k = j - P
+            t0 = 60 + 2 * abs(k) + 0.3 * g
+            s = np.exp(-0.5 * ((t - t0) / 5.0) ** 2)
+            if k != 0:
+                s = s - np.exp(-0.5 * ((t - t0 - 12) / 5.0) ** 2)
+            r[g, j, :len(t)] = s / (1 + abs(k)) * (1 + 0.02 * rng.normal())
+    return r
+
+
+def impact_charges(frame, num_group, ndepo, time_range, seed=11):
+    """Charge entries of ndepo depositions as get_charge_vec hands them
+    over: each deposition spreads over 3 neighbouring impact groups,
+    2 wires and 12 ticks with Gaussian weights.  Returns (offsets[G + 1],
+    channel, tick, charge) with the entries grouped by impact group."""
+    rng = np.random.default_rng(seed)
+    ch0 = rng.integers(frame["start_ch"] + 1, frame["end_ch"] - 2, size=ndepo)
+    t0 = rng.integers(time_range[0], max(time_range[0] + 1, time_range[1] - 13), size=ndepo)
+    g0 = rng.integers(0, num_group - 2, size=ndepo)
+    amp = rng.uniform(200.0, 8000.0, size=ndepo)
+    dg, dc, dt = np.meshgrid(np.arange(3), np.arange(2), np.arange(12), indexing="ij")
+    w = (np.exp(-0.5 * ((dt - 5.5) / 2.0) ** 2) * np.where(dc == 0, 1.0, 0.5) *
+         np.exp(-0.5 * (dg - 1.0) ** 2)).ravel()
+    grp = (g0[:, None] + dg.ravel()[None, :]).ravel()
+    ch = (ch0[:, None] + dc.ravel()[None, :]).ravel()
+    tk = (t0[:, None] + dt.ravel()[None, :]).ravel()
+    q = (amp[:, None] * w[None, :]).ravel()
+    order = np.argsort(grp, kind="stable")
+    counts = np.bincount(grp, minlength=num_group)
+    offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    return offsets, ch[order].astype(np.int32), tk[order].astype(np.int32), q[order]
