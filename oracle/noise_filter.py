@@ -1,0 +1,318 @@
+"""Restatement of Wire-Cell's noise filter for ProtoDUNE-SP: OmnibusNoiseFilter
+(sigproc/src/OmnibusNoiseFilter.cxx:95-260) with Protodune::OneChannelNoise
+(sigproc/src/Protodune.cxx:753-918) and the parts of OmniChannelNoiseDB it
+reads (sigproc/src/OmniChannelNoiseDB.cxx).
+
+TEST INFRASTRUCTURE ONLY.  Only tests/ and the CPU-baseline leg of the
+benchmark may import this module, and only as the checker.  Float
+arithmetic of the reference is done in numpy float32 scalars and arrays,
+double arithmetic in float64; the reference's FFTW float transforms are
+done in complex128 and rounded to complex64 where the reference holds a
+float spectrum.  Pure-Python loops follow the reference's sequential
+algorithms, so this is for test-sized inputs (tens of channels).
+"""
+import math
+
+import numpy as np
+
+F = np.float32
+FLAG = F(10000.0)
+
+
+# ---- util/src/Waveform.cxx --------------------------------------------
+
+def mean_rms(wf):
+    """Waveform::mean_rms (Waveform.cxx:19-37): double sums, population RMS."""
+    n = len(wf)
+    if n == 0:
+        return 0.0, 0.0
+    if n == 1:
+        return float(wf[0]), 0.0
+    s = s2 = 0.0
+    for x in np.asarray(wf, dtype=np.float64):
+        s += x
+        s2 += x * x
+    mean = s / n
+    v = (s2 - s * s / n) / n
+    return mean, math.sqrt(v) if v >= 0 else float("nan")
+
+
+def percentile_binned(wave, p):
+    """Waveform::percentile_binned (Waveform.cxx:108-131): an n-bin float
+    histogram between min and max, the lower edge of the bin where the
+    count passes n p."""
+    w = np.asarray(wave, dtype=F)
+    vmin, vmax = F(w.min()), F(w.max())
+    nbins = len(w)
+    binsize = F((vmax - vmin) / F(nbins))
+    hist = np.zeros(nbins, dtype=np.int64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for val in w:
+            r = F(F(val - vmin) / binsize)
+            if np.isnan(r):
+                b = -2 ** 31          # x86 conversion of NaN
+            else:
+                b = int(math.floor(abs(float(r)) + 0.5)) * (1 if r >= 0 else -1)   # round half away
+            b = max(0, b)
+            b = min(nbins - 1, b)
+            hist[b] += 1
+    imed = int(F(F(len(w)) * F(p)))   # size_t * float, in float
+    count = 0
+    for ind in range(nbins):
+        count += hist[ind]
+        if count > imed:
+            return F(vmin + F(F(ind) * binsize))
+    return F(vmin + (vmax - vmin) * F(p))
+
+
+def median_binned(wave):
+    return percentile_binned(wave, 0.5)
+
+
+# ---- sigproc/src/Microboone.cxx helpers ------------------------------
+
+def calc_rms_with_flags(sig):
+    """Microboone::CalcRMSWithFlags (:545-566): samples below 4096 only."""
+    temp = [x for x in np.asarray(sig, dtype=F) if x < 4096]
+    if not temp:
+        return F(0.0)
+    # the reference passes the double 0.5 -+ 0.34 to a float parameter
+    p0 = percentile_binned(temp, F(0.5 - 0.34))
+    p1 = percentile_binned(temp, F(0.5))
+    p2 = percentile_binned(temp, F(0.5 + 0.34))
+    return F(math.sqrt((float(p2 - p1) ** 2 + float(p1 - p0) ** 2) / 2.0))
+
+
+def signal_filter(sig):
+    """Microboone::SignalFilter (:568-610), in place: samples within 8 ticks
+    of one above 4 RMS get 20000 added (once)."""
+    sig = np.asarray(sig, dtype=F)
+    thr = F(4.0 * float(calc_rms_with_flags(sig)))
+    n = len(sig)
+    region = [(x > thr or float(x) < -1.0 * float(thr)) and x < 4096.0 for x in sig]
+    for i in range(n):
+        if region[i]:
+            for j in range(max(i - 8, 0), min(i + 8, n)):
+                if sig[j] < 4096.0:
+                    sig[j] = F(float(sig[j]) + 20000.0)
+    return sig
+
+
+def raw_adaptive_baseline(sig):
+    """Microboone::RawAdapativeBaselineAlg (:612-737), in place."""
+    sig = np.asarray(sig, dtype=F)
+    W, n = 20, len(sig)
+    minw = W // 2
+    if all(x == FLAG for x in sig):
+        return sig
+    base = [0.0] * n
+    filled = [False] * n
+    bval, wb = 0.0, 0
+    for j in range(0, W // 2 + 1):
+        if sig[j] < 4096.0:
+            bval += float(sig[j])
+            wb += 1
+    base[0] = 0.0 if wb == 0 else bval / wb
+    filled[0] = wb >= minw
+    for j in range(1, n):
+        old, new = j - W // 2 - 1, j + W // 2
+        if old >= 0 and sig[old] < 4096.0:
+            bval -= float(sig[old])
+            wb -= 1
+        if new < n and sig[new] < 4096:
+            bval += float(sig[new])
+            wb += 1
+        base[j] = 0.0 if wb == 0 else bval / wb
+        filled[j] = wb >= minw
+    for j in range(n):
+        x = sig[j]
+        if x != FLAG:
+            if not filled[j]:
+                d = j
+                while not filled[d] and d > 0 and sig[d] != FLAG:
+                    d -= 1
+                dflag = not filled[d]
+                u = j
+                while not filled[u] and u < n - 1 and sig[u] != FLAG:
+                    u += 1
+                uflag = not filled[u]
+                if not dflag and not uflag:
+                    base[j] = ((j - d) * base[d] + (u - j) * base[u]) / float(u - d)
+                elif dflag and not uflag:
+                    base[j] = base[u]
+                elif not dflag and uflag:
+                    base[j] = base[d]
+                else:
+                    base[j] = 0.0
+            sig[j] = F(float(x) - base[j])
+    return sig
+
+
+def remove_filter_flags(sig):
+    """Microboone::RemoveFilterFlags (:739-757), in place."""
+    sig = np.asarray(sig, dtype=F)
+    for i in range(len(sig)):
+        x = float(sig[i])
+        if x > 4096.0:
+            sig[i] = F(x - 20000.0) if x > 10000.0 else F(0.0)
+    return sig
+
+
+def noisy_filter(sig, min_rms, max_rms):
+    """Microboone::NoisyFilterAlg (:464-533), in place; True if noisy."""
+    sig = np.asarray(sig, dtype=F)
+    rms = float(calc_rms_with_flags(sig))
+    if rms > float(F(max_rms)) or rms < float(F(min_rms)):
+        sig[:] = FLAG
+        return True
+    return False
+
+
+# ---- sigproc/src/Diagnostics.cxx -----------------------------------
+
+def is_partial(spec, nfreqs=4, maxpower=6000.0):
+    """Diagnostics::Partial (:15-31): bins 1..nfreqs+1 falling and their
+    mean magnitude above maxpower."""
+    mag0 = abs(complex(spec[1]))
+    s = mag0
+    for ind in range(1, nfreqs + 1):
+        if ind >= len(spec):
+            break
+        magi = abs(complex(spec[ind + 1]))
+        if mag0 <= magi:
+            return False
+        s += magi
+    return s / (nfreqs + 1) > maxpower
+
+
+# ---- OmniChannelNoiseDB spectra ----------------------------------------
+
+def rcrc_spectrum(rcrc, tick, nsamples, layers=1):
+    """parse_rcrc (OmniChannelNoiseDB.cxx:171-208): Response::SimpleRC
+    sampled at t = i tick (util/src/Response.cxx:517-536, :267-276), its
+    float spectrum raised to the power rc_layers."""
+    t = np.arange(nsamples) * ((nsamples * tick) / nsamples)
+    w = np.where(rcrc > 0, -tick / rcrc * np.exp(-t / rcrc), 0.0)
+    w = w + (t < tick)
+    spec = np.fft.fft(w.astype(F).astype(np.float64)).astype(np.complex64)
+    out = spec.copy()
+    for _ in range(layers - 1):
+        out = (out * spec).astype(np.complex64)
+    return out
+
+
+def freqmask_spectrum(masks, nsamples):
+    """parse_freqmasks (:152-169): ones, with [lobin, hibin] (inclusive)
+    set to value, in list order."""
+    s = np.ones(nsamples, dtype=np.complex64)
+    for value, lo, hi in masks:
+        lo, hi = max(lo, 0), min(hi, nsamples - 1)
+        s[lo:hi + 1] = value
+    return s
+
+
+def flagged_bins(noise):
+    """The runs of bins whose spectrum value has a real part below 0.5
+    (Protodune.cxx:791-806), as inclusive (first, last) pairs."""
+    bins = []
+    for i, v in enumerate(noise):
+        if np.real(v) < 0.5:
+            if bins and i == bins[-1][1] + 1:
+                bins[-1][1] = i
+            else:
+                bins.append([i, i])
+    return [tuple(b) for b in bins]
+
+
+# ---- DftTools --------------------------------------------------------
+
+def fwd_r2c(x):
+    return np.fft.fft(np.asarray(x, dtype=F).astype(np.float64)).astype(np.complex64)
+
+
+def inv_c2r(spec):
+    """DftTools::inv_c2r (aux/src/DftTools.cxx:39-50) with
+    Spectrum::hermitian_mirror (Spectrum.h:56-89)."""
+    s = np.asarray(spec, dtype=np.complex128).copy()
+    n = len(s)
+    s[0] = s[0].real
+    half = n // 2
+    if n % 2 == 0:
+        s[half] = s[half].real
+        s[half + 1:] = np.conj(s[1:half][::-1])
+    else:
+        s[half + 1:] = np.conj(s[1:half + 1][::-1])
+    return np.fft.ifft(s).real.astype(F)
+
+
+def fft_scaling(signal, nsamples):
+    """Protodune::FftScaling (:532-552)."""
+    nsig = len(signal)
+    tran = list(fwd_r2c(signal)) + [0j] * max(0, nsamples - nsig)
+    tran = np.asarray(tran[:nsamples], dtype=np.complex64)
+    mid = nsig // 2 if nsig % 2 == 0 else (nsig + 1) // 2
+    tran = np.concatenate([tran[:mid], tran[nsig:], tran[mid:nsig]])
+    out = inv_c2r(tran)
+    return (out * F(nsamples // nsig)).astype(F)
+
+
+# ---- Protodune::OneChannelNoise::apply ---------------------------------
+
+def one_channel_noise(signal, plane, rcrc=None, noise=None, min_rms=0.5, max_rms=10.0, resmp=None):
+    """Returns (filtered signal, {mask name: True}) for one channel
+    (Protodune.cxx:753-918).  rcrc, noise: the DB spectra (None: empty)."""
+    sig = np.asarray(signal, dtype=F).copy()
+    masks = {}
+    if resmp is not None:
+        smpout = len(sig)
+        sig = fft_scaling(sig[:resmp], smpout)
+    spectrum = fwd_r2c(sig)
+    partial = is_partial(spectrum)
+    if not partial and rcrc is not None and len(rcrc) == len(spectrum):
+        with np.errstate(divide="ignore", invalid="ignore"):
+            spectrum = (spectrum.astype(np.complex128) / np.asarray(rcrc, np.complex128)).astype(np.complex64)
+    if plane == 2:
+        mag = np.abs(spectrum.astype(np.complex128)).astype(F)
+        mag[0] = 0
+        mag = raw_adaptive_baseline(mag)
+        spec_n = [] if noise is None else noise
+        fb = flagged_bins(spec_n)
+        n_harmonic = 0
+        m = len(mag)
+        for _ in range(5):
+            for lo, hi in fb:
+                iend = hi + 1
+                mean, rms = mean_rms(mag[lo:iend])
+                cut = mean + 2.7 * rms
+                if lo > 1050:
+                    cut = mean + 3 * rms
+                for j in range(lo, iend):
+                    if float(mag[j]) > cut:
+                        spectrum[j] = 0
+                        spectrum[m + 1 - j] = 0
+                        mag[j] = 0
+                        mag[m + 1 - j] = 0
+                        n_harmonic += 1
+        if n_harmonic > 5:
+            masks["harmonic"] = True
+    spectrum[0] = 0
+    sig = inv_c2r(spectrum)
+    mean, rms = mean_rms(sig)
+    temp = sig.copy()
+    for i in range(len(temp)):
+        if abs(float(temp[i]) - mean) > 6 * rms:
+            temp[i] = F(mean)
+    baseline = median_binned(temp)
+    sig = (sig + F(-baseline)).astype(F)
+    if partial:
+        if plane != 2:
+            masks["lf_noisy"] = True
+        signal_filter(sig)
+        raw_adaptive_baseline(sig)
+        remove_filter_flags(sig)
+    signal_filter(sig)
+    noisy = noisy_filter(sig, min_rms, max_rms)
+    remove_filter_flags(sig)
+    if noisy:
+        masks["noisy"] = True
+    return sig, masks

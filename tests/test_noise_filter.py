@@ -1,0 +1,213 @@
+"""The ProtoDUNE noise filter (OmnibusNoiseFilter + Protodune::OneChannelNoise,
+sigproc/src/OmnibusNoiseFilter.cxx, Protodune.cxx:753-918, Microboone.cxx
+helpers, OmniChannelNoiseDB spectra): the oracle (oracle/noise_filter.py)
+pinned on the CPU by hand-worked cases and closed forms.  The reference
+ships no test vectors for these filters.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import noise_filter as NF
+
+F = np.float32
+
+
+def test_percentile_binned_hand_worked():
+    # 0..9: bin width 0.9, 5 -> round(5.56) = 6, 9 -> 10 clamped to 9;
+    # counts per bin 1 1 1 1 1 0 1 1 1 2; the running count first passes
+    # 10 * 0.5 = 5 at bin 6: 6 * 0.9
+    w = np.arange(10, dtype=F)
+    assert NF.percentile_binned(w, 0.5) == F(F(6) * F(0.9))
+    # p = 0: passes 0 at the first occupied bin
+    assert NF.percentile_binned(w, 0.0) == 0
+    # constant wave: zero bin width, every sample in bin 0
+    assert NF.percentile_binned(np.full(7, 3.25, F), 0.5) == F(3.25)
+    # n p rounds in float: 6000 * 0.16f is 960 in float (959.99998 in double)
+    w = np.arange(6000, dtype=F)
+    got = NF.percentile_binned(w, F(0.5 - 0.34))
+    binsize = F(F(5999) / F(6000))
+    # bins of k: round(k / 0.99983) = k except where the ratio crosses .5;
+    # the 961st sample (960 counted before it) sits in bin 960
+    assert got == F(F(960) * binsize)
+
+
+def test_mean_rms_and_calc_rms():
+    assert NF.mean_rms([1, 2, 3, 4]) == (2.5, math.sqrt(1.25))
+    rng = np.random.default_rng(1)
+    x = rng.normal(0, 5, 20000).astype(F)
+    r = float(NF.calc_rms_with_flags(x))
+    assert abs(r - 5) < 0.25
+    # flagged samples (>= 4096) do not count
+    y = np.concatenate([x, np.full(500, 10000, F), np.full(300, 25000, F)])
+    assert float(NF.calc_rms_with_flags(y)) == pytest.approx(r, rel=0.02)
+    assert NF.calc_rms_with_flags(np.full(10, 5000, F)) == 0
+
+
+def test_signal_filter_pads_eight_each_side():
+    rng = np.random.default_rng(2)
+    x = rng.normal(0, 1, 200).astype(F)
+    x[50] = 100
+    y = NF.signal_filter(x.copy())
+    assert (y[42:58] > 10000).all()
+    assert (y[:42] < 4096).all() and (y[58:] < 4096).all()
+    np.testing.assert_array_equal(y[42:58], (x[42:58].astype(np.float64) + 20000.0).astype(F))
+    # already-flagged samples are left alone
+    x2 = x.copy(); x2[45] = 10000
+    assert NF.signal_filter(x2.copy())[45] == 10000
+
+
+def test_raw_adaptive_baseline_cases():
+    n = 300
+    # constant: zero everywhere
+    np.testing.assert_array_equal(NF.raw_adaptive_baseline(np.full(n, 7.0, F)), 0)
+    # linear ramp: interior ~0, first sample minus the mean of samples 0..10
+    a, b = F(0.5), F(3.0)
+    x = (a * np.arange(n, dtype=F) + b).astype(F)
+    y = NF.raw_adaptive_baseline(x.copy())
+    assert np.abs(y[10:n - 10]).max() < 1e-4
+    assert y[0] == pytest.approx(-5 * 0.5, abs=1e-5)
+    # flagged samples stay; all-flagged input is returned unchanged
+    x = np.full(n, 2.0, F); x[100] = 10000
+    y = NF.raw_adaptive_baseline(x.copy())
+    assert y[100] == 10000 and np.abs(np.delete(y, 100)).max() == 0
+    np.testing.assert_array_equal(NF.raw_adaptive_baseline(np.full(30, 10000, F)), 10000)
+    # a block of 5000s (>= 4096, not flags) between backgrounds 0 and 100:
+    # windows inside the block see fewer than 10 valid samples and take
+    # ((j - d) base[d] + (u - j) base[u]) / (u - d) from the nearest
+    # filled windows d < j < u (the reference's weights, the nearer
+    # background weighted by the far distance)
+    x = np.zeros(n, F); x[180:] = 100; x[120:180] = 5000
+    y = NF.raw_adaptive_baseline(x.copy())
+    # window [j-10, j+10] holds 130 - j valid samples left of the block
+    # (filled up to j = 120, mean 0) and j - 169 right of it (filled from
+    # j = 179, mean of x[180..189] = 100); j = 140 walks down to 120 and
+    # up to 179: base = ((140 - 120) * 0 + (179 - 140) * 100) / 59
+    assert y[140] == F(5000 - 3900 / 59)
+    assert y[120] == F(5000 - 0) and y[179] == F(5000 - 100)
+    assert y[121] == F(5000 - (1 * 0 + 58 * 100) / 59)
+
+
+def test_remove_filter_flags_and_noisy():
+    x = np.array([5000, 20005, 100, 10000, 4096, -3], F)
+    np.testing.assert_array_equal(NF.remove_filter_flags(x.copy()), [0, 5, 100, 0, 4096, -3])
+    rng = np.random.default_rng(3)
+    quiet = rng.normal(0, 0.1, 1000).astype(F)
+    assert NF.noisy_filter(quiet, 0.5, 10.0) and (quiet == 10000).all()
+    ok = rng.normal(0, 3, 1000).astype(F)
+    assert not NF.noisy_filter(ok.copy(), 0.5, 10.0)
+    loud = rng.normal(0, 40, 1000).astype(F)
+    assert NF.noisy_filter(loud, 0.5, 10.0)
+
+
+def test_partial_detection():
+    spec = np.zeros(100, np.complex64)
+    spec[1:7] = [9e4, 8e4, 7e4, 6e4, 5e4, 1]
+    assert NF.is_partial(spec)
+    spec[3] = 9.5e4                          # not falling
+    assert not NF.is_partial(spec)
+    spec[1:7] = [9e3, 8e3, 7e3, 6e3, 5e3, 1]   # falling but mean 7000 > 6000
+    assert NF.is_partial(spec)
+    spec[1:7] = [7e3, 6e3, 5e3, 4e3, 3e3, 1]   # mean 5000
+    assert not NF.is_partial(spec)
+
+
+def test_rcrc_spectrum_closed_form():
+    # x[0] = 1 - t/tau, x[k] = -(t/tau) e^{-k t/tau}: the DFT is
+    # 1 - (t/tau) (1 - e^{-N a}) / (1 - e^{-a} e^{-2 pi i f/N}), a = t/tau
+    tick, tau, n = 500.0, 1.1e6, 1000
+    got = NF.rcrc_spectrum(tau, tick, n).astype(np.complex128)
+    a = tick / tau
+    f = np.arange(n)
+    want = 1 - a * (1 - np.exp(-n * a)) / (1 - np.exp(-a) * np.exp(-2j * np.pi * f / n))
+    assert np.abs(got - want).max() < 1e-5
+    sq = NF.rcrc_spectrum(tau, tick, n, layers=2).astype(np.complex128)
+    assert np.abs(sq - want * want).max() < 1e-5
+
+
+def test_freqmasks_and_flagged_bins():
+    s = NF.freqmask_spectrum([(1.0, 0, 99), (0.0, 10, 12), (0.0, 40, 40), (0.0, 98, 200)], 100)
+    assert NF.flagged_bins(s) == [(10, 12), (40, 40), (98, 99)]
+    assert NF.flagged_bins(np.ones(10)) == []
+
+
+def test_inv_c2r_mirrors_the_lower_half():
+    rng = np.random.default_rng(4)
+    for n in (16, 17):
+        x = rng.normal(size=n).astype(F)
+        np.testing.assert_allclose(NF.inv_c2r(NF.fwd_r2c(x)), x, atol=1e-5)
+        s = NF.fwd_r2c(x)
+        s2 = s.copy(); s2[n // 2 + 1:] = 123 + 45j   # upper half is replaced by the mirror
+        np.testing.assert_allclose(NF.inv_c2r(s2), x, atol=1e-5)
+
+
+def test_fft_scaling_resamples():
+    # a cosine on bin k of 5996 samples comes back on bin k of 6000, scaled
+    # by 5996/6000 (forward unnormalised, inverse by 1/6000, integer factor 1)
+    k, n0, n1 = 37, 5996, 6000
+    x = np.cos(2 * np.pi * k * np.arange(n0) / n0).astype(F)
+    y = NF.fft_scaling(x, n1)
+    want = np.cos(2 * np.pi * k * np.arange(n1) / n1) * n0 / n1
+    assert np.abs(y - want).max() < 1e-4
+
+
+def test_one_channel_baseline_and_masks():
+    rng = np.random.default_rng(5)
+    n = 2000
+    x = (900 + rng.normal(0, 3, n)).astype(F)
+    y, m = NF.one_channel_noise(x, 0, min_rms=1.0, max_rms=30.0)
+    assert m == {} and abs(float(np.median(y))) < 0.5 and float(np.std(y)) == pytest.approx(3, rel=0.1)
+    quiet = (900 + rng.normal(0, 0.2, n)).astype(F)
+    y, m = NF.one_channel_noise(quiet, 1, min_rms=1.0, max_rms=30.0)
+    assert m == {"noisy": True} and not y.any()
+
+
+def test_one_channel_rc_undershoot_removed():
+    rng = np.random.default_rng(6)
+    n, tick, tau = 2000, 500.0, 1.1e6
+    rc = NF.rcrc_spectrum(tau, tick, n)
+    q = np.zeros(n); q[[300, 900, 1500]] = [800.0, 500.0, 1000.0]   # peaks up to 1000 ADC
+    q = np.convolve(q, np.exp(-0.5 * (np.arange(-20, 21) / 4.0) ** 2), "same")
+    x = np.fft.ifft(np.fft.fft(q) * rc.astype(np.complex128)).real + 700 + rng.normal(0, 1, n)
+    y, m = NF.one_channel_noise(x.astype(F), 2, rcrc=rc, min_rms=0.5, max_rms=30.0)
+    assert m == {}
+    # the undershoot after the last pulse is gone and the baseline is at
+    # zero: what is left is the noise
+    rms = lambda v: float(np.sqrt(np.mean(np.square(v))))
+    assert rms(y - q) < 1.1
+    assert abs(float(np.mean((y - q)[1530:1990]))) < 0.3
+    assert float(np.mean((x - 700 - q)[1530:1990])) < -1.0
+
+
+def test_one_channel_harmonics_removed_on_collection():
+    rng = np.random.default_rng(7)
+    n = 2000
+    t = np.arange(n)
+    bins = [300, 340, 380, 420, 460, 500]
+    x = 500 + rng.normal(0, 3, n)
+    for b in bins:
+        x = x + 6 * np.cos(2 * np.pi * b * t / n)
+    noise = NF.freqmask_spectrum([(1.0, 0, n - 1)] + [(0.0, b - 5, b + 5) for b in bins], n)
+    y, m = NF.one_channel_noise(x.astype(F), 2, noise=noise, min_rms=1.0, max_rms=30.0)
+    assert m == {"harmonic": True}
+    spec = np.abs(np.fft.fft(y))
+    assert spec[bins].max() < 1e-2 * np.abs(np.fft.fft(x - 500))[bins].min()
+    # the same on an induction plane leaves the lines alone
+    y2, m2 = NF.one_channel_noise(x.astype(F), 0, noise=noise, min_rms=1.0, max_rms=30.0)
+    assert m2 == {} and np.abs(np.fft.fft(y2))[bins].min() > 1000
+
+
+def test_one_channel_partial_waveform():
+    # a large slow decay: its spectrum falls over bins 1..5 with a mean
+    # above 6000, so the channel is partial: no RC correction, lf_noisy on
+    # induction planes, the adaptive baseline instead
+    rng = np.random.default_rng(8)
+    n = 2000
+    x = (3000 * np.exp(-np.arange(n) / 300.0) + rng.normal(0, 3, n)).astype(F)
+    assert NF.is_partial(NF.fwd_r2c(x))
+    y, m = NF.one_channel_noise(x, 0, rcrc=NF.rcrc_spectrum(1.1e6, 500.0, n), min_rms=1.0, max_rms=30.0)
+    assert m == {"lf_noisy": True}
+    assert abs(float(np.median(y[200:]))) < 3
+    y, m = NF.one_channel_noise(x, 2, min_rms=1.0, max_rms=30.0)
+    assert "lf_noisy" not in m
