@@ -437,6 +437,50 @@ int wct_impact_run(wct_impact h, const wct_impact_frame* frame, const long* grou
 int wct_impact_waveforms(wct_impact h, const wct_impact_frame* frame, const float* decon, int nsamples,
                          const float* long_aux, int nlong, int long_pad, float* out, void* stream);
 
+/* ---- noise filter ----
+ * Protodune::OneChannelNoise::apply (sigproc/src/Protodune.cxx:753-918)
+ * over every channel of a frame, the per-channel filter that
+ * OmnibusNoiseFilter::operator() runs (sigproc/src/OmnibusNoiseFilter.cxx:
+ * 170-188).  Per channel, as the noise DB and the anode give them:
+ *   plane      wire-plane index (2: collection, where the flagged
+ *              frequency runs are cleaned of harmonic lines)
+ *   rcrc       row of the rcrc spectra table (IChannelNoiseDatabase::rcrc),
+ *              -1 for none (an empty spectrum)
+ *   bins_begin, bins_end   the channel's runs of flagged bins (noise
+ *              spectrum real part < 0.5) in the bins table, [begin, end)
+ *              of its (first, last) inclusive pairs
+ *   min_rms, max_rms       min_rms_cut / max_rms_cut */
+typedef struct wct_nf_channel {
+    int plane;
+    int rcrc;
+    int bins_begin, bins_end;
+    float min_rms, max_rms;
+} wct_nf_channel;
+/* Flags per channel: the spectrum looked like a partial waveform
+ * (Diagnostics::Partial; the caller masks it "lf_noisy" off the
+ * collection plane), the RMS test failed ("noisy"), more than 5 harmonic
+ * bins were removed ("harmonic"). */
+#define WCT_NF_PARTIAL 1
+#define WCT_NF_NOISY 2
+#define WCT_NF_HARMONIC 4
+/* The rcrc spectrum of OmniChannelNoiseDB::parse_rcrc
+ * (sigproc/src/OmniChannelNoiseDB.cxx:171-208): Response::SimpleRC of
+ * width (time units) sampled at tick over nt samples, its spectrum (FP64
+ * on the device, rounded to float) raised to the power layers; out is nt
+ * complex floats (device). */
+int wct_nf_rcrc_spectrum(double width, double tick, int nt, int layers, float* out, void* stream);
+/* in: nch rows of nt samples (row stride ld_in, channels whose signal the
+ * filter sees, zeros for bad ones); out: the filtered rows (stride
+ * ld_out); chans: nch entries (device); rcrc: rows of nt complex floats
+ * (device, may be NULL when no channel has one); bins: (first, last)
+ * pairs (device, may be NULL when no channel has any); flags: nch ints
+ * (device).  The workspace holds wct_nf_workspace_size(nch, nt) bytes.
+ * Up to 12000 ticks (one channel's samples and scratch in LDS). */
+size_t wct_nf_workspace_size(int nch, int nt);
+int wct_nf_one_channel(const float* in, int ld_in, float* out, int ld_out, int nch, int nt,
+                       const wct_nf_channel* chans, const float* rcrc, const int* bins, int* flags,
+                       void* workspace, size_t ws_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,124 @@
+"""The per-channel noise filter on the device (wct_nf_one_channel) against
+the oracle (oracle/noise_filter.py), which is pinned on the CPU
+(tests/test_noise_filter.py).
+
+The outputs are float32 and go through FP32 transforms on both sides (the
+oracle's in FP64, rounded), so samples agree to about 1e-4 ADC.  The
+binned median and percentiles return bin edges; a transform rounding
+that moves one sample across a bin edge can move the crossing bin by
+one, which shifts a channel's whole baseline by one bin width (the
+sample range over the sample count, 0.01-0.5 ADC here); on noise-only
+channels, whose bins are narrow, that happens to about a third of the
+channels.  The test therefore requires, per channel, the difference minus
+its median within 1e-2 ADC, the median within one bin width (estimated
+from the output's range, with a margin of half a bin since the median's
+own input has its outliers clipped), and identical flags.
+"""
+import numpy as np
+import pytest
+
+import noise_filter as NF
+
+pytestmark = pytest.mark.gpu
+
+TICK = 500.0        # ns
+TAU = 1.1e6         # 1.1 ms
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def make_frame(rng, nt):
+    """A mixed set of channels: plain noise on each plane, RC-shaped
+    pulses, partial (slow-decay) waveforms, a quiet (noisy-tagged)
+    channel, a bad (all-zero) channel, collection channels with harmonic
+    lines in flagged runs."""
+    rc = NF.rcrc_spectrum(TAU, TICK, nt).astype(np.complex128)
+    t = np.arange(nt)
+    rows, chans = [], []
+    lines = [300, 340, 380, 420, 460, 500]
+    noise_mask = NF.freqmask_spectrum([(1.0, 0, nt - 1)] + [(0.0, b - 5, b + 5) for b in lines], nt)
+    runs = NF.flagged_bins(noise_mask)
+    for i in range(40):
+        plane = i % 3
+        base = 900.0 if plane < 2 else 400.0
+        x = base + rng.normal(0, 2.5 + 0.5 * plane, nt)
+        kind = i % 8
+        if kind in (1, 2):      # pulses through the RC
+            q = np.zeros(nt)
+            for p in rng.integers(100, nt - 100, size=3):
+                q[p] = rng.uniform(200, 900)
+            q = np.convolve(q, np.exp(-0.5 * (np.arange(-20, 21) / 4.0) ** 2), "same")
+            x = x + np.fft.ifft(np.fft.fft(q) * rc).real
+        if kind == 3:           # partial waveform
+            x = x + 3000 * np.exp(-t / 300.0)
+        if kind == 4 and i > 8:  # quiet: noisy-tagged
+            x = base + rng.normal(0, 0.2, nt)
+        if kind == 5:           # bad channel, zeroed by the caller
+            x = np.zeros(nt)
+        if plane == 2 and kind in (0, 6):
+            for b in lines:
+                x = x + 6 * np.cos(2 * np.pi * b * t / nt + rng.uniform(0, 6.28))
+        rows.append(x.astype(np.float32))
+        chans.append(dict(plane=plane, rcrc=0, bins=runs if plane == 2 else [], min_rms=1.0, max_rms=30.0,
+                          noise=noise_mask if plane == 2 else None))
+    return np.stack(rows), chans, rc.astype(np.complex64)
+
+
+def test_rcrc_spectrum_matches_oracle():
+    _torch()
+    from wct_amd import noise as N
+    for nt, layers in ((2000, 1), (6000, 1), (6000, 2), (1999, 1)):
+        got = N.rcrc_spectrum(TAU, TICK, nt, layers).astype(np.complex128)
+        want = NF.rcrc_spectrum(TAU, TICK, nt, layers).astype(np.complex128)
+        assert np.abs(got - want).max() <= 2e-6 * np.abs(want).max(), (nt, layers)
+
+
+@pytest.mark.parametrize("nt", [2000, 1998])
+def test_one_channel_matches_oracle(nt):
+    torch = _torch()
+    from wct_amd import noise as N
+    rng = np.random.default_rng(nt)
+    frame, chans, rc = make_frame(rng, nt)
+    print(f"noise filter {len(chans)} channels x {nt}", flush=True)
+    out, flags = N.one_channel(frame, chans, rcrc=rc[None, :])
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    for i, c in enumerate(chans):
+        want, masks = NF.one_channel_noise(frame[i], c["plane"], rcrc=rc, noise=c["noise"],
+                                           min_rms=c["min_rms"], max_rms=c["max_rms"])
+        assert sorted(N.mask_names(int(flags[i]), c["plane"])) == sorted(masks), (i, flags[i], masks)
+        d = got[i].astype(np.float64) - want
+        med = float(np.median(d))
+        width = (float(want.max()) - float(want.min())) / nt + 1e-6
+        assert np.abs(d - med).max() <= 1e-2 and abs(med) <= 1.5 * width, (i, np.abs(d - med).max(), med, width)
+    # the designed cases came out as designed
+    kinds = [i % 8 for i in range(len(chans))]
+    assert all(flags[i] & 1 for i in range(len(chans)) if kinds[i] == 3)
+    assert all(flags[i] & 2 for i in range(len(chans)) if kinds[i] == 5)
+    assert any(flags[i] & 4 for i in range(len(chans)))
+
+
+def test_one_channel_full_length_properties():
+    """2560 channels x 6000 ticks (one APA): every output finite, bad
+    channels zero and noisy, plain noise channels at zero baseline."""
+    torch = _torch()
+    from wct_amd import noise as N
+    rng = np.random.default_rng(3)
+    nt, nch = 6000, 2560
+    frame = (900 + rng.normal(0, 3, (nch, nt))).astype(np.float32)
+    frame[17] = 0
+    chans = [dict(plane=0 if c < 800 else (1 if c < 1600 else 2), rcrc=0, min_rms=1.0, max_rms=30.0)
+             for c in range(nch)]
+    rc = N.rcrc_spectrum(TAU, TICK, nt)
+    out, flags = N.one_channel(frame, chans, rcrc=rc[None, :])
+    o = out.cpu().numpy()
+    assert np.isfinite(o).all()
+    assert not o[17].any() and flags[17] == 2
+    med = np.median(np.delete(o, 17, axis=0), axis=1)
+    assert np.abs(med).max() < 0.5
+    assert (np.delete(flags, 17) == 0).all()

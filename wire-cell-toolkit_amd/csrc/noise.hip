@@ -1,0 +1,548 @@
+// Per-channel noise filter of ProtoDUNE-SP (Protodune::OneChannelNoise::apply,
+// sigproc/src/Protodune.cxx:753-918) for every channel of a frame at once.
+//
+// Stages (one stream): k_nf_load (samples into complex rows), forward
+// row transforms (dft.hip), k_nf_spec (one workgroup per channel: the
+// partial-waveform test, the RC-undershoot division, the collection
+// plane's harmonic removal, the DC bin and the Hermitian mirror of
+// DftTools::inv_c2r), inverse row transforms, k_nf_time (one workgroup
+// per channel: baseline by binned median, the adaptive baseline of
+// partial channels, the signal-protected RMS test, the flag removal).
+//
+// The per-channel algorithms of sigproc/src/Microboone.cxx are sequential
+// loops in the reference; here each is a workgroup-parallel pass over the
+// channel's samples in LDS with the same arithmetic: float histograms for
+// the binned percentiles (three percentiles of CalcRMSWithFlags from one
+// histogram), direct 21-sample window sums for RawAdapativeBaselineAlg
+// and the nearest filled/flagged/edge bin for its interpolation, the
+// 16-sample dilation of SignalFilter.
+#include "common.hpp"
+#include "gmem.hpp"
+
+namespace wct {
+
+namespace {
+
+constexpr int kNfBlock = 256;
+constexpr int kNfWaves = kNfBlock / 64;
+constexpr float kNfFlag = 10000.0f;
+
+struct NfShared {
+    double d[kNfWaves];
+    float f[kNfWaves];
+    int i[kNfWaves];
+    int chunk[kNfBlock];
+    float res[4];
+    int flag;
+};
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ int wave_sum(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ float wave_min(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Block reductions in a fixed order (deterministic).
+__device__ double block_sum(double v, NfShared& sh)
+{
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) sh.d[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0;
+    for (int w = 0; w < kNfWaves; ++w) r += sh.d[w];
+    __syncthreads();
+    return r;
+}
+__device__ int block_sum(int v, NfShared& sh)
+{
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) sh.i[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int r = 0;
+    for (int w = 0; w < kNfWaves; ++w) r += sh.i[w];
+    __syncthreads();
+    return r;
+}
+__device__ float block_min(float v, NfShared& sh)
+{
+    v = wave_min(v);
+    if ((threadIdx.x & 63) == 0) sh.f[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float r = sh.f[0];
+    for (int w = 1; w < kNfWaves; ++w) r = fminf(r, sh.f[w]);
+    __syncthreads();
+    return r;
+}
+__device__ float block_max(float v, NfShared& sh)
+{
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) sh.f[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float r = sh.f[0];
+    for (int w = 1; w < kNfWaves; ++w) r = fmaxf(r, sh.f[w]);
+    __syncthreads();
+    return r;
+}
+
+// Waveform::mean_rms (util/src/Waveform.cxx:19-37): double sums.
+__device__ void mean_rms(const float* s, int n, double& mean, double& rms, NfShared& sh)
+{
+    double a = 0, b = 0;
+    for (int i = threadIdx.x; i < n; i += kNfBlock) {
+        const double x = s[i];
+        a += x;
+        b += x * x;
+    }
+    a = block_sum(a, sh);
+    b = block_sum(b, sh);
+    mean = a / n;
+    rms = sqrt((b - a * a / n) / n);
+}
+
+// Waveform::percentile_binned (Waveform.cxx:108-131) over the samples
+// of s that pass valid (the reference's vector of those samples), for np
+// probabilities from one histogram: m = number of such samples bins of
+// (max - min) / m between their min and max (float), sample bin
+// round((v - min) / width) clamped to [0, m), the lower edge of the bin
+// where the running count first exceeds int(m p).  hist holds m ints.
+// Returns the count m (0: no samples, out untouched).
+template <class Valid>
+__device__ int pct_binned(const float* s, int n, Valid valid, const float* p, int np, float* out, int* hist,
+                          NfShared& sh)
+{
+    int cnt = 0;
+    float mn = INFINITY, mx = -INFINITY;
+    for (int i = threadIdx.x; i < n; i += kNfBlock) {
+        const float v = s[i];
+        if (valid(v)) {
+            ++cnt;
+            mn = fminf(mn, v);
+            mx = fmaxf(mx, v);
+        }
+    }
+    const int m = block_sum(cnt, sh);
+    if (m == 0) return 0;
+    mn = block_min(mn, sh);
+    mx = block_max(mx, sh);
+    const float width = __fdiv_rn(mx - mn, (float)m);
+    for (int i = threadIdx.x; i < m; i += kNfBlock) hist[i] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kNfBlock) {
+        const float v = s[i];
+        if (valid(v)) {
+            const float r = __fdiv_rn(v - mn, width);
+            int b = (r != r) ? 0 : (int)roundf(r);   // 0/0 (all equal): bin 0, as x86's INT_MIN clamps
+            b = max(0, min(m - 1, b));
+            atomicAdd(&hist[b], 1);
+        }
+    }
+    __syncthreads();
+    const int C = (m + kNfBlock - 1) / kNfBlock;
+    const int lo = min(m, threadIdx.x * C), hi = min(m, lo + C);
+    int csum = 0;
+    for (int i = lo; i < hi; ++i) csum += hist[i];
+    sh.chunk[threadIdx.x] = csum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int t = 0; t < kNfBlock; ++t) {
+            const int c = sh.chunk[t];
+            sh.chunk[t] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+    const int before = sh.chunk[threadIdx.x];
+    for (int k = 0; k < np; ++k) {
+        const int imed = (int)((float)m * p[k]);
+        if (threadIdx.x == 0) sh.res[k] = mn + (mx - mn) * p[k];   // not reached for p < 1
+        __syncthreads();
+        if (before <= imed && before + csum > imed) {
+            int c = before;
+            for (int i = lo; i < hi; ++i) {
+                c += hist[i];
+                if (c > imed) {
+                    sh.res[k] = mn + (float)i * width;
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+        out[k] = sh.res[k];
+        __syncthreads();
+    }
+    return m;
+}
+
+struct BelowAdcMax {
+    __device__ bool operator()(float v) const { return v < 4096.0f; }
+};
+struct AnySample {
+    __device__ bool operator()(float) const { return true; }
+};
+
+// Microboone::CalcRMSWithFlags (Microboone.cxx:545-566).
+__device__ float rms_with_flags(const float* s, int n, int* hist, NfShared& sh)
+{
+    const float p[3] = {(float)(0.5 - 0.34), 0.5f, (float)(0.5 + 0.34)};
+    float q[3];
+    if (!pct_binned(s, n, BelowAdcMax(), p, 3, q, hist, sh)) return 0.f;
+    const double a = q[2] - q[1], b = q[1] - q[0];
+    return (float)sqrt((a * a + b * b) / 2.);
+}
+
+// Microboone::SignalFilter (:568-610): samples within [i - 8, i + 8) of a
+// sample beyond 4 RMS (and below 4096) get 20000 added, once.
+__device__ void signal_filter(float* s, int n, int* hist, unsigned char* fl, NfShared& sh)
+{
+    const float thr = (float)(4.0 * rms_with_flags(s, n, hist, sh));
+    for (int i = threadIdx.x; i < n; i += kNfBlock) {
+        const float v = s[i];
+        fl[i] = ((v > thr) || ((double)v < -1.0 * (double)thr)) && v < 4096.0f;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < n; j += kNfBlock) {
+        const float v = s[j];
+        if (v < 4096.0f) {
+            bool near = false;
+            for (int i = max(j - 7, 0); i <= min(j + 8, n - 1); ++i) near |= fl[i] != 0;
+            if (near) s[j] = (float)((double)v + 20000.0);
+        }
+    }
+    __syncthreads();
+}
+
+// Microboone::RawAdapativeBaselineAlg (:612-737): subtract the mean of the
+// samples below 4096 in [j - 10, j + 10]; where that window holds fewer
+// than 10 of them, interpolate (with the reference's weights) between the
+// nearest bins d <= j <= u that are filled, at an edge or flagged
+// (10000), or take the one that is filled.  base: n doubles; fl: n bytes.
+__device__ void raw_adaptive_baseline(float* s, int n, double* base, unsigned char* fl, NfShared& sh)
+{
+    int nflag = 0;
+    for (int i = threadIdx.x; i < n; i += kNfBlock) nflag += s[i] == kNfFlag;
+    if (block_sum(nflag, sh) == n) return;
+    for (int j = threadIdx.x; j < n; j += kNfBlock) {
+        double a = 0;
+        int c = 0;
+        for (int k = max(j - 10, 0); k <= min(j + 10, n - 1); ++k) {
+            const float v = s[k];
+            if (v < 4096.0f) {
+                a += v;
+                ++c;
+            }
+        }
+        base[j] = c ? a / c : 0.0;
+        fl[j] = (c >= 10 ? 1 : 0) | (s[j] == kNfFlag ? 2 : 0);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < n; j += kNfBlock) {
+        if (fl[j] & 2) continue;
+        double b = base[j];
+        if (!(fl[j] & 1)) {
+            int d = j;
+            while (!(fl[d] & 1) && d > 0 && !(fl[d] & 2)) --d;
+            int u = j;
+            while (!(fl[u] & 1) && u < n - 1 && !(fl[u] & 2)) ++u;
+            const bool dn = !(fl[d] & 1), up = !(fl[u] & 1);
+            if (!dn && !up)
+                b = ((j - d) * base[d] + (u - j) * base[u]) / ((double)u - d);
+            else if (dn && !up)
+                b = base[u];
+            else if (!dn && up)
+                b = base[d];
+            else
+                b = 0.0;
+        }
+        s[j] = (float)((double)s[j] - b);
+    }
+    __syncthreads();
+}
+
+// Microboone::RemoveFilterFlags (:739-757).
+__device__ void remove_filter_flags(float* s, int n)
+{
+    for (int i = threadIdx.x; i < n; i += kNfBlock) {
+        const float v = s[i];
+        if (v > 4096.0f) s[i] = v > kNfFlag ? (float)((double)v - 20000.0) : 0.f;
+    }
+    __syncthreads();
+}
+
+// LDS of the per-channel passes: n floats (the samples), 8 n bytes of
+// scratch (doubles of the adaptive baseline, or a float copy plus an int
+// histogram), n flag bytes.
+__host__ __device__ __forceinline__ size_t nf_lds_bytes(int n) { return (size_t)13 * n + 32; }
+__device__ __forceinline__ void nf_lds(unsigned char* base, int n, float*& s, unsigned char*& scratch,
+                                       unsigned char*& fl)
+{
+    s = reinterpret_cast<float*>(base);
+    scratch = base + ((size_t)4 * n + 15) / 16 * 16;
+    fl = scratch + (size_t)8 * n;
+}
+
+__global__ __launch_bounds__(kNfBlock) void k_nf_load(const float* __restrict__ in, int ld_in,
+                                                      float2* __restrict__ rows, int nch, int n)
+{
+    const long i = (long)blockIdx.x * kNfBlock + threadIdx.x;
+    if (i >= (long)nch * n) return;
+    const int c = (int)(i / n), t = (int)(i % n);
+    G(rows)[i] = make_float2(G(in)[(long)c * ld_in + t], 0.f);
+}
+
+// Spectrum stage of one channel (Protodune.cxx:767-876).
+__global__ __launch_bounds__(kNfBlock) void k_nf_spec(float2* __restrict__ rows, int n,
+                                                      const wct_nf_channel* __restrict__ chans,
+                                                      const float2* __restrict__ rcrc, const int* __restrict__ bins,
+                                                      int* __restrict__ flags)
+{
+    extern __shared__ __align__(16) unsigned char nf_smem[];
+    __shared__ NfShared sh;
+    const int c = blockIdx.x;
+    const wct_nf_channel ch = chans[c];
+    const auto row = G(rows + (long)c * n);
+    // Diagnostics::Partial (sigproc/src/Diagnostics.cxx:15-31), nfreqs 4,
+    // maxpower 6000
+    if (threadIdx.x == 0) {
+        const float2 z1 = row[1];
+        const double mag0 = hypotf(z1.x, z1.y);
+        double sum = mag0;
+        int partial = 1;
+        for (int ind = 1; ind <= 4 && ind < n; ++ind) {
+            const float2 z = row[ind + 1];
+            const double magi = hypotf(z.x, z.y);
+            if (mag0 <= magi) {
+                partial = 0;
+                break;
+            }
+            sum += magi;
+        }
+        sh.flag = partial && sum / 5 > 6000.0;
+    }
+    __syncthreads();
+    const bool partial = sh.flag;
+    // RC undershoot: divide by the channel's rcrc spectrum (complex
+    // division, rounded from double)
+    if (!partial && ch.rcrc >= 0) {
+        const auto R = G(rcrc + (long)ch.rcrc * n);
+        for (int k = threadIdx.x; k < n; k += kNfBlock) {
+            const float2 a = row[k], b = R[k];
+            const double den = (double)b.x * b.x + (double)b.y * b.y;
+            const double re = ((double)a.x * b.x + (double)a.y * b.y) / den;
+            const double im = ((double)a.y * b.x - (double)a.x * b.y) / den;
+            row[k] = make_float2((float)re, (float)im);
+        }
+    }
+    __syncthreads();
+    int nharm = 0;
+    if (ch.plane == 2 && ch.bins_end > ch.bins_begin) {
+        float* mag;
+        unsigned char *scratch, *fl;
+        nf_lds(nf_smem, n, mag, scratch, fl);
+        for (int k = threadIdx.x; k < n; k += kNfBlock) {
+            const float2 z = row[k];
+            mag[k] = k == 0 ? 0.f : hypotf(z.x, z.y);
+        }
+        __syncthreads();
+        raw_adaptive_baseline(mag, n, reinterpret_cast<double*>(scratch), fl, sh);
+        // the flagged runs, five rounds (:808-846); sequential, as the
+        // zeroed mirror bins feed later runs' statistics
+        if (threadIdx.x == 0) {
+            for (int iter = 0; iter < 5; ++iter) {
+                for (int r = ch.bins_begin; r < ch.bins_end; ++r) {
+                    const int lo = G(bins)[2 * r], iend = G(bins)[2 * r + 1] + 1;
+                    double a = 0, b = 0;
+                    for (int j = lo; j < iend; ++j) {
+                        a += mag[j];
+                        b += (double)mag[j] * mag[j];
+                    }
+                    const int m = iend - lo;
+                    double mean = m ? a / m : 0, rms = m > 1 ? sqrt((b - a * a / m) / m) : 0;
+                    if (m == 1) mean = mag[lo];
+                    const double cut = lo > 1050 ? mean + 3 * rms : mean + 2.7 * rms;
+                    for (int j = lo; j < iend; ++j) {
+                        if (mag[j] > cut) {
+                            const int jm = n + 1 - j;
+                            row[j] = make_float2(0.f, 0.f);
+                            mag[j] = 0;
+                            if (jm >= 0 && jm < n) {
+                                row[jm] = make_float2(0.f, 0.f);
+                                mag[jm] = 0;
+                            }
+                            ++nharm;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // DC removed (:880), then inv_c2r's Hermitian mirror of bins 0..n/2
+    // (aux/src/DftTools.cxx:39-50, util/inc/WireCellUtil/Spectrum.h:56-89)
+    if (threadIdx.x == 0) row[0] = make_float2(0.f, 0.f);
+    for (int k = n / 2 + 1 + threadIdx.x; k < n; k += kNfBlock) {
+        const float2 z = row[n - k];
+        row[k] = make_float2(z.x, -z.y);
+    }
+    if (threadIdx.x == 0 && n % 2 == 0) row[n / 2] = make_float2(row[n / 2].x, 0.f);
+    if (threadIdx.x == 0) flags[c] = (partial ? WCT_NF_PARTIAL : 0) | (nharm > 5 ? WCT_NF_HARMONIC : 0);
+}
+
+// Time stage of one channel (Protodune.cxx:877-915).  rows hold the
+// inverse transforms (real parts used).
+__global__ __launch_bounds__(kNfBlock) void k_nf_time(const float2* __restrict__ rows, int n, float* __restrict__ out,
+                                                      int ld_out, const wct_nf_channel* __restrict__ chans,
+                                                      int* __restrict__ flags)
+{
+    extern __shared__ __align__(16) unsigned char nf_smem[];
+    __shared__ NfShared sh;
+    const int c = blockIdx.x;
+    const wct_nf_channel ch = chans[c];
+    float* s;
+    unsigned char *scratch, *fl;
+    nf_lds(nf_smem, n, s, scratch, fl);
+    const auto row = G(rows + (long)c * n);
+    for (int i = threadIdx.x; i < n; i += kNfBlock) s[i] = row[i].x;
+    __syncthreads();
+    int flag = flags[c];
+    // baseline: binned median of the samples with those beyond 6 RMS set
+    // to the mean (:880-890)
+    double mean, rms;
+    mean_rms(s, n, mean, rms, sh);
+    float* temp = reinterpret_cast<float*>(scratch);
+    int* hist = reinterpret_cast<int*>(scratch + (size_t)4 * n);
+    for (int i = threadIdx.x; i < n; i += kNfBlock) {
+        const float v = s[i];
+        temp[i] = fabs((double)v - mean) > 6 * rms ? (float)mean : v;
+    }
+    __syncthreads();
+    const float half = 0.5f;
+    float baseline = 0.f;
+    pct_binned(temp, n, AnySample(), &half, 1, &baseline, hist, sh);
+    const float shift = baseline * (-1);
+    for (int i = threadIdx.x; i < n; i += kNfBlock) s[i] = s[i] + shift;
+    __syncthreads();
+    int* hist2 = reinterpret_cast<int*>(scratch);
+    if (flag & WCT_NF_PARTIAL) {
+        signal_filter(s, n, hist2, fl, sh);
+        raw_adaptive_baseline(s, n, reinterpret_cast<double*>(scratch), fl, sh);
+        remove_filter_flags(s, n);
+    }
+    signal_filter(s, n, hist2, fl, sh);
+    // Microboone::NoisyFilterAlg (:464-533)
+    const double r = rms_with_flags(s, n, hist2, sh);
+    const bool noisy = r > (double)ch.max_rms || r < (double)ch.min_rms;
+    if (noisy)
+        for (int i = threadIdx.x; i < n; i += kNfBlock) s[i] = kNfFlag;
+    __syncthreads();
+    remove_filter_flags(s, n);
+    const auto dst = G(out + (long)c * ld_out);
+    for (int i = threadIdx.x; i < n; i += kNfBlock) dst[i] = s[i];
+    if (threadIdx.x == 0) flags[c] = flag | (noisy ? WCT_NF_NOISY : 0);
+}
+
+// SimpleRC samples at t = i tick, rounded to float as the reference's
+// realseq_t holds them (util/src/Response.cxx:517-536, :267-276), as a
+// double complex row.
+__global__ __launch_bounds__(kNfBlock) void k_nf_rc_samples(double2* __restrict__ row, int n, double width,
+                                                            double tick)
+{
+    const int i = blockIdx.x * kNfBlock + threadIdx.x;
+    if (i >= n) return;
+    const double t = i * ((n * tick) / n);
+    double v = 0;
+    if (width > 0) v += -tick / width * exp(-t / width);
+    if (t < tick) v += 1.0;
+    G(row)[i] = make_double2((double)(float)v, 0.0);
+}
+
+// The float spectrum raised to the power layers (OmniChannelNoiseDB.cxx:187-199).
+__global__ __launch_bounds__(kNfBlock) void k_nf_rc_power(const double2* __restrict__ spec, float2* __restrict__ out,
+                                                          int n, int layers)
+{
+    const int i = blockIdx.x * kNfBlock + threadIdx.x;
+    if (i >= n) return;
+    const double2 z = G(spec)[i];
+    const float2 s1 = make_float2((float)z.x, (float)z.y);
+    float2 acc = s1;
+    for (int l = 1; l < layers; ++l) acc = make_float2(acc.x * s1.x - acc.y * s1.y, acc.x * s1.y + acc.y * s1.x);
+    G(out)[i] = acc;
+}
+
+}  // namespace
+
+}  // namespace wct
+
+using namespace wct;
+
+extern "C" {
+
+size_t wct_nf_workspace_size(int nch, int nt)
+{
+    if (nch <= 0 || nt <= 0) return 0;
+    return ((size_t)nch * nt * sizeof(float2) + 255) / 256 * 256;
+}
+
+int wct_nf_rcrc_spectrum(double width, double tick, int nt, int layers, float* out, void* stream)
+{
+    WCT_ENTRY();
+    WCT_REQUIRE(out && nt >= 2 && tick > 0 && layers >= 1, WCT_ERR_ARG, "nf rcrc spectrum: bad arguments");
+    hipStream_t s = as_stream(stream);
+    DevTemp<double2> row(s);
+    WCT_TRY(row.alloc(nt));
+    const unsigned nb = (nt + kNfBlock - 1) / kNfBlock;
+    hipLaunchKernelGGL(k_nf_rc_samples, dim3(nb), dim3(kNfBlock), 0, s, row.p, nt, width, tick);
+    WCT_HIP_CHECK(hipGetLastError());
+    WCT_TRY(launch_rows<double>(row.p, row.p, nt, 1, nt, -1, 1.0, s));
+    hipLaunchKernelGGL(k_nf_rc_power, dim3(nb), dim3(kNfBlock), 0, s, row.p, reinterpret_cast<float2*>(out), nt,
+                       layers);
+    WCT_HIP_CHECK(hipGetLastError());
+    return WCT_OK;
+}
+
+int wct_nf_one_channel(const float* in, int ld_in, float* out, int ld_out, int nch, int nt,
+                       const wct_nf_channel* chans, const float* rcrc, const int* bins, int* flags,
+                       void* workspace, size_t ws_bytes, void* stream)
+{
+    WCT_ENTRY();
+    WCT_REQUIRE(in && out && chans && flags && nch >= 1 && nt >= 32 && ld_in >= nt && ld_out >= nt, WCT_ERR_ARG,
+                "nf: bad arguments (nch %d, nt %d, ld_in %d, ld_out %d)", nch, nt, ld_in, ld_out);
+    WCT_REQUIRE(workspace && ws_bytes >= wct_nf_workspace_size(nch, nt), WCT_ERR_ARG,
+                "nf: workspace of %zu bytes, %zu needed", ws_bytes, wct_nf_workspace_size(nch, nt));
+    const size_t lds = nf_lds_bytes(nt);
+    WCT_REQUIRE((long)lds + (long)sizeof(NfShared) <= lds_limit_bytes(), WCT_ERR_UNSUPPORTED,
+                "nf: %d ticks need %zu bytes of LDS per channel", nt, lds);
+    hipStream_t s = as_stream(stream);
+    float2* rows = reinterpret_cast<float2*>(workspace);
+    const long total = (long)nch * nt;
+    hipLaunchKernelGGL(k_nf_load, dim3((unsigned)((total + kNfBlock - 1) / kNfBlock)), dim3(kNfBlock), 0, s, in,
+                       ld_in, rows, nch, nt);
+    WCT_HIP_CHECK(hipGetLastError());
+    WCT_TRY(launch_rows<float>(rows, rows, nt, nch, nt, -1, 1.0f, s));
+    WCT_TRY(allow_lds(k_nf_spec, lds));
+    hipLaunchKernelGGL(k_nf_spec, dim3(nch), dim3(kNfBlock), lds, s, rows, nt, chans,
+                       reinterpret_cast<const float2*>(rcrc), bins, flags);
+    WCT_HIP_CHECK(hipGetLastError());
+    WCT_TRY(launch_rows<float>(rows, rows, nt, nch, nt, +1, 1.0f / nt, s));
+    WCT_TRY(allow_lds(k_nf_time, lds));
+    hipLaunchKernelGGL(k_nf_time, dim3(nch), dim3(kNfBlock), lds, s, rows, nt, out, ld_out, chans, flags);
+    WCT_HIP_CHECK(hipGetLastError());
+    return WCT_OK;
+}
+
+}  // extern "C"

@@ -54,6 +54,14 @@ class ImpactFrame(C.Structure):
                                         "frame_start_tick", "frame_end_tick", "nw", "nt")]
 
 
+class NfChannel(C.Structure):
+    _fields_ = [("plane", C.c_int), ("rcrc", C.c_int), ("bins_begin", C.c_int), ("bins_end", C.c_int),
+                ("min_rms", C.c_float), ("max_rms", C.c_float)]
+
+
+NF_PARTIAL, NF_NOISY, NF_HARMONIC = 1, 2, 4
+
+
 # name -> (restype, argtypes)
 _V, _I, _D, _F, _Z = C.c_void_p, C.c_int, C.c_double, C.c_float, C.c_size_t
 _SIGS = {
@@ -120,6 +128,9 @@ _SIGS = {
     "wct_impact_destroy": (_I, [_V]),
     "wct_impact_run": (_I, [_V, C.POINTER(ImpactFrame), C.POINTER(C.c_long), _V, _V, _V, _V, _V]),
     "wct_impact_waveforms": (_I, [_V, C.POINTER(ImpactFrame), _V, _I, _V, _I, _I, _V, _V]),
+    "wct_nf_rcrc_spectrum": (_I, [_D, _D, _I, _I, _V, _V]),
+    "wct_nf_workspace_size": (_Z, [_I, _I]),
+    "wct_nf_one_channel": (_I, [_V, _I, _V, _I, _I, _I, _V, _V, _V, _V, _V, _Z, _V]),
 }
 
 # Every symbol include/wct_hip.h declares (checked by the CPU tests).
