@@ -1,0 +1,663 @@
+// The noise filter components (see NoiseFilter.h).  OmnibusNoiseFilter's
+// control flow and masks are on the host, as in the reference; the
+// per-channel filter of ProtoDUNE runs for all channels of a frame in one
+// device batch (wct_nf_one_channel).
+#include "NoiseFilter.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <unordered_map>
+
+#include "../../include/wct_hip.h"
+#include "Buffers.h"
+#include "Factory.h"
+#include "Frame.h"
+
+namespace WireCell {
+
+namespace Waveform {
+
+BinRangeList merge(const BinRangeList& brl)
+{
+    BinRangeList tmp(brl.begin(), brl.end()), out;
+    if (tmp.empty()) return out;
+    std::sort(tmp.begin(), tmp.end());
+    out.push_back(tmp[0]);
+    for (size_t i = 1; i < tmp.size(); ++i) {
+        if (out.back().second >= tmp[i].first) {
+            out.back().second = tmp[i].second;
+            continue;
+        }
+        out.push_back(tmp[i]);
+    }
+    return out;
+}
+
+BinRangeList merge(const BinRangeList& a, const BinRangeList& b)
+{
+    BinRangeList all(a);
+    all.insert(all.end(), b.begin(), b.end());
+    return merge(all);
+}
+
+ChannelMasks merge(const ChannelMasks& a, const ChannelMasks& b)
+{
+    ChannelMasks out = a;
+    for (const auto& it : b) out[it.first] = merge(out[it.first], it.second);
+    return out;
+}
+
+void merge(ChannelMaskMap& one, const ChannelMaskMap& two, const std::map<std::string, std::string>& name_map)
+{
+    for (const auto& it : two) {
+        auto nm = name_map.find(it.first);
+        const std::string mapped = nm == name_map.end() ? it.first : nm->second;
+        auto dst = one.find(mapped);
+        if (dst != one.end())
+            dst->second = merge(dst->second, it.second);
+        else
+            one[mapped] = it.second;
+    }
+}
+
+}  // namespace Waveform
+
+namespace Hip {
+
+namespace {
+
+constexpr double kMillisecond = 1e6;   // ns
+
+void ncheck(int rc, const std::string& what)
+{
+    if (rc != WCT_OK) throw std::runtime_error(what + ": " + wct_last_error());
+}
+
+// Plane index of a channel from the anode's plane lists (-1: not on it).
+struct PlaneLookup {
+    std::unordered_map<int, int> plane;
+    void build(const IAnodeChannels& anode)
+    {
+        plane.clear();
+        for (int p = 0; p < 3; ++p)
+            for (int ch : anode.plane_channels(p)) plane[ch] = p;
+    }
+    int operator()(int ch) const
+    {
+        auto it = plane.find(ch);
+        return it == plane.end() ? -1 : it->second;
+    }
+};
+
+}  // namespace
+
+// ---- OmniChannelNoiseDB (sigproc/src/OmniChannelNoiseDB.cxx) ----------
+class OmniChannelNoiseDB : public IChannelNoiseDatabase, public IConfigurable {
+  public:
+    struct ChannelInfo {   // defaults of :30-50
+        double nominal_baseline{0.0}, gain_correction{1.0}, response_offset{0.0};
+        double min_rms_cut{0.5}, max_rms_cut{10.0};
+        int pad_window_front{0}, pad_window_back{0};
+        float decon_limit{0.02f}, decon_lf_cutoff{0.08f}, adc_limit{0.0f}, decon_limit1{0.08f};
+        float protection_factor{5.0f}, min_adc_limit{50}, roi_min_max_ratio{0.8f};
+        std::shared_ptr<filter_t> rcrc, noise;
+    };
+
+    Configuration default_configuration() const override
+    {
+        Configuration cfg;
+        cfg["tick"] = m_tick;
+        cfg["nsamples"] = m_nsamples;
+        cfg["anode"] = m_anode_tn;
+        cfg["field_response"] = "FieldResponse";
+        cfg["dft"] = "HipDFT";
+        cfg["groups"] = Configuration::makeArray();
+        cfg["bad"] = Configuration::makeArray();
+        cfg["channel_info"] = Configuration::makeArray();
+        return cfg;
+    }
+
+    void configure(const Configuration& cfg) override
+    {
+        m_tick = get(cfg, "tick", m_tick);
+        m_nsamples = get(cfg, "nsamples", m_nsamples);
+        m_anode_tn = get<std::string>(cfg, "anode", m_anode_tn);
+        m_anode = Factory::find_tn<IAnodeChannels>(m_anode_tn);
+        m_db.clear();
+        for (int p = 0; p < 3; ++p)
+            for (int ch : m_anode->plane_channels(p)) m_db[ch] = ChannelInfo();
+        m_groups.clear();
+        for (const auto& jg : cfg["groups"]) {
+            channel_group_t g;
+            for (const auto& jch : jg) g.push_back(jch.asInt());
+            m_groups.push_back(g);
+        }
+        m_bad.clear();
+        for (const auto& jch : cfg["bad"]) m_bad.push_back(jch.asInt());
+        std::sort(m_bad.begin(), m_bad.end());
+        m_rc_layers = 2;
+        m_rcrc_cache.clear();
+        for (const auto& jci : cfg["channel_info"]) update_channels(jci);
+    }
+
+    double sample_time() const override { return m_tick; }
+    double nominal_baseline(int ch) const override { return dbget(ch).nominal_baseline; }
+    double gain_correction(int ch) const override { return dbget(ch).gain_correction; }
+    double min_rms_cut(int ch) const override { return dbget(ch).min_rms_cut; }
+    double max_rms_cut(int ch) const override { return dbget(ch).max_rms_cut; }
+    const filter_t& rcrc(int ch) const override
+    {
+        static const filter_t dummy;
+        const auto& f = dbget(ch).rcrc;
+        return f ? *f : dummy;
+    }
+    const filter_t& noise(int ch) const override
+    {
+        static const filter_t dummy;
+        const auto& f = dbget(ch).noise;
+        return f ? *f : dummy;
+    }
+    const std::vector<channel_group_t> coherent_channels() const override { return m_groups; }
+    std::vector<int> bad_channels() const override { return m_bad; }
+
+  private:
+    const ChannelInfo& dbget(int ch) const
+    {
+        auto it = m_db.find(ch);
+        if (it == m_db.end()) throw std::out_of_range("OmniChannelNoiseDB: no db info for channel " + std::to_string(ch));
+        return it->second;
+    }
+    ChannelInfo& get_ci(int ch)
+    {
+        auto it = m_db.find(ch);
+        if (it == m_db.end()) throw std::out_of_range("OmniChannelNoiseDB: no db info for channel " + std::to_string(ch));
+        return it->second;
+    }
+
+    // parse_channels (:85-138): a number, a list, {first, last} or {wpid}
+    // (the low three bits of the plane id are the layer: 1 U, 2 V, 4 W).
+    std::vector<int> parse_channels(const Configuration& j) const
+    {
+        std::vector<int> ret;
+        if (j.isNumeric()) return {j.asInt()};
+        if (j.isArray()) {
+            for (const auto& c : j) ret.push_back(c.asInt());
+            return ret;
+        }
+        if (j.isMember("first") && j.isMember("last")) {
+            for (int c = j["first"].asInt(); c <= j["last"].asInt(); ++c) ret.push_back(c);
+            return ret;
+        }
+        if (j.isMember("wpid")) {
+            const int layer = j["wpid"].asInt() & 7;
+            const int plane = layer == 1 ? 0 : layer == 2 ? 1 : layer == 4 ? 2 : -1;
+            if (plane >= 0) ret = m_anode->plane_channels(plane);
+        }
+        return ret;
+    }
+
+    // parse_rcrc (:171-208): cached by round(1000 rcrc / ms) only, as the
+    // reference's cache is (a later rc_layers for the same value reuses
+    // the first spectrum).
+    std::shared_ptr<filter_t> parse_rcrc(double val, int layers)
+    {
+        const int key = (int)std::round(1000 * val / kMillisecond);
+        auto it = m_rcrc_cache.find(key);
+        if (it != m_rcrc_cache.end()) return it->second;
+        auto spec = std::make_shared<filter_t>(m_nsamples);
+        DevBuf d;
+        d.ensure(sizeof(float) * 2 * m_nsamples);
+        void* s = nullptr;
+        ncheck(wct_stream_create(&s), "OmniChannelNoiseDB stream");
+        int rc = wct_nf_rcrc_spectrum(val, m_tick, m_nsamples, layers, (float*)d.p, s);
+        if (rc == WCT_OK) rc = wct_memcpy_dtoh(spec->data(), d.p, sizeof(float) * 2 * m_nsamples, s);
+        wct_stream_destroy(s);
+        ncheck(rc, "OmniChannelNoiseDB rcrc spectrum");
+        m_rcrc_cache[key] = spec;
+        return spec;
+    }
+
+    // parse_freqmasks (:152-169)
+    std::shared_ptr<filter_t> parse_freqmasks(const Configuration& j) const
+    {
+        auto spec = std::make_shared<filter_t>(m_nsamples, std::complex<float>(1, 0));
+        for (const auto& one : j) {
+            const double value = one["value"].asDouble();
+            const int lo = std::max(one["lobin"].asInt(), 0);
+            const int hi = std::min(one["hibin"].asInt(), m_nsamples - 1);
+            for (int i = lo; i <= hi; ++i) (*spec)[i] = (float)value;
+        }
+        return spec;
+    }
+
+    // update_channels (:376-573): the keys the ProtoDUNE filter and the
+    // accessors above read; response/reconfig (coherent and MicroBooNE
+    // filters) are not kept.
+    void update_channels(const Configuration& cfg)
+    {
+        auto chans = parse_channels(cfg["channels"]);
+        auto setd = [&](const char* key, auto member) {
+            if (!cfg.isMember(key)) return;
+            const double v = cfg[key].asDouble();
+            for (int ch : chans) get_ci(ch).*member = v;
+        };
+        setd("nominal_baseline", &ChannelInfo::nominal_baseline);
+        setd("gain_correction", &ChannelInfo::gain_correction);
+        setd("response_offset", &ChannelInfo::response_offset);
+        setd("min_rms_cut", &ChannelInfo::min_rms_cut);
+        setd("max_rms_cut", &ChannelInfo::max_rms_cut);
+        auto seti = [&](const char* key, int ChannelInfo::*member) {
+            if (!cfg.isMember(key)) return;
+            const int v = cfg[key].asInt();
+            for (int ch : chans) get_ci(ch).*member = v;
+        };
+        seti("pad_window_front", &ChannelInfo::pad_window_front);
+        seti("pad_window_back", &ChannelInfo::pad_window_back);
+        auto setf = [&](const char* key, float ChannelInfo::*member) {
+            if (!cfg.isMember(key)) return;
+            const float v = (float)cfg[key].asDouble();
+            for (int ch : chans) get_ci(ch).*member = v;
+        };
+        setf("decon_limit", &ChannelInfo::decon_limit);
+        setf("decon_lf_cutoff", &ChannelInfo::decon_lf_cutoff);
+        setf("decon_limit1", &ChannelInfo::decon_limit1);
+        setf("adc_limit", &ChannelInfo::adc_limit);
+        setf("protection_factor", &ChannelInfo::protection_factor);
+        setf("min_adc_limit", &ChannelInfo::min_adc_limit);
+        setf("roi_min_max_ratio", &ChannelInfo::roi_min_max_ratio);
+        if (!cfg["rcrc"].isNull()) {
+            if (cfg.isMember("rc_layers")) m_rc_layers = cfg["rc_layers"].asInt();
+            auto val = parse_rcrc(cfg["rcrc"].asDouble(), m_rc_layers);
+            for (int ch : chans) get_ci(ch).rcrc = val;
+        }
+        if (!cfg["freqmasks"].isNull()) {
+            auto val = parse_freqmasks(cfg["freqmasks"]);
+            for (int ch : chans) get_ci(ch).noise = val;
+        }
+    }
+
+    double m_tick{0.5 * 1000.0};
+    int m_nsamples{9600};
+    int m_rc_layers{2};
+    std::string m_anode_tn{"AnodeChannels"};
+    std::shared_ptr<IAnodeChannels> m_anode;
+    std::map<int, ChannelInfo> m_db;
+    std::vector<channel_group_t> m_groups;
+    std::vector<int> m_bad;
+    std::map<int, std::shared_ptr<filter_t>> m_rcrc_cache;
+};
+
+// ---- pdOneChannelNoise (sigproc/src/Protodune.cxx:712-923) -------------
+class OneChannelNoise : public IChannelFilter, public IBatchChannelFilter, public IConfigurable {
+  public:
+    typedef IChannelNoiseDatabase::filter_t filter_t;
+    ~OneChannelNoise() override
+    {
+        if (m_stream) wct_stream_destroy(m_stream);
+    }
+    Configuration default_configuration() const override
+    {
+        Configuration cfg;
+        cfg["anode"] = m_anode_tn;
+        cfg["noisedb"] = m_noisedb_tn;
+        cfg["dft"] = "HipDFT";
+        return cfg;
+    }
+    void configure(const Configuration& cfg) override
+    {
+        m_anode_tn = get<std::string>(cfg, "anode", m_anode_tn);
+        m_anode = Factory::find_tn<IAnodeChannels>(m_anode_tn);
+        m_planes.build(*m_anode);
+        m_noisedb_tn = get<std::string>(cfg, "noisedb", m_noisedb_tn);
+        m_noisedb = Factory::find_tn<IChannelNoiseDatabase>(m_noisedb_tn);
+        m_resmp.clear();
+        for (const auto& jone : cfg["resmp"]) {
+            const int smpin = jone["sample_from"].asInt();
+            for (const auto& jch : jone["channels"]) m_resmp[jch.asInt()] = smpin;
+        }
+    }
+
+    Waveform::ChannelMaskMap apply(int ch, signal_t& signal) const override
+    {
+        std::vector<float> rows(signal.begin(), signal.end());
+        std::vector<Waveform::ChannelMaskMap> masks;
+        apply_batch({ch}, rows, (int)signal.size(), masks);
+        signal.assign(rows.begin(), rows.end());
+        return masks[0];
+    }
+    Waveform::ChannelMaskMap apply(channel_signals_t&) const override { return Waveform::ChannelMaskMap(); }
+
+    void apply_batch(const std::vector<int>& channels, std::vector<float>& rows, int nt,
+                     std::vector<Waveform::ChannelMaskMap>& masks) const override
+    {
+        std::lock_guard<std::mutex> lock(m_mutex);
+        const int nch = (int)channels.size();
+        masks.assign(nch, Waveform::ChannelMaskMap());
+        if (nch == 0) return;
+        if (!m_stream) ncheck(wct_stream_create(&m_stream), "pdOneChannelNoise stream");
+        resample(channels, rows, nt);
+
+        // per-channel entries, rcrc rows and flagged runs, shared spectra once
+        std::vector<wct_nf_channel> ent(nch);
+        std::vector<const filter_t*> rc_rows;
+        std::map<const filter_t*, int> rc_id;
+        std::vector<int> bins;
+        std::map<const filter_t*, std::pair<int, int>> bin_id;
+        for (int i = 0; i < nch; ++i) {
+            const int ch = channels[i];
+            wct_nf_channel& e = ent[i];
+            e.plane = m_planes(ch);
+            e.min_rms = (float)m_noisedb->min_rms_cut(ch);
+            e.max_rms = (float)m_noisedb->max_rms_cut(ch);
+            const filter_t& rc = m_noisedb->rcrc(ch);
+            e.rcrc = -1;
+            if ((int)rc.size() == nt) {
+                auto it = rc_id.find(&rc);
+                if (it == rc_id.end()) {
+                    it = rc_id.emplace(&rc, (int)rc_rows.size()).first;
+                    rc_rows.push_back(&rc);
+                }
+                e.rcrc = it->second;
+            }
+            e.bins_begin = e.bins_end = 0;
+            if (e.plane == 2) {
+                const filter_t& nz = m_noisedb->noise(ch);
+                auto it = bin_id.find(&nz);
+                if (it == bin_id.end()) {
+                    const int begin = (int)bins.size() / 2;
+                    for (int k = 0; k < (int)nz.size(); ++k) {   // :791-806
+                        if (nz[k].real() < 0.5f) {
+                            if ((int)bins.size() / 2 > begin && bins.back() == k - 1)
+                                bins.back() = k;
+                            else {
+                                bins.push_back(k);
+                                bins.push_back(k);
+                            }
+                        }
+                    }
+                    it = bin_id.emplace(&nz, std::make_pair(begin, (int)bins.size() / 2)).first;
+                }
+                e.bins_begin = it->second.first;
+                e.bins_end = it->second.second;
+            }
+        }
+        const size_t row_bytes = sizeof(float) * (size_t)nch * nt;
+        m_d_in.ensure(row_bytes);
+        m_d_out.ensure(row_bytes);
+        m_d_chans.ensure(sizeof(wct_nf_channel) * nch);
+        m_d_flags.ensure(sizeof(int) * nch);
+        m_d_rcrc.ensure(sizeof(float) * 2 * std::max<size_t>(1, rc_rows.size()) * nt);
+        m_d_bins.ensure(sizeof(int) * std::max<size_t>(2, bins.size()));
+        const size_t ws = wct_nf_workspace_size(nch, nt);
+        m_d_ws.ensure(ws);
+        ncheck(wct_memcpy_htod_async(m_d_in.p, rows.data(), row_bytes, m_stream), "pdOneChannelNoise upload");
+        ncheck(wct_memcpy_htod_async(m_d_chans.p, ent.data(), sizeof(wct_nf_channel) * nch, m_stream),
+               "pdOneChannelNoise upload");
+        for (size_t r = 0; r < rc_rows.size(); ++r)
+            ncheck(wct_memcpy_htod_async((char*)m_d_rcrc.p + sizeof(float) * 2 * nt * r, rc_rows[r]->data(),
+                                         sizeof(float) * 2 * nt, m_stream),
+                   "pdOneChannelNoise upload");
+        if (!bins.empty())
+            ncheck(wct_memcpy_htod_async(m_d_bins.p, bins.data(), sizeof(int) * bins.size(), m_stream),
+                   "pdOneChannelNoise upload");
+        ncheck(wct_nf_one_channel((const float*)m_d_in.p, nt, (float*)m_d_out.p, nt, nch, nt,
+                                  (const wct_nf_channel*)m_d_chans.p, (const float*)m_d_rcrc.p,
+                                  (const int*)m_d_bins.p, (int*)m_d_flags.p, m_d_ws.p, ws, m_stream),
+               "pdOneChannelNoise");
+        std::vector<int> flags(nch);
+        ncheck(wct_memcpy_dtoh_async(rows.data(), m_d_out.p, row_bytes, m_stream), "pdOneChannelNoise download");
+        ncheck(wct_memcpy_dtoh_async(flags.data(), m_d_flags.p, sizeof(int) * nch, m_stream),
+               "pdOneChannelNoise download");
+        ncheck(wct_stream_sync(m_stream), "pdOneChannelNoise sync");
+        for (int i = 0; i < nch; ++i) {
+            const Waveform::BinRange all(0, nt);
+            if (flags[i] & WCT_NF_HARMONIC) masks[i]["harmonic"][channels[i]].push_back(all);
+            if ((flags[i] & WCT_NF_PARTIAL) && ent[i].plane != 2) masks[i]["lf_noisy"][channels[i]].push_back(all);
+            if (flags[i] & WCT_NF_NOISY) masks[i]["noisy"][channels[i]].push_back(all);
+        }
+    }
+
+  private:
+    // The FEMB clock correction (:760-768, FftScaling :532-552): the
+    // channel's first sample_from samples resampled to nt, through the
+    // device transforms.
+    void resample(const std::vector<int>& channels, std::vector<float>& rows, int nt) const
+    {
+        std::map<int, std::vector<int>> by_len;
+        for (int i = 0; i < (int)channels.size(); ++i) {
+            auto it = m_resmp.find(channels[i]);
+            if (it != m_resmp.end()) by_len[it->second].push_back(i);
+        }
+        for (const auto& bl : by_len) {
+            const int nin = bl.first, k = (int)bl.second.size();
+            if (nin <= 0) continue;
+            std::vector<std::complex<float>> buf((size_t)k * std::max(nin, nt));
+            for (int r = 0; r < k; ++r) {
+                const float* src = rows.data() + (size_t)bl.second[r] * nt;
+                for (int t = 0; t < nin; ++t) buf[(size_t)r * nin + t] = t < nt ? src[t] : 0.f;
+            }
+            DevBuf d;
+            const size_t bytes = sizeof(std::complex<float>) * buf.size();
+            d.ensure(bytes);
+            ncheck(wct_memcpy_htod_async(d.p, buf.data(), sizeof(std::complex<float>) * k * nin, m_stream),
+                   "resample upload");
+            ncheck(wct_dft_fwd1b((const float*)d.p, (float*)d.p, k, nin, 1, m_stream), "resample fwd");
+            std::vector<std::complex<float>> spec((size_t)k * nin);
+            ncheck(wct_memcpy_dtoh_async(spec.data(), d.p, sizeof(std::complex<float>) * k * nin, m_stream),
+                   "resample download");
+            ncheck(wct_stream_sync(m_stream), "resample sync");
+            const int mid = nin % 2 == 0 ? nin / 2 : (nin + 1) / 2;
+            for (int r = 0; r < k; ++r) {
+                std::vector<std::complex<float>> tran(nt, 0.f);
+                const std::complex<float>* s = spec.data() + (size_t)r * nin;
+                // resize to nt, then rotate [mid, nin) to the end
+                std::vector<std::complex<float>> padded(s, s + std::min(nin, nt));
+                padded.resize(nt, 0.f);
+                std::rotate(padded.begin() + std::min(mid, nt), padded.begin() + std::min(nin, nt), padded.end());
+                // inv_c2r's Hermitian mirror (Spectrum.h:56-89)
+                padded[0] = padded[0].real();
+                const int half = nt / 2;
+                if (nt % 2 == 0) {
+                    padded[half] = padded[half].real();
+                    for (int j = 1; j < half; ++j) padded[nt - j] = std::conj(padded[j]);
+                } else {
+                    for (int j = 1; j <= half; ++j) padded[nt - j] = std::conj(padded[j]);
+                }
+                std::copy(padded.begin(), padded.end(), buf.begin() + (size_t)r * nt);
+            }
+            ncheck(wct_memcpy_htod_async(d.p, buf.data(), sizeof(std::complex<float>) * k * nt, m_stream),
+                   "resample upload");
+            ncheck(wct_dft_inv1b((const float*)d.p, (float*)d.p, k, nt, 1, m_stream), "resample inv");
+            ncheck(wct_memcpy_dtoh_async(buf.data(), d.p, sizeof(std::complex<float>) * k * nt, m_stream),
+                   "resample download");
+            ncheck(wct_stream_sync(m_stream), "resample sync");
+            const float scale = (float)(nt / nin);   // integer ratio, as Waveform::scale gets it
+            for (int r = 0; r < k; ++r) {
+                float* dst = rows.data() + (size_t)bl.second[r] * nt;
+                for (int t = 0; t < nt; ++t) dst[t] = buf[(size_t)r * nt + t].real() * scale;
+            }
+        }
+    }
+
+    std::string m_anode_tn{"AnodeChannels"}, m_noisedb_tn{"OmniChannelNoiseDB"};
+    std::shared_ptr<IAnodeChannels> m_anode;
+    std::shared_ptr<IChannelNoiseDatabase> m_noisedb;
+    PlaneLookup m_planes;
+    std::map<int, int> m_resmp;
+    mutable std::mutex m_mutex;
+    mutable void* m_stream{nullptr};
+    mutable DevBuf m_d_in, m_d_out, m_d_chans, m_d_flags, m_d_rcrc, m_d_bins, m_d_ws;
+};
+
+// ---- OmnibusNoiseFilter (sigproc/src/OmnibusNoiseFilter.cxx) -----------
+class OmnibusNoiseFilter : public IFrameFilter, public IConfigurable {
+  public:
+    Configuration default_configuration() const override   // :77-97
+    {
+        Configuration cfg;
+        cfg["nticks"] = (int)m_nticks;
+        cfg["maskmap"]["chirp"] = "bad";
+        cfg["maskmap"]["noisy"] = "bad";
+        cfg["channel_filters"][0] = "mbOneChannelNoise";
+        cfg["channel_status_filters"][0] = "mbOneChannelStatus";
+        cfg["grouped_filters"][0] = "mbCoherentNoiseSub";
+        cfg["noisedb"] = "";
+        cfg["intraces"] = m_intag;
+        cfg["outtraces"] = m_outtag;
+        return cfg;
+    }
+    void configure(const Configuration& cfg) override   // :37-75
+    {
+        m_nticks = (size_t)get(cfg, "nticks", (int)m_nticks);
+        m_maskmap.clear();
+        const auto& jmm = cfg["maskmap"];
+        if (jmm.isObject())
+            for (const auto& name : jmm.getMemberNames()) m_maskmap[name] = jmm[name].asString();
+        m_perchan.clear();
+        m_perchan_status.clear();
+        m_grouped.clear();
+        for (const auto& jf : cfg["channel_filters"]) m_perchan.push_back(Factory::find_tn<IChannelFilter>(jf.asString()));
+        for (const auto& jf : cfg["channel_status_filters"])
+            m_perchan_status.push_back(Factory::find_tn<IChannelFilter>(jf.asString()));
+        for (const auto& jf : cfg["grouped_filters"]) m_grouped.push_back(Factory::find_tn<IChannelFilter>(jf.asString()));
+        m_noisedb = Factory::find_tn<IChannelNoiseDatabase>(cfg["noisedb"].asString());
+        m_intag = get<std::string>(cfg, "intraces", m_intag);
+        m_outtag = get<std::string>(cfg, "outtraces", m_outtag);
+    }
+
+    bool operator()(const input_pointer& inframe, output_pointer& outframe) override   // :99-260
+    {
+        outframe = nullptr;
+        if (!inframe) return true;
+        const auto traces = tagged_traces(*inframe, m_intag);
+        if (traces.empty()) {
+            outframe = std::make_shared<Aux::SimpleFrame>(inframe->ident(), inframe->time(),
+                                                          std::make_shared<ITrace::vector>(), inframe->tick());
+            return true;
+        }
+        if (!m_nticks) m_nticks = traces.at(0)->charge().size();
+        const int nt = (int)m_nticks;
+
+        Waveform::ChannelMaskMap cmm;
+        Waveform::merge(cmm, inframe->masks(), m_maskmap);
+        const std::vector<int> bad = m_noisedb->bad_channels();
+        {
+            Waveform::ChannelMasks temp;
+            for (int ch : bad) temp[ch].push_back(Waveform::BinRange(0, nt));
+            Waveform::ChannelMaskMap tm;
+            tm["bad"] = temp;
+            Waveform::merge(cmm, tm, m_maskmap);
+        }
+
+        // working rows, one per channel (a later trace of the same channel
+        // takes its row), good channels' samples from tick 0
+        std::vector<int> channels;
+        std::unordered_map<int, int> row_of;
+        for (const auto& t : traces) {
+            if (!row_of.count(t->channel())) {
+                row_of[t->channel()] = (int)channels.size();
+                channels.push_back(t->channel());
+            }
+        }
+        std::vector<float> rows((size_t)channels.size() * nt, 0.f);
+        for (const auto& t : traces) {
+            float* row = rows.data() + (size_t)row_of[t->channel()] * nt;
+            std::fill(row, row + nt, 0.f);
+            if (std::binary_search(bad.begin(), bad.end(), t->channel())) continue;
+            const auto& q = t->charge();
+            std::copy(q.begin(), q.begin() + std::min((size_t)nt, q.size()), row);
+        }
+
+        for (const auto& filter : m_perchan) run_filter(*filter, channels, rows, nt, cmm);
+        for (const auto& group : m_noisedb->coherent_channels()) {   // :193-224
+            bool complete = true;
+            IChannelFilter::channel_signals_t chgrp;
+            for (int ch : group) {
+                auto it = row_of.find(ch);
+                if (it == row_of.end()) {
+                    complete = false;
+                    continue;
+                }
+                const float* row = rows.data() + (size_t)it->second * nt;
+                chgrp[ch].assign(row, row + nt);
+            }
+            if (!complete) continue;
+            for (const auto& filter : m_grouped) Waveform::merge(cmm, filter->apply(chgrp), m_maskmap);
+            for (const auto& cs : chgrp) std::copy(cs.second.begin(), cs.second.end(), rows.data() + (size_t)row_of[cs.first] * nt);
+        }
+        for (const auto& filter : m_perchan_status) run_filter(*filter, channels, rows, nt, cmm);
+
+        auto itraces = std::make_shared<ITrace::vector>();
+        for (size_t i = 0; i < channels.size(); ++i)
+            itraces->push_back(std::make_shared<Aux::SimpleTrace>(
+                channels[i], 0, ITrace::ChargeSequence(rows.begin() + i * nt, rows.begin() + (i + 1) * nt)));
+        auto sframe = std::make_shared<Aux::SimpleFrame>(inframe->ident(), inframe->time(), itraces, inframe->tick(),
+                                                         cmm);
+        IFrame::trace_list_t indices(itraces->size());
+        for (size_t i = 0; i < indices.size(); ++i) indices[i] = i;
+        if (!m_outtag.empty()) sframe->tag_traces(m_outtag, indices);
+        sframe->tag_frame("noisefilter");
+        outframe = sframe;
+        return true;
+    }
+
+  private:
+    // Aux::tagged_traces (aux/src/FrameTools.cxx:103-150).
+    static ITrace::vector tagged_traces(const IFrame& frame, const std::string& tag)
+    {
+        ITrace::vector ret;
+        const auto all = frame.traces();
+        if (!all) return ret;
+        if (tag.empty()) {   // untagged traces
+            std::vector<size_t> tagged;
+            for (const auto& t : frame.trace_tags()) {
+                const auto& l = frame.tagged_traces(t);
+                tagged.insert(tagged.end(), l.begin(), l.end());
+            }
+            std::sort(tagged.begin(), tagged.end());
+            for (size_t i = 0; i < all->size(); ++i)
+                if (!std::binary_search(tagged.begin(), tagged.end(), i)) ret.push_back(all->at(i));
+            return ret;
+        }
+        for (size_t i : frame.tagged_traces(tag)) ret.push_back(all->at(i));
+        if (!ret.empty()) return ret;
+        const auto& ft = frame.frame_tags();
+        if (std::find(ft.begin(), ft.end(), tag) == ft.end()) return ret;
+        return *all;
+    }
+
+    // One per-channel filter over every row: batched when the filter
+    // takes batches, else channel by channel; masks merged in channel
+    // order (:176-187).
+    void run_filter(const IChannelFilter& f, const std::vector<int>& channels, std::vector<float>& rows, int nt,
+                    Waveform::ChannelMaskMap& cmm) const
+    {
+        if (auto* batch = dynamic_cast<const IBatchChannelFilter*>(&f)) {
+            std::vector<Waveform::ChannelMaskMap> masks;
+            batch->apply_batch(channels, rows, nt, masks);
+            for (auto& m : masks) Waveform::merge(cmm, m, m_maskmap);
+            return;
+        }
+        for (size_t i = 0; i < channels.size(); ++i) {
+            IChannelFilter::signal_t sig(rows.begin() + i * nt, rows.begin() + (i + 1) * nt);
+            Waveform::merge(cmm, f.apply(channels[i], sig), m_maskmap);
+            std::copy(sig.begin(), sig.begin() + std::min((size_t)nt, sig.size()), rows.begin() + i * nt);
+        }
+    }
+
+    size_t m_nticks{0};
+    std::map<std::string, std::string> m_maskmap;
+    std::vector<std::shared_ptr<IChannelFilter>> m_perchan, m_perchan_status, m_grouped;
+    std::shared_ptr<IChannelNoiseDatabase> m_noisedb;
+    std::string m_intag{"orig"}, m_outtag{"raw"};
+};
+
+}  // namespace Hip
+}  // namespace WireCell
+
+WIRECELL_HIP_FACTORY(OmniChannelNoiseDB, WireCell::Hip::OmniChannelNoiseDB)
+WIRECELL_HIP_FACTORY(pdOneChannelNoise, WireCell::Hip::OneChannelNoise)
+WIRECELL_HIP_FACTORY(OmnibusNoiseFilter, WireCell::Hip::OmnibusNoiseFilter)
