@@ -63,6 +63,10 @@ def parse():
                     help="1: time the forward simulation convolution (one 896 x 6272 plane frame) as a side leg")
     ap.add_argument("--impact-only", action="store_true",
                     help="run only the forward simulation leg and print its own JSON line")
+    ap.add_argument("--noise-apas", type=int, default=4,
+                    help="APAs (2560 channels x 6000 ticks) in the timed noise-filter side leg (0 = skip)")
+    ap.add_argument("--noise-only", action="store_true",
+                    help="run only the noise-filter leg and print its own JSON line")
     ap.add_argument("--adjoint-frames", type=int, default=16,
                     help="frames of 800 x 6000 in the timed adjoint gradient step (0 = skip)")
     return ap.parse_args()
@@ -367,6 +371,95 @@ def impact_leg(dev, reps=20, cpu=True):
     return res
 
 
+def noise_leg(dev, napas=4, reps=10, cpu=True):
+    """ProtoDUNE's per-channel noise filter (pdOneChannelNoise as
+    OmnibusNoiseFilter runs it) on napas APAs of 2560 channels x 6000
+    ticks resident on the device: rcrc 1.1 ms on every channel (rc_layers
+    1), RMS cuts 1-30 ADC, seeded noise with RC-shaped pulses on a tenth of
+    the channels and a slow partial waveform on one channel in 200.  The
+    whole call is timed with HIP events on its stream, and one row
+    transform of the same shape (wct_dft_fwd1b, the kernel of both
+    transform stages) beside it.  CPU baseline: the oracle's one-channel
+    restatement (pure Python and numpy) on the first 100 channels (about
+    10 s)."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from wct_amd import noise as N
+    from wct_amd.capi import NfChannel, check, lib
+    nt, per_apa = 6000, 2560
+    nch = napas * per_apa
+    rng = np.random.default_rng(17)
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    x = torch.randn((nch, nt), generator=g, device=dev) * 3.0 + 900.0
+    t = torch.arange(nt, device=dev, dtype=torch.float32)
+    rc = N.rcrc_spectrum(1.1e6, 500.0, nt)
+    pulse_rows = rng.choice(nch, size=nch // 10, replace=False)
+    q = np.zeros(nt)
+    q[[1000, 3000, 4500]] = [600.0, 300.0, 900.0]
+    q = np.convolve(q, np.exp(-0.5 * (np.arange(-20, 21) / 4.0) ** 2), "same")
+    qrc = torch.as_tensor(np.fft.ifft(np.fft.fft(q) * rc.astype(np.complex128)).real, dtype=torch.float32,
+                          device=dev)
+    x[torch.as_tensor(pulse_rows, device=dev)] += qrc
+    x[::200] += 3000.0 * torch.exp(-t / 300.0)
+    planes = [(c % per_apa) // 800 if (c % per_apa) < 1600 else 2 for c in range(nch)]
+    entries = (NfChannel * nch)(*[NfChannel(p, 0, 0, 0, 1.0, 30.0) for p in planes])
+    dch = torch.frombuffer(bytearray(bytes(entries)), dtype=torch.uint8).to(dev)
+    drc = torch.as_tensor(rc, device=dev)
+    out = torch.empty_like(x)
+    flags = torch.zeros(nch, dtype=torch.int32, device=dev)
+    ws_bytes = lib().wct_nf_workspace_size(nch, nt)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+    run = lambda: check(lib().wct_nf_one_channel(
+        C.c_void_p(x.data_ptr()), nt, C.c_void_p(out.data_ptr()), nt, nch, nt, C.c_void_p(dch.data_ptr()),
+        C.c_void_p(drc.data_ptr()), None, C.c_void_p(flags.data_ptr()), C.c_void_p(ws.data_ptr()), ws_bytes,
+        stream.cuda_stream))
+    rows = torch.empty((nch, nt), dtype=torch.complex64, device=dev)
+    rows.normal_()
+    fwd = lambda: check(lib().wct_dft_fwd1b(C.c_void_p(rows.data_ptr()), C.c_void_p(rows.data_ptr()), nch, nt, 1,
+                                            stream.cuda_stream))
+
+    def timed(fn, n):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(n):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    run()
+    ms = timed(run, reps)
+    fwd()
+    fft_ms = timed(fwd, reps)
+    fl = flags.cpu().numpy()
+    ct = nch * nt
+    res = {
+        "workload": f"pdOneChannelNoise on {napas} APAs x {per_apa} channels x {nt} ticks",
+        "channels": nch, "ms": round(ms, 4), "channel_ticks_per_s": ct / (ms / 1e3),
+        "alg_bytes_per_channel_tick": 8, "achieved_gbs": 8 * ct / (ms / 1e3) / 1e9,
+        "frac": 8 * ct / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+        "row_transform_ms": round(fft_ms, 4),
+        "partial": int(((fl & 1) != 0).sum()), "noisy": int(((fl & 2) != 0).sum()), "dtype": "f32",
+    }
+    if cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import noise_filter as NFO
+        sample = list(range(100))
+        xs = x[sample].cpu().numpy()
+        t0 = time.perf_counter()
+        for i, c in enumerate(sample):
+            NFO.one_channel_noise(xs[i], planes[c], rcrc=rc, min_rms=1.0, max_rms=30.0)
+        sec = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": len(sample) * nt / sec, "unit": "channel-ticks/s", "cores": 1,
+                               "kind": "port", "sample": f"{len(sample)} channels of 6000 ticks through the "
+                                                         "oracle's one-channel restatement",
+                               "seconds": round(sec, 3)}
+    return res
+
+
 def plain_leg(planes, frames, outs, masks, rmss, kern, napas, reps=2):
     """The plain per-plane device path (one launch per reference step,
     FP32, response spectrum rebuilt per call) over the first napas APAs
@@ -519,6 +612,10 @@ def main():
     import torch
     from wct_amd import decon, workload as W
     dev = torch.device("cuda:0")
+    if args.noise_only:
+        print(json.dumps({"metric": "noise-filtered channel-ticks/s (pdOneChannelNoise, 1 GPU)",
+                          **noise_leg(dev, napas=max(1, args.noise_apas), cpu=not args.no_cpu)}), flush=True)
+        return
     if args.impact_only:
         print(json.dumps({"metric": "forward simulation samples/s (ImpactTransform, 1 GPU)",
                           **impact_leg(dev, cpu=not args.no_cpu)}), flush=True)
@@ -614,6 +711,10 @@ def main():
     if args.impact:
         log("bench: forward simulation (ImpactTransform) leg")
         impact = impact_leg(dev, cpu=not args.no_cpu)
+    noise = None
+    if args.noise_apas:
+        log("bench: noise-filter leg")
+        noise = noise_leg(dev, napas=args.noise_apas, cpu=not args.no_cpu)
     adjoint = None
     if args.adjoint_frames > 0:
         adjoint = adjoint_leg(dev, args.adjoint_frames)
@@ -687,6 +788,7 @@ def main():
         "plain_gpu": plain,
         "adjoint_step": adjoint,
         "forward_sim": impact,
+        "noise_filter": noise,
         "chargefit": chargefit,
         "batched_vs_plain": None if plain is None else plain["ratio"],
         "cpu_baseline": cpu,
