@@ -1,13 +1,19 @@
 // Per-channel noise filter of ProtoDUNE-SP (Protodune::OneChannelNoise::apply,
 // sigproc/src/Protodune.cxx:753-918) for every channel of a frame at once.
 //
-// Stages (one stream): k_nf_load (samples into complex rows), forward
-// row transforms (dft.hip), k_nf_spec (one workgroup per channel: the
-// partial-waveform test, the RC-undershoot division, the collection
-// plane's harmonic removal, the DC bin and the Hermitian mirror of
-// DftTools::inv_c2r), inverse row transforms, k_nf_time (one workgroup
-// per channel: baseline by binned median, the adaptive baseline of
-// partial channels, the signal-protected RMS test, the flag removal).
+// Stages (one stream): k_nf_load (two channels' samples into one complex
+// row, the first as real part, the second as imaginary part), forward
+// row transforms (dft.hip), k_nf_spec (one workgroup per channel pair:
+// each channel's spectrum separated from the pair's, X_a(k) = (Z(k) +
+// Z*(n-k)) / 2, X_b(k) = (Z(k) - Z*(n-k)) / 2i; the partial-waveform
+// test, the RC-undershoot division, the collection plane's harmonic
+// removal, the DC bin, and the Hermitian mirror of DftTools::inv_c2r
+// written back as Y_a + i Y_b), inverse row transforms (whose real and
+// imaginary parts are then the two channels' real waveforms), k_nf_time
+// (one workgroup per channel: baseline by binned median, the adaptive
+// baseline of partial channels, the signal-protected RMS test, the flag
+// removal).  Two channels per transform halve the transform work of a
+// real-input c2c transform per channel, which the reference does.
 //
 // The per-channel algorithms of sigproc/src/Microboone.cxx are sequential
 // loops in the reference; here each is a workgroup-parallel pass over the
@@ -23,7 +29,7 @@ namespace wct {
 
 namespace {
 
-constexpr int kNfBlock = 256;
+constexpr int kNfBlock = 512;   // 8 waves per channel: 16 per CU at two channels' LDS
 constexpr int kNfWaves = kNfBlock / 64;
 constexpr float kNfFlag = 10000.0f;
 
@@ -31,7 +37,6 @@ struct NfShared {
     double d[kNfWaves];
     float f[kNfWaves];
     int i[kNfWaves];
-    int chunk[kNfBlock];
     float res[4];
     int flag;
 };
@@ -156,18 +161,18 @@ __device__ int pct_binned(const float* s, int n, Valid valid, const float* p, in
     const int lo = min(m, threadIdx.x * C), hi = min(m, lo + C);
     int csum = 0;
     for (int i = lo; i < hi; ++i) csum += hist[i];
-    sh.chunk[threadIdx.x] = csum;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int t = 0; t < kNfBlock; ++t) {
-            const int c = sh.chunk[t];
-            sh.chunk[t] = acc;
-            acc += c;
-        }
+    // exclusive prefix of the chunk sums: scans within waves, then the
+    // totals of the waves before this one
+    int incl = csum;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if ((threadIdx.x & 63) >= o) incl += y;
     }
+    if ((threadIdx.x & 63) == 63) sh.i[threadIdx.x >> 6] = incl;
     __syncthreads();
-    const int before = sh.chunk[threadIdx.x];
+    int before = incl - csum;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) before += sh.i[w];
+    __syncthreads();
     for (int k = 0; k < np; ++k) {
         const int imed = (int)((float)m * p[k]);
         if (threadIdx.x == 0) sh.res[k] = mn + (mx - mn) * p[k];   // not reached for p < 1
@@ -220,7 +225,11 @@ __device__ void signal_filter(float* s, int n, int* hist, unsigned char* fl, NfS
         const float v = s[j];
         if (v < 4096.0f) {
             bool near = false;
-            for (int i = max(j - 7, 0); i <= min(j + 8, n - 1); ++i) near |= fl[i] != 0;
+#pragma unroll
+            for (int d = -7; d <= 8; ++d) {
+                const int i = j + d;
+                near |= i >= 0 && i < n && fl[i] != 0;
+            }
             if (near) s[j] = (float)((double)v + 20000.0);
         }
     }
@@ -240,8 +249,10 @@ __device__ void raw_adaptive_baseline(float* s, int n, double* base, unsigned ch
     for (int j = threadIdx.x; j < n; j += kNfBlock) {
         double a = 0;
         int c = 0;
-        for (int k = max(j - 10, 0); k <= min(j + 10, n - 1); ++k) {
-            const float v = s[k];
+#pragma unroll
+        for (int d = -10; d <= 10; ++d) {
+            const int k = j + d;
+            const float v = (k >= 0 && k < n) ? s[k] : 4096.0f;
             if (v < 4096.0f) {
                 a += v;
                 ++c;
@@ -299,32 +310,63 @@ __device__ __forceinline__ void nf_lds(unsigned char* base, int n, float*& s, un
 __global__ __launch_bounds__(kNfBlock) void k_nf_load(const float* __restrict__ in, int ld_in,
                                                       float2* __restrict__ rows, int nch, int n)
 {
+    const int npair = (nch + 1) / 2;
     const long i = (long)blockIdx.x * kNfBlock + threadIdx.x;
-    if (i >= (long)nch * n) return;
-    const int c = (int)(i / n), t = (int)(i % n);
-    G(rows)[i] = make_float2(G(in)[(long)c * ld_in + t], 0.f);
+    if (i >= (long)npair * n) return;
+    const int p = (int)(i / n), t = (int)(i % n);
+    const float a = G(in)[(long)(2 * p) * ld_in + t];
+    const float b = 2 * p + 1 < nch ? G(in)[(long)(2 * p + 1) * ld_in + t] : 0.f;
+    G(rows)[i] = make_float2(a, b);
 }
 
-// Spectrum stage of one channel (Protodune.cxx:767-876).
-__global__ __launch_bounds__(kNfBlock) void k_nf_spec(float2* __restrict__ rows, int n,
+// One channel's bin k of the pair row Z: X_a(k) = (Z(k) + Z*(n-k)) / 2
+// (which = 0) or X_b(k) = (Z(k) - Z*(n-k)) / 2i (which = 1).
+template <typename P>
+__device__ __forceinline__ float2 unpack_bin(P row, int n, int k, int which)
+{
+    const float2 z = row[k], w = row[k ? n - k : 0];
+    return which == 0 ? make_float2(0.5f * (z.x + w.x), 0.5f * (z.y - w.y))
+                      : make_float2(0.5f * (z.y + w.y), -0.5f * (z.x - w.x));
+}
+
+// a / b, rounded from double.
+__device__ __forceinline__ float2 cdiv(float2 a, float2 b)
+{
+    const double den = (double)b.x * b.x + (double)b.y * b.y;
+    return make_float2((float)(((double)a.x * b.x + (double)a.y * b.y) / den),
+                       (float)(((double)a.y * b.x - (double)a.x * b.y) / den));
+}
+
+// LDS of the spectrum stage when some channel has flagged runs: the
+// magnitudes, the adaptive-baseline scratch, and per channel of the pair
+// a flag per lower-half bin that the harmonic removal zeroed.
+__host__ __device__ __forceinline__ size_t nf_spec_lds_bytes(int n) { return (size_t)13 * n + 2 * (n / 2 + 1) + 64; }
+
+// Spectrum stage of one channel pair (Protodune.cxx:767-876).
+__global__ __launch_bounds__(kNfBlock) void k_nf_spec(float2* __restrict__ rows, int n, int nch,
                                                       const wct_nf_channel* __restrict__ chans,
                                                       const float2* __restrict__ rcrc, const int* __restrict__ bins,
                                                       int* __restrict__ flags)
 {
     extern __shared__ __align__(16) unsigned char nf_smem[];
     __shared__ NfShared sh;
-    const int c = blockIdx.x;
-    const wct_nf_channel ch = chans[c];
-    const auto row = G(rows + (long)c * n);
+    __shared__ int part[2], nharm[2];
+    const int p = blockIdx.x;
+    const int nc = 2 * p + 1 < nch ? 2 : 1;
+    wct_nf_channel ch[2];
+    ch[0] = chans[2 * p];
+    ch[1] = nc == 2 ? chans[2 * p + 1] : ch[0];
+    const auto row = G(rows + (long)p * n);
     // Diagnostics::Partial (sigproc/src/Diagnostics.cxx:15-31), nfreqs 4,
-    // maxpower 6000
-    if (threadIdx.x == 0) {
-        const float2 z1 = row[1];
+    // maxpower 6000, one lane per channel
+    if (threadIdx.x < 2) {
+        const int c = threadIdx.x;
+        const float2 z1 = unpack_bin(row, n, 1, c);
         const double mag0 = hypotf(z1.x, z1.y);
         double sum = mag0;
         int partial = 1;
         for (int ind = 1; ind <= 4 && ind < n; ++ind) {
-            const float2 z = row[ind + 1];
+            const float2 z = unpack_bin(row, n, ind + 1, c);
             const double magi = hypotf(z.x, z.y);
             if (mag0 <= magi) {
                 partial = 0;
@@ -332,59 +374,62 @@ __global__ __launch_bounds__(kNfBlock) void k_nf_spec(float2* __restrict__ rows,
             }
             sum += magi;
         }
-        sh.flag = partial && sum / 5 > 6000.0;
+        part[c] = partial && sum / 5 > 6000.0;
+        nharm[c] = 0;
     }
     __syncthreads();
-    const bool partial = sh.flag;
-    // RC undershoot: divide by the channel's rcrc spectrum (complex
-    // division, rounded from double)
-    if (!partial && ch.rcrc >= 0) {
-        const auto R = G(rcrc + (long)ch.rcrc * n);
-        for (int k = threadIdx.x; k < n; k += kNfBlock) {
-            const float2 a = row[k], b = R[k];
-            const double den = (double)b.x * b.x + (double)b.y * b.y;
-            const double re = ((double)a.x * b.x + (double)a.y * b.y) / den;
-            const double im = ((double)a.y * b.x - (double)a.x * b.y) / den;
-            row[k] = make_float2((float)re, (float)im);
-        }
-    }
-    __syncthreads();
-    int nharm = 0;
-    if (ch.plane == 2 && ch.bins_end > ch.bins_begin) {
+    const bool divide[2] = {!part[0] && ch[0].rcrc >= 0, !part[1] && ch[1].rcrc >= 0};
+    const int half = n / 2;
+    unsigned char* zf = nullptr;
+    bool harm[2] = {false, false};
+    for (int c = 0; c < nc; ++c) harm[c] = bins && ch[c].plane == 2 && ch[c].bins_end > ch[c].bins_begin;
+    if (harm[0] || harm[1]) {
         float* mag;
         unsigned char *scratch, *fl;
         nf_lds(nf_smem, n, mag, scratch, fl);
-        for (int k = threadIdx.x; k < n; k += kNfBlock) {
-            const float2 z = row[k];
-            mag[k] = k == 0 ? 0.f : hypotf(z.x, z.y);
-        }
-        __syncthreads();
-        raw_adaptive_baseline(mag, n, reinterpret_cast<double*>(scratch), fl, sh);
-        // the flagged runs, five rounds (:808-846); sequential, as the
-        // zeroed mirror bins feed later runs' statistics
-        if (threadIdx.x == 0) {
-            for (int iter = 0; iter < 5; ++iter) {
-                for (int r = ch.bins_begin; r < ch.bins_end; ++r) {
-                    const int lo = G(bins)[2 * r], iend = G(bins)[2 * r + 1] + 1;
-                    double a = 0, b = 0;
-                    for (int j = lo; j < iend; ++j) {
-                        a += mag[j];
-                        b += (double)mag[j] * mag[j];
-                    }
-                    const int m = iend - lo;
-                    double mean = m ? a / m : 0, rms = m > 1 ? sqrt((b - a * a / m) / m) : 0;
-                    if (m == 1) mean = mag[lo];
-                    const double cut = lo > 1050 ? mean + 3 * rms : mean + 2.7 * rms;
-                    for (int j = lo; j < iend; ++j) {
-                        if (mag[j] > cut) {
-                            const int jm = n + 1 - j;
-                            row[j] = make_float2(0.f, 0.f);
-                            mag[j] = 0;
-                            if (jm >= 0 && jm < n) {
-                                row[jm] = make_float2(0.f, 0.f);
-                                mag[jm] = 0;
+        zf = fl + n;
+        for (int k = threadIdx.x; k < 2 * (half + 1); k += kNfBlock) zf[k] = 0;
+        for (int c = 0; c < nc; ++c) {
+            if (!harm[c]) continue;
+            __syncthreads();
+            const auto R = G(rcrc + (long)max(ch[c].rcrc, 0) * n);
+            for (int k = threadIdx.x; k < n; k += kNfBlock) {
+                float2 x = unpack_bin(row, n, k, c);
+                if (divide[c]) x = cdiv(x, R[k]);
+                mag[k] = k == 0 ? 0.f : hypotf(x.x, x.y);
+            }
+            __syncthreads();
+            raw_adaptive_baseline(mag, n, reinterpret_cast<double*>(scratch), fl, sh);
+            // the flagged runs, five rounds (:808-846), on one lane: the
+            // zeroed mirror bins feed later runs' statistics.  Only bins
+            // 0..n/2 reach the output (the mirror replaces the rest), so a
+            // zeroing is recorded there: bin j itself, or its partner
+            // n + 1 - j when that one is in the lower half.
+            if (threadIdx.x == 0) {
+                unsigned char* z = zf + c * (half + 1);
+                for (int iter = 0; iter < 5; ++iter) {
+                    for (int r = ch[c].bins_begin; r < ch[c].bins_end; ++r) {
+                        const int lo = G(bins)[2 * r], iend = G(bins)[2 * r + 1] + 1;
+                        double a = 0, b = 0;
+                        for (int j = lo; j < iend; ++j) {
+                            a += mag[j];
+                            b += (double)mag[j] * mag[j];
+                        }
+                        const int m = iend - lo;
+                        const double mean = m == 1 ? (double)mag[lo] : (m ? a / m : 0.0);
+                        const double rms = m > 1 ? sqrt((b - a * a / m) / m) : 0.0;
+                        const double cut = lo > 1050 ? mean + 3 * rms : mean + 2.7 * rms;
+                        for (int j = lo; j < iend; ++j) {
+                            if (mag[j] > cut) {
+                                const int jm = n + 1 - j;
+                                mag[j] = 0;
+                                if (j <= half) z[j] = 1;
+                                if (jm >= 0 && jm < n) {
+                                    mag[jm] = 0;
+                                    if (jm <= half) z[jm] = 1;
+                                }
+                                ++nharm[c];
                             }
-                            ++nharm;
                         }
                     }
                 }
@@ -392,15 +437,29 @@ __global__ __launch_bounds__(kNfBlock) void k_nf_spec(float2* __restrict__ rows,
         }
     }
     __syncthreads();
-    // DC removed (:880), then inv_c2r's Hermitian mirror of bins 0..n/2
-    // (aux/src/DftTools.cxx:39-50, util/inc/WireCellUtil/Spectrum.h:56-89)
-    if (threadIdx.x == 0) row[0] = make_float2(0.f, 0.f);
-    for (int k = n / 2 + 1 + threadIdx.x; k < n; k += kNfBlock) {
-        const float2 z = row[n - k];
-        row[k] = make_float2(z.x, -z.y);
+    // per bin pair (k, n - k) on one thread: the channels' corrected bins,
+    // DC removed (:880), mirrored as inv_c2r mirrors them
+    const auto Ra = G(rcrc + (long)max(ch[0].rcrc, 0) * n);
+    const auto Rb = G(rcrc + (long)max(ch[1].rcrc, 0) * n);
+    for (int k = threadIdx.x; k <= half; k += kNfBlock) {
+        float2 xa = unpack_bin(row, n, k, 0), xb = unpack_bin(row, n, k, 1);
+        if (divide[0]) xa = cdiv(xa, Ra[k]);
+        if (divide[1]) xb = cdiv(xb, Rb[k]);
+        if (zf && harm[0] && zf[k]) xa = make_float2(0.f, 0.f);
+        if (zf && harm[1] && zf[half + 1 + k]) xb = make_float2(0.f, 0.f);
+        if (nc == 1) xb = make_float2(0.f, 0.f);
+        if (k == 0) {
+            row[0] = make_float2(0.f, 0.f);
+        } else if (2 * k == n) {
+            row[k] = make_float2(xa.x, xb.x);
+        } else {
+            row[k] = make_float2(xa.x - xb.y, xa.y + xb.x);
+            row[n - k] = make_float2(xa.x + xb.y, xb.x - xa.y);
+        }
     }
-    if (threadIdx.x == 0 && n % 2 == 0) row[n / 2] = make_float2(row[n / 2].x, 0.f);
-    if (threadIdx.x == 0) flags[c] = (partial ? WCT_NF_PARTIAL : 0) | (nharm > 5 ? WCT_NF_HARMONIC : 0);
+    if (threadIdx.x < nc)
+        flags[2 * p + threadIdx.x] =
+            (part[threadIdx.x] ? WCT_NF_PARTIAL : 0) | (nharm[threadIdx.x] > 5 ? WCT_NF_HARMONIC : 0);
 }
 
 // Time stage of one channel (Protodune.cxx:877-915).  rows hold the
@@ -416,8 +475,8 @@ __global__ __launch_bounds__(kNfBlock) void k_nf_time(const float2* __restrict__
     float* s;
     unsigned char *scratch, *fl;
     nf_lds(nf_smem, n, s, scratch, fl);
-    const auto row = G(rows + (long)c * n);
-    for (int i = threadIdx.x; i < n; i += kNfBlock) s[i] = row[i].x;
+    const auto row = G(reinterpret_cast<const float*>(rows + (long)(c / 2) * n) + (c & 1));
+    for (int i = threadIdx.x; i < n; i += kNfBlock) s[i] = row[2 * i];
     __syncthreads();
     int flag = flags[c];
     // baseline: binned median of the samples with those beyond 6 RMS set
@@ -495,7 +554,7 @@ extern "C" {
 size_t wct_nf_workspace_size(int nch, int nt)
 {
     if (nch <= 0 || nt <= 0) return 0;
-    return ((size_t)nch * nt * sizeof(float2) + 255) / 256 * 256;
+    return ((size_t)((nch + 1) / 2) * nt * sizeof(float2) + 255) / 256 * 256;
 }
 
 int wct_nf_rcrc_spectrum(double width, double tick, int nt, int layers, float* out, void* stream)
@@ -529,16 +588,21 @@ int wct_nf_one_channel(const float* in, int ld_in, float* out, int ld_out, int n
                 "nf: %d ticks need %zu bytes of LDS per channel", nt, lds);
     hipStream_t s = as_stream(stream);
     float2* rows = reinterpret_cast<float2*>(workspace);
-    const long total = (long)nch * nt;
+    const int npair = (nch + 1) / 2;
+    const long total = (long)npair * nt;
     hipLaunchKernelGGL(k_nf_load, dim3((unsigned)((total + kNfBlock - 1) / kNfBlock)), dim3(kNfBlock), 0, s, in,
                        ld_in, rows, nch, nt);
     WCT_HIP_CHECK(hipGetLastError());
-    WCT_TRY(launch_rows<float>(rows, rows, nt, nch, nt, -1, 1.0f, s));
-    WCT_TRY(allow_lds(k_nf_spec, lds));
-    hipLaunchKernelGGL(k_nf_spec, dim3(nch), dim3(kNfBlock), lds, s, rows, nt, chans,
+    WCT_TRY(launch_rows<float>(rows, rows, nt, npair, nt, -1, 1.0f, s));
+    // LDS only when some channel has flagged runs (bins given)
+    const size_t spec_lds = bins ? nf_spec_lds_bytes(nt) : 0;
+    WCT_REQUIRE((long)spec_lds + (long)sizeof(NfShared) <= lds_limit_bytes(), WCT_ERR_UNSUPPORTED,
+                "nf: %d ticks need %zu bytes of LDS per channel pair", nt, spec_lds);
+    WCT_TRY(allow_lds(k_nf_spec, spec_lds));
+    hipLaunchKernelGGL(k_nf_spec, dim3(npair), dim3(kNfBlock), spec_lds, s, rows, nt, nch, chans,
                        reinterpret_cast<const float2*>(rcrc), bins, flags);
     WCT_HIP_CHECK(hipGetLastError());
-    WCT_TRY(launch_rows<float>(rows, rows, nt, nch, nt, +1, 1.0f / nt, s));
+    WCT_TRY(launch_rows<float>(rows, rows, nt, npair, nt, +1, 1.0f / nt, s));
     WCT_TRY(allow_lds(k_nf_time, lds));
     hipLaunchKernelGGL(k_nf_time, dim3(nch), dim3(kNfBlock), lds, s, rows, nt, out, ld_out, chans, flags);
     WCT_HIP_CHECK(hipGetLastError());
