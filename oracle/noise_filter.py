@@ -1,7 +1,8 @@
-"""Restatement of Wire-Cell's noise filter for ProtoDUNE-SP: OmnibusNoiseFilter
+"""Restatement of Wire-Cell's per-channel noise filters: OmnibusNoiseFilter
 (sigproc/src/OmnibusNoiseFilter.cxx:95-260) with Protodune::OneChannelNoise
-(sigproc/src/Protodune.cxx:753-918) and the parts of OmniChannelNoiseDB it
-reads (sigproc/src/OmniChannelNoiseDB.cxx).
+(sigproc/src/Protodune.cxx:753-918) and RelGainCalib (:975-1009),
+Microboone::OneChannelNoise (sigproc/src/Microboone.cxx:902-1064) and the
+parts of OmniChannelNoiseDB they read (sigproc/src/OmniChannelNoiseDB.cxx).
 
 TEST INFRASTRUCTURE ONLY.  Only tests/ and the CPU-baseline leg of the
 benchmark may import this module, and only as the checker.  Float
@@ -316,3 +317,119 @@ def one_channel_noise(signal, plane, rcrc=None, noise=None, min_rms=0.5, max_rms
     if noisy:
         masks["noisy"] = True
     return sig, masks
+
+
+# ---- MicroBooNE's one-channel filter -------------------------------------
+
+def chirp(sig, window=20, min_rms=0.9, max_frac=0.20):
+    """Diagnostics::Chirp (sigproc/src/Diagnostics.cxx:33-137): RMS of
+    consecutive 20-sample windows (double), the low-RMS run bounds; returns
+    (is_chirp, beg, end) with [beg, end) the chirped samples."""
+    n = len(sig)
+    counter, mean, rms2 = 0, 0.0, 0.0
+    num_low, first, last, lowflag = 0, -1, -1, False
+    r1 = r2 = r3 = 0.0
+    normal = 0
+    for ibin in range(n):
+        x = float(sig[ibin])
+        mean += x
+        rms2 += x * x
+        counter += 1
+        if counter == window:
+            mean /= window
+            rms2 /= window
+            v = rms2 - mean * mean
+            rms = math.sqrt(v) if v >= 0 else float("nan")
+            r1, r2, r3 = r2, r3, rms
+            if rms < min_rms:
+                num_low += 1
+            if ibin >= 3 * window - 1:
+                if r2 < min_rms and (r1 > min_rms or r3 > min_rms):
+                    normal += 1
+                if not lowflag:
+                    if r2 < min_rms and r3 < min_rms:
+                        lowflag = True
+                        first, last = ibin - 2 * window + 1, ibin - window + 1
+                    if ibin == 3 * window - 1 and r1 < min_rms and r2 < min_rms:
+                        lowflag = True
+                        first, last = ibin - 3 * window + 1, ibin - 2 * window + 1
+                elif r2 < min_rms and r3 < min_rms:
+                    last = ibin - window + 1
+            counter, mean, rms2 = 0, 0.0, 0.0
+    chirp_frac = num_low / (n / window)
+    normal_frac = normal / num_low if num_low else float("nan")
+    if num_low > 4 and (normal_frac < max_frac or (num_low < 2.0 / max_frac and last - first == num_low * window)):
+        first = max(1, first - window)
+        last = min(n, last + 2 * window)
+        if n - last < window:
+            last = n
+        if chirp_frac > 0.99:
+            first, last = 1, n
+        return True, first - 1, last
+    return False, -1, -1
+
+
+def chirp_raise_baseline(sig, bin1, bin2):
+    """Microboone::Chirp_raise_baseline (:535-543), in place."""
+    sig[max(bin1, 0):min(bin2, len(sig))] = FLAG
+    return sig
+
+
+def mb_one_channel_noise(signal, plane, baseline=0.0, gain=1.0, rcrc=None, config=None, noise=None,
+                         min_rms=0.5, max_rms=10.0):
+    """Microboone::OneChannelNoise::apply (sigproc/src/Microboone.cxx:902-1064);
+    returns (signal, {mask name: [(first, second), ...]}).  rcrc, config
+    and noise are the DB spectra (None: not applied; rcrc only when the
+    waveform is not partial)."""
+    sig = (np.asarray(signal, dtype=F) + F(F(baseline) * -1)).astype(F)
+    n = len(sig)
+    sig_gc = (sig * F(gain)).astype(F)
+    masks = {}
+    is_chirp, beg, end = chirp(sig_gc)
+    if is_chirp:
+        masks.setdefault("chirp", []).append((beg, end))
+        if plane != 2 and (beg > 0 or end < n):
+            masks.setdefault("lf_noisy", []).append((beg, end))
+    spectrum = fwd_r2c(sig)
+    partial = is_partial(spectrum)
+    if not partial and rcrc is not None:
+        spectrum = (spectrum.astype(np.complex128) / np.asarray(rcrc, np.complex128)).astype(np.complex64)
+    if config is not None:
+        spectrum = (spectrum * np.asarray(config, np.complex64)).astype(np.complex64)
+    if noise is not None:
+        spectrum = (spectrum * np.asarray(noise, np.complex64)).astype(np.complex64)
+    spectrum[0] = 0
+    sig = inv_c2r(spectrum)
+    mean, rms = mean_rms(sig)
+    temp = sig.copy()
+    for i in range(n):
+        if abs(float(temp[i]) - mean) > 6 * rms:
+            temp[i] = F(mean)
+    base = median_binned(temp)
+    sig = (sig + F(-base)).astype(F)
+    if is_chirp:
+        chirp_raise_baseline(sig, beg, end)
+        signal_filter(sig)
+        raw_adaptive_baseline(sig)
+    if partial:
+        if plane != 2:
+            masks.setdefault("lf_noisy", []).append((0, n))
+        signal_filter(sig)
+        raw_adaptive_baseline(sig)
+    signal_filter(sig)
+    noisy = noisy_filter(sig, min_rms, max_rms)
+    remove_filter_flags(sig)
+    if noisy:
+        masks["noisy"] = [(0, n)]
+        masks.pop("lf_noisy", None)
+    return sig, masks
+
+
+def rel_gain_calib(signal, gain_fac, gain_def=1.0, gain_min=0.8, gain_max=1.25):
+    """Protodune::RelGainCalib::apply (Protodune.cxx:975-1009): gain_fac
+    (signal - 0) + 0 in float, unless the factor is outside the cuts."""
+    sig = np.asarray(signal, dtype=F).copy()
+    g = F(gain_fac)
+    if g > F(gain_max) or g < F(gain_min):
+        return sig
+    return (g * (sig - F(0))).astype(F) + F(0)

@@ -211,3 +211,62 @@ def test_one_channel_partial_waveform():
     assert abs(float(np.median(y[200:]))) < 3
     y, m = NF.one_channel_noise(x, 2, min_rms=1.0, max_rms=30.0)
     assert "lf_noisy" not in m
+
+
+def _chirp_signal(rng, n, lo, hi, sigma=3.0):
+    x = rng.normal(0, sigma, n).astype(F)
+    x[lo:hi] = rng.normal(0, 0.1, hi - lo)      # a dead (chirped) stretch
+    return x
+
+
+def test_chirp_hand_worked():
+    rng = np.random.default_rng(21)
+    n = 2000
+    # low-RMS windows 20..49 (samples 400..999): the run is first seen at
+    # window 21 (first = 400, last = 420) and last extended at window 49
+    # (last = 980); 30 low windows, 2 next to normal ones (0.067 < 0.2);
+    # widened to first - 20 = 380 and last + 40 = 1020, and the 1-based
+    # first bin turned into beg = 379
+    ok, beg, end = NF.chirp(_chirp_signal(rng, n, 400, 1000))
+    assert ok and (beg, end) == (379, 1020)
+    # a dead channel (every window low): the whole readout
+    ok, beg, end = NF.chirp(rng.normal(0, 0.1, n).astype(F))
+    assert ok and (beg, end) == (0, n)
+    # too few low windows (4): no chirp
+    ok, _, _ = NF.chirp(_chirp_signal(rng, n, 400, 480))
+    assert not ok
+    # ordinary noise
+    assert not NF.chirp(rng.normal(0, 3, n).astype(F))[0]
+
+
+def test_mb_one_channel_chirp_and_masks():
+    rng = np.random.default_rng(22)
+    n = 2000
+    x = (2048 + _chirp_signal(rng, n, 400, 1000)).astype(F)
+    y, m = NF.mb_one_channel_noise(x, 0, baseline=2048, min_rms=1.0, max_rms=30.0)
+    assert m["chirp"] == [(379, 1020)] and m["lf_noisy"] == [(379, 1020)] and "noisy" not in m
+    assert not y[379:1020].any()                                 # raised to the flag, then removed
+    assert abs(float(np.median(y[:300]))) < 0.5 and abs(float(np.median(y[1100:]))) < 0.5
+    # collection: chirp but no lf_noisy
+    y, m = NF.mb_one_channel_noise(x, 2, baseline=2048, min_rms=1.0, max_rms=30.0)
+    assert "chirp" in m and "lf_noisy" not in m
+    # a notch from the noise spectrum removes a line; the config spectrum scales
+    t = np.arange(n)
+    x = (400 + rng.normal(0, 3, n) + 20 * np.cos(2 * np.pi * 300 * t / n)).astype(F)
+    notch = NF.freqmask_spectrum([(1.0, 0, n - 1), (0.0, 295, 305), (0.0, n - 305, n - 295)], n)
+    y, m = NF.mb_one_channel_noise(x, 2, baseline=400, noise=notch, min_rms=1.0, max_rms=30.0)
+    assert m == {} and np.abs(np.fft.fft(y))[300] < 1.0
+    half = np.full(n, 0.5, np.complex64)
+    y2, _ = NF.mb_one_channel_noise(x, 2, baseline=400, config=half, noise=notch, min_rms=0.5, max_rms=30.0)
+    assert float(np.std(y2)) == pytest.approx(float(np.std(y)) / 2, rel=0.05)
+    # noisy clears lf_noisy
+    q = (900 + rng.normal(0, 0.05, n)).astype(F)
+    y, m = NF.mb_one_channel_noise(q, 0, baseline=900, min_rms=1.0, max_rms=30.0)
+    assert "noisy" in m and "lf_noisy" not in m and not y.any()
+
+
+def test_rel_gain_calib():
+    x = np.array([1.0, -2.0, 3.5], F)
+    np.testing.assert_array_equal(NF.rel_gain_calib(x, 1.1), (F(1.1) * x).astype(F))
+    np.testing.assert_array_equal(NF.rel_gain_calib(x, 1.3), x)      # above the 1.25 cut: unchanged
+    np.testing.assert_array_equal(NF.rel_gain_calib(x, 0.7), x)      # below 0.8
