@@ -404,11 +404,11 @@ def noise_leg(dev, napas=4, reps=10, cpu=True):
     x[torch.as_tensor(pulse_rows, device=dev)] += qrc
     x[::200] += 3000.0 * torch.exp(-t / 300.0)
     planes = [(c % per_apa) // 800 if (c % per_apa) < 1600 else 2 for c in range(nch)]
-    entries = (NfChannel * nch)(*[NfChannel(p, 0, 0, 0, 1.0, 30.0) for p in planes])
+    entries = (NfChannel * nch)(*[NfChannel(p, 0, 0, 0, 1.0, 30.0, 0, 0.0, 1.0, -1, -1) for p in planes])
     dch = torch.frombuffer(bytearray(bytes(entries)), dtype=torch.uint8).to(dev)
     drc = torch.as_tensor(rc, device=dev)
     out = torch.empty_like(x)
-    flags = torch.zeros(nch, dtype=torch.int32, device=dev)
+    flags = torch.zeros((nch, 3), dtype=torch.int32, device=dev)
     ws_bytes = lib().wct_nf_workspace_size(nch, nt)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream()
@@ -434,7 +434,7 @@ def noise_leg(dev, napas=4, reps=10, cpu=True):
     ms = timed(run, reps)
     fwd()
     fft_ms = timed(fwd, reps)
-    fl = flags.cpu().numpy()
+    fl = flags.cpu().numpy()[:, 0]
     ct = nch * nt
     res = {
         "workload": f"pdOneChannelNoise on {napas} APAs x {per_apa} channels x {nt} ticks",

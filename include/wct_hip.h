@@ -438,31 +438,49 @@ int wct_impact_waveforms(wct_impact h, const wct_impact_frame* frame, const floa
                          const float* long_aux, int nlong, int long_pad, float* out, void* stream);
 
 /* ---- noise filter ----
- * Protodune::OneChannelNoise::apply (sigproc/src/Protodune.cxx:753-918)
- * over every channel of a frame, the per-channel filter that
- * OmnibusNoiseFilter::operator() runs (sigproc/src/OmnibusNoiseFilter.cxx:
- * 170-188).  Per channel, as the noise DB and the anode give them:
- *   plane      wire-plane index (2: collection, where the flagged
- *              frequency runs are cleaned of harmonic lines)
- *   rcrc       row of the rcrc spectra table (IChannelNoiseDatabase::rcrc),
- *              -1 for none (an empty spectrum)
- *   bins_begin, bins_end   the channel's runs of flagged bins (noise
- *              spectrum real part < 0.5) in the bins table, [begin, end)
- *              of its (first, last) inclusive pairs
- *   min_rms, max_rms       min_rms_cut / max_rms_cut */
+ * The one-channel filters that OmnibusNoiseFilter::operator() runs per
+ * channel (sigproc/src/OmnibusNoiseFilter.cxx:170-188), over every
+ * channel of a frame at once: Protodune::OneChannelNoise::apply
+ * (sigproc/src/Protodune.cxx:753-918, mode WCT_NF_PROTODUNE) and
+ * Microboone::OneChannelNoise::apply (sigproc/src/Microboone.cxx:902-1064,
+ * mode WCT_NF_MICROBOONE).  Per channel, as the noise DB and the anode
+ * give them:
+ *   plane      wire-plane index (2: collection)
+ *   rcrc       row of the spectra table dividing the spectrum
+ *              (IChannelNoiseDatabase::rcrc; not for partial waveforms),
+ *              -1 for none
+ *   bins_begin, bins_end   ProtoDUNE, collection plane: the channel's runs
+ *              of flagged bins (noise spectrum real part < 0.5) cleaned of
+ *              harmonic lines, [begin, end) of the bins table's (first,
+ *              last) inclusive pairs
+ *   min_rms, max_rms       min_rms_cut / max_rms_cut
+ *   mode       WCT_NF_PROTODUNE or WCT_NF_MICROBOONE
+ *   baseline, gain         MicroBooNE: nominal_baseline subtracted first,
+ *              gain_correction applied for the chirp test
+ *   config, noise          MicroBooNE: rows of the spectra table the
+ *              spectrum is multiplied by (IChannelNoiseDatabase::config,
+ *              ::noise), -1 for none */
 typedef struct wct_nf_channel {
     int plane;
     int rcrc;
     int bins_begin, bins_end;
     float min_rms, max_rms;
+    int mode;
+    float baseline, gain;
+    int config, noise;
 } wct_nf_channel;
-/* Flags per channel: the spectrum looked like a partial waveform
- * (Diagnostics::Partial; the caller masks it "lf_noisy" off the
- * collection plane), the RMS test failed ("noisy"), more than 5 harmonic
- * bins were removed ("harmonic"). */
+#define WCT_NF_PROTODUNE 0
+#define WCT_NF_MICROBOONE 1
+/* Flags, three ints per channel: bits, then the chirped samples [begin,
+ * end) (-1, -1 without).  Bits: the spectrum looked like a partial
+ * waveform (Diagnostics::Partial; masked "lf_noisy" off the collection
+ * plane by the caller), the RMS test failed ("noisy"), more than 5
+ * harmonic bins were removed ("harmonic"), the waveform has a chirp
+ * (Diagnostics::Chirp, MicroBooNE; "chirp"). */
 #define WCT_NF_PARTIAL 1
 #define WCT_NF_NOISY 2
 #define WCT_NF_HARMONIC 4
+#define WCT_NF_CHIRP 8
 /* The rcrc spectrum of OmniChannelNoiseDB::parse_rcrc
  * (sigproc/src/OmniChannelNoiseDB.cxx:171-208): Response::SimpleRC of
  * width (time units) sampled at tick over nt samples, its spectrum (FP64
@@ -471,15 +489,21 @@ typedef struct wct_nf_channel {
 int wct_nf_rcrc_spectrum(double width, double tick, int nt, int layers, float* out, void* stream);
 /* in: nch rows of nt samples (row stride ld_in, channels whose signal the
  * filter sees, zeros for bad ones); out: the filtered rows (stride
- * ld_out); chans: nch entries (device); rcrc: rows of nt complex floats
- * (device, may be NULL when no channel has one); bins: (first, last)
- * pairs (device, may be NULL when no channel has any); flags: nch ints
- * (device).  The workspace holds wct_nf_workspace_size(nch, nt) bytes.
- * Up to 12000 ticks (one channel's samples and scratch in LDS). */
+ * ld_out); chans: nch entries (device); spectra: rows of nt complex floats
+ * that the channels' rcrc, config and noise index (device, may be NULL
+ * when no channel uses one); bins: (first, last) pairs (device, may be
+ * NULL when no channel has any); flags: 3 nch ints (device).  The
+ * workspace holds wct_nf_workspace_size(nch, nt) bytes.  Up to 12000
+ * ticks (one channel's samples and scratch in LDS). */
 size_t wct_nf_workspace_size(int nch, int nt);
 int wct_nf_one_channel(const float* in, int ld_in, float* out, int ld_out, int nch, int nt,
-                       const wct_nf_channel* chans, const float* rcrc, const int* bins, int* flags,
+                       const wct_nf_channel* chans, const float* spectra, const int* bins, int* flags,
                        void* workspace, size_t ws_bytes, void* stream);
+/* Protodune::RelGainCalib::apply (Protodune.cxx:975-1009) on nch rows in
+ * place: row c scaled by gain[c] (float) when gain[c] lies within
+ * [gain_min, gain_max]; gain is a device array. */
+int wct_nf_rel_gain(float* rows, int ld, int nch, int nt, const float* gain, float gain_min, float gain_max,
+                    void* stream);
 
 #ifdef __cplusplus
 }

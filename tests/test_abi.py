@@ -65,7 +65,7 @@ def test_plugin_factories_exported():
 def test_abi_version_and_error_string():
     from wct_amd import capi
     L = capi.lib()
-    assert L.wct_abi_version() == 4
+    assert L.wct_abi_version() == 5
     assert L.wct_dft_supported(6000) == 1 and L.wct_dft_supported(17) == 1
     assert L.wct_dft_supported(0) == 0 and L.wct_dft_supported((1 << 20) + 1) == 0
     assert L.wct_dft_native_length(6000) == 1 and L.wct_dft_native_length(17) == 0

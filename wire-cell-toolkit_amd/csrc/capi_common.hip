@@ -257,7 +257,7 @@ using namespace wct;
 extern "C" {
 
 const char* wct_last_error(void) { return g_err; }
-int wct_abi_version(void) { return 4; }
+int wct_abi_version(void) { return 5; }
 
 int wct_device_count(int* n)
 {

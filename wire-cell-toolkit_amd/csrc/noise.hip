@@ -307,16 +307,109 @@ __device__ __forceinline__ void nf_lds(unsigned char* base, int n, float*& s, un
     fl = scratch + (size_t)8 * n;
 }
 
+// A channel's sample as its filter starts from it: MicroBooNE's filter
+// first subtracts the nominal baseline (Microboone.cxx:914-918, float).
+__device__ __forceinline__ float nf_input(const float* in, int ld_in, const wct_nf_channel& ch, int c, int t)
+{
+    const float x = G(in)[(long)c * ld_in + t];
+    return ch.mode == WCT_NF_MICROBOONE ? x + ch.baseline * (-1) : x;
+}
+
 __global__ __launch_bounds__(kNfBlock) void k_nf_load(const float* __restrict__ in, int ld_in,
+                                                      const wct_nf_channel* __restrict__ chans,
                                                       float2* __restrict__ rows, int nch, int n)
 {
     const int npair = (nch + 1) / 2;
     const long i = (long)blockIdx.x * kNfBlock + threadIdx.x;
     if (i >= (long)npair * n) return;
     const int p = (int)(i / n), t = (int)(i % n);
-    const float a = G(in)[(long)(2 * p) * ld_in + t];
-    const float b = 2 * p + 1 < nch ? G(in)[(long)(2 * p + 1) * ld_in + t] : 0.f;
+    const float a = nf_input(in, ld_in, chans[2 * p], 2 * p, t);
+    const float b = 2 * p + 1 < nch ? nf_input(in, ld_in, chans[2 * p + 1], 2 * p + 1, t) : 0.f;
     G(rows)[i] = make_float2(a, b);
+}
+
+// Diagnostics::Chirp (sigproc/src/Diagnostics.cxx:33-137) on the
+// gain-corrected samples of MicroBooNE channels: the RMS of each complete
+// 20-sample window (double) in parallel, the run bookkeeping over the
+// windows on one lane.  Writes every channel's three flag ints.
+constexpr int kChirpWindow = 20;
+__global__ __launch_bounds__(kNfBlock) void k_nf_chirp(const float* __restrict__ in, int ld_in, int n,
+                                                       const wct_nf_channel* __restrict__ chans,
+                                                       int* __restrict__ flags)
+{
+    extern __shared__ __align__(16) unsigned char nf_smem[];
+    double* wrms = reinterpret_cast<double*>(nf_smem);
+    const int c = blockIdx.x;
+    const wct_nf_channel ch = chans[c];
+    if (ch.mode != WCT_NF_MICROBOONE) {
+        if (threadIdx.x == 0) {
+            flags[3 * c] = 0;
+            flags[3 * c + 1] = -1;
+            flags[3 * c + 2] = -1;
+        }
+        return;
+    }
+    const int nwin = n / kChirpWindow;
+    for (int w = threadIdx.x; w < nwin; w += kNfBlock) {
+        double m = 0, m2 = 0;
+        for (int k = 0; k < kChirpWindow; ++k) {
+            const double x = nf_input(in, ld_in, ch, c, w * kChirpWindow + k) * ch.gain;
+            m += x;
+            m2 += x * x;
+        }
+        m /= kChirpWindow;
+        m2 /= kChirpWindow;
+        wrms[w] = sqrt(m2 - m * m);
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double min_rms = 0.9, max_frac = 0.20;
+    int num_low = 0, first = -1, last = -1, normal = 0;
+    bool low = false;
+    double r1 = 0, r2 = 0, r3 = 0;
+    for (int w = 0; w < nwin; ++w) {
+        const int ibin = w * kChirpWindow + kChirpWindow - 1;
+        r1 = r2;
+        r2 = r3;
+        r3 = wrms[w];
+        if (r3 < min_rms) ++num_low;
+        if (ibin >= 3 * kChirpWindow - 1) {
+            if (r2 < min_rms && (r1 > min_rms || r3 > min_rms)) ++normal;
+            if (!low) {
+                if (r2 < min_rms && r3 < min_rms) {
+                    low = true;
+                    first = ibin - 2 * kChirpWindow + 1;
+                    last = ibin - kChirpWindow + 1;
+                }
+                if (ibin == 3 * kChirpWindow - 1 && r1 < min_rms && r2 < min_rms) {
+                    low = true;
+                    first = ibin - 3 * kChirpWindow + 1;
+                    last = ibin - 2 * kChirpWindow + 1;
+                }
+            } else if (r2 < min_rms && r3 < min_rms) {
+                last = ibin - kChirpWindow + 1;
+            }
+        }
+    }
+    const double chirp_frac = (double)num_low / ((double)n / (double)kChirpWindow);
+    const double normal_frac = (double)normal / (double)num_low;
+    int bits = 0, beg = -1, end = -1;
+    if (num_low > 4 && (normal_frac < max_frac ||
+                        (num_low < 2.0 / max_frac && last - first == num_low * kChirpWindow))) {
+        first = max(1, first - kChirpWindow);
+        last = min(n, last + 2 * kChirpWindow);
+        if (n - last < kChirpWindow) last = n;
+        if (chirp_frac > 0.99) {
+            first = 1;
+            last = n;
+        }
+        bits = WCT_NF_CHIRP;
+        beg = first - 1;
+        end = last;
+    }
+    flags[3 * c] = bits;
+    flags[3 * c + 1] = beg;
+    flags[3 * c + 2] = end;
 }
 
 // One channel's bin k of the pair row Z: X_a(k) = (Z(k) + Z*(n-k)) / 2
@@ -335,6 +428,11 @@ __device__ __forceinline__ float2 cdiv(float2 a, float2 b)
     const double den = (double)b.x * b.x + (double)b.y * b.y;
     return make_float2((float)(((double)a.x * b.x + (double)a.y * b.y) / den),
                        (float)(((double)a.y * b.x - (double)a.x * b.y) / den));
+}
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
 // LDS of the spectrum stage when some channel has flagged runs: the
@@ -382,7 +480,8 @@ __global__ __launch_bounds__(kNfBlock) void k_nf_spec(float2* __restrict__ rows,
     const int half = n / 2;
     unsigned char* zf = nullptr;
     bool harm[2] = {false, false};
-    for (int c = 0; c < nc; ++c) harm[c] = bins && ch[c].plane == 2 && ch[c].bins_end > ch[c].bins_begin;
+    for (int c = 0; c < nc; ++c)
+        harm[c] = bins && ch[c].mode == WCT_NF_PROTODUNE && ch[c].plane == 2 && ch[c].bins_end > ch[c].bins_begin;
     if (harm[0] || harm[1]) {
         float* mag;
         unsigned char *scratch, *fl;
@@ -441,10 +540,19 @@ __global__ __launch_bounds__(kNfBlock) void k_nf_spec(float2* __restrict__ rows,
     // DC removed (:880), mirrored as inv_c2r mirrors them
     const auto Ra = G(rcrc + (long)max(ch[0].rcrc, 0) * n);
     const auto Rb = G(rcrc + (long)max(ch[1].rcrc, 0) * n);
+    // MicroBooNE: the config and noise spectra multiply the spectrum
+    // (Microboone.cxx:958-974); only bins 0..n/2 reach the output
+    const bool mb[2] = {ch[0].mode == WCT_NF_MICROBOONE, ch[1].mode == WCT_NF_MICROBOONE};
+    const auto Ca = G(rcrc + (long)max(ch[0].config, 0) * n), Cb = G(rcrc + (long)max(ch[1].config, 0) * n);
+    const auto Na = G(rcrc + (long)max(ch[0].noise, 0) * n), Nb = G(rcrc + (long)max(ch[1].noise, 0) * n);
     for (int k = threadIdx.x; k <= half; k += kNfBlock) {
         float2 xa = unpack_bin(row, n, k, 0), xb = unpack_bin(row, n, k, 1);
         if (divide[0]) xa = cdiv(xa, Ra[k]);
         if (divide[1]) xb = cdiv(xb, Rb[k]);
+        if (mb[0] && ch[0].config >= 0) xa = cmulf(xa, Ca[k]);
+        if (mb[0] && ch[0].noise >= 0) xa = cmulf(xa, Na[k]);
+        if (mb[1] && ch[1].config >= 0) xb = cmulf(xb, Cb[k]);
+        if (mb[1] && ch[1].noise >= 0) xb = cmulf(xb, Nb[k]);
         if (zf && harm[0] && zf[k]) xa = make_float2(0.f, 0.f);
         if (zf && harm[1] && zf[half + 1 + k]) xb = make_float2(0.f, 0.f);
         if (nc == 1) xb = make_float2(0.f, 0.f);
@@ -458,7 +566,7 @@ __global__ __launch_bounds__(kNfBlock) void k_nf_spec(float2* __restrict__ rows,
         }
     }
     if (threadIdx.x < nc)
-        flags[2 * p + threadIdx.x] =
+        flags[3 * (2 * p + threadIdx.x)] |=
             (part[threadIdx.x] ? WCT_NF_PARTIAL : 0) | (nharm[threadIdx.x] > 5 ? WCT_NF_HARMONIC : 0);
 }
 
@@ -478,7 +586,7 @@ __global__ __launch_bounds__(kNfBlock) void k_nf_time(const float2* __restrict__
     const auto row = G(reinterpret_cast<const float*>(rows + (long)(c / 2) * n) + (c & 1));
     for (int i = threadIdx.x; i < n; i += kNfBlock) s[i] = row[2 * i];
     __syncthreads();
-    int flag = flags[c];
+    int flag = flags[3 * c];
     // baseline: binned median of the samples with those beyond 6 RMS set
     // to the mean (:880-890)
     double mean, rms;
@@ -497,7 +605,21 @@ __global__ __launch_bounds__(kNfBlock) void k_nf_time(const float2* __restrict__
     for (int i = threadIdx.x; i < n; i += kNfBlock) s[i] = s[i] + shift;
     __syncthreads();
     int* hist2 = reinterpret_cast<int*>(scratch);
-    if (flag & WCT_NF_PARTIAL) {
+    if (ch.mode == WCT_NF_MICROBOONE) {
+        // chirped samples raised to the flag, then the adaptive baseline
+        // (Microboone.cxx:1000-1024; no flag removal until the end)
+        if (flag & WCT_NF_CHIRP) {
+            const int b0 = max(flags[3 * c + 1], 0), b1 = min(flags[3 * c + 2], n);
+            for (int i = b0 + (int)threadIdx.x; i < b1; i += kNfBlock) s[i] = kNfFlag;
+            __syncthreads();
+            signal_filter(s, n, hist2, fl, sh);
+            raw_adaptive_baseline(s, n, reinterpret_cast<double*>(scratch), fl, sh);
+        }
+        if (flag & WCT_NF_PARTIAL) {
+            signal_filter(s, n, hist2, fl, sh);
+            raw_adaptive_baseline(s, n, reinterpret_cast<double*>(scratch), fl, sh);
+        }
+    } else if (flag & WCT_NF_PARTIAL) {
         signal_filter(s, n, hist2, fl, sh);
         raw_adaptive_baseline(s, n, reinterpret_cast<double*>(scratch), fl, sh);
         remove_filter_flags(s, n);
@@ -512,7 +634,7 @@ __global__ __launch_bounds__(kNfBlock) void k_nf_time(const float2* __restrict__
     remove_filter_flags(s, n);
     const auto dst = G(out + (long)c * ld_out);
     for (int i = threadIdx.x; i < n; i += kNfBlock) dst[i] = s[i];
-    if (threadIdx.x == 0) flags[c] = flag | (noisy ? WCT_NF_NOISY : 0);
+    if (threadIdx.x == 0) flags[3 * c] = flag | (noisy ? WCT_NF_NOISY : 0);
 }
 
 // SimpleRC samples at t = i tick, rounded to float as the reference's
@@ -541,6 +663,20 @@ __global__ __launch_bounds__(kNfBlock) void k_nf_rc_power(const double2* __restr
     float2 acc = s1;
     for (int l = 1; l < layers; ++l) acc = make_float2(acc.x * s1.x - acc.y * s1.y, acc.x * s1.y + acc.y * s1.x);
     G(out)[i] = acc;
+}
+
+// Protodune::RelGainCalib::apply (Protodune.cxx:975-1009).
+__global__ __launch_bounds__(kNfBlock) void k_nf_rel_gain(float* __restrict__ rows, int ld, int nch, int n,
+                                                          const float* __restrict__ gain, float gmin, float gmax)
+{
+    const long i = (long)blockIdx.x * kNfBlock + threadIdx.x;
+    if (i >= (long)nch * n) return;
+    const int c = (int)(i / n), t = (int)(i % n);
+    const float g = G(gain)[c];
+    if (g > gmax || g < gmin) return;
+    float* p = rows + (long)c * ld + t;
+    const float v = g * (*p - 0.f);
+    *p = v + 0.f;
 }
 
 }  // namespace
@@ -590,8 +726,12 @@ int wct_nf_one_channel(const float* in, int ld_in, float* out, int ld_out, int n
     float2* rows = reinterpret_cast<float2*>(workspace);
     const int npair = (nch + 1) / 2;
     const long total = (long)npair * nt;
+    const size_t chirp_lds = sizeof(double) * (nt / kChirpWindow + 1);
+    WCT_TRY(allow_lds(k_nf_chirp, chirp_lds));
+    hipLaunchKernelGGL(k_nf_chirp, dim3(nch), dim3(kNfBlock), chirp_lds, s, in, ld_in, nt, chans, flags);
+    WCT_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_nf_load, dim3((unsigned)((total + kNfBlock - 1) / kNfBlock)), dim3(kNfBlock), 0, s, in,
-                       ld_in, rows, nch, nt);
+                       ld_in, chans, rows, nch, nt);
     WCT_HIP_CHECK(hipGetLastError());
     WCT_TRY(launch_rows<float>(rows, rows, nt, npair, nt, -1, 1.0f, s));
     // LDS only when some channel has flagged runs (bins given)
@@ -610,3 +750,16 @@ int wct_nf_one_channel(const float* in, int ld_in, float* out, int ld_out, int n
 }
 
 }  // extern "C"
+
+extern "C" int wct_nf_rel_gain(float* rows, int ld, int nch, int nt, const float* gain, float gain_min,
+                               float gain_max, void* stream)
+{
+    WCT_ENTRY();
+    WCT_REQUIRE(rows && gain && nch >= 0 && nt >= 0 && ld >= nt, WCT_ERR_ARG, "nf rel gain: bad arguments");
+    if (nch == 0 || nt == 0) return WCT_OK;
+    const long total = (long)nch * nt;
+    hipLaunchKernelGGL(k_nf_rel_gain, dim3((unsigned)((total + kNfBlock - 1) / kNfBlock)), dim3(kNfBlock), 0,
+                       as_stream(stream), rows, ld, nch, nt, gain, gain_min, gain_max);
+    WCT_HIP_CHECK(hipGetLastError());
+    return WCT_OK;
+}

@@ -103,7 +103,7 @@ class OmniChannelNoiseDB : public IChannelNoiseDatabase, public IConfigurable {
         int pad_window_front{0}, pad_window_back{0};
         float decon_limit{0.02f}, decon_lf_cutoff{0.08f}, adc_limit{0.0f}, decon_limit1{0.08f};
         float protection_factor{5.0f}, min_adc_limit{50}, roi_min_max_ratio{0.8f};
-        std::shared_ptr<filter_t> rcrc, noise;
+        std::shared_ptr<filter_t> rcrc, config, noise;
     };
 
     Configuration default_configuration() const override
@@ -140,6 +140,8 @@ class OmniChannelNoiseDB : public IChannelNoiseDatabase, public IConfigurable {
         std::sort(m_bad.begin(), m_bad.end());
         m_rc_layers = 2;
         m_rcrc_cache.clear();
+        m_reconfig_cache.clear();
+        m_unity.reset();
         for (const auto& jci : cfg["channel_info"]) update_channels(jci);
     }
 
@@ -152,6 +154,12 @@ class OmniChannelNoiseDB : public IChannelNoiseDatabase, public IConfigurable {
     {
         static const filter_t dummy;
         const auto& f = dbget(ch).rcrc;
+        return f ? *f : dummy;
+    }
+    const filter_t& config(int ch) const override
+    {
+        static const filter_t dummy;
+        const auto& f = dbget(ch).config;
         return f ? *f : dummy;
     }
     const filter_t& noise(int ch) const override
@@ -220,6 +228,60 @@ class OmniChannelNoiseDB : public IChannelNoiseDatabase, public IConfigurable {
         return spec;
     }
 
+    // A flat unit spectrum of nsamples bins (default_filter, :141-150).
+    std::shared_ptr<filter_t> unity()
+    {
+        if (!m_unity) m_unity = std::make_shared<filter_t>(m_nsamples, std::complex<float>(1, 0));
+        return m_unity;
+    }
+
+    // The float spectrum of nsamples samples (FP64 transform on the device).
+    filter_t spectrum_of(const std::vector<float>& wave) const
+    {
+        const int n = (int)wave.size();
+        std::vector<double> buf(2 * (size_t)n, 0.0);
+        for (int i = 0; i < n; ++i) buf[2 * i] = wave[i];
+        DevBuf d;
+        d.ensure(sizeof(double) * buf.size());
+        void* s = nullptr;
+        ncheck(wct_stream_create(&s), "OmniChannelNoiseDB stream");
+        int rc = wct_memcpy_htod(d.p, buf.data(), sizeof(double) * buf.size(), s);
+        if (rc == WCT_OK) rc = wct_dft_fwd1b_f64((const double*)d.p, (double*)d.p, 1, n, 1, s);
+        if (rc == WCT_OK) rc = wct_memcpy_dtoh(buf.data(), d.p, sizeof(double) * buf.size(), s);
+        wct_stream_destroy(s);
+        ncheck(rc, "OmniChannelNoiseDB spectrum");
+        filter_t out(n);
+        for (int i = 0; i < n; ++i) out[i] = std::complex<float>((float)buf[2 * i], (float)buf[2 * i + 1]);
+        return out;
+    }
+
+    // parse_reconfig / get_reconfig (:220-273): the ratio of the "to" and
+    // "from" cold-electronics spectra (Response::ColdElec sampled at
+    // i tick), cached by the four rounded parameters as the reference
+    // caches them; an empty object gives the unit spectrum.
+    std::shared_ptr<filter_t> parse_reconfig(const Configuration& j)
+    {
+        if (!j.isObject() || j.getMemberNames().empty()) return unity();
+        const double mV_fC = units::mV / units::fC, us = units::microsecond;
+        const double fg = j["from"]["gain"].asDouble(), fs = j["from"]["shaping"].asDouble();
+        const double tg = j["to"]["gain"].asDouble(), ts = j["to"]["shaping"].asDouble();
+        const int key = (int)std::round(10.0 * fg / mV_fC) << 24 | (int)std::round(10.0 * fs / us) << 16 |
+                        (int)std::round(10.0 * tg / mV_fC) << 8 | (int)std::round(10.0 * ts / us);
+        auto it = m_reconfig_cache.find(key);
+        if (it != m_reconfig_cache.end()) return it->second;
+        std::vector<float> to_sig(m_nsamples), from_sig(m_nsamples);
+        const double dt = (m_nsamples * m_tick) / m_nsamples;
+        for (int i = 0; i < m_nsamples; ++i) {
+            to_sig[i] = (float)coldelec(i * dt, tg, ts);
+            from_sig[i] = (float)coldelec(i * dt, fg, fs);
+        }
+        const filter_t to_f = spectrum_of(to_sig), from_f = spectrum_of(from_sig);
+        auto filt = std::make_shared<filter_t>(m_nsamples);
+        for (int i = 0; i < m_nsamples; ++i) (*filt)[i] = to_f[i] / from_f[i];
+        m_reconfig_cache[key] = filt;
+        return filt;
+    }
+
     // parse_freqmasks (:152-169)
     std::shared_ptr<filter_t> parse_freqmasks(const Configuration& j) const
     {
@@ -233,9 +295,9 @@ class OmniChannelNoiseDB : public IChannelNoiseDatabase, public IConfigurable {
         return spec;
     }
 
-    // update_channels (:376-573): the keys the ProtoDUNE filter and the
-    // accessors above read; response/reconfig (coherent and MicroBooNE
-    // filters) are not kept.
+    // update_channels (:376-573): the keys the one-channel filters and the
+    // accessors above read; the response spectra (read by the coherent
+    // filter) are not kept.
     void update_channels(const Configuration& cfg)
     {
         auto chans = parse_channels(cfg["channels"]);
@@ -273,6 +335,10 @@ class OmniChannelNoiseDB : public IChannelNoiseDatabase, public IConfigurable {
             auto val = parse_rcrc(cfg["rcrc"].asDouble(), m_rc_layers);
             for (int ch : chans) get_ci(ch).rcrc = val;
         }
+        if (!cfg["reconfig"].isNull()) {
+            auto val = parse_reconfig(cfg["reconfig"]);
+            for (int ch : chans) get_ci(ch).config = val;
+        }
         if (!cfg["freqmasks"].isNull()) {
             auto val = parse_freqmasks(cfg["freqmasks"]);
             for (int ch : chans) get_ci(ch).noise = val;
@@ -287,13 +353,16 @@ class OmniChannelNoiseDB : public IChannelNoiseDatabase, public IConfigurable {
     std::map<int, ChannelInfo> m_db;
     std::vector<channel_group_t> m_groups;
     std::vector<int> m_bad;
-    std::map<int, std::shared_ptr<filter_t>> m_rcrc_cache;
+    std::map<int, std::shared_ptr<filter_t>> m_rcrc_cache, m_reconfig_cache;
+    std::shared_ptr<filter_t> m_unity;
 };
 
-// ---- pdOneChannelNoise (sigproc/src/Protodune.cxx:712-923) -------------
+// ---- pdOneChannelNoise (sigproc/src/Protodune.cxx:712-923) and
+//      mbOneChannelNoise (sigproc/src/Microboone.cxx:894-1069) -----------
 class OneChannelNoise : public IChannelFilter, public IBatchChannelFilter, public IConfigurable {
   public:
     typedef IChannelNoiseDatabase::filter_t filter_t;
+    explicit OneChannelNoise(int mode) : m_mode(mode) {}
     ~OneChannelNoise() override
     {
         if (m_stream) wct_stream_destroy(m_stream);
@@ -314,6 +383,7 @@ class OneChannelNoise : public IChannelFilter, public IBatchChannelFilter, publi
         m_noisedb_tn = get<std::string>(cfg, "noisedb", m_noisedb_tn);
         m_noisedb = Factory::find_tn<IChannelNoiseDatabase>(m_noisedb_tn);
         m_resmp.clear();
+        if (m_mode != WCT_NF_PROTODUNE) return;   // FEMB resampling is ProtoDUNE's
         for (const auto& jone : cfg["resmp"]) {
             const int smpin = jone["sample_from"].asInt();
             for (const auto& jch : jone["channels"]) m_resmp[jch.asInt()] = smpin;
@@ -340,10 +410,25 @@ class OneChannelNoise : public IChannelFilter, public IBatchChannelFilter, publi
         if (!m_stream) ncheck(wct_stream_create(&m_stream), "pdOneChannelNoise stream");
         resample(channels, rows, nt);
 
-        // per-channel entries, rcrc rows and flagged runs, shared spectra once
+        // per-channel entries, spectra rows and flagged runs, shared spectra once
         std::vector<wct_nf_channel> ent(nch);
         std::vector<const filter_t*> rc_rows;
         std::map<const filter_t*, int> rc_id;
+        auto row_of = [&](const filter_t& f, const char* what, int ch) {
+            if (f.empty()) return -1;
+            if ((int)f.size() != nt) {
+                if (m_mode == WCT_NF_PROTODUNE) return -1;   // :771-779: skipped (with a warning)
+                throw std::runtime_error(std::string("mbOneChannelNoise: ") + what + " spectrum of channel " +
+                                         std::to_string(ch) + " has " + std::to_string(f.size()) +
+                                         " bins for " + std::to_string(nt) + " ticks");
+            }
+            auto it = rc_id.find(&f);
+            if (it == rc_id.end()) {
+                it = rc_id.emplace(&f, (int)rc_rows.size()).first;
+                rc_rows.push_back(&f);
+            }
+            return it->second;
+        };
         std::vector<int> bins;
         std::map<const filter_t*, std::pair<int, int>> bin_id;
         for (int i = 0; i < nch; ++i) {
@@ -352,17 +437,19 @@ class OneChannelNoise : public IChannelFilter, public IBatchChannelFilter, publi
             e.plane = m_planes(ch);
             e.min_rms = (float)m_noisedb->min_rms_cut(ch);
             e.max_rms = (float)m_noisedb->max_rms_cut(ch);
-            const filter_t& rc = m_noisedb->rcrc(ch);
-            e.rcrc = -1;
-            if ((int)rc.size() == nt) {
-                auto it = rc_id.find(&rc);
-                if (it == rc_id.end()) {
-                    it = rc_id.emplace(&rc, (int)rc_rows.size()).first;
-                    rc_rows.push_back(&rc);
-                }
-                e.rcrc = it->second;
-            }
+            e.mode = m_mode;
+            e.rcrc = row_of(m_noisedb->rcrc(ch), "rcrc", ch);
+            e.baseline = 0.f;
+            e.gain = 1.f;
+            e.config = e.noise = -1;
             e.bins_begin = e.bins_end = 0;
+            if (m_mode == WCT_NF_MICROBOONE) {
+                e.baseline = (float)m_noisedb->nominal_baseline(ch);
+                e.gain = (float)m_noisedb->gain_correction(ch);
+                e.config = row_of(m_noisedb->config(ch), "config", ch);
+                e.noise = row_of(m_noisedb->noise(ch), "noise", ch);
+                continue;
+            }
             if (e.plane == 2) {
                 const filter_t& nz = m_noisedb->noise(ch);
                 auto it = bin_id.find(&nz);
@@ -388,7 +475,7 @@ class OneChannelNoise : public IChannelFilter, public IBatchChannelFilter, publi
         m_d_in.ensure(row_bytes);
         m_d_out.ensure(row_bytes);
         m_d_chans.ensure(sizeof(wct_nf_channel) * nch);
-        m_d_flags.ensure(sizeof(int) * nch);
+        m_d_flags.ensure(sizeof(int) * 3 * nch);
         m_d_rcrc.ensure(sizeof(float) * 2 * std::max<size_t>(1, rc_rows.size()) * nt);
         m_d_bins.ensure(sizeof(int) * std::max<size_t>(2, bins.size()));
         const size_t ws = wct_nf_workspace_size(nch, nt);
@@ -407,16 +494,35 @@ class OneChannelNoise : public IChannelFilter, public IBatchChannelFilter, publi
                                   (const wct_nf_channel*)m_d_chans.p, (const float*)m_d_rcrc.p,
                                   (const int*)m_d_bins.p, (int*)m_d_flags.p, m_d_ws.p, ws, m_stream),
                "pdOneChannelNoise");
-        std::vector<int> flags(nch);
+        std::vector<int> flags(3 * (size_t)nch);
         ncheck(wct_memcpy_dtoh_async(rows.data(), m_d_out.p, row_bytes, m_stream), "pdOneChannelNoise download");
-        ncheck(wct_memcpy_dtoh_async(flags.data(), m_d_flags.p, sizeof(int) * nch, m_stream),
+        ncheck(wct_memcpy_dtoh_async(flags.data(), m_d_flags.p, sizeof(int) * 3 * nch, m_stream),
                "pdOneChannelNoise download");
         ncheck(wct_stream_sync(m_stream), "pdOneChannelNoise sync");
         for (int i = 0; i < nch; ++i) {
+            const int bits = flags[3 * i], ch = channels[i];
             const Waveform::BinRange all(0, nt);
-            if (flags[i] & WCT_NF_HARMONIC) masks[i]["harmonic"][channels[i]].push_back(all);
-            if ((flags[i] & WCT_NF_PARTIAL) && ent[i].plane != 2) masks[i]["lf_noisy"][channels[i]].push_back(all);
-            if (flags[i] & WCT_NF_NOISY) masks[i]["noisy"][channels[i]].push_back(all);
+            auto& m = masks[i];
+            if (m_mode == WCT_NF_MICROBOONE) {   // Microboone.cxx:924-935, 1007-1016, 1044-1061
+                const Waveform::BinRange chirped(flags[3 * i + 1], flags[3 * i + 2]);
+                if (bits & WCT_NF_CHIRP) {
+                    m["chirp"][ch].push_back(chirped);
+                    if (ent[i].plane != 2 && (chirped.first > 0 || chirped.second < nt))
+                        m["lf_noisy"][ch].push_back(chirped);
+                }
+                if ((bits & WCT_NF_PARTIAL) && ent[i].plane != 2) m["lf_noisy"][ch].push_back(all);
+                if (bits & WCT_NF_NOISY) {
+                    m["noisy"][ch].push_back(all);
+                    auto lf = m.find("lf_noisy");
+                    if (lf != m.end()) {
+                        lf->second.erase(ch);
+                    }
+                }
+                continue;
+            }
+            if (bits & WCT_NF_HARMONIC) m["harmonic"][ch].push_back(all);
+            if ((bits & WCT_NF_PARTIAL) && ent[i].plane != 2) m["lf_noisy"][ch].push_back(all);
+            if (bits & WCT_NF_NOISY) m["noisy"][ch].push_back(all);
         }
     }
 
@@ -482,6 +588,7 @@ class OneChannelNoise : public IChannelFilter, public IBatchChannelFilter, publi
         }
     }
 
+    int m_mode;
     std::string m_anode_tn{"AnodeChannels"}, m_noisedb_tn{"OmniChannelNoiseDB"};
     std::shared_ptr<IAnodeChannels> m_anode;
     std::shared_ptr<IChannelNoiseDatabase> m_noisedb;
@@ -490,6 +597,86 @@ class OneChannelNoise : public IChannelFilter, public IBatchChannelFilter, publi
     mutable std::mutex m_mutex;
     mutable void* m_stream{nullptr};
     mutable DevBuf m_d_in, m_d_out, m_d_chans, m_d_flags, m_d_rcrc, m_d_bins, m_d_ws;
+};
+
+class PdOneChannelNoise : public OneChannelNoise {
+  public:
+    PdOneChannelNoise() : OneChannelNoise(WCT_NF_PROTODUNE) {}
+};
+class MbOneChannelNoise : public OneChannelNoise {
+  public:
+    MbOneChannelNoise() : OneChannelNoise(WCT_NF_MICROBOONE) {}
+};
+
+// ---- pdRelGainCalib (sigproc/src/Protodune.cxx:925-1014) ---------------
+class RelGainCalib : public IChannelFilter, public IBatchChannelFilter, public IConfigurable {
+  public:
+    ~RelGainCalib() override
+    {
+        if (m_stream) wct_stream_destroy(m_stream);
+    }
+    Configuration default_configuration() const override
+    {
+        Configuration cfg;
+        cfg["anode"] = m_anode_tn;
+        cfg["noisedb"] = m_noisedb_tn;
+        cfg["gain_def"] = (double)m_gain_def;
+        cfg["gain_min_cut"] = (double)m_gain_min;
+        cfg["gain_max_cut"] = (double)m_gain_max;
+        return cfg;
+    }
+    void configure(const Configuration& cfg) override
+    {
+        m_anode_tn = get<std::string>(cfg, "anode", m_anode_tn);
+        Factory::find_tn<IAnodeChannels>(m_anode_tn);
+        m_noisedb_tn = get<std::string>(cfg, "noisedb", m_noisedb_tn);
+        Factory::find_tn<IChannelNoiseDatabase>(m_noisedb_tn);
+        m_gain_def = get(cfg, "gain_def", m_gain_def);
+        m_gain_min = get(cfg, "gain_min_cut", m_gain_min);
+        m_gain_max = get(cfg, "gain_max_cut", m_gain_max);
+        m_rel_gain.clear();
+        for (const auto& j : cfg["rel_gain"]) m_rel_gain.push_back((float)j.asDouble());
+    }
+    Waveform::ChannelMaskMap apply(int ch, signal_t& signal) const override
+    {
+        std::vector<float> rows(signal.begin(), signal.end());
+        std::vector<Waveform::ChannelMaskMap> masks;
+        apply_batch({ch}, rows, (int)signal.size(), masks);
+        signal.assign(rows.begin(), rows.end());
+        return Waveform::ChannelMaskMap();
+    }
+    Waveform::ChannelMaskMap apply(channel_signals_t&) const override { return Waveform::ChannelMaskMap(); }
+    // The factor of channel ch is rel_gain[ch] (an array indexed by channel
+    // number, :984); out of range throws as the reference's at() does.
+    void apply_batch(const std::vector<int>& channels, std::vector<float>& rows, int nt,
+                     std::vector<Waveform::ChannelMaskMap>& masks) const override
+    {
+        std::lock_guard<std::mutex> lock(m_mutex);
+        const int nch = (int)channels.size();
+        masks.assign(nch, Waveform::ChannelMaskMap());
+        if (nch == 0 || nt == 0) return;
+        std::vector<float> g(nch);
+        for (int i = 0; i < nch; ++i) g[i] = m_rel_gain.at(channels[i]);
+        if (!m_stream) ncheck(wct_stream_create(&m_stream), "pdRelGainCalib stream");
+        const size_t bytes = sizeof(float) * (size_t)nch * nt;
+        m_d_rows.ensure(bytes);
+        m_d_gain.ensure(sizeof(float) * nch);
+        ncheck(wct_memcpy_htod_async(m_d_rows.p, rows.data(), bytes, m_stream), "pdRelGainCalib upload");
+        ncheck(wct_memcpy_htod_async(m_d_gain.p, g.data(), sizeof(float) * nch, m_stream), "pdRelGainCalib upload");
+        ncheck(wct_nf_rel_gain((float*)m_d_rows.p, nt, nch, nt, (const float*)m_d_gain.p, m_gain_min, m_gain_max,
+                               m_stream),
+               "pdRelGainCalib");
+        ncheck(wct_memcpy_dtoh_async(rows.data(), m_d_rows.p, bytes, m_stream), "pdRelGainCalib download");
+        ncheck(wct_stream_sync(m_stream), "pdRelGainCalib sync");
+    }
+
+  private:
+    std::string m_anode_tn{"AnodeChannels"}, m_noisedb_tn{"OmniChannelNoiseDB"};
+    float m_gain_def{1.0f}, m_gain_min{0.8f}, m_gain_max{1.25f};
+    std::vector<float> m_rel_gain;
+    mutable std::mutex m_mutex;
+    mutable void* m_stream{nullptr};
+    mutable DevBuf m_d_rows, m_d_gain;
 };
 
 // ---- OmnibusNoiseFilter (sigproc/src/OmnibusNoiseFilter.cxx) -----------
@@ -659,5 +846,7 @@ class OmnibusNoiseFilter : public IFrameFilter, public IConfigurable {
 }  // namespace WireCell
 
 WIRECELL_HIP_FACTORY(OmniChannelNoiseDB, WireCell::Hip::OmniChannelNoiseDB)
-WIRECELL_HIP_FACTORY(pdOneChannelNoise, WireCell::Hip::OneChannelNoise)
+WIRECELL_HIP_FACTORY(pdOneChannelNoise, WireCell::Hip::PdOneChannelNoise)
+WIRECELL_HIP_FACTORY(mbOneChannelNoise, WireCell::Hip::MbOneChannelNoise)
+WIRECELL_HIP_FACTORY(pdRelGainCalib, WireCell::Hip::RelGainCalib)
 WIRECELL_HIP_FACTORY(OmnibusNoiseFilter, WireCell::Hip::OmnibusNoiseFilter)

@@ -5,6 +5,8 @@
 //   OmniChannelNoiseDB     sigproc/src/OmniChannelNoiseDB.cxx
 //   pdOneChannelNoise      sigproc/src/Protodune.cxx:712-923, every
 //                          channel of a frame in one device batch
+//   mbOneChannelNoise      sigproc/src/Microboone.cxx:894-1069, the same
+//   pdRelGainCalib         sigproc/src/Protodune.cxx:925-1014
 //   OmnibusNoiseFilter     sigproc/src/OmnibusNoiseFilter.cxx
 #pragma once
 #include <complex>
@@ -28,6 +30,7 @@ class IChannelNoiseDatabase : public IComponent<IChannelNoiseDatabase> {
     virtual double min_rms_cut(int channel) const = 0;
     virtual double max_rms_cut(int channel) const = 0;
     virtual const filter_t& rcrc(int channel) const = 0;
+    virtual const filter_t& config(int channel) const = 0;
     virtual const filter_t& noise(int channel) const = 0;
     virtual const std::vector<channel_group_t> coherent_channels() const = 0;
     virtual std::vector<int> bad_channels() const = 0;

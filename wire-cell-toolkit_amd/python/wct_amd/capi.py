@@ -56,10 +56,12 @@ class ImpactFrame(C.Structure):
 
 class NfChannel(C.Structure):
     _fields_ = [("plane", C.c_int), ("rcrc", C.c_int), ("bins_begin", C.c_int), ("bins_end", C.c_int),
-                ("min_rms", C.c_float), ("max_rms", C.c_float)]
+                ("min_rms", C.c_float), ("max_rms", C.c_float), ("mode", C.c_int), ("baseline", C.c_float),
+                ("gain", C.c_float), ("config", C.c_int), ("noise", C.c_int)]
 
 
-NF_PARTIAL, NF_NOISY, NF_HARMONIC = 1, 2, 4
+NF_PARTIAL, NF_NOISY, NF_HARMONIC, NF_CHIRP = 1, 2, 4, 8
+NF_PROTODUNE, NF_MICROBOONE = 0, 1
 
 
 # name -> (restype, argtypes)
@@ -131,6 +133,7 @@ _SIGS = {
     "wct_nf_rcrc_spectrum": (_I, [_D, _D, _I, _I, _V, _V]),
     "wct_nf_workspace_size": (_Z, [_I, _I]),
     "wct_nf_one_channel": (_I, [_V, _I, _V, _I, _I, _I, _V, _V, _V, _V, _V, _Z, _V]),
+    "wct_nf_rel_gain": (_I, [_V, _I, _I, _I, _V, _F, _F, _V]),
 }
 
 # Every symbol include/wct_hip.h declares (checked by the CPU tests).

@@ -1,13 +1,14 @@
-"""The ProtoDUNE per-channel noise filter (Protodune::OneChannelNoise,
-sigproc/src/Protodune.cxx:753-918) through the C ABI, for every channel
-of a frame at once.  Device buffers are torch tensors used as plain
-device memory.
+"""The per-channel noise filters (Protodune::OneChannelNoise,
+sigproc/src/Protodune.cxx:753-918; Microboone::OneChannelNoise,
+sigproc/src/Microboone.cxx:902-1064; Protodune::RelGainCalib,
+Protodune.cxx:975-1009) through the C ABI, for every channel of a frame at
+once.  Device buffers are torch tensors used as plain device memory.
 """
 import ctypes as C
 
 import numpy as np
 
-from .capi import NF_HARMONIC, NF_NOISY, NF_PARTIAL, NfChannel, check, lib
+from .capi import NF_CHIRP, NF_HARMONIC, NF_MICROBOONE, NF_NOISY, NF_PARTIAL, NF_PROTODUNE, NfChannel, check, lib
 
 
 def _stream():
@@ -39,9 +40,12 @@ def flagged_bins(noise):
 
 def one_channel(frame, channels, rcrc=None, stream=None):
     """frame: (nch, nt) float32 samples (numpy or CUDA tensor); channels:
-    one dict per row with plane, rcrc (row of the rcrc table or -1), bins
-    (flagged (first, last) runs), min_rms, max_rms; rcrc: (nr, nt)
-    complex64 table.  Returns (filtered CUDA tensor, flags numpy)."""
+    one dict per row with plane, mode ("protodune" or "microboone"), rcrc
+    (row of the spectra table or -1), bins (ProtoDUNE: flagged (first,
+    last) runs), min_rms, max_rms, and for MicroBooNE baseline, gain,
+    config and noise_row (rows of the spectra table or -1); rcrc: the
+    (nr, nt) complex64 spectra table.  Returns (filtered CUDA tensor,
+    flags (nch, 3) numpy: bits, chirp begin, chirp end)."""
     import torch
     x = torch.as_tensor(frame, dtype=torch.float32, device="cuda").contiguous()
     nch, nt = x.shape
@@ -50,8 +54,12 @@ def one_channel(frame, channels, rcrc=None, stream=None):
     pairs = []
     for i, c in enumerate(channels):
         b = list(c.get("bins", []))
+        mode = c.get("mode", "protodune")
+        mode = NF_MICROBOONE if mode in ("microboone", NF_MICROBOONE) else NF_PROTODUNE
         entries[i] = NfChannel(int(c.get("plane", 0)), int(c.get("rcrc", -1)), len(pairs), len(pairs) + len(b),
-                               float(c.get("min_rms", 0.5)), float(c.get("max_rms", 10.0)))
+                               float(c.get("min_rms", 0.5)), float(c.get("max_rms", 10.0)), mode,
+                               float(c.get("baseline", 0.0)), float(c.get("gain", 1.0)), int(c.get("config", -1)),
+                               int(c.get("noise_row", -1)))
         pairs.extend(b)
     dch = torch.frombuffer(bytearray(bytes(entries)), dtype=torch.uint8).to("cuda")
     dbins = torch.as_tensor(np.asarray(pairs, np.int32).reshape(-1), device="cuda") if pairs else None
@@ -59,7 +67,7 @@ def one_channel(frame, channels, rcrc=None, stream=None):
     if rcrc is not None and len(rcrc):
         drc = torch.as_tensor(np.ascontiguousarray(rcrc, np.complex64), device="cuda")
     out = torch.empty_like(x)
-    flags = torch.zeros(nch, dtype=torch.int32, device="cuda")
+    flags = torch.zeros((nch, 3), dtype=torch.int32, device="cuda")
     ws_bytes = lib().wct_nf_workspace_size(nch, nt)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
     check(lib().wct_nf_one_channel(
@@ -69,13 +77,43 @@ def one_channel(frame, channels, rcrc=None, stream=None):
     return out, flags.cpu().numpy()
 
 
+def masks(flag, plane, nt, mode="protodune"):
+    """The mask map one channel's flags (bits, chirp begin, chirp end)
+    stand for: {name: [(first, second), ...]}, as Protodune.cxx:846-915
+    and Microboone.cxx:924-1061 fill it."""
+    bits, beg, end = (int(v) for v in flag)
+    out = {}
+    if mode in ("microboone", NF_MICROBOONE):
+        if bits & NF_CHIRP:
+            out["chirp"] = [(beg, end)]
+            if plane != 2 and (beg > 0 or end < nt):
+                out.setdefault("lf_noisy", []).append((beg, end))
+        if bits & NF_PARTIAL and plane != 2:
+            out.setdefault("lf_noisy", []).append((0, nt))
+        if bits & NF_NOISY:
+            out["noisy"] = [(0, nt)]
+            out.pop("lf_noisy", None)
+        return out
+    if bits & NF_HARMONIC:
+        out["harmonic"] = [(0, nt)]
+    if bits & NF_PARTIAL and plane != 2:
+        out["lf_noisy"] = [(0, nt)]
+    if bits & NF_NOISY:
+        out["noisy"] = [(0, nt)]
+    return out
+
+
 def mask_names(flag, plane):
-    """The ChannelMaskMap names one channel's flags stand for."""
-    names = []
-    if flag & NF_HARMONIC:
-        names.append("harmonic")
-    if flag & NF_PARTIAL and plane != 2:
-        names.append("lf_noisy")
-    if flag & NF_NOISY:
-        names.append("noisy")
-    return names
+    """Names of the masks of a ProtoDUNE channel's flag bits."""
+    return sorted(masks((flag, -1, -1), plane, 0))
+
+
+def rel_gain(frame, gains, gain_min=0.8, gain_max=1.25, stream=None):
+    """Protodune::RelGainCalib on every row (in place on a CUDA tensor)."""
+    import torch
+    x = torch.as_tensor(frame, dtype=torch.float32, device="cuda").contiguous()
+    g = torch.as_tensor(np.asarray(gains, np.float32), device="cuda")
+    nch, nt = x.shape
+    check(lib().wct_nf_rel_gain(C.c_void_p(x.data_ptr()), nt, nch, nt, C.c_void_p(g.data_ptr()), float(gain_min),
+                                float(gain_max), _stream() if stream is None else stream))
+    return x
