@@ -433,3 +433,219 @@ def rel_gain_calib(signal, gain_fac, gain_def=1.0, gain_min=0.8, gain_max=1.25):
     if g > F(gain_max) or g < F(gain_min):
         return sig
     return (g * (sig - F(0))).astype(F) + F(0)
+
+
+# ---- MicroBooNE's coherent noise subtraction (grouped filter) -----------
+
+def filter_time(freq):
+    """Microboone.cxx:33-38."""
+    return (freq > 0) * math.exp(-0.5 * (freq / 0.143555) ** 4.95096)
+
+
+def filter_low(freq, cut_off=0.08):
+    """Microboone.cxx:40-49."""
+    if (0.177 < freq < 0.18) or (0.2143 < freq < 0.215) or (0.106 <= freq <= 0.109) or (0.25 < freq < 0.251):
+        return 0.0
+    return 1 - math.exp(-((freq / cut_off) ** 8))
+
+
+def filter_low_loose(freq):
+    """Microboone.cxx:51."""
+    return 1 - math.exp(-((freq / 0.005) ** 2))
+
+
+def _freq(i, n):
+    return i / (1.0 * n) * 2.0 if i < n / 2.0 else (n - i) / (1.0 * n) * 2.0
+
+
+def calc_rms(signal):
+    """Derivations::CalcRMS (sigproc/src/Derivations.cxx:7-21): mean and RMS
+    of the samples within 4.5 RMS of the mean."""
+    mean, rms = mean_rms(signal)
+    keep = [x for x in np.asarray(signal, dtype=F) if abs(float(x) - mean) < 4.5 * rms]
+    return mean_rms(keep)
+
+
+def calc_median(signals):
+    """Derivations::CalcMedian (:23-69): per tick, the binned median of the
+    channels' samples within 5 (float) mean RMS and above 0.001 in size."""
+    max_rms, count = F(0), F(0)
+    for s in signals:
+        r = mean_rms(s)[1]
+        if r > 0:
+            max_rms = F(float(max_rms) + r)
+            count = F(count + F(1))
+    if count > 0:
+        max_rms = F(max_rms / count)
+    lim = F(F(5) * max_rms)
+    arr = np.stack([np.asarray(s, dtype=F) for s in signals])
+    med = np.zeros(arr.shape[1], dtype=F)
+    for t in range(arr.shape[1]):
+        col = [v for v in arr[:, t] if abs(v) < lim and abs(float(v)) > 0.001]
+        med[t] = median_binned(col) if col else F(0)
+    return med
+
+
+def _decon(sig, respec, lowfilt):
+    spec = fwd_r2c(sig).astype(np.complex128) / np.asarray(respec, np.complex128)
+    spec = spec.astype(np.complex64)
+    n = len(spec)
+    fac = np.array([F(filter_time(_freq(i, n)) * lowfilt(_freq(i, n))) for i in range(n)], dtype=np.complex64)
+    return inv_c2r((spec * fac).astype(np.complex64))
+
+
+def _uses_response(respec, res_offset):
+    return respec is not None and len(respec) > 0 and (np.real(respec[0]) != 1 or np.imag(respec[0]) != 0) \
+        and res_offset != 0
+
+
+def _runs(flags):
+    rois, inside = [], False
+    for i, f in enumerate(flags):
+        if f:
+            if inside:
+                rois[-1].append(i)
+            else:
+                rois.append([i])
+                inside = True
+        else:
+            inside = False
+    return rois
+
+
+def _interp_rois(vals, rois, which):
+    n = len(vals)
+    for roi in rois:
+        if not which(roi):
+            continue
+        b0, bf = max(roi[0] - 1, 0), min(roi[-1] + 1, n - 1)
+        m0, mf = float(vals[b0]), float(vals[bf])
+        run, rise = float(bf - b0), mf - m0
+        for b in roi:
+            vals[b] = F(m0 + (b - b0) / run * rise)
+    return vals
+
+
+def signal_protection(medians, respec, res_offset, pad_f, pad_b, upper_decon_limit, decon_lf_cutoff,
+                      upper_adc_limit, protection_factor, min_adc_limit):
+    """Microboone::SignalProtection (Microboone.cxx:276-462); medians
+    changed in place; returns the ROIs (lists of ticks)."""
+    n = len(medians)
+    flags = [False] * n
+    mean, rms = calc_rms(medians)
+
+    def mark(j):
+        flags[j] = True
+        for k in range(pad_b):
+            flags[min(j + k + 1, n - 1)] = True
+        for k in range(pad_f):
+            flags[max(j - k - 1, 0)] = True
+
+    pf = float(F(protection_factor))
+    limit = F(pf * rms) if pf * rms > float(F(upper_adc_limit)) else F(upper_adc_limit)
+    if F(min_adc_limit) < limit:
+        limit = F(min_adc_limit)
+    for j in range(n):
+        if abs(float(medians[j]) - mean) > float(limit):
+            mark(j)
+    use = _uses_response(respec, res_offset)
+    if use:
+        dec = _decon(medians, respec, lambda f: filter_low(f, float(F(decon_lf_cutoff))))
+        mean, rms = calc_rms(dec)
+        limit = F(pf * rms) if pf * rms > float(F(upper_decon_limit)) else F(upper_decon_limit)
+        for j in range(n):
+            if float(dec[j]) - mean > float(limit):
+                tb = j + res_offset
+                if tb >= n:
+                    tb -= n
+                mark(tb)
+    rois = _runs(flags)
+    _interp_rois(medians, rois, lambda roi: not use)
+    return rois
+
+
+def subtract_wscaling(signals, medians, respec, res_offset, rois, decon_limit1, roi_min_max_ratio,
+                      rms_threshold=0.0):
+    """Microboone::Subtract_WScaling (Microboone.cxx:53-274); signals (in
+    channel order) changed in place."""
+    n = len(medians)
+    coefs = []
+    ave, cnt = 0.0, 0.0
+    for s in signals:
+        r = calc_rms(s)[1]
+        s2 = s3 = 0.0
+        for j in range(n):
+            if abs(float(s[j])) < 4 * r:
+                s2 += float(F(s[j] * medians[j]))
+                s3 += float(F(medians[j] * medians[j]))
+        c = s2 / s3 if s3 > 0 else 0.0
+        coefs.append(c)
+        if c != 0:
+            ave += c
+            cnt += 1
+    if cnt > 0:
+        ave /= cnt
+    use = _uses_response(respec, res_offset)
+    lim1 = float(F(decon_limit1))
+    for i, s in enumerate(signals):
+        if ave != 0:
+            sc = F(coefs[i] / ave)
+            sc = max(sc, F(0))
+            sc = min(sc, F(1.5))
+        else:
+            sc = F(0)
+        sub = medians
+        if use:
+            if rms_threshold:
+                roi_sig = np.asarray(s, dtype=F).copy()
+            else:
+                roi_sig = np.zeros(n, dtype=F)
+                for roi in rois:
+                    b0, bf = max(roi[0] - 1, 0), min(roi[-1] + 1, n - 1)
+                    m0, mf = float(s[b0]), float(s[bf])
+                    run, rise = float(bf - b0), mf - m0
+                    for b in roi:
+                        roi_sig[b] = F(float(s[b]) - (m0 + (b - b0) / run * rise))
+            dec = _decon(roi_sig, respec, filter_low_loose)
+            if rms_threshold:
+                m, r = calc_rms(dec)
+                dec = np.asarray([F(float(v) - m) for v in dec], dtype=F)
+                rl = float(F(rms_threshold)) * r
+                if 0 < rl < 0.02:
+                    rl = 0.02
+                if rl > lim1:
+                    rl = lim1
+                lim1 = float(F(rl))
+            good = set()
+            for roi in rois:
+                b0, bf = max(roi[0] - 1, 0), min(roi[-1] + 1, n - 1)
+                vals = []
+                for t in range(b0, bf + 1):
+                    tb = t - res_offset
+                    if tb < 0:
+                        tb += n
+                    if tb >= n:
+                        tb -= n
+                    vals.append(float(dec[tb]))
+                mx, mn = max(vals), min(vals)
+                if mx > lim1 and abs(mn) < mx * float(F(roi_min_max_ratio)):
+                    good.add(roi[0])
+            sub = _interp_rois(np.asarray(medians, dtype=F).copy(), rois, lambda roi: roi[0] in good)
+        for t in range(n):
+            if abs(float(s[t])) > 0.001:
+                s[t] = F(s[t] - F(sub[t] * sc))
+    return signals
+
+
+def coherent_noise_sub(signals, respec=None, res_offset=0, pad_f=0, pad_b=0, decon_limit=0.02,
+                       decon_lf_cutoff=0.08, adc_limit=0.0, decon_limit1=0.08, roi_min_max_ratio=0.8,
+                       protection_factor=5.0, min_adc_limit=50.0, rms_threshold=0.0):
+    """Microboone::CoherentNoiseSub::apply (Microboone.cxx:798-872) on one
+    group (signals in channel order, float32 arrays changed in place); the
+    DB values are those of the group's first channel."""
+    signals = [np.asarray(s, dtype=F).copy() for s in signals]
+    med = calc_median(signals)
+    rois = signal_protection(med, respec, res_offset, pad_f, pad_b, decon_limit, decon_lf_cutoff, adc_limit,
+                             protection_factor, min_adc_limit)
+    subtract_wscaling(signals, med, respec, res_offset, rois, decon_limit1, roi_min_max_ratio, rms_threshold)
+    return signals

@@ -270,3 +270,72 @@ def test_rel_gain_calib():
     np.testing.assert_array_equal(NF.rel_gain_calib(x, 1.1), (F(1.1) * x).astype(F))
     np.testing.assert_array_equal(NF.rel_gain_calib(x, 1.3), x)      # above the 1.25 cut: unchanged
     np.testing.assert_array_equal(NF.rel_gain_calib(x, 0.7), x)      # below 0.8
+
+
+def test_microboone_spectral_filters():
+    assert NF.filter_time(0.0) == 0.0
+    assert NF.filter_time(0.143555) == pytest.approx(math.exp(-0.5))
+    assert NF.filter_low(0.178) == 0.0 and NF.filter_low(0.107) == 0.0 and NF.filter_low(0.2145) == 0.0
+    assert NF.filter_low(0.08) == pytest.approx(1 - math.exp(-1))
+    assert NF.filter_low(0.05, 0.05) == pytest.approx(1 - math.exp(-1))
+    assert NF.filter_low_loose(0.005) == pytest.approx(1 - math.exp(-1))
+
+
+def test_calc_rms_and_median_hand_worked():
+    # CalcRMS drops samples beyond 4.5 RMS: 20 zeros and 100 give mean 4.76,
+    # RMS 21.3; 100 is 4.47 RMS out, so kept; with 40 zeros (RMS 15.4) it goes
+    x = np.zeros(41, F); x[0] = 100
+    assert NF.calc_rms(x) == (0.0, 0.0)
+    y = np.zeros(21, F); y[0] = 100
+    assert NF.calc_rms(y) == NF.mean_rms(y)
+    # per tick: [1, 2, 3] -> bins of 2/3 at 0, 2 (1.5 rounds up), 2
+    # (clamped); counts 1 0 2 pass int(3 * 0.5) = 1 at bin 2: 1 + 2 * 2/3
+    sig = [np.array([1.0, 5], F), np.array([2.0, 5], F), np.array([3.0, 0.0005], F)]
+    med = NF.calc_median(sig)
+    assert med[0] == F(F(1) + F(2) * F(F(2) / F(3)))
+    assert med[1] == 5     # 0.0005 is below 0.001 and left out; 5 is within 5 RMS
+    # samples beyond 5 mean-RMS are left out: mean RMS of the channels
+    # below is 0.5 (two of 0..1 alternating), so 3 is out
+    a = np.tile(np.array([0.0, 1.0], F), 50)
+    c = a.copy(); c[10] = 3
+    assert NF.calc_median([a, a, c])[10] == F(0)
+
+
+def test_signal_protection_without_response():
+    n = 200
+    med = np.zeros(n, F); med[100] = 80; med[[99, 101]] = 2
+    rois = NF.signal_protection(med, None, 0, pad_f=3, pad_b=4, upper_decon_limit=0.02, decon_lf_cutoff=0.08,
+                                upper_adc_limit=15, protection_factor=5, min_adc_limit=50)
+    # |80 - mean| > 15: ticks 97..104 protected and replaced by the line
+    # between ticks 96 and 105 (both 0)
+    assert rois == [list(range(97, 105))]
+    assert not med.any()
+
+
+def test_subtract_wscaling_removes_scaled_common_mode():
+    rng = np.random.default_rng(41)
+    n = 1500
+    common = (10 * np.sin(np.arange(n) / 30.0)).astype(F)
+    scales = [0.8, 1.0, 1.2, 1.0]
+    sig = [(s * common + rng.normal(0, 0.5, n)).astype(F) for s in scales]
+    med = NF.calc_median(sig)
+    out = NF.subtract_wscaling([x.copy() for x in sig], med, None, 0, [], 0.08, 0.8)
+    for s, o in zip(scales, out):
+        assert float(np.std(o)) < 1.0      # the common mode is gone, scaled per channel
+    # the scaling follows each channel's share of the common mode
+    x = NF.subtract_wscaling([sig[0].copy()] + [x.copy() for x in sig[1:]], med, None, 0, [], 0.08, 0.8)
+    assert abs(float(np.mean(x[0] * common)) / float(np.mean(common * common))) < 0.05
+
+
+def test_coherent_noise_sub_protects_signal():
+    rng = np.random.default_rng(42)
+    n, nch = 1500, 16
+    common = (8 * np.sin(np.arange(n) / 25.0)).astype(F)
+    sig = [(common + rng.normal(0, 1.0, n)).astype(F) for _ in range(nch)]
+    pulse = (300 * np.exp(-0.5 * ((np.arange(n) - 700) / 5.0) ** 2)).astype(F)
+    for c in range(nch):     # a track crossing every channel at the same time
+        sig[c] = (sig[c] + pulse).astype(F)
+    out = NF.coherent_noise_sub(sig, adc_limit=15, protection_factor=5, min_adc_limit=50, pad_f=10, pad_b=10)
+    for o in out:
+        assert float(np.std(o[:600])) < 1.5               # common mode removed away from the pulse
+        assert float(o[700]) > 250                         # the pulse itself survives (protected)
