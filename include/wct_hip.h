@@ -499,6 +499,29 @@ size_t wct_nf_workspace_size(int nch, int nt);
 int wct_nf_one_channel(const float* in, int ld_in, float* out, int ld_out, int nch, int nt,
                        const wct_nf_channel* chans, const float* spectra, const int* bins, int* flags,
                        void* workspace, size_t ws_bytes, void* stream);
+/* Microboone::CoherentNoiseSub::apply (sigproc/src/Microboone.cxx:798-872)
+ * on every channel group of a frame at once: per group the binned median
+ * over its channels (Derivations::CalcMedian), the signal protection
+ * (SignalProtection, :276-462), the scaled subtraction
+ * (Subtract_WScaling, :53-274).  A group lists rows [first, first +
+ * count) of group_rows (row indices into rows, in channel order, up to
+ * 128 per group; groups disjoint); row_group gives each row's group (-1:
+ * in none).  Per group, from the DB values of its first channel: respec
+ * (row of the spectra table, or -1 when the reference skips the response
+ * path: no response, a unit first bin or a zero offset), res_offset,
+ * pad_f / pad_b (pad_window_front / back) and the coherent_nf_* limits.
+ * rows are filtered in place.  rms_threshold is taken as 0 (the
+ * reference's default; its per-channel limit chain is not built). */
+typedef struct wct_nf_group {
+    int first, count;
+    int respec, res_offset, pad_f, pad_b;
+    float decon_limit, decon_lf_cutoff, adc_limit, decon_limit1, roi_min_max_ratio, protection_factor,
+        min_adc_limit;
+} wct_nf_group;
+size_t wct_nf_coherent_workspace_size(int ngroups, int nrows, int nt);
+int wct_nf_coherent(float* rows, int ld, int nrows, int nt, const wct_nf_group* groups, int ngroups,
+                    const int* group_rows, const int* row_group, const float* spectra, int any_response,
+                    void* workspace, size_t ws_bytes, void* stream);
 /* Protodune::RelGainCalib::apply (Protodune.cxx:975-1009) on nch rows in
  * place: row c scaled by gain[c] (float) when gain[c] lies within
  * [gain_min, gain_max]; gain is a device array. */

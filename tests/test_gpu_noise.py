@@ -333,3 +333,129 @@ def test_mb_component_chain():
         assert set(m) == set(chans), (name, set(m), set(chans))
     assert "noisefilter" in out.frame_tags() and list(out.tagged_traces("quiet")) == list(range(24))
     CO.reset()
+
+
+def _coherent_case(rng, nt, nch, with_response):
+    """One group: a common waveform on every channel (scaled per channel),
+    independent noise, and a pulse at the same tick on every channel (a
+    track parallel to the wire plane), through an induction-like response
+    when with_response."""
+    t = np.arange(nt)
+    common = 6 * np.sin(t / 23.0) + 3 * np.sin(t / 7.0)
+    resp = np.zeros(nt)
+    resp[100:140] = np.sin(np.linspace(0, 2 * np.pi, 40))          # bipolar, offset ~120
+    pulse = np.zeros(nt)
+    pulse[900] = 400.0
+    pulse = np.convolve(pulse, np.exp(-0.5 * (np.arange(-15, 16) / 3.0) ** 2), "same")
+    if with_response:
+        pulse = np.fft.ifft(np.fft.fft(pulse) * np.fft.fft(resp)).real
+    rows = []
+    for c in range(nch):
+        x = rng.uniform(0.8, 1.2) * common + rng.normal(0, 1.0, nt)
+        if c % 3 == 0:
+            x = x + pulse
+        rows.append(x.astype(np.float32))
+    respec = np.fft.fft(resp.astype(np.float32).astype(np.float64)).astype(np.complex64)
+    return np.stack(rows), respec
+
+
+def test_coherent_noise_sub_matches_oracle():
+    """Three disjoint groups in interleaved rows: two with the response
+    path (U-plane settings), one without (collection)."""
+    torch = _torch()
+    from wct_amd import noise as N
+    rng = np.random.default_rng(77)
+    nt = 2000
+    blocks, respecs = [], []
+    for k, resp in enumerate((True, True, False)):
+        rows, rs = _coherent_case(rng, nt, 16 + 8 * k, resp)
+        blocks.append(rows)
+        respecs.append(rs)
+    nrows = sum(len(b) for b in blocks)
+    order = rng.permutation(nrows)             # frame row of each (group, channel)
+    frame = np.zeros((nrows, nt), np.float32)
+    groups, pos = [], 0
+    spectra = np.stack([respecs[0], respecs[1]])
+    for k, b in enumerate(blocks):
+        idx = [int(order[pos + i]) for i in range(len(b))]
+        pos += len(b)
+        for i, r in enumerate(idx):
+            frame[r] = b[i]
+        p = dict(rows=idx, pad_f=10, pad_b=10, adc_limit=15.0, protection_factor=5.0, min_adc_limit=50.0,
+                 decon_limit=0.02, decon_limit1=0.07, roi_min_max_ratio=3.0)
+        if k < 2:
+            p.update(respec=k, res_offset=120)
+        groups.append(p)
+    print(f"coherent {len(groups)} groups, {nrows} rows x {nt}", flush=True)
+    got = N.coherent(frame, groups, spectra).cpu().numpy()
+    for k, grp in enumerate(groups):
+        want = NF.coherent_noise_sub([frame[r] for r in grp["rows"]],
+                                     respec=spectra[k] if k < 2 else None, res_offset=120 if k < 2 else 0,
+                                     pad_f=10, pad_b=10, decon_limit=0.02, decon_lf_cutoff=0.08, adc_limit=15.0,
+                                     decon_limit1=0.07, roi_min_max_ratio=3.0, protection_factor=5.0,
+                                     min_adc_limit=50.0)
+        for i, r in enumerate(grp["rows"]):
+            d = np.abs(got[r].astype(np.float64) - want[i])
+            assert d.max() <= 1e-3, (k, i, d.max(), int(d.argmax()))
+        # the common mode is gone and the pulse kept on the pulsed channels
+        for i, r in enumerate(grp["rows"]):
+            assert float(np.std(got[r][:600])) < 1.6
+    # rows of no group are untouched
+    assert np.array_equal(got.shape, frame.shape)
+
+
+def test_grouped_filter_component():
+    """OmnibusNoiseFilter with mbCoherentNoiseSub as a grouped filter over
+    the DB's groups (U/V groups with an explicit response waveform and
+    offset, W groups without), all groups in one batch: traces equal the
+    oracle's coherent subtraction group by group; a group with a missing
+    channel is left alone."""
+    _torch()
+    from wct_amd import components as CO
+    rng = np.random.default_rng(91)
+    nt = 2000
+    resp = np.zeros(nt)
+    resp[100:140] = np.sin(np.linspace(0, 2 * np.pi, 40))
+    rows = np.zeros((24, nt), np.float32)
+    for k in range(3):
+        r, _ = _coherent_case(rng, nt, 8, k < 2)
+        rows[8 * k:8 * (k + 1)] = r
+    groups = [list(range(0, 8)), list(range(8, 16)), list(range(16, 24)), [0, 1, 99]]   # the last one incomplete
+    common = {"pad_window_front": 10, "pad_window_back": 10, "adc_limit": 15, "protection_factor": 5.0,
+              "min_adc_limit": 50, "decon_limit": 0.02, "decon_limit1": 0.07, "roi_min_max_ratio": 3.0,
+              "min_rms_cut": 0.0, "max_rms_cut": 1000.0}
+    cfg = [
+        {"type": "AnodeChannels", "name": "ca", "data": {"ident": 0, "plane_ranges": [[0, 8], [8, 8], [16, 8]]}},
+        {"type": "OmniChannelNoiseDB", "name": "cdb", "data": {
+            "anode": "AnodeChannels:ca", "tick": TICK, "nsamples": nt, "bad": [], "groups": groups,
+            "channel_info": [
+                dict(common, channels={"first": 0, "last": 15}, response_offset=120,
+                     response={"waveform": [float(v) for v in resp[:200]], "waveformid": 1}),
+                dict(common, channels={"first": 16, "last": 23}),
+            ]}},
+        {"type": "mbCoherentNoiseSub", "name": "coh", "data": {"anode": "AnodeChannels:ca",
+                                                               "noisedb": "OmniChannelNoiseDB:cdb"}},
+        {"type": "OmnibusNoiseFilter", "name": "cnf", "data": {
+            "nticks": 0, "maskmap": {}, "channel_filters": [], "grouped_filters": ["mbCoherentNoiseSub:coh"],
+            "channel_status_filters": [], "noisedb": "OmniChannelNoiseDB:cdb", "intraces": "orig",
+            "outtraces": "quiet"}},
+    ]
+    CO.reset()
+    CO.configure(cfg)
+    fr = CO.Frame(ident=5, time=0.0, tick=TICK)
+    for ch in range(24):
+        fr.add_trace(ch, rows[ch], 0)
+    fr.tag_traces("orig", list(range(24)))
+    out = CO.frame_filter("OmnibusNoiseFilter:cnf", fr)
+    got = {ch: q for ch, tb, q in out.traces()}
+    respec = np.fft.fft(np.concatenate([resp[:200], np.zeros(nt - 200)]).astype(np.float32).astype(np.float64))
+    for k in range(3):
+        want = NF.coherent_noise_sub([rows[c] for c in range(8 * k, 8 * k + 8)],
+                                     respec=respec.astype(np.complex64) if k < 2 else None,
+                                     res_offset=120 if k < 2 else 0, pad_f=10, pad_b=10, decon_limit=0.02,
+                                     decon_lf_cutoff=0.08, adc_limit=15.0, decon_limit1=0.07, roi_min_max_ratio=3.0,
+                                     protection_factor=5.0, min_adc_limit=50.0)
+        for i in range(8):
+            d = np.abs(got[8 * k + i].astype(np.float64) - want[i])
+            assert d.max() <= 1e-3, (k, i, d.max())
+    CO.reset()

@@ -679,6 +679,436 @@ __global__ __launch_bounds__(kNfBlock) void k_nf_rel_gain(float* __restrict__ ro
     *p = v + 0.f;
 }
 
+// ---- MicroBooNE's coherent noise subtraction (Microboone.cxx:53-872) ----
+
+// Derivations::CalcRMS (sigproc/src/Derivations.cxx:7-21): mean_rms, then
+// mean_rms of the samples within 4.5 RMS of that mean.
+__device__ void calc_rms(const float* s, int n, double& mean, double& rms, NfShared& sh)
+{
+    double m0, r0;
+    mean_rms(s, n, m0, r0, sh);
+    double a = 0, b = 0;
+    int c = 0;
+    for (int i = threadIdx.x; i < n; i += kNfBlock) {
+        const double x = s[i];
+        if (fabs(x - m0) < 4.5 * r0) {
+            a += x;
+            b += x * x;
+            ++c;
+        }
+    }
+    a = block_sum(a, sh);
+    b = block_sum(b, sh);
+    c = block_sum(c, sh);
+    if (c == 0) {
+        mean = rms = 0;
+    } else if (c == 1) {
+        mean = a;
+        rms = 0;
+    } else {
+        mean = a / c;
+        rms = sqrt((b - a * a / c) / c);
+    }
+}
+
+// filter_time, filter_low, filter_low_loose (Microboone.cxx:31-51) at the
+// frequency of bin i of n (2 MHz digitisation, :164-171), as the float
+// factor the reference multiplies by.
+__device__ __forceinline__ double cn_freq(int i, int n)
+{
+    return i < n / 2. ? i / (1. * n) * 2. : (n - i) / (1. * n) * 2.;
+}
+__device__ __forceinline__ double cn_filter_time(double f)
+{
+    return (f > 0) * exp(-0.5 * pow(f / 0.143555, 4.95096));
+}
+__device__ __forceinline__ double cn_filter_low(double f, double cut)
+{
+    if ((f > 0.177 && f < 0.18) || (f > 0.2143 && f < 0.215) || (f >= 0.106 && f <= 0.109) || (f > 0.25 && f < 0.251))
+        return 0;
+    return 1 - exp(-pow(f / cut, 8));
+}
+__device__ __forceinline__ double cn_filter_low_loose(double f) { return 1 - exp(-pow(f / 0.005, 2)); }
+
+constexpr int kCnTicks = 128;        // ticks per median workgroup
+constexpr int kMaxGroupRows = 128;   // channels per coherent group
+
+// Per-channel mean_rms and CalcRMS (for the group's mean RMS and the
+// coefficient's 4 RMS window).
+__global__ __launch_bounds__(kNfBlock) void k_cn_rms(const float* __restrict__ rows, int ld, int n,
+                                                     double* __restrict__ rms0, double* __restrict__ crms)
+{
+    extern __shared__ __align__(16) unsigned char nf_smem[];
+    __shared__ NfShared sh;
+    float* s = reinterpret_cast<float*>(nf_smem);
+    const int c = blockIdx.x;
+    const auto src = G(rows + (long)c * ld);
+    for (int i = threadIdx.x; i < n; i += kNfBlock) s[i] = src[i];
+    __syncthreads();
+    double m, r;
+    mean_rms(s, n, m, r, sh);
+    double cm, cr;
+    calc_rms(s, n, cm, cr, sh);
+    if (threadIdx.x == 0) {
+        rms0[c] = r;
+        crms[c] = cr;
+    }
+}
+
+// Derivations::CalcMedian (:23-69) per group and tick: the group's mean
+// RMS accumulated in float in channel order, then per tick the binned
+// median of the samples within 5 of it and above 0.001 in size, found as
+// the int(m / 2)-th smallest bin index (the bin where the running count
+// first passes it).
+__global__ __launch_bounds__(kCnTicks) void k_cn_median(const float* __restrict__ rows, int ld, int n,
+                                                        const wct_nf_group* __restrict__ groups,
+                                                        const int* __restrict__ grows,
+                                                        const double* __restrict__ rms0, float* __restrict__ med)
+{
+    extern __shared__ __align__(16) unsigned char nf_smem[];
+    const int g = blockIdx.x;
+    const wct_nf_group grp = groups[g];
+    const int t = blockIdx.y * kCnTicks + threadIdx.x;
+    float maxrms = 0.f, count = 0.f;
+    for (int k = 0; k < grp.count; ++k) {
+        const double r = rms0[grows[grp.first + k]];
+        if (r > 0) {
+            maxrms = (float)(maxrms + r);
+            count = count + 1.f;
+        }
+    }
+    if (count > 0) maxrms = maxrms / count;
+    const float lim = 5 * maxrms;
+    float* tile = reinterpret_cast<float*>(nf_smem);   // [channel][tick]
+    int* bin = reinterpret_cast<int*>(nf_smem);
+    const bool tin = t < n;
+    for (int k = 0; k < grp.count; ++k)
+        tile[k * kCnTicks + threadIdx.x] = tin ? G(rows + (long)grows[grp.first + k] * ld)[t] : 0.f;
+    if (!tin) return;   // no barrier below: each thread owns its column
+    int m = 0;
+    float vmin = INFINITY, vmax = -INFINITY;
+    for (int k = 0; k < grp.count; ++k) {
+        const float v = tile[k * kCnTicks + threadIdx.x];
+        if (fabsf(v) < lim && fabs((double)v) > 0.001) {
+            ++m;
+            vmin = fminf(vmin, v);
+            vmax = fmaxf(vmax, v);
+        }
+    }
+    float out = 0.f;
+    if (m > 0) {
+        const float width = __fdiv_rn(vmax - vmin, (float)m);
+        for (int k = 0; k < grp.count; ++k) {
+            const float v = tile[k * kCnTicks + threadIdx.x];
+            int b = -1;
+            if (fabsf(v) < lim && fabs((double)v) > 0.001) {
+                const float r = __fdiv_rn(v - vmin, width);
+                b = (r != r) ? 0 : (int)roundf(r);
+                b = max(0, min(m - 1, b));
+            }
+            bin[k * kCnTicks + threadIdx.x] = b;
+        }
+        const int imed = (int)((float)m * 0.5f);
+        int ind = 0;
+        for (int i = 0; i < grp.count; ++i) {
+            const int bi = bin[i * kCnTicks + threadIdx.x];
+            if (bi < 0) continue;
+            int less = 0, le = 0;
+            for (int j = 0; j < grp.count; ++j) {
+                const int bj = bin[j * kCnTicks + threadIdx.x];
+                if (bj < 0) continue;
+                less += bj < bi;
+                le += bj <= bi;
+            }
+            if (less <= imed && imed < le) {
+                ind = bi;
+                break;
+            }
+        }
+        out = vmin + (float)ind * width;
+    }
+    G(med)[(long)g * n + t] = out;
+}
+
+// Run bounds of bin b in the marks (marked bins only).
+__device__ __forceinline__ void run_of(const unsigned char* mk, int n, int b, int& s, int& e)
+{
+    s = b;
+    while (s > 0 && mk[s - 1]) --s;
+    e = b;
+    while (e < n - 1 && mk[e + 1]) ++e;
+}
+
+// Marks of the samples beyond the limit, padded (:324-346, :388-410):
+// bin b is marked when a source lies in [b - pad_b, b + pad_f] (the
+// reference's padding clamps at both ends, which this covers).
+__device__ void cn_pad_marks(const unsigned char* src, unsigned char* mk, int n, int pad_f, int pad_b)
+{
+    for (int b = threadIdx.x; b < n; b += kNfBlock) {
+        bool any = mk[b] != 0;
+        for (int j = max(b - pad_b, 0); j <= min(b + pad_f, n - 1) && !any; ++j) any = src[j] != 0;
+        mk[b] = any;
+    }
+    __syncthreads();
+}
+
+// SignalProtection, first part (:276-346), one workgroup per group: the
+// limit from the medians' CalcRMS, the marks; groups without the response
+// path get their marked runs replaced by the line between the bins just
+// outside (:439-459).  Groups with it keep the medians and hand them to
+// the deconvolution (complex rows).
+__global__ __launch_bounds__(kNfBlock) void k_cn_protect(float* __restrict__ med, int n,
+                                                         const wct_nf_group* __restrict__ groups,
+                                                         unsigned char* __restrict__ marks,
+                                                         float2* __restrict__ grow_c)
+{
+    extern __shared__ __align__(16) unsigned char nf_smem[];
+    __shared__ NfShared sh;
+    const int g = blockIdx.x;
+    const wct_nf_group grp = groups[g];
+    float* m = reinterpret_cast<float*>(nf_smem);
+    unsigned char* src = nf_smem + ((size_t)4 * n + 15) / 16 * 16;
+    unsigned char* mk = src + n;
+    const auto row = G(med + (long)g * n);
+    for (int i = threadIdx.x; i < n; i += kNfBlock) m[i] = row[i];
+    __syncthreads();
+    double mean, rms;
+    calc_rms(m, n, mean, rms, sh);
+    const double pf = grp.protection_factor;
+    float limit = pf * rms > (double)grp.adc_limit ? (float)(pf * rms) : grp.adc_limit;
+    if (grp.min_adc_limit < limit) limit = grp.min_adc_limit;
+    for (int j = threadIdx.x; j < n; j += kNfBlock) {
+        src[j] = fabs((double)m[j] - mean) > (double)limit;
+        mk[j] = 0;
+    }
+    __syncthreads();
+    cn_pad_marks(src, mk, n, grp.pad_f, grp.pad_b);
+    for (int j = threadIdx.x; j < n; j += kNfBlock) G(marks + (long)g * n)[j] = mk[j];
+    if (grp.respec >= 0) {
+        for (int j = threadIdx.x; j < n; j += kNfBlock) G(grow_c + (long)g * n)[j] = make_float2(m[j], 0.f);
+        return;
+    }
+    for (int b = threadIdx.x; b < n; b += kNfBlock) {
+        if (!mk[b]) continue;
+        int s0, e0;
+        run_of(mk, n, b, s0, e0);
+        const int b0 = max(s0 - 1, 0), bf = min(e0 + 1, n - 1);
+        const double m0 = m[b0], mf = m[bf];
+        row[b] = (float)(m0 + (b - b0) / (double)(bf - b0) * (mf - m0));
+    }
+}
+
+// A spectrum row divided by the response, times a float filter factor,
+// then DftTools::inv_c2r's mirror (bins above n/2 from the lower half).
+__device__ void cn_decon_spectrum(float2* rowp, int n, const float2* respec, bool loose, double cut)
+{
+    const auto row = G(rowp);
+    const auto R = G(respec);
+    for (int k = threadIdx.x; k <= n / 2; k += kNfBlock) {
+        const double f = cn_freq(k, n);
+        const float fac = (float)(cn_filter_time(f) * (loose ? cn_filter_low_loose(f) : cn_filter_low(f, cut)));
+        float2 x = cdiv(row[k], R[k]);
+        x = make_float2(x.x * fac, x.y * fac);
+        if (k == 0) x.y = 0.f;
+        if (2 * k == n) x.y = 0.f;
+        row[k] = x;
+        if (k > 0 && 2 * k != n) row[n - k] = make_float2(x.x, -x.y);
+    }
+}
+
+__global__ __launch_bounds__(kNfBlock) void k_cn_median_decon(float2* __restrict__ grow_c, int n,
+                                                              const wct_nf_group* __restrict__ groups,
+                                                              const float2* __restrict__ spectra)
+{
+    const wct_nf_group grp = groups[blockIdx.x];
+    if (grp.respec < 0) return;
+    cn_decon_spectrum(grow_c + (long)blockIdx.x * n, n, spectra + (long)grp.respec * n, false, grp.decon_lf_cutoff);
+}
+
+// SignalProtection, the deconvolution part (:348-411): marks where the
+// deconvolved medians exceed their limit, shifted by the response offset.
+__global__ __launch_bounds__(kNfBlock) void k_cn_protect_decon(const float2* __restrict__ grow_c, int n,
+                                                               const wct_nf_group* __restrict__ groups,
+                                                               unsigned char* __restrict__ marks)
+{
+    extern __shared__ __align__(16) unsigned char nf_smem[];
+    __shared__ NfShared sh;
+    const int g = blockIdx.x;
+    const wct_nf_group grp = groups[g];
+    if (grp.respec < 0) return;
+    float* d = reinterpret_cast<float*>(nf_smem);
+    unsigned char* src = nf_smem + ((size_t)4 * n + 15) / 16 * 16;
+    unsigned char* mk = src + n;
+    for (int i = threadIdx.x; i < n; i += kNfBlock) {
+        d[i] = G(grow_c + (long)g * n)[i].x;
+        mk[i] = G(marks + (long)g * n)[i];
+        src[i] = 0;
+    }
+    __syncthreads();
+    double mean, rms;
+    calc_rms(d, n, mean, rms, sh);
+    const double pf = grp.protection_factor;
+    const float limit = pf * rms > (double)grp.decon_limit ? (float)(pf * rms) : grp.decon_limit;
+    for (int j = threadIdx.x; j < n; j += kNfBlock) {
+        if ((double)d[j] - mean > (double)limit) {
+            int tb = j + grp.res_offset;
+            if (tb >= n) tb -= n;
+            if (tb >= 0 && tb < n) src[tb] = 1;
+        }
+    }
+    __syncthreads();
+    cn_pad_marks(src, mk, n, grp.pad_f, grp.pad_b);
+    for (int j = threadIdx.x; j < n; j += kNfBlock) G(marks + (long)g * n)[j] = mk[j];
+}
+
+// Subtract_WScaling's coefficient per channel (:67-96): sums over the
+// samples within 4 CalcRMS of zero, products in float.
+__global__ __launch_bounds__(kNfBlock) void k_cn_coef(const float* __restrict__ rows, int ld, int n,
+                                                      const int* __restrict__ row_group,
+                                                      const float* __restrict__ med,
+                                                      const double* __restrict__ crms, double* __restrict__ coef)
+{
+    __shared__ NfShared sh;
+    const int c = blockIdx.x;
+    const int g = row_group[c];
+    if (g < 0) return;
+    const auto s = G(rows + (long)c * ld);
+    const auto m = G(med + (long)g * n);
+    const double r4 = 4 * crms[c];
+    double a = 0, b = 0;
+    for (int j = threadIdx.x; j < n; j += kNfBlock) {
+        const float x = s[j], y = m[j];
+        if (fabsf(x) < r4) {
+            a += x * y;
+            b += y * y;
+        }
+    }
+    a = block_sum(a, sh);
+    b = block_sum(b, sh);
+    if (threadIdx.x == 0) coef[c] = b > 0 ? a / b : 0.0;
+}
+
+// The group's mean coefficient and each channel's scaling (:97-123).
+__global__ void k_cn_scaling(const wct_nf_group* __restrict__ groups, int ngroups, const int* __restrict__ grows,
+                             const double* __restrict__ coef, float* __restrict__ scaling)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const wct_nf_group grp = groups[g];
+    double ave = 0, cnt = 0;
+    for (int k = 0; k < grp.count; ++k) {
+        const double c = coef[grows[grp.first + k]];
+        if (c != 0) {
+            ave += c;
+            cnt += 1;
+        }
+    }
+    if (cnt > 0) ave /= cnt;
+    for (int k = 0; k < grp.count; ++k) {
+        const int r = grows[grp.first + k];
+        float sc = 0.f;
+        if (ave != 0) {
+            sc = (float)(coef[r] / ave);
+            if (sc < 0) sc = 0;
+            if (sc > 1.5f) sc = 1.5f;
+        }
+        scaling[r] = sc;
+    }
+}
+
+// The channel's samples in the group's marked runs with the line between
+// the bins just outside subtracted, zero elsewhere (:130-145), as a
+// complex row.
+__global__ __launch_bounds__(kNfBlock) void k_cn_roi_rows(const float* __restrict__ rows, int ld, int n,
+                                                          const int* __restrict__ row_group,
+                                                          const wct_nf_group* __restrict__ groups,
+                                                          const unsigned char* __restrict__ marks,
+                                                          float2* __restrict__ crow)
+{
+    const int c = blockIdx.x;
+    const int g = row_group[c];
+    if (g < 0 || groups[g].respec < 0) return;
+    const auto s = G(rows + (long)c * ld);
+    const auto mk = G(marks + (long)g * n);
+    const auto dst = G(crow + (long)c * n);
+    for (int b = threadIdx.x; b < n; b += kNfBlock) {
+        float v = 0.f;
+        if (mk[b]) {
+            int s0 = b, e0 = b;
+            while (s0 > 0 && mk[s0 - 1]) --s0;
+            while (e0 < n - 1 && mk[e0 + 1]) ++e0;
+            const int b0 = max(s0 - 1, 0), bf = min(e0 + 1, n - 1);
+            const double m0 = s[b0], mf = s[bf];
+            v = (float)(s[b] - (m0 + (b - b0) / (double)(bf - b0) * (mf - m0)));
+        }
+        dst[b] = make_float2(v, 0.f);
+    }
+}
+
+__global__ __launch_bounds__(kNfBlock) void k_cn_roi_decon(float2* __restrict__ crow, int n,
+                                                           const int* __restrict__ row_group,
+                                                           const wct_nf_group* __restrict__ groups,
+                                                           const float2* __restrict__ spectra)
+{
+    const int c = blockIdx.x;
+    if (row_group[c] < 0) return;
+    const wct_nf_group grp = groups[row_group[c]];
+    if (grp.respec < 0) return;
+    cn_decon_spectrum(crow + (long)c * n, n, spectra + (long)grp.respec * n, true, 0.0);
+}
+
+// The subtraction (:124-263): with the response path, the median's runs
+// whose deconvolved channel (shifted back by the offset) peaks above
+// decon_limit1 without a comparable dip are replaced by the line between
+// the bins just outside; samples above 0.001 in size lose the scaled
+// median.
+__global__ __launch_bounds__(kNfBlock) void k_cn_subtract(float* __restrict__ rows, int ld, int n,
+                                                          const int* __restrict__ row_group,
+                                                          const wct_nf_group* __restrict__ groups,
+                                                          const unsigned char* __restrict__ marks,
+                                                          const float* __restrict__ med,
+                                                          const float2* __restrict__ crow,
+                                                          const float* __restrict__ scaling)
+{
+    const int c = blockIdx.x;
+    const int g = row_group[c];
+    if (g < 0) return;
+    const wct_nf_group grp = groups[g];
+    const auto s = G(rows + (long)c * ld);
+    const auto m = G(med + (long)g * n);
+    const auto mk = G(marks + (long)g * n);
+    const float sc = scaling[c];
+    const bool resp = grp.respec >= 0;
+    for (int b = threadIdx.x; b < n; b += kNfBlock) {
+        float mv = m[b];
+        if (resp && mk[b]) {
+            int s0 = b, e0 = b;
+            while (s0 > 0 && mk[s0 - 1]) --s0;
+            while (e0 < n - 1 && mk[e0 + 1]) ++e0;
+            const int b0 = max(s0 - 1, 0), bf = min(e0 + 1, n - 1);
+            double mx = 0, mn = 0;
+            for (int i = b0; i <= bf; ++i) {
+                int tb = i - grp.res_offset;
+                if (tb < 0) tb += n;
+                if (tb >= n) tb -= n;
+                const double v = G(crow + (long)c * n)[tb].x;
+                if (i == b0) {
+                    mx = mn = v;
+                } else {
+                    mx = fmax(mx, v);
+                    mn = fmin(mn, v);
+                }
+            }
+            if (mx > (double)grp.decon_limit1 && fabs(mn) < mx * (double)grp.roi_min_max_ratio) {
+                const double m0 = m[b0], mf = m[bf];
+                mv = (float)(m0 + (b - b0) / (double)(bf - b0) * (mf - m0));
+            }
+        }
+        const float x = s[b];
+        if (fabs((double)x) > 0.001) s[b] = x - mv * sc;
+    }
+}
+
 }  // namespace
 
 }  // namespace wct
@@ -760,6 +1190,92 @@ extern "C" int wct_nf_rel_gain(float* rows, int ld, int nch, int nt, const float
     const long total = (long)nch * nt;
     hipLaunchKernelGGL(k_nf_rel_gain, dim3((unsigned)((total + kNfBlock - 1) / kNfBlock)), dim3(kNfBlock), 0,
                        as_stream(stream), rows, ld, nch, nt, gain, gain_min, gain_max);
+    WCT_HIP_CHECK(hipGetLastError());
+    return WCT_OK;
+}
+
+extern "C" size_t wct_nf_coherent_workspace_size(int ngroups, int nrows, int nt)
+{
+    if (ngroups <= 0 || nrows <= 0 || nt <= 0) return 0;
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    return al(sizeof(float) * (size_t)ngroups * nt)          // medians
+           + al((size_t)ngroups * nt)                         // marks
+           + al(sizeof(float2) * (size_t)ngroups * nt)        // group rows (complex)
+           + al(sizeof(float2) * (size_t)nrows * nt)          // channel rows (complex)
+           + al(sizeof(double) * 3 * (size_t)nrows)           // rms, CalcRMS, coefficients
+           + al(sizeof(float) * (size_t)nrows)                // scaling
+           + al(sizeof(int) * (size_t)nrows);                 // group of each row
+}
+
+extern "C" int wct_nf_coherent(float* rows, int ld, int nrows, int nt, const wct_nf_group* groups, int ngroups,
+                               const int* group_rows, const int* row_group, const float* spectra, int any_response,
+                               void* workspace, size_t ws_bytes, void* stream)
+{
+    WCT_ENTRY();
+    WCT_REQUIRE(rows && groups && group_rows && row_group && nrows >= 1 && ngroups >= 1 && nt >= 32 && ld >= nt,
+                WCT_ERR_ARG, "nf coherent: bad arguments");
+    WCT_REQUIRE(!any_response || spectra, WCT_ERR_ARG, "nf coherent: response spectra missing");
+    WCT_REQUIRE(workspace && ws_bytes >= wct_nf_coherent_workspace_size(ngroups, nrows, nt), WCT_ERR_ARG,
+                "nf coherent: workspace of %zu bytes, %zu needed", ws_bytes,
+                wct_nf_coherent_workspace_size(ngroups, nrows, nt));
+    hipStream_t s = as_stream(stream);
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    unsigned char* p = reinterpret_cast<unsigned char*>(workspace);
+    float* med = reinterpret_cast<float*>(p);
+    p += al(sizeof(float) * (size_t)ngroups * nt);
+    unsigned char* marks = p;
+    p += al((size_t)ngroups * nt);
+    float2* grow_c = reinterpret_cast<float2*>(p);
+    p += al(sizeof(float2) * (size_t)ngroups * nt);
+    float2* crow = reinterpret_cast<float2*>(p);
+    p += al(sizeof(float2) * (size_t)nrows * nt);
+    double* rms0 = reinterpret_cast<double*>(p);
+    double* crms = rms0 + nrows;
+    double* coef = crms + nrows;
+    p += al(sizeof(double) * 3 * (size_t)nrows);
+    float* scaling = reinterpret_cast<float*>(p);
+    // every row's group (row_group: -1 for rows in no group)
+    const size_t lds_row = ((size_t)4 * nt + 15) / 16 * 16 + 2 * (size_t)nt + 16;
+    WCT_REQUIRE((long)lds_row + (long)sizeof(NfShared) <= lds_limit_bytes(), WCT_ERR_UNSUPPORTED,
+                "nf coherent: %d ticks need %zu bytes of LDS", nt, lds_row);
+    WCT_TRY(allow_lds(k_cn_rms, lds_row));
+    hipLaunchKernelGGL(k_cn_rms, dim3(nrows), dim3(kNfBlock), lds_row, s, rows, ld, nt, rms0, crms);
+    WCT_HIP_CHECK(hipGetLastError());
+    const size_t lds_med = sizeof(float) * kCnTicks * kMaxGroupRows;
+    WCT_TRY(allow_lds(k_cn_median, lds_med));
+    hipLaunchKernelGGL(k_cn_median, dim3(ngroups, (nt + kCnTicks - 1) / kCnTicks), dim3(kCnTicks), lds_med, s, rows,
+                       ld, nt, groups, group_rows, rms0, med);
+    WCT_HIP_CHECK(hipGetLastError());
+    WCT_TRY(allow_lds(k_cn_protect, lds_row));
+    hipLaunchKernelGGL(k_cn_protect, dim3(ngroups), dim3(kNfBlock), lds_row, s, med, nt, groups, marks, grow_c);
+    WCT_HIP_CHECK(hipGetLastError());
+    if (any_response) {
+        WCT_TRY(launch_rows<float>(grow_c, grow_c, nt, ngroups, nt, -1, 1.0f, s));
+        hipLaunchKernelGGL(k_cn_median_decon, dim3(ngroups), dim3(kNfBlock), 0, s, grow_c, nt, groups,
+                           reinterpret_cast<const float2*>(spectra));
+        WCT_HIP_CHECK(hipGetLastError());
+        WCT_TRY(launch_rows<float>(grow_c, grow_c, nt, ngroups, nt, +1, 1.0f / nt, s));
+        WCT_TRY(allow_lds(k_cn_protect_decon, lds_row));
+        hipLaunchKernelGGL(k_cn_protect_decon, dim3(ngroups), dim3(kNfBlock), lds_row, s, grow_c, nt, groups, marks);
+        WCT_HIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_cn_coef, dim3(nrows), dim3(kNfBlock), 0, s, rows, ld, nt, row_group, med, crms, coef);
+    WCT_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_cn_scaling, dim3((ngroups + 63) / 64), dim3(64), 0, s, groups, ngroups, group_rows, coef,
+                       scaling);
+    WCT_HIP_CHECK(hipGetLastError());
+    if (any_response) {
+        hipLaunchKernelGGL(k_cn_roi_rows, dim3(nrows), dim3(kNfBlock), 0, s, rows, ld, nt, row_group, groups, marks,
+                           crow);
+        WCT_HIP_CHECK(hipGetLastError());
+        WCT_TRY(launch_rows<float>(crow, crow, nt, nrows, nt, -1, 1.0f, s));
+        hipLaunchKernelGGL(k_cn_roi_decon, dim3(nrows), dim3(kNfBlock), 0, s, crow, nt, row_group, groups,
+                           reinterpret_cast<const float2*>(spectra));
+        WCT_HIP_CHECK(hipGetLastError());
+        WCT_TRY(launch_rows<float>(crow, crow, nt, nrows, nt, +1, 1.0f / nt, s));
+    }
+    hipLaunchKernelGGL(k_cn_subtract, dim3(nrows), dim3(kNfBlock), 0, s, rows, ld, nt, row_group, groups, marks, med,
+                       crow, scaling);
     WCT_HIP_CHECK(hipGetLastError());
     return WCT_OK;
 }

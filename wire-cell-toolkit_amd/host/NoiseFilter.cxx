@@ -103,7 +103,7 @@ class OmniChannelNoiseDB : public IChannelNoiseDatabase, public IConfigurable {
         int pad_window_front{0}, pad_window_back{0};
         float decon_limit{0.02f}, decon_lf_cutoff{0.08f}, adc_limit{0.0f}, decon_limit1{0.08f};
         float protection_factor{5.0f}, min_adc_limit{50}, roi_min_max_ratio{0.8f};
-        std::shared_ptr<filter_t> rcrc, config, noise;
+        std::shared_ptr<filter_t> rcrc, config, noise, response;
     };
 
     Configuration default_configuration() const override
@@ -141,6 +141,7 @@ class OmniChannelNoiseDB : public IChannelNoiseDatabase, public IConfigurable {
         m_rc_layers = 2;
         m_rcrc_cache.clear();
         m_reconfig_cache.clear();
+        m_response_cache.clear();
         m_unity.reset();
         for (const auto& jci : cfg["channel_info"]) update_channels(jci);
     }
@@ -149,6 +150,22 @@ class OmniChannelNoiseDB : public IChannelNoiseDatabase, public IConfigurable {
     double nominal_baseline(int ch) const override { return dbget(ch).nominal_baseline; }
     double gain_correction(int ch) const override { return dbget(ch).gain_correction; }
     double min_rms_cut(int ch) const override { return dbget(ch).min_rms_cut; }
+    double response_offset(int ch) const override { return dbget(ch).response_offset; }
+    int pad_window_front(int ch) const override { return dbget(ch).pad_window_front; }
+    int pad_window_back(int ch) const override { return dbget(ch).pad_window_back; }
+    float coherent_nf_decon_limit(int ch) const override { return dbget(ch).decon_limit; }
+    float coherent_nf_decon_lf_cutoff(int ch) const override { return dbget(ch).decon_lf_cutoff; }
+    float coherent_nf_decon_limit1(int ch) const override { return dbget(ch).decon_limit1; }
+    float coherent_nf_adc_limit(int ch) const override { return dbget(ch).adc_limit; }
+    float coherent_nf_protection_factor(int ch) const override { return dbget(ch).protection_factor; }
+    float coherent_nf_min_adc_limit(int ch) const override { return dbget(ch).min_adc_limit; }
+    float coherent_nf_roi_min_max_ratio(int ch) const override { return dbget(ch).roi_min_max_ratio; }
+    const filter_t& response(int ch) const override
+    {
+        static const filter_t dummy;
+        const auto& f = dbget(ch).response;
+        return f ? *f : dummy;
+    }
     double max_rms_cut(int ch) const override { return dbget(ch).max_rms_cut; }
     const filter_t& rcrc(int ch) const override
     {
@@ -282,6 +299,28 @@ class OmniChannelNoiseDB : public IChannelNoiseDatabase, public IConfigurable {
         return filt;
     }
 
+    // parse_response (:302-350), the explicit-waveform form: the first
+    // nsamples samples (float), zero-padded, as a spectrum, cached by
+    // waveformid.  The form averaging the anode's field response ({wpid})
+    // is refused (this build's anode has no field response).  An object
+    // with neither gives an empty spectrum.
+    std::shared_ptr<filter_t> parse_response(const Configuration& j)
+    {
+        if (j.isMember("wpid"))
+            throw std::runtime_error("OmniChannelNoiseDB: a response from the anode's field response ({wpid}) is "
+                                     "not built; give {waveform, waveformid}");
+        if (!(j.isMember("waveform") && j.isMember("waveformid"))) return std::make_shared<filter_t>();
+        const int id = j["waveformid"].asInt();
+        auto it = m_response_cache.find(id);
+        if (it != m_response_cache.end()) return it->second;
+        std::vector<float> wave(m_nsamples, 0.f);
+        const int nsamp = std::min(m_nsamples, (int)j["waveform"].size());
+        for (int i = 0; i < nsamp; ++i) wave[i] = (float)j["waveform"][i].asDouble();
+        auto spec = std::make_shared<filter_t>(spectrum_of(wave));
+        m_response_cache[id] = spec;
+        return spec;
+    }
+
     // parse_freqmasks (:152-169)
     std::shared_ptr<filter_t> parse_freqmasks(const Configuration& j) const
     {
@@ -339,6 +378,10 @@ class OmniChannelNoiseDB : public IChannelNoiseDatabase, public IConfigurable {
             auto val = parse_reconfig(cfg["reconfig"]);
             for (int ch : chans) get_ci(ch).config = val;
         }
+        if (!cfg["response"].isNull()) {
+            auto val = parse_response(cfg["response"]);
+            for (int ch : chans) get_ci(ch).response = val;
+        }
         if (!cfg["freqmasks"].isNull()) {
             auto val = parse_freqmasks(cfg["freqmasks"]);
             for (int ch : chans) get_ci(ch).noise = val;
@@ -353,7 +396,7 @@ class OmniChannelNoiseDB : public IChannelNoiseDatabase, public IConfigurable {
     std::map<int, ChannelInfo> m_db;
     std::vector<channel_group_t> m_groups;
     std::vector<int> m_bad;
-    std::map<int, std::shared_ptr<filter_t>> m_rcrc_cache, m_reconfig_cache;
+    std::map<int, std::shared_ptr<filter_t>> m_rcrc_cache, m_reconfig_cache, m_response_cache;
     std::shared_ptr<filter_t> m_unity;
 };
 
@@ -679,6 +722,142 @@ class RelGainCalib : public IChannelFilter, public IBatchChannelFilter, public I
     mutable DevBuf m_d_rows, m_d_gain;
 };
 
+// ---- mbCoherentNoiseSub (sigproc/src/Microboone.cxx:790-892) -----------
+class CoherentNoiseSub : public IChannelFilter, public IBatchGroupFilter, public IConfigurable {
+  public:
+    typedef IChannelNoiseDatabase::filter_t filter_t;
+    ~CoherentNoiseSub() override
+    {
+        if (m_stream) wct_stream_destroy(m_stream);
+    }
+    Configuration default_configuration() const override
+    {
+        Configuration cfg;
+        cfg["anode"] = m_anode_tn;
+        cfg["noisedb"] = m_noisedb_tn;
+        cfg["dft"] = "HipDFT";
+        cfg["rms_threshold"] = (double)m_rms_threshold;
+        return cfg;
+    }
+    void configure(const Configuration& cfg) override
+    {
+        m_anode_tn = get<std::string>(cfg, "anode", m_anode_tn);
+        Factory::find_tn<IAnodeChannels>(m_anode_tn);
+        m_noisedb_tn = get<std::string>(cfg, "noisedb", m_noisedb_tn);
+        m_noisedb = Factory::find_tn<IChannelNoiseDatabase>(m_noisedb_tn);
+        m_rms_threshold = get(cfg, "rms_threshold", m_rms_threshold);
+        if (m_rms_threshold != 0)
+            throw ConfigError("mbCoherentNoiseSub: rms_threshold other than 0 (the per-channel limit chain of "
+                              "Subtract_WScaling) is not built");
+    }
+    Waveform::ChannelMaskMap apply(int, signal_t&) const override { return Waveform::ChannelMaskMap(); }
+    Waveform::ChannelMaskMap apply(channel_signals_t& chansig) const override
+    {
+        if (chansig.empty()) return Waveform::ChannelMaskMap();
+        std::vector<int> channels;
+        int nt = (int)chansig.begin()->second.size();
+        std::vector<float> rows;
+        std::vector<int> grp;
+        for (auto& cs : chansig) {
+            grp.push_back((int)channels.size());
+            channels.push_back(cs.first);
+            rows.insert(rows.end(), cs.second.begin(), cs.second.end());
+        }
+        apply_groups(channels, {grp}, rows, nt);
+        size_t i = 0;
+        for (auto& cs : chansig) {
+            std::copy(rows.begin() + i * nt, rows.begin() + (i + 1) * nt, cs.second.begin());
+            ++i;
+        }
+        return Waveform::ChannelMaskMap();   // :871
+    }
+    // Every group in one batch; a group's rows are taken in channel order
+    // and its parameters from its first channel (:807-822).
+    void apply_groups(const std::vector<int>& channels, const std::vector<std::vector<int>>& groups,
+                      std::vector<float>& rows, int nt) const override
+    {
+        std::lock_guard<std::mutex> lock(m_mutex);
+        if (groups.empty() || nt == 0) return;
+        const int nrows = (int)channels.size();
+        std::vector<wct_nf_group> ent;
+        std::vector<int> grows, row_group(nrows, -1);
+        std::vector<const filter_t*> spec_rows;
+        std::map<const filter_t*, int> spec_id;
+        for (const auto& g0 : groups) {
+            std::vector<int> g(g0);
+            std::sort(g.begin(), g.end(), [&](int a, int b) { return channels[a] < channels[b]; });
+            if (g.size() > 128) throw std::runtime_error("mbCoherentNoiseSub: groups of up to 128 channels");
+            wct_nf_group e{};
+            e.first = (int)grows.size();
+            e.count = (int)g.size();
+            const int ach = channels[g[0]];
+            const filter_t& rs = m_noisedb->response(ach);
+            e.res_offset = (int)m_noisedb->response_offset(ach);
+            e.pad_f = m_noisedb->pad_window_front(ach);
+            e.pad_b = m_noisedb->pad_window_back(ach);
+            e.decon_limit = m_noisedb->coherent_nf_decon_limit(ach);
+            e.decon_lf_cutoff = m_noisedb->coherent_nf_decon_lf_cutoff(ach);
+            e.adc_limit = m_noisedb->coherent_nf_adc_limit(ach);
+            e.decon_limit1 = m_noisedb->coherent_nf_decon_limit1(ach);
+            e.roi_min_max_ratio = m_noisedb->coherent_nf_roi_min_max_ratio(ach);
+            e.protection_factor = m_noisedb->coherent_nf_protection_factor(ach);
+            e.min_adc_limit = m_noisedb->coherent_nf_min_adc_limit(ach);
+            e.respec = -1;
+            if (!rs.empty() && (rs[0].real() != 1 || rs[0].imag() != 0) && e.res_offset != 0) {
+                if ((int)rs.size() != nt)
+                    throw std::runtime_error("mbCoherentNoiseSub: response spectrum of " + std::to_string(rs.size()) +
+                                             " bins for " + std::to_string(nt) + " ticks");
+                auto it = spec_id.find(&rs);
+                if (it == spec_id.end()) {
+                    it = spec_id.emplace(&rs, (int)spec_rows.size()).first;
+                    spec_rows.push_back(&rs);
+                }
+                e.respec = it->second;
+            }
+            for (int r : g) {
+                if (row_group[r] >= 0) throw std::runtime_error("mbCoherentNoiseSub: a channel in two groups");
+                row_group[r] = (int)ent.size();
+                grows.push_back(r);
+            }
+            ent.push_back(e);
+        }
+        if (!m_stream) ncheck(wct_stream_create(&m_stream), "mbCoherentNoiseSub stream");
+        const size_t row_bytes = sizeof(float) * (size_t)nrows * nt;
+        m_d_rows.ensure(row_bytes);
+        m_d_groups.ensure(sizeof(wct_nf_group) * ent.size());
+        m_d_grows.ensure(sizeof(int) * grows.size());
+        m_d_rowgroup.ensure(sizeof(int) * nrows);
+        m_d_spectra.ensure(sizeof(float) * 2 * std::max<size_t>(1, spec_rows.size()) * nt);
+        const size_t ws = wct_nf_coherent_workspace_size((int)ent.size(), nrows, nt);
+        m_d_ws.ensure(ws);
+        ncheck(wct_memcpy_htod_async(m_d_rows.p, rows.data(), row_bytes, m_stream), "mbCoherentNoiseSub upload");
+        ncheck(wct_memcpy_htod_async(m_d_groups.p, ent.data(), sizeof(wct_nf_group) * ent.size(), m_stream),
+               "mbCoherentNoiseSub upload");
+        ncheck(wct_memcpy_htod_async(m_d_grows.p, grows.data(), sizeof(int) * grows.size(), m_stream),
+               "mbCoherentNoiseSub upload");
+        ncheck(wct_memcpy_htod_async(m_d_rowgroup.p, row_group.data(), sizeof(int) * nrows, m_stream),
+               "mbCoherentNoiseSub upload");
+        for (size_t r = 0; r < spec_rows.size(); ++r)
+            ncheck(wct_memcpy_htod_async((char*)m_d_spectra.p + sizeof(float) * 2 * nt * r, spec_rows[r]->data(),
+                                         sizeof(float) * 2 * nt, m_stream),
+                   "mbCoherentNoiseSub upload");
+        ncheck(wct_nf_coherent((float*)m_d_rows.p, nt, nrows, nt, (const wct_nf_group*)m_d_groups.p, (int)ent.size(),
+                               (const int*)m_d_grows.p, (const int*)m_d_rowgroup.p, (const float*)m_d_spectra.p,
+                               spec_rows.empty() ? 0 : 1, m_d_ws.p, ws, m_stream),
+               "mbCoherentNoiseSub");
+        ncheck(wct_memcpy_dtoh_async(rows.data(), m_d_rows.p, row_bytes, m_stream), "mbCoherentNoiseSub download");
+        ncheck(wct_stream_sync(m_stream), "mbCoherentNoiseSub sync");
+    }
+
+  private:
+    std::string m_anode_tn{"AnodeChannels"}, m_noisedb_tn{"OmniChannelNoiseDB"};
+    std::shared_ptr<IChannelNoiseDatabase> m_noisedb;
+    float m_rms_threshold{0.0f};
+    mutable std::mutex m_mutex;
+    mutable void* m_stream{nullptr};
+    mutable DevBuf m_d_rows, m_d_groups, m_d_grows, m_d_rowgroup, m_d_spectra, m_d_ws;
+};
+
 // ---- OmnibusNoiseFilter (sigproc/src/OmnibusNoiseFilter.cxx) -----------
 class OmnibusNoiseFilter : public IFrameFilter, public IConfigurable {
   public:
@@ -759,7 +938,31 @@ class OmnibusNoiseFilter : public IFrameFilter, public IConfigurable {
         }
 
         for (const auto& filter : m_perchan) run_filter(*filter, channels, rows, nt, cmm);
-        for (const auto& group : m_noisedb->coherent_channels()) {   // :193-224
+        // grouped filters (:193-224): the complete groups (every channel
+        // present), all at once when the filter takes batches
+        std::vector<std::vector<int>> complete_groups;
+        for (const auto& group : m_noisedb->coherent_channels()) {
+            std::vector<int> g;
+            bool ok = true;
+            for (int ch : group) {
+                auto it = row_of.find(ch);
+                if (it == row_of.end()) {
+                    ok = false;
+                    break;
+                }
+                g.push_back(it->second);
+            }
+            if (ok) complete_groups.push_back(g);
+        }
+        std::vector<std::shared_ptr<IChannelFilter>> unbatched;
+        for (const auto& filter : m_grouped) {
+            if (auto* batch = dynamic_cast<const IBatchGroupFilter*>(filter.get()))
+                batch->apply_groups(channels, complete_groups, rows, nt);
+            else
+                unbatched.push_back(filter);
+        }
+        for (const auto& group : unbatched.empty() ? std::vector<std::vector<int>>()
+                                                   : m_noisedb->coherent_channels()) {
             bool complete = true;
             IChannelFilter::channel_signals_t chgrp;
             for (int ch : group) {
@@ -772,7 +975,7 @@ class OmnibusNoiseFilter : public IFrameFilter, public IConfigurable {
                 chgrp[ch].assign(row, row + nt);
             }
             if (!complete) continue;
-            for (const auto& filter : m_grouped) Waveform::merge(cmm, filter->apply(chgrp), m_maskmap);
+            for (const auto& filter : unbatched) Waveform::merge(cmm, filter->apply(chgrp), m_maskmap);
             for (const auto& cs : chgrp) std::copy(cs.second.begin(), cs.second.end(), rows.data() + (size_t)row_of[cs.first] * nt);
         }
         for (const auto& filter : m_perchan_status) run_filter(*filter, channels, rows, nt, cmm);
@@ -849,4 +1052,5 @@ WIRECELL_HIP_FACTORY(OmniChannelNoiseDB, WireCell::Hip::OmniChannelNoiseDB)
 WIRECELL_HIP_FACTORY(pdOneChannelNoise, WireCell::Hip::PdOneChannelNoise)
 WIRECELL_HIP_FACTORY(mbOneChannelNoise, WireCell::Hip::MbOneChannelNoise)
 WIRECELL_HIP_FACTORY(pdRelGainCalib, WireCell::Hip::RelGainCalib)
+WIRECELL_HIP_FACTORY(mbCoherentNoiseSub, WireCell::Hip::CoherentNoiseSub)
 WIRECELL_HIP_FACTORY(OmnibusNoiseFilter, WireCell::Hip::OmnibusNoiseFilter)

@@ -7,6 +7,8 @@
 //                          channel of a frame in one device batch
 //   mbOneChannelNoise      sigproc/src/Microboone.cxx:894-1069, the same
 //   pdRelGainCalib         sigproc/src/Protodune.cxx:925-1014
+//   mbCoherentNoiseSub     sigproc/src/Microboone.cxx:790-892, every
+//                          channel group of a frame in one device batch
 //   OmnibusNoiseFilter     sigproc/src/OmnibusNoiseFilter.cxx
 #pragma once
 #include <complex>
@@ -29,9 +31,20 @@ class IChannelNoiseDatabase : public IComponent<IChannelNoiseDatabase> {
     virtual double gain_correction(int channel) const = 0;
     virtual double min_rms_cut(int channel) const = 0;
     virtual double max_rms_cut(int channel) const = 0;
+    virtual double response_offset(int channel) const = 0;
+    virtual int pad_window_front(int channel) const = 0;
+    virtual int pad_window_back(int channel) const = 0;
+    virtual float coherent_nf_decon_limit(int channel) const = 0;
+    virtual float coherent_nf_decon_lf_cutoff(int channel) const = 0;
+    virtual float coherent_nf_decon_limit1(int channel) const = 0;
+    virtual float coherent_nf_adc_limit(int channel) const = 0;
+    virtual float coherent_nf_protection_factor(int channel) const = 0;
+    virtual float coherent_nf_min_adc_limit(int channel) const = 0;
+    virtual float coherent_nf_roi_min_max_ratio(int channel) const = 0;
     virtual const filter_t& rcrc(int channel) const = 0;
     virtual const filter_t& config(int channel) const = 0;
     virtual const filter_t& noise(int channel) const = 0;
+    virtual const filter_t& response(int channel) const = 0;
     virtual const std::vector<channel_group_t> coherent_channels() const = 0;
     virtual std::vector<int> bad_channels() const = 0;
 };
@@ -65,6 +78,15 @@ class IBatchChannelFilter {
     virtual ~IBatchChannelFilter() = default;
     virtual void apply_batch(const std::vector<int>& channels, std::vector<float>& rows, int nticks,
                              std::vector<Waveform::ChannelMaskMap>& masks) const = 0;
+};
+
+// A frame's channel groups through a batched grouped filter: groups lists
+// row indices of rows (nticks samples each), filtered in place.
+class IBatchGroupFilter {
+  public:
+    virtual ~IBatchGroupFilter() = default;
+    virtual void apply_groups(const std::vector<int>& channels, const std::vector<std::vector<int>>& groups,
+                              std::vector<float>& rows, int nticks) const = 0;
 };
 
 }  // namespace Hip

@@ -60,6 +60,12 @@ class NfChannel(C.Structure):
                 ("gain", C.c_float), ("config", C.c_int), ("noise", C.c_int)]
 
 
+class NfGroup(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("first", "count", "respec", "res_offset", "pad_f", "pad_b")] + \
+               [(n, C.c_float) for n in ("decon_limit", "decon_lf_cutoff", "adc_limit", "decon_limit1",
+                                         "roi_min_max_ratio", "protection_factor", "min_adc_limit")]
+
+
 NF_PARTIAL, NF_NOISY, NF_HARMONIC, NF_CHIRP = 1, 2, 4, 8
 NF_PROTODUNE, NF_MICROBOONE = 0, 1
 
@@ -134,6 +140,8 @@ _SIGS = {
     "wct_nf_workspace_size": (_Z, [_I, _I]),
     "wct_nf_one_channel": (_I, [_V, _I, _V, _I, _I, _I, _V, _V, _V, _V, _V, _Z, _V]),
     "wct_nf_rel_gain": (_I, [_V, _I, _I, _I, _V, _F, _F, _V]),
+    "wct_nf_coherent_workspace_size": (_Z, [_I, _I, _I]),
+    "wct_nf_coherent": (_I, [_V, _I, _I, _I, _V, _I, _V, _V, _V, _I, _V, _Z, _V]),
 }
 
 # Every symbol include/wct_hip.h declares (checked by the CPU tests).

@@ -8,7 +8,7 @@ import ctypes as C
 
 import numpy as np
 
-from .capi import NF_CHIRP, NF_HARMONIC, NF_MICROBOONE, NF_NOISY, NF_PARTIAL, NF_PROTODUNE, NfChannel, check, lib
+from .capi import NF_CHIRP, NF_HARMONIC, NF_MICROBOONE, NF_NOISY, NF_PARTIAL, NF_PROTODUNE, NfChannel, NfGroup, check, lib
 
 
 def _stream():
@@ -116,4 +116,42 @@ def rel_gain(frame, gains, gain_min=0.8, gain_max=1.25, stream=None):
     nch, nt = x.shape
     check(lib().wct_nf_rel_gain(C.c_void_p(x.data_ptr()), nt, nch, nt, C.c_void_p(g.data_ptr()), float(gain_min),
                                 float(gain_max), _stream() if stream is None else stream))
+    return x
+
+
+GROUP_DEFAULTS = dict(respec=-1, res_offset=0, pad_f=0, pad_b=0, decon_limit=0.02, decon_lf_cutoff=0.08,
+                      adc_limit=0.0, decon_limit1=0.08, roi_min_max_ratio=0.8, protection_factor=5.0,
+                      min_adc_limit=50.0)
+
+
+def coherent(frame, groups, spectra=None, stream=None):
+    """Microboone::CoherentNoiseSub on every group (in place on a CUDA copy
+    of frame, returned).  groups: dicts with rows (row indices in channel
+    order) and the GROUP_DEFAULTS keys (respec: row of spectra or -1)."""
+    import torch
+    x = torch.as_tensor(frame, dtype=torch.float32, device="cuda").contiguous().clone()
+    nrows, nt = x.shape
+    entries = (NfGroup * len(groups))()
+    grows, row_group = [], np.full(nrows, -1, np.int32)
+    any_resp = 0
+    for g, grp in enumerate(groups):
+        p = dict(GROUP_DEFAULTS, **{k: v for k, v in grp.items() if k != "rows"})
+        entries[g] = NfGroup(len(grows), len(grp["rows"]), int(p["respec"]), int(p["res_offset"]), int(p["pad_f"]),
+                             int(p["pad_b"]), *(float(p[k]) for k in ("decon_limit", "decon_lf_cutoff", "adc_limit",
+                                                                     "decon_limit1", "roi_min_max_ratio",
+                                                                     "protection_factor", "min_adc_limit")))
+        any_resp |= int(p["respec"]) >= 0
+        for r in grp["rows"]:
+            row_group[r] = g
+        grows.extend(grp["rows"])
+    dg = torch.frombuffer(bytearray(bytes(entries)), dtype=torch.uint8).to("cuda")
+    dgr = torch.as_tensor(np.asarray(grows, np.int32), device="cuda")
+    drg = torch.as_tensor(row_group, device="cuda")
+    dsp = torch.as_tensor(np.ascontiguousarray(spectra, np.complex64), device="cuda") if spectra is not None else None
+    ws_bytes = lib().wct_nf_coherent_workspace_size(len(groups), nrows, nt)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+    check(lib().wct_nf_coherent(C.c_void_p(x.data_ptr()), nt, nrows, nt, C.c_void_p(dg.data_ptr()), len(groups),
+                                C.c_void_p(dgr.data_ptr()), C.c_void_p(drg.data_ptr()),
+                                C.c_void_p(dsp.data_ptr() if dsp is not None else 0), any_resp,
+                                C.c_void_p(ws.data_ptr()), ws_bytes, _stream() if stream is None else stream))
     return x
