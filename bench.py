@@ -435,6 +435,37 @@ def noise_leg(dev, napas=4, reps=10, cpu=True):
     fwd()
     fft_ms = timed(fwd, reps)
     fl = flags.cpu().numpy()[:, 0]
+    # the coherent subtraction (mbCoherentNoiseSub) on the filtered rows:
+    # ProtoDUNE's groups (20 x 40 U, 20 x 40 V, 20 x 48 W channels per APA),
+    # U and V groups through the response path (a 40-tick bipolar response
+    # at offset 120)
+    from wct_amd.capi import NfGroup
+    groups, grows = [], []
+    row_group = np.full(nch, -1, np.int32)
+    for a in range(napas):
+        for first, size, count in ((0, 40, 20), (800, 40, 20), (1600, 48, 20)):
+            for k in range(count):
+                rr = [a * per_apa + first + k * size + i for i in range(size)]
+                resp = first < 1600
+                groups.append(NfGroup(len(grows), size, 0 if resp else -1, 120 if resp else 0, 10, 10, 0.02, 0.08,
+                                      15.0, 0.07, 3.0, 5.0, 50.0))
+                row_group[rr] = len(groups) - 1
+                grows.extend(rr)
+    resp = np.zeros(nt)
+    resp[100:140] = np.sin(np.linspace(0, 2 * np.pi, 40))
+    dspec = torch.as_tensor(np.fft.fft(resp).astype(np.complex64), device=dev)
+    dg = torch.frombuffer(bytearray(bytes((NfGroup * len(groups))(*groups))), dtype=torch.uint8).to(dev)
+    dgr = torch.as_tensor(np.asarray(grows, np.int32), device=dev)
+    drg = torch.as_tensor(row_group, device=dev)
+    cws_bytes = lib().wct_nf_coherent_workspace_size(len(groups), nch, nt)
+    cws = torch.empty(cws_bytes, dtype=torch.uint8, device=dev)
+    work = out.clone()
+    coh = lambda: check(lib().wct_nf_coherent(
+        C.c_void_p(work.data_ptr()), nt, nch, nt, C.c_void_p(dg.data_ptr()), len(groups), C.c_void_p(dgr.data_ptr()),
+        C.c_void_p(drg.data_ptr()), C.c_void_p(dspec.data_ptr()), 1, C.c_void_p(cws.data_ptr()), cws_bytes,
+        stream.cuda_stream))
+    coh()
+    coh_ms = timed(coh, reps)
     ct = nch * nt
     res = {
         "workload": f"pdOneChannelNoise on {napas} APAs x {per_apa} channels x {nt} ticks",
@@ -442,6 +473,8 @@ def noise_leg(dev, napas=4, reps=10, cpu=True):
         "alg_bytes_per_channel_tick": 8, "achieved_gbs": 8 * ct / (ms / 1e3) / 1e9,
         "frac": 8 * ct / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
         "row_transform_ms": round(fft_ms, 4),
+        "coherent": {"groups": len(groups), "ms": round(coh_ms, 4),
+                     "channel_ticks_per_s": ct / (coh_ms / 1e3)},
         "partial": int(((fl & 1) != 0).sum()), "noisy": int(((fl & 2) != 0).sum()), "dtype": "f32",
     }
     if cpu:

@@ -765,7 +765,6 @@ __global__ __launch_bounds__(kCnTicks) void k_cn_median(const float* __restrict_
                                                         const int* __restrict__ grows,
                                                         const double* __restrict__ rms0, float* __restrict__ med)
 {
-    extern __shared__ __align__(16) unsigned char nf_smem[];
     const int g = blockIdx.x;
     const wct_nf_group grp = groups[g];
     const int t = blockIdx.y * kCnTicks + threadIdx.x;
@@ -779,16 +778,13 @@ __global__ __launch_bounds__(kCnTicks) void k_cn_median(const float* __restrict_
     }
     if (count > 0) maxrms = maxrms / count;
     const float lim = 5 * maxrms;
-    float* tile = reinterpret_cast<float*>(nf_smem);   // [channel][tick]
-    int* bin = reinterpret_cast<int*>(nf_smem);
-    const bool tin = t < n;
-    for (int k = 0; k < grp.count; ++k)
-        tile[k * kCnTicks + threadIdx.x] = tin ? G(rows + (long)grows[grp.first + k] * ld)[t] : 0.f;
-    if (!tin) return;   // no barrier below: each thread owns its column
+    if (t >= n) return;
+    const int* gr = grows + grp.first;
+    const int cnt = min(grp.count, kMaxGroupRows);   // the documented limit
     int m = 0;
     float vmin = INFINITY, vmax = -INFINITY;
-    for (int k = 0; k < grp.count; ++k) {
-        const float v = tile[k * kCnTicks + threadIdx.x];
+    for (int k = 0; k < cnt; ++k) {
+        const float v = G(rows + (long)gr[k] * ld)[t];
         if (fabsf(v) < lim && fabs((double)v) > 0.001) {
             ++m;
             vmin = fminf(vmin, v);
@@ -797,35 +793,53 @@ __global__ __launch_bounds__(kCnTicks) void k_cn_median(const float* __restrict_
     }
     float out = 0.f;
     if (m > 0) {
+        // bin indices as bytes, four to a register; 127 marks a sample
+        // outside the selection (and pads the last word): no search
+        // threshold below exceeds m - 1 <= 127, so it is never counted
+        constexpr int kWords = kMaxGroupRows / 4;
+        const int nw = (cnt + 3) / 4;
         const float width = __fdiv_rn(vmax - vmin, (float)m);
-        for (int k = 0; k < grp.count; ++k) {
-            const float v = tile[k * kCnTicks + threadIdx.x];
-            int b = -1;
-            if (fabsf(v) < lim && fabs((double)v) > 0.001) {
-                const float r = __fdiv_rn(v - vmin, width);
-                b = (r != r) ? 0 : (int)roundf(r);
-                b = max(0, min(m - 1, b));
+        unsigned packed[kWords];
+#pragma unroll
+        for (int w = 0; w < kWords; ++w) {
+            unsigned word = 0x7f7f7f7fu;
+            if (w < nw) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int k = 4 * w + j;
+                    if (k < cnt) {
+                        const float v = G(rows + (long)gr[k] * ld)[t];
+                        unsigned b = 127;
+                        if (fabsf(v) < lim && fabs((double)v) > 0.001) {
+                            const float r = __fdiv_rn(v - vmin, width);
+                            const int bi = (r != r) ? 0 : (int)roundf(r);
+                            b = (unsigned)max(0, min(m - 1, bi));
+                        }
+                        word = (word & ~(0xffu << (8 * j))) | (b << (8 * j));
+                    }
+                }
             }
-            bin[k * kCnTicks + threadIdx.x] = b;
+            packed[w] = word;
         }
+        // the int(m / 2)-th smallest bin: the least b with more than imed
+        // bins <= b, by bisection over [0, m - 1]; a byte x in [0, 127]
+        // is below q in [1, 127] exactly when (x | 0x80) - q clears bit 7,
+        // and no byte borrows from its neighbour
         const int imed = (int)((float)m * 0.5f);
-        int ind = 0;
-        for (int i = 0; i < grp.count; ++i) {
-            const int bi = bin[i * kCnTicks + threadIdx.x];
-            if (bi < 0) continue;
-            int less = 0, le = 0;
-            for (int j = 0; j < grp.count; ++j) {
-                const int bj = bin[j * kCnTicks + threadIdx.x];
-                if (bj < 0) continue;
-                less += bj < bi;
-                le += bj <= bi;
-            }
-            if (less <= imed && imed < le) {
-                ind = bi;
-                break;
-            }
+        int lo = 0, hi = m - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const unsigned q = (unsigned)(mid + 1) * 0x01010101u;
+            int below = 0;
+#pragma unroll
+            for (int w = 0; w < kWords; ++w)
+                if (w < nw) below += 4 - __popc(((packed[w] | 0x80808080u) - q) & 0x80808080u);
+            if (below > imed)
+                hi = mid;
+            else
+                lo = mid + 1;
         }
-        out = vmin + (float)ind * width;
+        out = vmin + (float)lo * width;
     }
     G(med)[(long)g * n + t] = out;
 }
@@ -1241,9 +1255,7 @@ extern "C" int wct_nf_coherent(float* rows, int ld, int nrows, int nt, const wct
     WCT_TRY(allow_lds(k_cn_rms, lds_row));
     hipLaunchKernelGGL(k_cn_rms, dim3(nrows), dim3(kNfBlock), lds_row, s, rows, ld, nt, rms0, crms);
     WCT_HIP_CHECK(hipGetLastError());
-    const size_t lds_med = sizeof(float) * kCnTicks * kMaxGroupRows;
-    WCT_TRY(allow_lds(k_cn_median, lds_med));
-    hipLaunchKernelGGL(k_cn_median, dim3(ngroups, (nt + kCnTicks - 1) / kCnTicks), dim3(kCnTicks), lds_med, s, rows,
+    hipLaunchKernelGGL(k_cn_median, dim3(ngroups, (nt + kCnTicks - 1) / kCnTicks), dim3(kCnTicks), 0, s, rows,
                        ld, nt, groups, group_rows, rms0, med);
     WCT_HIP_CHECK(hipGetLastError());
     WCT_TRY(allow_lds(k_cn_protect, lds_row));

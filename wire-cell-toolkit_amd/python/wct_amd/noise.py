@@ -141,7 +141,11 @@ def coherent(frame, groups, spectra=None, stream=None):
                                                                      "decon_limit1", "roi_min_max_ratio",
                                                                      "protection_factor", "min_adc_limit")))
         any_resp |= int(p["respec"]) >= 0
+        if not 1 <= len(grp["rows"]) <= 128:
+            raise ValueError(f"coherent: group {g} has {len(grp['rows'])} channels (1 to 128 supported)")
         for r in grp["rows"]:
+            if row_group[r] >= 0:
+                raise ValueError(f"coherent: row {r} is in groups {row_group[r]} and {g}")
             row_group[r] = g
         grows.extend(grp["rows"])
     dg = torch.frombuffer(bytearray(bytes(entries)), dtype=torch.uint8).to("cuda")
