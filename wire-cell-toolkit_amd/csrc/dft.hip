@@ -58,6 +58,44 @@ __global__ __launch_bounds__(kBlock) void k_fft_rows(const typename C2<T>::type*
     }
 }
 
+// ---- rows of a length with a compile-time plan ----
+// One row of N entries per workgroup of NT threads, unpadded in LDS, on
+// the odd-first radix schedule (every LDS address a base plus an
+// immediate); the schedule's twiddles (about N entries) are read through
+// the caches so the row alone sets the LDS a workgroup takes.
+template <int N, int NT>
+__global__ __launch_bounds__(NT) void k_fft_rows_lin(const float2* __restrict__ in, float2* __restrict__ out,
+                                                     long dist, const float2* __restrict__ tw, int sign, float scale)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* buf = reinterpret_cast<float2*>(smem_raw);
+    const long row = blockIdx.x;
+    const auto src = G(in + row * dist);
+    for (int i = threadIdx.x; i < N; i += NT) buf[i] = src[i];
+    __syncthreads();
+    fft_lin<float2, NT, N>(buf, G(tw), sign, threadIdx.x);
+    const auto dst = G(out + row * dist);
+    for (int i = threadIdx.x; i < N; i += NT) {
+        float2 v = buf[i];
+        v.x *= scale;
+        v.y *= scale;
+        dst[i] = v;
+    }
+}
+
+template <int N, int NT>
+static int launch_rows_lin(const void* in, void* out, int nrows, long dist, int sign, float scale, hipStream_t s)
+{
+    const DevPlan* dl = nullptr;
+    WCT_TRY(get_plan(N, false, &dl, true));
+    const size_t lds = sizeof(float2) * N;
+    WCT_TRY(allow_lds(k_fft_rows_lin<N, NT>, lds));
+    hipLaunchKernelGGL((k_fft_rows_lin<N, NT>), dim3(nrows), dim3(NT), lds, s, (const float2*)in, (float2*)out, dist,
+                       (const float2*)dl->tw, sign, scale);
+    WCT_HIP_CHECK(hipGetLastError());
+    return WCT_OK;
+}
+
 // ---- columns: a tile of TC columns, each transformed by its own slice ----
 template <typename T, int TC, int NMAX>
 __global__ __launch_bounds__(kBlock) void k_fft_cols(const typename C2<T>::type* __restrict__ in,
@@ -135,6 +173,16 @@ static int launch_rows_lds(const void* in, void* out, int n, int nrows, long dis
     const T2* src = (const T2*)in;
     T2* dst = (T2*)out;
     constexpr bool dbl = sizeof(T) == 8;
+    if constexpr (!dbl) {
+        // lengths of common frames on compile-time plans: 6000 ticks (the
+        // noise filters, the per-plane path), 6272 (the forward simulation's
+        // padded 6000-tick window), 8000
+        if (nrows >= 1) {
+            if (n == 6000) return launch_rows_lin<6000, 512>(in, out, nrows, dist, sign, (float)scale, s);
+            if (n == 6272) return launch_rows_lin<6272, 512>(in, out, nrows, dist, sign, (float)scale, s);
+            if (n == 8000) return launch_rows_lin<8000, 512>(in, out, nrows, dist, sign, (float)scale, s);
+        }
+    }
 #define WCT_ROWS(SL, NM, TWG)                                                                        \
     do {                                                                                             \
         const int rpb = kBlock / SL;                                                                 \
