@@ -347,7 +347,7 @@ def impact_leg(dev, reps=20, cpu=True):
                     f"{P['ndepo']} depositions, frame {nw} x {nt}",
         "charges": int(offs[-1]), "frame": [nw, nt], "ms_per_plane": round(ms, 4),
         "samples_per_s": nw * nt / (ms / 1e3),
-        "dominant_kernel": {"kernel": "k_fft_rows (forward time transform of the packed frames)",
+        "dominant_kernel": {"kernel": "k_fft_rows_lin<6272> (forward time transform of the packed frames)",
                             "launch_ms": round(row_ms, 4), "alg_bytes": row_bytes,
                             "achieved_gbs": row_bytes / (row_ms / 1e3) / 1e9,
                             "frac": row_bytes / (row_ms / 1e3) / 1e9 / HBM_PEAK_GBS},

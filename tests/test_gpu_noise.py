@@ -372,8 +372,13 @@ def test_coherent_noise_sub_matches_oracle():
         blocks.append(rows)
         respecs.append(rs)
     nrows = sum(len(b) for b in blocks)
-    order = rng.permutation(nrows)             # frame row of each (group, channel)
-    frame = np.zeros((nrows, nt), np.float32)
+    # frame row of each (group, channel), plus one row in no group; the
+    # odd last row is in a response group (its complex row has no partner)
+    order = rng.permutation(nrows + 1)
+    j = int(np.flatnonzero(order == nrows)[0])
+    order[[0, j]] = order[[j, 0]]              # into group 0 (response path)
+    frame = np.zeros((nrows + 1, nt), np.float32)
+    frame[order[nrows]] = rng.normal(0, 3, nt).astype(np.float32)
     groups, pos = [], 0
     spectra = np.stack([respecs[0], respecs[1]])
     for k, b in enumerate(blocks):
@@ -401,7 +406,7 @@ def test_coherent_noise_sub_matches_oracle():
         for i, r in enumerate(grp["rows"]):
             assert float(np.std(got[r][:600])) < 1.6
     # rows of no group are untouched
-    assert np.array_equal(got.shape, frame.shape)
+    assert np.array_equal(got[order[nrows]], frame[order[nrows]])
 
 
 def test_grouped_filter_component():

@@ -1033,42 +1033,81 @@ __global__ void k_cn_scaling(const wct_nf_group* __restrict__ groups, int ngroup
 // The channel's samples in the group's marked runs with the line between
 // the bins just outside subtracted, zero elsewhere (:130-145), as a
 // complex row.
+// A channel row on the response path with its median-protected runs kept
+// and everything else zero: in a marked run, the sample less the line
+// between the samples just outside the run.
+__device__ __forceinline__ float cn_roi_value(const float* s, const unsigned char* mk, int n, int b)
+{
+    if (!mk[b]) return 0.f;
+    int s0 = b, e0 = b;
+    while (s0 > 0 && mk[s0 - 1]) --s0;
+    while (e0 < n - 1 && mk[e0 + 1]) ++e0;
+    const int b0 = max(s0 - 1, 0), bf = min(e0 + 1, n - 1);
+    const double m0 = s[b0], mf = s[bf];
+    return (float)(s[b] - (m0 + (b - b0) / (double)(bf - b0) * (mf - m0)));
+}
+
+// Rows 2p and 2p + 1 as the real and imaginary parts of complex row p;
+// rows off the response path (and the missing partner of an odd last row)
+// are zero.
 __global__ __launch_bounds__(kNfBlock) void k_cn_roi_rows(const float* __restrict__ rows, int ld, int n,
-                                                          const int* __restrict__ row_group,
+                                                          int nrows, const int* __restrict__ row_group,
                                                           const wct_nf_group* __restrict__ groups,
                                                           const unsigned char* __restrict__ marks,
                                                           float2* __restrict__ crow)
 {
-    const int c = blockIdx.x;
-    const int g = row_group[c];
-    if (g < 0 || groups[g].respec < 0) return;
-    const auto s = G(rows + (long)c * ld);
-    const auto mk = G(marks + (long)g * n);
-    const auto dst = G(crow + (long)c * n);
-    for (int b = threadIdx.x; b < n; b += kNfBlock) {
-        float v = 0.f;
-        if (mk[b]) {
-            int s0 = b, e0 = b;
-            while (s0 > 0 && mk[s0 - 1]) --s0;
-            while (e0 < n - 1 && mk[e0 + 1]) ++e0;
-            const int b0 = max(s0 - 1, 0), bf = min(e0 + 1, n - 1);
-            const double m0 = s[b0], mf = s[bf];
-            v = (float)(s[b] - (m0 + (b - b0) / (double)(bf - b0) * (mf - m0)));
-        }
-        dst[b] = make_float2(v, 0.f);
-    }
+    const int p = blockIdx.x;
+    const int ca = 2 * p, cb = 2 * p + 1;
+    const int ga = row_group[ca], gb = cb < nrows ? row_group[cb] : -1;
+    const bool ra = ga >= 0 && groups[ga].respec >= 0, rb = gb >= 0 && groups[gb].respec >= 0;
+    const float* sa = rows + (long)ca * ld;
+    const float* sb = rows + (long)min(cb, nrows - 1) * ld;
+    const unsigned char* ma = marks + (long)(ra ? ga : 0) * n;
+    const unsigned char* mb = marks + (long)(rb ? gb : 0) * n;
+    const auto dst = G(crow + (long)p * n);
+    for (int b = threadIdx.x; b < n; b += kNfBlock)
+        dst[b] = make_float2(ra ? cn_roi_value(sa, ma, n, b) : 0.f, rb ? cn_roi_value(sb, mb, n, b) : 0.f);
 }
 
-__global__ __launch_bounds__(kNfBlock) void k_cn_roi_decon(float2* __restrict__ crow, int n,
+// The two rows of a pair deconvolved for the ROI judgement
+// (cn_decon_spectrum's steps on each): their spectra separated from the
+// pair's, X_a(k) = (Z(k) + Z*(n - k)) / 2 and X_b(k) = (Z(k) - Z*(n - k))
+// / 2i, each divided by its group's response and filtered, then repacked
+// as Y_a + i Y_b (both Hermitian), so the inverse gives the two rows as
+// its real and imaginary parts.  Bins k and n - k belong to one thread.
+__global__ __launch_bounds__(kNfBlock) void k_cn_roi_decon(float2* __restrict__ crow, int n, int nrows,
                                                            const int* __restrict__ row_group,
                                                            const wct_nf_group* __restrict__ groups,
                                                            const float2* __restrict__ spectra)
 {
-    const int c = blockIdx.x;
-    if (row_group[c] < 0) return;
-    const wct_nf_group grp = groups[row_group[c]];
-    if (grp.respec < 0) return;
-    cn_decon_spectrum(crow + (long)c * n, n, spectra + (long)grp.respec * n, true, 0.0);
+    const int p = blockIdx.x;
+    const int ca = 2 * p, cb = 2 * p + 1;
+    const int ga = row_group[ca], gb = cb < nrows ? row_group[cb] : -1;
+    const int ra = ga >= 0 ? groups[ga].respec : -1, rb = gb >= 0 ? groups[gb].respec : -1;
+    if (ra < 0 && rb < 0) return;
+    const auto row = G(crow + (long)p * n);
+    const auto Ra = G(spectra + (long)max(ra, 0) * n), Rb = G(spectra + (long)max(rb, 0) * n);
+    for (int k = threadIdx.x; k <= n / 2; k += kNfBlock) {
+        const int kn = k == 0 ? 0 : n - k;
+        const float2 z = row[k], zn = row[kn];
+        const double f = cn_freq(k, n);
+        const float fac = (float)(cn_filter_time(f) * cn_filter_low_loose(f));
+        float2 ya = make_float2(0.f, 0.f), yb = make_float2(0.f, 0.f);
+        if (ra >= 0) {
+            const float2 xa = make_float2(0.5f * (z.x + zn.x), 0.5f * (z.y - zn.y));
+            ya = cdiv(xa, Ra[k]);
+            ya = make_float2(ya.x * fac, ya.y * fac);
+            if (k == 0 || 2 * k == n) ya.y = 0.f;
+        }
+        if (rb >= 0) {
+            const float2 xb = make_float2(0.5f * (z.y + zn.y), -0.5f * (z.x - zn.x));
+            yb = cdiv(xb, Rb[k]);
+            yb = make_float2(yb.x * fac, yb.y * fac);
+            if (k == 0 || 2 * k == n) yb.y = 0.f;
+        }
+        row[k] = make_float2(ya.x - yb.y, ya.y + yb.x);
+        if (k > 0 && 2 * k != n) row[kn] = make_float2(ya.x + yb.y, -ya.y + yb.x);
+    }
 }
 
 // The subtraction (:124-263): with the response path, the median's runs
@@ -1093,6 +1132,7 @@ __global__ __launch_bounds__(kNfBlock) void k_cn_subtract(float* __restrict__ ro
     const auto mk = G(marks + (long)g * n);
     const float sc = scaling[c];
     const bool resp = grp.respec >= 0;
+    const auto cr = G(reinterpret_cast<const float*>(crow) + (long)(c >> 1) * n * 2 + (c & 1));
     for (int b = threadIdx.x; b < n; b += kNfBlock) {
         float mv = m[b];
         if (resp && mk[b]) {
@@ -1105,7 +1145,7 @@ __global__ __launch_bounds__(kNfBlock) void k_cn_subtract(float* __restrict__ ro
                 int tb = i - grp.res_offset;
                 if (tb < 0) tb += n;
                 if (tb >= n) tb -= n;
-                const double v = G(crow + (long)c * n)[tb].x;
+                const double v = cr[2 * tb];
                 if (i == b0) {
                     mx = mn = v;
                 } else {
@@ -1277,14 +1317,16 @@ extern "C" int wct_nf_coherent(float* rows, int ld, int nrows, int nt, const wct
                        scaling);
     WCT_HIP_CHECK(hipGetLastError());
     if (any_response) {
-        hipLaunchKernelGGL(k_cn_roi_rows, dim3(nrows), dim3(kNfBlock), 0, s, rows, ld, nt, row_group, groups, marks,
-                           crow);
+        // two channel rows per complex row
+        const int npair = (nrows + 1) / 2;
+        hipLaunchKernelGGL(k_cn_roi_rows, dim3(npair), dim3(kNfBlock), 0, s, rows, ld, nt, nrows, row_group, groups,
+                           marks, crow);
         WCT_HIP_CHECK(hipGetLastError());
-        WCT_TRY(launch_rows<float>(crow, crow, nt, nrows, nt, -1, 1.0f, s));
-        hipLaunchKernelGGL(k_cn_roi_decon, dim3(nrows), dim3(kNfBlock), 0, s, crow, nt, row_group, groups,
+        WCT_TRY(launch_rows<float>(crow, crow, nt, npair, nt, -1, 1.0f, s));
+        hipLaunchKernelGGL(k_cn_roi_decon, dim3(npair), dim3(kNfBlock), 0, s, crow, nt, nrows, row_group, groups,
                            reinterpret_cast<const float2*>(spectra));
         WCT_HIP_CHECK(hipGetLastError());
-        WCT_TRY(launch_rows<float>(crow, crow, nt, nrows, nt, +1, 1.0f / nt, s));
+        WCT_TRY(launch_rows<float>(crow, crow, nt, npair, nt, +1, 1.0f / nt, s));
     }
     hipLaunchKernelGGL(k_cn_subtract, dim3(nrows), dim3(kNfBlock), 0, s, rows, ld, nt, row_group, groups, marks, med,
                        crow, scaling);
