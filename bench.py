@@ -211,6 +211,31 @@ def stage_times(batches, reps):
     return [a / reps for a in acc]
 
 
+def signal_split_leg(planes, make_jobs, tuning, reps=5):
+    """The three stages timed on one chunk of noise-only APAs and on one
+    chunk of APAs with tracks (every other APA carries them), each as its
+    own batch of the tuned shape: the c2r stage skips its +-500 window pass
+    on rows whose histogram proves every sample inside the window, which
+    every noise-only row does and rows crossed by tracks may not."""
+    from wct_amd import decon
+    per = 3 * tuning["chunk_apas"]
+    sets = {"noise_only": [i for i in range(len(planes)) if (i // 3) % 2 == 0][:per],
+            "tracks": [i for i in range(len(planes)) if (i // 3) % 2 == 1][:per]}
+    if min(len(v) for v in sets.values()) < per:
+        return None
+    out = {}
+    for name, idx in sets.items():
+        rp = tuning["row_plan"]
+        b = decon.Batch(make_jobs(idx), wire_tile=tuning["wire_tile"], row_grid_pct=tuning["row_grid_pct"],
+                        row_plan=rp[0], r2c_blocks=rp[1], c2r_blocks=rp[2])
+        b.run()
+        ms = stage_times([b], reps)
+        out[name] = {"apas": per // 3, "stage_ms": [round(m, 4) for m in ms]}
+        b.close()
+    out["c2r_ratio"] = round(out["tracks"]["stage_ms"][2] / out["noise_only"]["stage_ms"][2], 4)
+    return out
+
+
 def stage_alg_bytes(planes, kern):
     """Algorithmic bytes of each stage per step: r2c reads the frame (4 B
     per sample) and writes the kept bins (8 B each); the wire pass reads
@@ -734,6 +759,9 @@ def main():
                  "sample": f"{pn} planes ({pn // 3} APAs) of the same frames, one launch per "
                            f"reference step, {pms:.2f} ms per pass"}
 
+    split = signal_split_leg(planes, make_jobs, tuning)
+    log(f"bench: stages on noise-only and track APAs {split}")
+
     accuracy, acc_ok = None, True
     if not args.no_accuracy:
         accuracy = accuracy_leg(planes, frames, kern, dev, apa=1 if len(planes) >= 6 else 0)
@@ -819,6 +847,7 @@ def main():
         "accuracy_fp64": accuracy,
         "response_build_fp64": resp_build,
         "plain_gpu": plain,
+        "stats_split": split,
         "adjoint_step": adjoint,
         "forward_sim": impact,
         "noise_filter": noise,
