@@ -28,7 +28,7 @@ def _relerr(a, b):
 
 
 SIZES = [1, 2, 3, 5, 6, 8, 15, 16, 25, 30, 60, 64, 77, 120, 121, 128, 143, 169, 480, 800,
-         960, 1000, 1024, 3000, 4096, 6000, 8000]
+         960, 1000, 1024, 3000, 4096, 6000, 6272, 8000]
 
 
 @pytest.mark.parametrize("n", SIZES)
@@ -72,7 +72,7 @@ def test_impulse_and_zero(dev):
     assert np.abs(x2.cpu().numpy() - r2).max() < 1e-6
 
 
-@pytest.mark.parametrize("shape", [(2, 8), (8, 2), (3, 5), (800, 6000), (960, 1000), (33, 77)])
+@pytest.mark.parametrize("shape", [(2, 8), (8, 2), (3, 5), (800, 6000), (960, 1000), (33, 77), (9, 6272), (5, 8000)])
 @pytest.mark.parametrize("axis", [0, 1])
 def test_1b(dev, shape, axis):
     # test_idft.cxx:79-131 impulse on (2,8)/(8,2) plus random data
