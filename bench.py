@@ -10,7 +10,7 @@ writes the decon frame (fp32) and the 3-sigma ROI mask (u8).  The planes
 are processed in chunks of --chunk APAs whose spectra share one
 workspace, three launches per chunk.
 
-Prints one JSON line (the driver's contract) with the roofline of the
+Prints one JSON line (metric, value, config) with the roofline of the
 dominant kernel (the stage with the longest launch, timed on its own
 with HIP events after the timed steps; the whole pipeline's figures sit
 under roofline.pipeline) and the CPU baseline (the oracle's FP32
