@@ -491,6 +491,19 @@ def noise_leg(dev, napas=4, reps=10, cpu=True):
         stream.cuda_stream))
     coh()
     coh_ms = timed(coh, reps)
+    # the status filter (mbOneChannelStatus) on the induction channels of
+    # the filtered rows: Threshold 3.5, Window 5, Nbins 250, Cut 14
+    ind = torch.cat([out[a * per_apa:a * per_apa + 1600] for a in range(napas)]).contiguous()
+    nind = ind.shape[0]
+    sws_bytes = lib().wct_nf_status_workspace_size(nind, nt)
+    sws = torch.empty(sws_bytes, dtype=torch.uint8, device=dev)
+    sflags = torch.empty(nind, dtype=torch.int32, device=dev)
+    stat = lambda: check(lib().wct_nf_lf_noisy(
+        C.c_void_p(ind.data_ptr()), nt, nind, nt, 3.5, 5, 250, 14.0, C.c_void_p(sflags.data_ptr()), None,
+        C.c_void_p(sws.data_ptr()), sws_bytes, stream.cuda_stream))
+    stat()
+    stat_ms = timed(stat, reps)
+    nlf = int(sflags.sum().item())
     ct = nch * nt
     res = {
         "workload": f"pdOneChannelNoise on {napas} APAs x {per_apa} channels x {nt} ticks",
@@ -500,6 +513,8 @@ def noise_leg(dev, napas=4, reps=10, cpu=True):
         "row_transform_ms": round(fft_ms, 4),
         "coherent": {"groups": len(groups), "ms": round(coh_ms, 4),
                      "channel_ticks_per_s": ct / (coh_ms / 1e3)},
+        "status": {"channels": nind, "ms": round(stat_ms, 4), "lf_noisy": nlf,
+                   "channel_ticks_per_s": nind * nt / (stat_ms / 1e3)},
         "partial": int(((fl & 1) != 0).sum()), "noisy": int(((fl & 2) != 0).sum()), "dtype": "f32",
     }
     if cpu:

@@ -527,6 +527,21 @@ int wct_nf_coherent(float* rows, int ld, int nrows, int nt, const wct_nf_group* 
  * [gain_min, gain_max]; gain is a device array. */
 int wct_nf_rel_gain(float* rows, int ld, int nch, int nt, const float* gain, float gain_min, float gain_max,
                     void* stream);
+/* Microboone::OneChannelStatus::ID_lf_noisy (sigproc/src/Microboone.cxx:
+ * 1281-1334) on nch rows of nt samples (device, row stride ld): per row
+ * the binned median and 16/84 % points (Waveform::percentile_binned), the
+ * samples within [i - window, i + window) of a sample i beyond threshold x
+ * rms from the median set to the median, the float magnitudes of bins
+ * 1..nbins of their spectrum summed, over the row's count of non-zero
+ * samples; lf_noisy[c] (device, nch ints) = 1 when that ratio exceeds cut
+ * (0 for a row with no non-zero sample), ratio (device, nch doubles, may
+ * be NULL) receives it.  window >= 0, 1 <= nbins < nt (the reference's
+ * Threshold, Window, Nbins and Cut).  The caller picks the channels
+ * (OneChannelStatus::apply, :1263-1279, tests channels off the collection
+ * plane only). */
+size_t wct_nf_status_workspace_size(int nch, int nt);
+int wct_nf_lf_noisy(const float* rows, int ld, int nch, int nt, double threshold, int window, int nbins, double cut,
+                    int* lf_noisy, double* ratio, void* workspace, size_t ws_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,8 +1,10 @@
 """Restatement of Wire-Cell's per-channel noise filters: OmnibusNoiseFilter
 (sigproc/src/OmnibusNoiseFilter.cxx:95-260) with Protodune::OneChannelNoise
 (sigproc/src/Protodune.cxx:753-918) and RelGainCalib (:975-1009),
-Microboone::OneChannelNoise (sigproc/src/Microboone.cxx:902-1064) and the
-parts of OmniChannelNoiseDB they read (sigproc/src/OmniChannelNoiseDB.cxx).
+Microboone::OneChannelNoise (sigproc/src/Microboone.cxx:902-1064), the
+coherent subtraction (:53-462, :798-872), OneChannelStatus (:1263-1334)
+and the parts of OmniChannelNoiseDB they read
+(sigproc/src/OmniChannelNoiseDB.cxx).
 
 TEST INFRASTRUCTURE ONLY.  Only tests/ and the CPU-baseline leg of the
 benchmark may import this module, and only as the checker.  Float
@@ -649,3 +651,53 @@ def coherent_noise_sub(signals, respec=None, res_offset=0, pad_f=0, pad_b=0, dec
                              protection_factor, min_adc_limit)
     subtract_wscaling(signals, med, respec, res_offset, rois, decon_limit1, roi_min_max_ratio, rms_threshold)
     return signals
+
+
+# ---- Microboone::OneChannelStatus (Microboone.cxx:1263-1334) -----------
+
+def lf_noisy_masked(sig, threshold=3.5, window=5):
+    """ID_lf_noisy's first part (:1287-1318): the binned median and 16/84 %
+    points, rms from them (double), the non-zero count, and a copy of the
+    samples in which every sample within [i - window, i + window) of a
+    sample i beyond threshold x rms from the median is set to the median
+    (marks from the original samples).  Returns (masked, median, rms,
+    valid)."""
+    s = np.asarray(sig, dtype=F)
+    mean = float(percentile_binned(s, 0.5))
+    val1 = float(percentile_binned(s, 0.5 - 0.34))
+    val2 = float(percentile_binned(s, 0.5 + 0.34))
+    rms = math.sqrt(((val1 - mean) ** 2 + (val2 - mean) ** 2) / 2.0)
+    valid = int(np.count_nonzero(s))
+    out = s.copy()
+    n = len(s)
+    for i in range(n):
+        if abs(float(s[i]) - mean) > threshold * rms:
+            for k in range(-window, window):
+                if 0 <= k + i < n:
+                    out[k + i] = F(mean)
+    return out, mean, rms, valid
+
+
+def lf_noisy(sig, threshold=3.5, window=5, nbins=250, cut=14.0):
+    """OneChannelStatus::ID_lf_noisy (:1281-1334): the summed magnitude of
+    bins 1..nbins of the masked samples' spectrum (float transform) over
+    the count of non-zero samples, against cut.  Returns (flag, ratio);
+    no non-zero sample: (False, 0)."""
+    masked, _, _, valid = lf_noisy_masked(sig, threshold, window)
+    if not valid:
+        return False, 0.0
+    spec = fwd_r2c(masked)
+    content = 0.0
+    for i in range(nbins):
+        content += float(F(abs(spec[i + 1])))
+    return content / valid > cut, content / valid
+
+
+def one_channel_status(signal, plane, threshold=3.5, window=5, nbins=250, cut=14.0):
+    """OneChannelStatus::apply (:1263-1279): off the collection plane, the
+    whole channel is masked lf_noisy when ID_lf_noisy holds.  Returns the
+    mask list of the channel ([(0, n)] or [])."""
+    if plane == 2:
+        return []
+    flag, _ = lf_noisy(signal, threshold, window, nbins, cut)
+    return [(0, len(signal))] if flag else []

@@ -464,3 +464,98 @@ def test_grouped_filter_component():
             d = np.abs(got[8 * k + i].astype(np.float64) - want[i])
             assert d.max() <= 1e-3, (k, i, d.max())
     CO.reset()
+
+
+def _status_rows(rng, nt, n=43):
+    """Rows for OneChannelStatus: unit noise (a ratio near 5 at 2000
+    ticks), plus a sine inside the 250 summed bins at amplitude 40 (ratio
+    near 25) or 12 (near 11), a sine past them at 60, or a sine of 40 with
+    five 500 ADC spikes; then a zero row and a constant row."""
+    i = np.arange(nt)
+    rows = []
+    for k in range(n - 2):
+        r = rng.normal(0, 1.0, nt)
+        kind = k % 5
+        ph = rng.uniform(0, 2 * np.pi)
+        if kind in (1, 4):
+            r += 40 * np.sin(2 * np.pi * rng.integers(1, 251) * i / nt + ph)
+        elif kind == 2:
+            r += 12 * np.sin(2 * np.pi * rng.integers(1, 251) * i / nt + ph)
+        elif kind == 3:
+            r += 60 * np.sin(2 * np.pi * rng.integers(300, 900) * i / nt + ph)
+        if kind == 4:
+            r[rng.integers(0, nt, 5)] += 500.0
+        rows.append(r)
+    rows.append(np.zeros(nt))
+    rows.append(np.full(nt, 300.0))
+    return np.asarray(rows, np.float32)
+
+
+def test_lf_noisy_matches_oracle():
+    """wct_nf_lf_noisy on an odd number of rows against the oracle's
+    ID_lf_noisy: ratios within 1e-4 (FP32 transforms of two channels at
+    once against the oracle's), flags equal away from the cut."""
+    _torch()
+    from wct_amd import noise as N
+    rng = np.random.default_rng(1281)
+    nt = 2000
+    rows = _status_rows(rng, nt)
+    print(f"lf_noisy {rows.shape}", flush=True)
+    flags, ratio = N.lf_noisy(rows)
+    nflag = 0
+    for c in range(len(rows)):
+        want, wr = NF.lf_noisy(rows[c])
+        assert abs(ratio[c] - wr) <= 1e-4 * max(wr, 1.0), (c, ratio[c], wr)
+        if abs(wr - 14.0) > 0.01:
+            assert bool(flags[c]) == want, (c, flags[c], want, wr)
+        nflag += want
+    assert 10 <= nflag <= 30
+    assert not flags[-2] and ratio[-2] == 0             # the zero row
+    assert not flags[-1] and ratio[-1] < 1e-3           # the constant row: only bin 0
+    # other parameters
+    flags, ratio = N.lf_noisy(rows[:8], threshold=2.5, window=9, nbins=600, cut=8.0)
+    for c in range(8):
+        want, wr = NF.lf_noisy(rows[c], 2.5, 9, 600, 8.0)
+        assert abs(ratio[c] - wr) <= 1e-4 * max(wr, 1.0), (c, ratio[c], wr)
+        if abs(wr - 8.0) > 0.01:
+            assert bool(flags[c]) == want
+
+
+def test_status_filter_component():
+    """OmnibusNoiseFilter with mbOneChannelStatus as its status filter:
+    the induction channels whose oracle ID_lf_noisy holds are masked
+    lf_noisy over the whole waveform, collection channels never; the
+    samples pass unchanged."""
+    _torch()
+    from wct_amd import components as CO
+    rng = np.random.default_rng(1263)
+    nt = 2000
+    rows = _status_rows(rng, nt, n=26)[:24]
+    cfg = [
+        {"type": "AnodeChannels", "name": "sa", "data": {"ident": 0, "plane_ranges": [[0, 8], [8, 8], [16, 8]]}},
+        {"type": "OmniChannelNoiseDB", "name": "sdb", "data": {
+            "anode": "AnodeChannels:sa", "tick": TICK, "nsamples": nt, "bad": [],
+            "channel_info": [{"channels": {"first": 0, "last": 23}, "min_rms_cut": 0.0, "max_rms_cut": 1000.0}]}},
+        {"type": "mbOneChannelStatus", "name": "st", "data": {"anode": "AnodeChannels:sa", "Threshold": 3.5,
+                                                              "Window": 5, "Nbins": 250, "Cut": 14}},
+        {"type": "OmnibusNoiseFilter", "name": "snf", "data": {
+            "nticks": 0, "maskmap": {}, "channel_filters": [], "grouped_filters": [],
+            "channel_status_filters": ["mbOneChannelStatus:st"], "noisedb": "OmniChannelNoiseDB:sdb",
+            "intraces": "orig", "outtraces": "raw"}},
+    ]
+    CO.reset()
+    CO.configure(cfg)
+    fr = CO.Frame(ident=7, time=0.0, tick=TICK)
+    for ch in range(24):
+        fr.add_trace(ch, rows[ch], 0)
+    fr.tag_traces("orig", list(range(24)))
+    out = CO.frame_filter("OmnibusNoiseFilter:snf", fr)
+    got = {ch: q for ch, tb, q in out.traces()}
+    want = {ch for ch in range(16) if NF.one_channel_status(rows[ch], ch // 8)}
+    assert want and any(NF.lf_noisy(rows[ch])[0] for ch in range(16, 24))   # a collection channel that would be
+    m = out.masks("lf_noisy")
+    assert set(m) == want, (set(m), want)
+    assert all(v == [(0, nt)] for v in m.values())
+    for ch in range(24):
+        assert np.array_equal(got[ch], rows[ch])
+    CO.reset()

@@ -339,3 +339,64 @@ def test_coherent_noise_sub_protects_signal():
     for o in out:
         assert float(np.std(o[:600])) < 1.5               # common mode removed away from the pulse
         assert float(o[700]) > 250                         # the pulse itself survives (protected)
+
+
+# ---- Microboone::OneChannelStatus (Microboone.cxx:1263-1334) ------------
+
+def _sine(n, k0, amp, phase=0.3):
+    return (amp * np.sin(2 * np.pi * k0 * np.arange(n) / n + phase)).astype(F)
+
+
+def test_lf_noisy_sine_closed_form():
+    """A sine at bin k0 of amplitude A has |X(k0)| = A n / 2 and no other
+    bin in 1..n/2: the ratio is A / 2 when 1 <= k0 <= nbins (every sample
+    non-zero at this phase, none beyond 3.5 rms of a sine), 0 when k0 is
+    past nbins."""
+    n = 1000
+    for k0, inside in ((1, True), (250, True), (251, False), (400, False)):
+        s = _sine(n, k0, 40.0)
+        assert np.count_nonzero(s) == n
+        flag, ratio = NF.lf_noisy(s)
+        if inside:
+            assert abs(ratio - 20.0) < 2e-3, (k0, ratio)
+            assert flag
+        else:
+            assert ratio < 1e-3, (k0, ratio)
+            assert not flag
+    flag, ratio = NF.lf_noisy(_sine(n, 5, 20.0))        # ratio 10 below the cut of 14
+    assert not flag and abs(ratio - 10.0) < 1e-3
+    flag, ratio = NF.lf_noisy(_sine(n, 5, 20.0), cut=9.5)
+    assert flag
+
+
+def test_lf_noisy_dc_and_empty():
+    # only bin 0 (never summed): a constant row, whatever its level
+    flag, ratio = NF.lf_noisy(np.full(1000, 500.0, F))
+    assert not flag and ratio < 1e-6
+    # no non-zero sample: not noisy (:1299-1301)
+    assert NF.lf_noisy(np.zeros(1000, F)) == (False, 0.0)
+
+
+def test_lf_noisy_mask_window():
+    """A spike at i sets samples [i - 5, i + 5) to the median, clipped at
+    the row's ends; marks come from the original samples."""
+    n = 2000
+    for at, lo, hi in ((100, 95, 105), (2, 0, 7), (n - 3, n - 8, n)):
+        s = _sine(n, 3, 10.0)
+        s[at] = 1000.0
+        masked, med, rms, valid = NF.lf_noisy_masked(s, 3.5, 5)
+        assert valid == n
+        assert np.flatnonzero(np.abs(s.astype(np.float64) - med) > 3.5 * rms).tolist() == [at]
+        assert np.flatnonzero(masked != s).tolist() == [i for i in range(lo, hi) if s[i] != F(med)]
+        assert np.all(masked[lo:hi] == F(med))
+    # window 0: nothing changes
+    masked, *_ = NF.lf_noisy_masked(s, 3.5, 0)
+    assert np.array_equal(masked, s)
+
+
+def test_one_channel_status_planes():
+    s = _sine(1000, 3, 40.0)
+    assert NF.one_channel_status(s, plane=0) == [(0, 1000)]
+    assert NF.one_channel_status(s, plane=1) == [(0, 1000)]
+    assert NF.one_channel_status(s, plane=2) == []      # collection is not tested
+    assert NF.one_channel_status(_sine(1000, 3, 10.0), plane=0) == []

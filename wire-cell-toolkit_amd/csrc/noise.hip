@@ -1163,6 +1163,84 @@ __global__ __launch_bounds__(kNfBlock) void k_cn_subtract(float* __restrict__ ro
     }
 }
 
+// ---- Microboone::OneChannelStatus (Microboone.cxx:1263-1334) ----------
+
+// ID_lf_noisy's samples (:1287-1318) for channel c: the binned median and
+// 16/84 % points of the row from one histogram, the rms from them in
+// double, the count of non-zero samples, and the row with every sample
+// within [i - window, i + window) of a sample i beyond threshold x rms
+// from the median set to the median (marks from the original samples);
+// written as the real (even c) or imaginary (odd c) part of complex row
+// c / 2 (the partner of an odd last row zero).
+__global__ __launch_bounds__(kNfBlock) void k_st_load(const float* __restrict__ in, int ld, int nch, int n,
+                                                      double threshold, int window, float2* __restrict__ rows,
+                                                      int* __restrict__ valid)
+{
+    extern __shared__ __align__(16) unsigned char nf_smem[];
+    __shared__ NfShared sh;
+    const int c = blockIdx.x;
+    const size_t a4 = ((size_t)4 * n + 15) / 16 * 16;
+    float* s = reinterpret_cast<float*>(nf_smem);
+    int* hist = reinterpret_cast<int*>(nf_smem + a4);
+    unsigned char* mk = nf_smem + 2 * a4;
+    const auto src = G(in + (long)c * ld);
+    int nz = 0;
+    for (int i = threadIdx.x; i < n; i += kNfBlock) {
+        const float v = src[i];
+        s[i] = v;
+        nz += v != 0.f;
+    }
+    __syncthreads();
+    const float p[3] = {0.5f, (float)(0.5 - 0.34), (float)(0.5 + 0.34)};
+    float q[3];
+    pct_binned(s, n, AnySample(), p, 3, q, hist, sh);
+    nz = block_sum(nz, sh);
+    const double mean = q[0], da = (double)q[1] - mean, db = (double)q[2] - mean;
+    const double lim = threshold * sqrt((da * da + db * db) / 2.);
+    for (int i = threadIdx.x; i < n; i += kNfBlock) mk[i] = fabs((double)s[i] - mean) > lim;
+    __syncthreads();
+    const auto dst = G(reinterpret_cast<float*>(rows) + (long)(c >> 1) * n * 2 + (c & 1));
+    const bool lone = (c & 1) == 0 && c + 1 == nch;
+    for (int j = threadIdx.x; j < n; j += kNfBlock) {
+        bool hit = false;   // a marked i with j in [i - window, i + window)
+        const int i1 = min(n - 1, j + window);
+        for (int i = max(0, j - window + 1); i <= i1; ++i) hit |= mk[i] != 0;
+        dst[2 * j] = hit ? (float)mean : s[j];
+        if (lone) dst[2 * j + 1] = 0.f;
+    }
+    if (threadIdx.x == 0) valid[c] = nz;
+}
+
+// :1320-1333 for the two channels of complex row p: each spectrum
+// separated from the pair's (X_a(k) = (Z(k) + Z*(n - k)) / 2, X_b(k) =
+// (Z(k) - Z*(n - k)) / 2i), the float magnitudes of bins 1..nbins summed
+// in double, over the channel's non-zero count, against cut.
+__global__ __launch_bounds__(kNfBlock) void k_st_ratio(const float2* __restrict__ rows, int nch, int n, int nbins,
+                                                       double cut, const int* __restrict__ valid,
+                                                       int* __restrict__ flag, double* __restrict__ ratio)
+{
+    __shared__ NfShared sh;
+    const int p = blockIdx.x;
+    const auto row = G(rows + (long)p * n);
+    double ca = 0, cb = 0;
+    for (int k = 1 + (int)threadIdx.x; k <= nbins; k += kNfBlock) {
+        const float2 z = row[k], zn = row[n - k];
+        ca += (double)hypotf(0.5f * (z.x + zn.x), 0.5f * (z.y - zn.y));
+        cb += (double)hypotf(0.5f * (z.y + zn.y), -0.5f * (z.x - zn.x));
+    }
+    ca = block_sum(ca, sh);
+    cb = block_sum(cb, sh);
+    if (threadIdx.x < 2) {
+        const int c = 2 * p + (int)threadIdx.x;
+        if (c < nch) {
+            const int v = G(valid)[c];
+            const double r = v ? (threadIdx.x ? cb : ca) / v : 0.0;
+            G(flag)[c] = v && r > cut;
+            if (ratio) G(ratio)[c] = r;
+        }
+    }
+}
+
 }  // namespace
 
 }  // namespace wct
@@ -1330,6 +1408,43 @@ extern "C" int wct_nf_coherent(float* rows, int ld, int nrows, int nt, const wct
     }
     hipLaunchKernelGGL(k_cn_subtract, dim3(nrows), dim3(kNfBlock), 0, s, rows, ld, nt, row_group, groups, marks, med,
                        crow, scaling);
+    WCT_HIP_CHECK(hipGetLastError());
+    return WCT_OK;
+}
+
+extern "C" size_t wct_nf_status_workspace_size(int nch, int nt)
+{
+    if (nch <= 0 || nt <= 0) return 0;
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    return al(sizeof(float2) * (size_t)((nch + 1) / 2) * nt) + al(sizeof(int) * (size_t)nch);
+}
+
+extern "C" int wct_nf_lf_noisy(const float* rows, int ld, int nch, int nt, double threshold, int window, int nbins,
+                               double cut, int* lf_noisy, double* ratio, void* workspace, size_t ws_bytes,
+                               void* stream)
+{
+    WCT_ENTRY();
+    WCT_REQUIRE(rows && lf_noisy && nch >= 1 && nt >= 2 && ld >= nt && window >= 0 && nbins >= 1 && nbins < nt,
+                WCT_ERR_ARG, "nf lf_noisy: bad arguments (nch %d, nt %d, ld %d, window %d, nbins %d)", nch, nt, ld,
+                window, nbins);
+    WCT_REQUIRE(workspace && ws_bytes >= wct_nf_status_workspace_size(nch, nt), WCT_ERR_ARG,
+                "nf lf_noisy: workspace of %zu bytes, %zu needed", ws_bytes, wct_nf_status_workspace_size(nch, nt));
+    const size_t lds = 2 * (((size_t)4 * nt + 15) / 16 * 16) + (size_t)nt;
+    WCT_REQUIRE((long)lds + (long)sizeof(NfShared) <= lds_limit_bytes(), WCT_ERR_UNSUPPORTED,
+                "nf lf_noisy: %d ticks need %zu bytes of LDS per channel", nt, lds);
+    hipStream_t s = as_stream(stream);
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const int npair = (nch + 1) / 2;
+    float2* crow = reinterpret_cast<float2*>(workspace);
+    int* valid = reinterpret_cast<int*>(reinterpret_cast<unsigned char*>(workspace) +
+                                        al(sizeof(float2) * (size_t)npair * nt));
+    WCT_TRY(allow_lds(k_st_load, lds));
+    hipLaunchKernelGGL(k_st_load, dim3(nch), dim3(kNfBlock), lds, s, rows, ld, nch, nt, threshold, window, crow,
+                       valid);
+    WCT_HIP_CHECK(hipGetLastError());
+    WCT_TRY(launch_rows<float>(crow, crow, nt, npair, nt, -1, 1.0f, s));
+    hipLaunchKernelGGL(k_st_ratio, dim3(npair), dim3(kNfBlock), 0, s, crow, nch, nt, nbins, cut, valid, lf_noisy,
+                       ratio);
     WCT_HIP_CHECK(hipGetLastError());
     return WCT_OK;
 }

@@ -722,6 +722,93 @@ class RelGainCalib : public IChannelFilter, public IBatchChannelFilter, public I
     mutable DevBuf m_d_rows, m_d_gain;
 };
 
+// ---- mbOneChannelStatus (sigproc/src/Microboone.cxx:1218-1334) ---------
+// A status filter: channels off the collection plane whose low-frequency
+// content (ID_lf_noisy) exceeds the cut are masked lf_noisy over the whole
+// waveform; the samples are not changed.  Every tested channel of a frame
+// in one device batch (wct_nf_lf_noisy).
+class OneChannelStatus : public IChannelFilter, public IBatchChannelFilter, public IConfigurable {
+  public:
+    ~OneChannelStatus() override
+    {
+        if (m_stream) wct_stream_destroy(m_stream);
+    }
+    Configuration default_configuration() const override   // :1245-1255
+    {
+        Configuration cfg;
+        cfg["Threshold"] = m_threshold;
+        cfg["Window"] = m_window;
+        cfg["Nbins"] = m_nbins;
+        cfg["Cut"] = m_cut;
+        cfg["anode"] = m_anode_tn;
+        cfg["dft"] = "HipDFT";
+        return cfg;
+    }
+    void configure(const Configuration& cfg) override   // :1229-1243
+    {
+        m_threshold = get(cfg, "Threshold", m_threshold);
+        m_window = get(cfg, "Window", m_window);
+        m_nbins = (int)get<double>(cfg, "Nbins", m_nbins);
+        m_cut = get(cfg, "Cut", m_cut);
+        if (m_window < 0) throw ConfigError("mbOneChannelStatus: Window below 0");
+        if (m_nbins < 1) throw ConfigError("mbOneChannelStatus: Nbins below 1");
+        m_anode_tn = get<std::string>(cfg, "anode", m_anode_tn);
+        m_planes.build(*Factory::find_tn<IAnodeChannels>(m_anode_tn));
+    }
+    Waveform::ChannelMaskMap apply(int ch, signal_t& signal) const override
+    {
+        std::vector<float> rows(signal.begin(), signal.end());
+        std::vector<Waveform::ChannelMaskMap> masks;
+        apply_batch({ch}, rows, (int)signal.size(), masks);
+        return masks[0];
+    }
+    Waveform::ChannelMaskMap apply(channel_signals_t&) const override { return Waveform::ChannelMaskMap(); }
+    void apply_batch(const std::vector<int>& channels, std::vector<float>& rows, int nt,
+                     std::vector<Waveform::ChannelMaskMap>& masks) const override
+    {
+        std::lock_guard<std::mutex> lock(m_mutex);
+        const int nch = (int)channels.size();
+        masks.assign(nch, Waveform::ChannelMaskMap());
+        std::vector<int> pick;   // rows off the collection plane (:1267)
+        for (int i = 0; i < nch; ++i)
+            if (m_planes(channels[i]) != 2) pick.push_back(i);
+        if (pick.empty() || nt == 0) return;
+        if (m_nbins >= nt)   // the reference's at(nbins) would throw
+            throw std::runtime_error("mbOneChannelStatus: Nbins " + std::to_string(m_nbins) + " needs more than " +
+                                     std::to_string(nt) + " ticks");
+        const int np = (int)pick.size();
+        std::vector<float> sel((size_t)np * nt);
+        for (int k = 0; k < np; ++k)
+            std::copy(rows.begin() + (size_t)pick[k] * nt, rows.begin() + (size_t)(pick[k] + 1) * nt,
+                      sel.begin() + (size_t)k * nt);
+        if (!m_stream) ncheck(wct_stream_create(&m_stream), "mbOneChannelStatus stream");
+        const size_t bytes = sizeof(float) * sel.size();
+        const size_t ws = wct_nf_status_workspace_size(np, nt);
+        m_d_rows.ensure(bytes);
+        m_d_flags.ensure(sizeof(int) * np);
+        m_d_ws.ensure(ws);
+        ncheck(wct_memcpy_htod_async(m_d_rows.p, sel.data(), bytes, m_stream), "mbOneChannelStatus upload");
+        ncheck(wct_nf_lf_noisy((const float*)m_d_rows.p, nt, np, nt, m_threshold, m_window, m_nbins, m_cut,
+                               (int*)m_d_flags.p, nullptr, m_d_ws.p, ws, m_stream),
+               "mbOneChannelStatus");
+        std::vector<int> flags(np);
+        ncheck(wct_memcpy_dtoh_async(flags.data(), m_d_flags.p, sizeof(int) * np, m_stream),
+               "mbOneChannelStatus download");
+        ncheck(wct_stream_sync(m_stream), "mbOneChannelStatus sync");
+        for (int k = 0; k < np; ++k)
+            if (flags[k]) masks[pick[k]]["lf_noisy"][channels[pick[k]]].push_back({0, nt});
+    }
+
+  private:
+    std::string m_anode_tn{"AnodeChannels"};
+    double m_threshold{3.5}, m_cut{14.0};
+    int m_window{5}, m_nbins{250};
+    PlaneLookup m_planes;
+    mutable std::mutex m_mutex;
+    mutable void* m_stream{nullptr};
+    mutable DevBuf m_d_rows, m_d_flags, m_d_ws;
+};
+
 // ---- mbCoherentNoiseSub (sigproc/src/Microboone.cxx:790-892) -----------
 class CoherentNoiseSub : public IChannelFilter, public IBatchGroupFilter, public IConfigurable {
   public:
@@ -1053,4 +1140,5 @@ WIRECELL_HIP_FACTORY(pdOneChannelNoise, WireCell::Hip::PdOneChannelNoise)
 WIRECELL_HIP_FACTORY(mbOneChannelNoise, WireCell::Hip::MbOneChannelNoise)
 WIRECELL_HIP_FACTORY(pdRelGainCalib, WireCell::Hip::RelGainCalib)
 WIRECELL_HIP_FACTORY(mbCoherentNoiseSub, WireCell::Hip::CoherentNoiseSub)
+WIRECELL_HIP_FACTORY(mbOneChannelStatus, WireCell::Hip::OneChannelStatus)
 WIRECELL_HIP_FACTORY(OmnibusNoiseFilter, WireCell::Hip::OmnibusNoiseFilter)

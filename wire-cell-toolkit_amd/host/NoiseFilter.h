@@ -9,6 +9,8 @@
 //   pdRelGainCalib         sigproc/src/Protodune.cxx:925-1014
 //   mbCoherentNoiseSub     sigproc/src/Microboone.cxx:790-892, every
 //                          channel group of a frame in one device batch
+//   mbOneChannelStatus     sigproc/src/Microboone.cxx:1218-1334, the
+//                          same
 //   OmnibusNoiseFilter     sigproc/src/OmnibusNoiseFilter.cxx
 #pragma once
 #include <complex>

@@ -119,6 +119,23 @@ def rel_gain(frame, gains, gain_min=0.8, gain_max=1.25, stream=None):
     return x
 
 
+def lf_noisy(frame, threshold=3.5, window=5, nbins=250, cut=14.0, stream=None):
+    """Microboone::OneChannelStatus::ID_lf_noisy on every row of frame:
+    (flags, ratios) as numpy arrays (bool, float64)."""
+    import torch
+    x = torch.as_tensor(frame, dtype=torch.float32, device="cuda").contiguous()
+    nch, nt = x.shape
+    flags = torch.empty(nch, dtype=torch.int32, device="cuda")
+    ratio = torch.empty(nch, dtype=torch.float64, device="cuda")
+    ws_bytes = lib().wct_nf_status_workspace_size(nch, nt)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+    check(lib().wct_nf_lf_noisy(C.c_void_p(x.data_ptr()), nt, nch, nt, float(threshold), int(window), int(nbins),
+                                float(cut), C.c_void_p(flags.data_ptr()), C.c_void_p(ratio.data_ptr()),
+                                C.c_void_p(ws.data_ptr()), ws_bytes, _stream() if stream is None else stream))
+    torch.cuda.synchronize()
+    return flags.cpu().numpy().astype(bool), ratio.cpu().numpy()
+
+
 GROUP_DEFAULTS = dict(respec=-1, res_offset=0, pad_f=0, pad_b=0, decon_limit=0.02, decon_lf_cutoff=0.08,
                       adc_limit=0.0, decon_limit1=0.08, roi_min_max_ratio=0.8, protection_factor=5.0,
                       min_adc_limit=50.0)
