@@ -504,6 +504,15 @@ def noise_leg(dev, napas=4, reps=10, cpu=True):
     stat()
     stat_ms = timed(stat, reps)
     nlf = int(sflags.sum().item())
+    # the bit-shift test (mbADCBitShift, 12 bits, 500 ticks examined) on
+    # every raw row: none is shifted, so this is the detection alone
+    bsx = x.clone()
+    bns = torch.empty(nch, dtype=torch.int32, device=dev)
+    bshift = lambda: check(lib().wct_nf_adc_bit_shift(
+        C.c_void_p(bsx.data_ptr()), nt, nch, nt, 12, 500, 7.5, 0.8, C.c_void_p(bns.data_ptr()), stream.cuda_stream))
+    bshift()
+    bs_ms = timed(bshift, reps)
+    nbs = int((bns > 0).sum().item())
     ct = nch * nt
     res = {
         "workload": f"pdOneChannelNoise on {napas} APAs x {per_apa} channels x {nt} ticks",
@@ -515,6 +524,7 @@ def noise_leg(dev, napas=4, reps=10, cpu=True):
                      "channel_ticks_per_s": ct / (coh_ms / 1e3)},
         "status": {"channels": nind, "ms": round(stat_ms, 4), "lf_noisy": nlf,
                    "channel_ticks_per_s": nind * nt / (stat_ms / 1e3)},
+        "bit_shift": {"channels": nch, "ms": round(bs_ms, 4), "shifted": nbs},
         "partial": int(((fl & 1) != 0).sum()), "noisy": int(((fl & 2) != 0).sum()), "dtype": "f32",
     }
     if cpu:

@@ -539,6 +539,17 @@ int wct_nf_rel_gain(float* rows, int ld, int nch, int nt, const float* gain, flo
  * Threshold, Window, Nbins and Cut).  The caller picks the channels
  * (OneChannelStatus::apply, :1263-1279, tests channels off the collection
  * plane only). */
+/* Microboone::ADCBitShift::apply (sigproc/src/Microboone.cxx:1106-1216)
+ * on nch rows of nt samples in place (device, row stride ld): the low
+ * bits that rarely flip over the first exam_nticks samples give nshift;
+ * for 0 < nshift < 11 every sample is rotated back (Bits::shift_right with
+ * its own low nshift bits as filling, util/src/Bits.cxx:4-9) and samples
+ * far from the mean and a neighbour take the filling nearest the
+ * neighbours' extrapolation.  nshift (device, nch ints) receives each
+ * row's count (0: unchanged).  1 <= nbits <= 31, 1 <= exam_nticks <= nt
+ * (the reference's Number_of_ADC_bits and Exam_number_of_ticks_test). */
+int wct_nf_adc_bit_shift(float* rows, int ld, int nch, int nt, int nbits, int exam_nticks, double threshold_sigma,
+                         double threshold_fix, int* nshift, void* stream);
 size_t wct_nf_status_workspace_size(int nch, int nt);
 int wct_nf_lf_noisy(const float* rows, int ld, int nch, int nt, double threshold, int window, int nbins, double cut,
                     int* lf_noisy, double* ratio, void* workspace, size_t ws_bytes, void* stream);

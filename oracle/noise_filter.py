@@ -2,8 +2,8 @@
 (sigproc/src/OmnibusNoiseFilter.cxx:95-260) with Protodune::OneChannelNoise
 (sigproc/src/Protodune.cxx:753-918) and RelGainCalib (:975-1009),
 Microboone::OneChannelNoise (sigproc/src/Microboone.cxx:902-1064), the
-coherent subtraction (:53-462, :798-872), OneChannelStatus (:1263-1334)
-and the parts of OmniChannelNoiseDB they read
+coherent subtraction (:53-462, :798-872), OneChannelStatus (:1263-1334),
+ADCBitShift (:1106-1216) and the parts of OmniChannelNoiseDB they read
 (sigproc/src/OmniChannelNoiseDB.cxx).
 
 TEST INFRASTRUCTURE ONLY.  Only tests/ and the CPU-baseline leg of the
@@ -701,3 +701,90 @@ def one_channel_status(signal, plane, threshold=3.5, window=5, nbins=250, cut=14
         return []
     flag, _ = lf_noisy(signal, threshold, window, nbins, cut)
     return [(0, len(signal))] if flag else []
+
+
+# ---- Microboone::ADCBitShift (Microboone.cxx:1106-1216) -----------------
+
+def _i32(v):
+    """C int wrap-around of a Python int."""
+    v &= 0xFFFFFFFF
+    return v - (1 << 32) if v >= 1 << 31 else v
+
+
+def bits_shift_right(value, n, filling, total_bits):
+    """Bits::shift_right (util/src/Bits.cxx:4-7): (value >> n) | (filling
+    << (total_bits - n)), C int arithmetic."""
+    return _i32((value >> n) | _i32(filling << (total_bits - n)))
+
+
+def bits_lowest(value, n):
+    """Bits::lowest_bits (util/src/Bits.cxx:9): value & (2^n - 1)."""
+    return value & ((1 << n) - 1)
+
+
+def _trunc(x):
+    """C conversion of a double to int (toward zero)."""
+    return int(math.trunc(x))
+
+
+def adc_bit_shift(signal, nbits=12, exam_nticks=500, threshold_sigma=7.5, threshold_fix=0.8):
+    """ADCBitShift::apply (:1106-1216).  The low nbits bits of each of the
+    first exam_nticks samples (as int, truncated) are compared with the
+    previous sample's (zero before the first); nshift is the number of
+    low bits, from bit 0 up, that flip fewer than nbin / 2 - nbin / 2
+    sqrt(1 / nbin) threshold_sigma times.  For 0 < nshift < 11 every
+    sample becomes shift_right(x, nshift, lowest_bits(x, nshift), 12), and
+    a sample far (pow(2, nbits - nshift) threshold_fix, as int) from the
+    integer mean and from a neighbour takes, in the fillings' increasing
+    order, each shift_right(x, nshift, filling, 12) closer to the
+    neighbours' extrapolation (sequential: the previous samples are the
+    corrected ones; the first sample's next two are the mean).  Returns
+    (signal as float32, nshift; 0 when unchanged).  The reference does not
+    terminate on a negative sample in the examined ticks (its bit loop
+    shifts a negative int forever); this takes the two's-complement bits."""
+    sig = np.asarray(signal, dtype=F)
+    nbin = exam_nticks
+    counter = [0] * nbits
+    prev = [0] * nbits
+    for i in range(nbin):
+        x = _trunc(float(sig[i]))
+        cur = [(x >> j) & 1 for j in range(nbits)]
+        for j in range(nbits):
+            counter[j] += abs(cur[j] - prev[j])
+        prev = cur
+    lim = nbin / 2. - nbin / 2. * math.sqrt(1. / nbin) * threshold_sigma
+    nshift = 0
+    for j in range(nbits):
+        if counter[j] < lim:
+            nshift += 1
+        else:
+            break
+    if not (nshift != 0 and nshift < 11):
+        return sig.copy(), 0
+    nl = len(sig)
+    x_orig = [_trunc(float(v)) for v in sig]
+    x = [0] * nl
+    fillings = set()
+    total = 0
+    for i in range(nl):
+        f = bits_lowest(x_orig[i], nshift)
+        x[i] = bits_shift_right(x_orig[i], nshift, f, 12)
+        fillings.add(f)
+        total += x[i]
+    mean = _i32(total)
+    mean = int(mean / nl) if mean >= 0 else -int(-mean / nl)   # C int division
+    exp_diff = _trunc(math.pow(2, nbits - nshift) * threshold_fix)
+    prev1 = prev_c = nxt = nxt1 = mean
+    for i in range(nl):
+        cur = x[i]
+        if abs(cur - mean) > exp_diff and (abs(cur - prev_c) > exp_diff or abs(cur - nxt) > exp_diff):
+            ev = _trunc(((2 * prev_c - prev1) + (prev_c + nxt) / 2. + (prev_c * 2. / 3. + nxt1 / 3.)) / 3.)
+            for f in sorted(fillings):
+                y = bits_shift_right(x_orig[i], nshift, f, 12)
+                if abs(y - ev) < abs(x[i] - ev):
+                    x[i] = y
+        prev1 = prev_c
+        prev_c = x[i]
+        nxt = x[i + 2] if i + 2 < nl else mean
+        nxt1 = x[i + 3] if i + 3 < nl else mean
+    return np.asarray(x, dtype=F), nshift

@@ -559,3 +559,90 @@ def test_status_filter_component():
     for ch in range(24):
         assert np.array_equal(got[ch], rows[ch])
     CO.reset()
+
+
+def _rotl12(x, n):
+    x = np.asarray(x, dtype=np.int64)
+    return (((x << n) | (x >> (12 - n))) & 0xFFF).astype(np.int64)
+
+
+def _bitshift_rows(rng, nt, nrows=21):
+    """Baselines whose top bits hold still (600-900 +- 3 or 20: bits 9-11
+    fixed; 1100 +- 3 for a 5-bit rotation: bits 7-11 fixed), rotated left by 0, 1, 2, 3 or 5 bits in 12,
+    some with samples whose bit 11 is set (glitches the correction must
+    catch), and a zero row."""
+    rows = []
+    for k in range(nrows - 1):
+        n = (0, 1, 2, 3, 5)[k % 5]
+        base, sd = (1100, 3.0) if n == 5 else (rng.uniform(600, 900), (3.0, 20.0)[k % 2])
+        x = np.clip(np.rint(base + rng.normal(0, sd, nt)), 0, 2047).astype(np.int64)
+        if k % 3 == 0:
+            x[rng.integers(600, nt, 4)] += 2048
+        rows.append(_rotl12(x, n) if n else x)
+    rows.append(np.zeros(nt, np.int64))
+    return np.asarray(rows, np.float32)
+
+
+def test_adc_bit_shift_matches_oracle():
+    """wct_nf_adc_bit_shift against the oracle: shift counts equal and
+    every sample equal (integer arithmetic), at the default and at other
+    thresholds."""
+    _torch()
+    from wct_amd import noise as N
+    rng = np.random.default_rng(1071)
+    nt = 3000
+    rows = _bitshift_rows(rng, nt)
+    print(f"adc bit shift {rows.shape}", flush=True)
+    got, ns = N.adc_bit_shift(rows)
+    got = got.cpu().numpy()
+    seen = set()
+    for c in range(len(rows)):
+        want, wn = NF.adc_bit_shift(rows[c])
+        assert ns[c] == wn, (c, ns[c], wn)
+        assert np.array_equal(got[c], want), (c, np.flatnonzero(got[c] != want)[:5])
+        seen.add(wn)
+    assert {0, 1, 2, 3, 5} <= seen
+    got, ns = N.adc_bit_shift(rows[:10], exam_nticks=800, threshold_sigma=5.0, threshold_fix=0.5)
+    got = got.cpu().numpy()
+    for c in range(10):
+        want, wn = NF.adc_bit_shift(rows[c], exam_nticks=800, threshold_sigma=5.0, threshold_fix=0.5)
+        assert ns[c] == wn and np.array_equal(got[c], want), c
+
+
+def test_bit_shift_filter_component():
+    """OmnibusNoiseFilter with mbADCBitShift as a channel filter: traces
+    equal the oracle's, corrected channels masked ADCBitShift (n, n)."""
+    _torch()
+    from wct_amd import components as CO
+    rng = np.random.default_rng(1106)
+    nt = 3000
+    rows = _bitshift_rows(rng, nt, nrows=12)
+    cfg = [
+        {"type": "AnodeChannels", "name": "ba", "data": {"ident": 0, "plane_ranges": [[0, 4], [4, 4], [8, 4]]}},
+        {"type": "OmniChannelNoiseDB", "name": "bdb", "data": {
+            "anode": "AnodeChannels:ba", "tick": TICK, "nsamples": nt, "bad": [],
+            "channel_info": [{"channels": {"first": 0, "last": 11}, "min_rms_cut": 0.0, "max_rms_cut": 1000.0}]}},
+        {"type": "mbADCBitShift", "name": "bs", "data": {"Number_of_ADC_bits": 12, "Exam_number_of_ticks_test": 500,
+                                                          "Threshold_sigma_test": 7.5, "Threshold_fix": 0.8}},
+        {"type": "OmnibusNoiseFilter", "name": "bnf", "data": {
+            "nticks": 0, "maskmap": {}, "channel_filters": ["mbADCBitShift:bs"], "grouped_filters": [],
+            "channel_status_filters": [], "noisedb": "OmniChannelNoiseDB:bdb", "intraces": "orig",
+            "outtraces": "raw"}},
+    ]
+    CO.reset()
+    CO.configure(cfg)
+    fr = CO.Frame(ident=3, time=0.0, tick=TICK)
+    for ch in range(12):
+        fr.add_trace(ch, rows[ch], 0)
+    fr.tag_traces("orig", list(range(12)))
+    out = CO.frame_filter("OmnibusNoiseFilter:bnf", fr)
+    got = {ch: q for ch, tb, q in out.traces()}
+    want_mask = {}
+    for ch in range(12):
+        want, wn = NF.adc_bit_shift(rows[ch])
+        assert np.array_equal(got[ch], want), ch
+        if wn:
+            want_mask[ch] = [(wn, wn)]
+    assert want_mask
+    assert out.masks("ADCBitShift") == want_mask
+    CO.reset()

@@ -400,3 +400,53 @@ def test_one_channel_status_planes():
     assert NF.one_channel_status(s, plane=1) == [(0, 1000)]
     assert NF.one_channel_status(s, plane=2) == []      # collection is not tested
     assert NF.one_channel_status(_sine(1000, 3, 10.0), plane=0) == []
+
+
+# ---- Microboone::ADCBitShift (Microboone.cxx:1106-1216) -----------------
+
+def _rotl12(x, n):
+    """The readout fault the filter undoes: the 12 ADC bits rotated left
+    by n (the top n bits land in the low n)."""
+    x = np.asarray(x, dtype=np.int64)
+    return (((x << n) | (x >> (12 - n))) & 0xFFF).astype(np.int64)
+
+
+def test_bits_hand_worked():
+    # Bits::shift_right(0b101101, 2, 0b11, 12) = 0b1011 | 0b11 << 10
+    assert NF.bits_shift_right(0b101101, 2, 0b11, 12) == 11 + 3072
+    assert NF.bits_lowest(0b101101, 3) == 0b101
+    # shift_right with the value's own low bits undoes a left rotation
+    for x in (0, 1, 1500, 2048, 4095):
+        for n in (1, 2, 5, 10):
+            v = int(_rotl12(x, n))
+            assert NF.bits_shift_right(v, n, NF.bits_lowest(v, n), 12) == x
+
+
+def test_adc_bit_shift_undoes_rotation():
+    """A baseline of 1500 +- 20 rotated by n = 1..3: its low n bits (the
+    top bits of 1500, all zero) never flip in the first 500 ticks, bit n
+    (bit 0 of the sample) flips about half the time, so nshift = n and
+    every sample is rotated back exactly (nothing is far from the mean)."""
+    rng = np.random.default_rng(1106)
+    x = np.rint(1500 + rng.normal(0, 20, 3000)).astype(np.int64)
+    for n in (1, 2, 3):
+        out, ns = NF.adc_bit_shift(_rotl12(x, n).astype(F))
+        assert ns == n
+        assert np.array_equal(out, x.astype(F))
+    out, ns = NF.adc_bit_shift(x.astype(F))            # no fault: unchanged
+    assert ns == 0 and np.array_equal(out, x.astype(F))
+
+
+def test_adc_bit_shift_restores_a_glitch():
+    """One sample with bit 11 set (1500 + 2048) rotated by 1 lands, rotated
+    back with its own filling, 2048 away from the mean and its neighbours
+    (beyond 2^11 x 0.8 = 1638): the other filling (0) is nearer the
+    neighbours' extrapolation, so it becomes 1500 again."""
+    x = np.full(2000, 1500, np.int64)
+    x[1::2] += 1          # bit 0 of the samples flips every tick
+    x[700] = 1500 + 2048
+    out, ns = NF.adc_bit_shift(_rotl12(x, 1).astype(F))
+    assert ns == 1
+    want = x.copy()
+    want[700] = 1500
+    assert np.array_equal(out, want.astype(F))

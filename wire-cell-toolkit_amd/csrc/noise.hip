@@ -1241,6 +1241,92 @@ __global__ __launch_bounds__(kNfBlock) void k_st_ratio(const float2* __restrict_
     }
 }
 
+// ---- Microboone::ADCBitShift (Microboone.cxx:1106-1216) ---------------
+
+// One channel per workgroup, in place.  Detection: the low nbits bits of
+// each of the first exam samples (as int, truncated) against the previous
+// sample's (zero before the first), flips counted per bit by wave ballots;
+// nshift = the low bits, from bit 0 up, flipping fewer than nbin / 2 -
+// nbin / 2 sqrt(1 / nbin) threshold_sigma times.  For 0 < nshift < 11:
+// every sample rotated back with its own low bits as filling
+// (Bits::shift_right(x, nshift, lowest_bits(x, nshift), 12)), the set of
+// fillings and the integer mean formed in parallel, then the examination
+// of :1164-1208 on one lane (each sample's previous two are the corrected
+// ones, so it is sequential): a sample far from the mean and from a
+// neighbour takes, fillings in increasing order, each candidate closer to
+// the neighbours' extrapolation.
+__global__ __launch_bounds__(kNfBlock) void k_bs(float* __restrict__ rows, int ld, int n, int nbits, int exam,
+                                                 double threshold_sigma, double threshold_fix,
+                                                 int* __restrict__ nshift_out)
+{
+    extern __shared__ __align__(16) unsigned char nf_smem[];
+    __shared__ NfShared sh;
+    __shared__ int cnt[32];
+    __shared__ unsigned fill[32];   // up to 2^10 fillings
+    const int c = blockIdx.x;
+    const auto row = G(rows + (long)c * ld);
+    int* x = reinterpret_cast<int*>(nf_smem);
+    if (threadIdx.x < 32) {
+        cnt[threadIdx.x] = 0;
+        fill[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (int i0 = 0; i0 < exam; i0 += kNfBlock) {
+        const int i = i0 + (int)threadIdx.x;
+        unsigned d = 0;
+        if (i < exam) d = (unsigned)((int)row[i] ^ (i ? (int)row[i - 1] : 0));
+        for (int j = 0; j < nbits; ++j) {
+            const unsigned long long m = __ballot((d >> j) & 1u);
+            if ((threadIdx.x & 63) == 0 && m) atomicAdd(&cnt[j], __popcll(m));
+        }
+    }
+    __syncthreads();
+    const double lim = exam / 2. - exam / 2. * sqrt(1. / exam) * threshold_sigma;
+    int ns = 0;
+    while (ns < nbits && cnt[ns] < lim) ++ns;
+    if (ns == 0 || ns >= 11) {   // uniform: every thread reads the same counts
+        if (threadIdx.x == 0) G(nshift_out)[c] = 0;
+        return;
+    }
+    int part = 0;
+    for (int i = threadIdx.x; i < n; i += kNfBlock) {
+        const int xo = (int)row[i];
+        const int f = xo & ((1 << ns) - 1);
+        const int y = (xo >> ns) | (f << (12 - ns));
+        x[i] = y;
+        part += y;
+        atomicOr(&fill[f >> 5], 1u << (f & 31));
+    }
+    const int total = block_sum(part, sh);   // int arithmetic, as the reference sums
+    const int mean = total / n;
+    const int exp_diff = (int)(ldexp(1.0, nbits - ns) * threshold_fix);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int prev1 = mean, prev = mean, next = mean, next1 = mean;
+        const int nf = 1 << ns;
+        for (int i = 0; i < n; ++i) {
+            const int cur = x[i];
+            if (abs(cur - mean) > exp_diff && (abs(cur - prev) > exp_diff || abs(cur - next) > exp_diff)) {
+                const int ev =
+                    (int)(((2 * prev - prev1) + (prev + next) / 2. + (prev * 2. / 3. + next1 / 3.)) / 3.);
+                const int xo = (int)row[i];
+                for (int f = 0; f < nf; ++f) {
+                    if (!((fill[f >> 5] >> (f & 31)) & 1u)) continue;
+                    const int y = (xo >> ns) | (f << (12 - ns));
+                    if (abs(y - ev) < abs(x[i] - ev)) x[i] = y;
+                }
+            }
+            prev1 = prev;
+            prev = x[i];
+            next = i + 2 < n ? x[i + 2] : mean;
+            next1 = i + 3 < n ? x[i + 3] : mean;
+        }
+        G(nshift_out)[c] = ns;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kNfBlock) row[i] = (float)x[i];
+}
+
 }  // namespace
 
 }  // namespace wct
@@ -1445,6 +1531,25 @@ extern "C" int wct_nf_lf_noisy(const float* rows, int ld, int nch, int nt, doubl
     WCT_TRY(launch_rows<float>(crow, crow, nt, npair, nt, -1, 1.0f, s));
     hipLaunchKernelGGL(k_st_ratio, dim3(npair), dim3(kNfBlock), 0, s, crow, nch, nt, nbins, cut, valid, lf_noisy,
                        ratio);
+    WCT_HIP_CHECK(hipGetLastError());
+    return WCT_OK;
+}
+
+extern "C" int wct_nf_adc_bit_shift(float* rows, int ld, int nch, int nt, int nbits, int exam_nticks,
+                                    double threshold_sigma, double threshold_fix, int* nshift, void* stream)
+{
+    WCT_ENTRY();
+    WCT_REQUIRE(rows && nshift && nch >= 1 && nt >= 1 && ld >= nt && nbits >= 1 && nbits <= 31 &&
+                    exam_nticks >= 1 && exam_nticks <= nt,
+                WCT_ERR_ARG, "nf adc bit shift: bad arguments (nch %d, nt %d, ld %d, nbits %d, exam %d)", nch, nt,
+                ld, nbits, exam_nticks);
+    const size_t lds = sizeof(int) * (size_t)nt;
+    WCT_REQUIRE((long)lds + (long)sizeof(NfShared) + 256 <= lds_limit_bytes(), WCT_ERR_UNSUPPORTED,
+                "nf adc bit shift: %d ticks need %zu bytes of LDS per channel", nt, lds);
+    hipStream_t s = as_stream(stream);
+    WCT_TRY(allow_lds(k_bs, lds));
+    hipLaunchKernelGGL(k_bs, dim3(nch), dim3(kNfBlock), lds, s, rows, ld, nt, nbits, exam_nticks, threshold_sigma,
+                       threshold_fix, nshift);
     WCT_HIP_CHECK(hipGetLastError());
     return WCT_OK;
 }

@@ -722,6 +722,77 @@ class RelGainCalib : public IChannelFilter, public IBatchChannelFilter, public I
     mutable DevBuf m_d_rows, m_d_gain;
 };
 
+// ---- mbADCBitShift (sigproc/src/Microboone.cxx:1071-1216) --------------
+// Every channel of a frame in one device batch (wct_nf_adc_bit_shift); a
+// corrected channel gets the mask ADCBitShift with the range (nshift,
+// nshift), as the reference records it.
+class ADCBitShift : public IChannelFilter, public IBatchChannelFilter, public IConfigurable {
+  public:
+    ~ADCBitShift() override
+    {
+        if (m_stream) wct_stream_destroy(m_stream);
+    }
+    Configuration default_configuration() const override   // :1088-1097
+    {
+        Configuration cfg;
+        cfg["Number_of_ADC_bits"] = m_nbits;
+        cfg["Exam_number_of_ticks_test"] = m_exam;
+        cfg["Threshold_sigma_test"] = m_sigma;
+        cfg["Threshold_fix"] = m_fix;
+        return cfg;
+    }
+    void configure(const Configuration& cfg) override   // :1080-1086
+    {
+        m_nbits = get(cfg, "Number_of_ADC_bits", m_nbits);
+        m_exam = get(cfg, "Exam_number_of_ticks_test", m_exam);
+        m_sigma = get(cfg, "Threshold_sigma_test", m_sigma);
+        m_fix = get(cfg, "Threshold_fix", m_fix);
+        if (m_nbits < 1 || m_nbits > 31) throw ConfigError("mbADCBitShift: Number_of_ADC_bits outside 1..31");
+        if (m_exam < 1) throw ConfigError("mbADCBitShift: Exam_number_of_ticks_test below 1");
+    }
+    Waveform::ChannelMaskMap apply(int ch, signal_t& signal) const override
+    {
+        std::vector<float> rows(signal.begin(), signal.end());
+        std::vector<Waveform::ChannelMaskMap> masks;
+        apply_batch({ch}, rows, (int)signal.size(), masks);
+        signal.assign(rows.begin(), rows.end());
+        return masks[0];
+    }
+    Waveform::ChannelMaskMap apply(channel_signals_t&) const override { return Waveform::ChannelMaskMap(); }
+    void apply_batch(const std::vector<int>& channels, std::vector<float>& rows, int nt,
+                     std::vector<Waveform::ChannelMaskMap>& masks) const override
+    {
+        std::lock_guard<std::mutex> lock(m_mutex);
+        const int nch = (int)channels.size();
+        masks.assign(nch, Waveform::ChannelMaskMap());
+        if (nch == 0 || nt == 0) return;
+        if (m_exam > nt)   // the reference's at() would throw
+            throw std::runtime_error("mbADCBitShift: Exam_number_of_ticks_test " + std::to_string(m_exam) +
+                                     " above " + std::to_string(nt) + " ticks");
+        if (!m_stream) ncheck(wct_stream_create(&m_stream), "mbADCBitShift stream");
+        const size_t bytes = sizeof(float) * (size_t)nch * nt;
+        m_d_rows.ensure(bytes);
+        m_d_ns.ensure(sizeof(int) * nch);
+        ncheck(wct_memcpy_htod_async(m_d_rows.p, rows.data(), bytes, m_stream), "mbADCBitShift upload");
+        ncheck(wct_nf_adc_bit_shift((float*)m_d_rows.p, nt, nch, nt, m_nbits, m_exam, m_sigma, m_fix,
+                                    (int*)m_d_ns.p, m_stream),
+               "mbADCBitShift");
+        std::vector<int> ns(nch);
+        ncheck(wct_memcpy_dtoh_async(rows.data(), m_d_rows.p, bytes, m_stream), "mbADCBitShift download");
+        ncheck(wct_memcpy_dtoh_async(ns.data(), m_d_ns.p, sizeof(int) * nch, m_stream), "mbADCBitShift download");
+        ncheck(wct_stream_sync(m_stream), "mbADCBitShift sync");
+        for (int i = 0; i < nch; ++i)
+            if (ns[i]) masks[i]["ADCBitShift"][channels[i]].push_back({ns[i], ns[i]});
+    }
+
+  private:
+    int m_nbits{12}, m_exam{500};
+    double m_sigma{7.5}, m_fix{0.8};
+    mutable std::mutex m_mutex;
+    mutable void* m_stream{nullptr};
+    mutable DevBuf m_d_rows, m_d_ns;
+};
+
 // ---- mbOneChannelStatus (sigproc/src/Microboone.cxx:1218-1334) ---------
 // A status filter: channels off the collection plane whose low-frequency
 // content (ID_lf_noisy) exceeds the cut are masked lf_noisy over the whole
@@ -1141,4 +1212,5 @@ WIRECELL_HIP_FACTORY(mbOneChannelNoise, WireCell::Hip::MbOneChannelNoise)
 WIRECELL_HIP_FACTORY(pdRelGainCalib, WireCell::Hip::RelGainCalib)
 WIRECELL_HIP_FACTORY(mbCoherentNoiseSub, WireCell::Hip::CoherentNoiseSub)
 WIRECELL_HIP_FACTORY(mbOneChannelStatus, WireCell::Hip::OneChannelStatus)
+WIRECELL_HIP_FACTORY(mbADCBitShift, WireCell::Hip::ADCBitShift)
 WIRECELL_HIP_FACTORY(OmnibusNoiseFilter, WireCell::Hip::OmnibusNoiseFilter)

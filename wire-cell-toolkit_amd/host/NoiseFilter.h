@@ -11,6 +11,7 @@
 //                          channel group of a frame in one device batch
 //   mbOneChannelStatus     sigproc/src/Microboone.cxx:1218-1334, the
 //                          same
+//   mbADCBitShift          sigproc/src/Microboone.cxx:1071-1216, the same
 //   OmnibusNoiseFilter     sigproc/src/OmnibusNoiseFilter.cxx
 #pragma once
 #include <complex>

@@ -140,6 +140,7 @@ _SIGS = {
     "wct_nf_workspace_size": (_Z, [_I, _I]),
     "wct_nf_one_channel": (_I, [_V, _I, _V, _I, _I, _I, _V, _V, _V, _V, _V, _Z, _V]),
     "wct_nf_rel_gain": (_I, [_V, _I, _I, _I, _V, _F, _F, _V]),
+    "wct_nf_adc_bit_shift": (_I, [_V, _I, _I, _I, _I, _I, _D, _D, _V, _V]),
     "wct_nf_status_workspace_size": (_Z, [_I, _I]),
     "wct_nf_lf_noisy": (_I, [_V, _I, _I, _I, _D, _I, _I, _D, _V, _V, _V, _Z, _V]),
     "wct_nf_coherent_workspace_size": (_Z, [_I, _I, _I]),

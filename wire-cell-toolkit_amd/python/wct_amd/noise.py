@@ -119,6 +119,20 @@ def rel_gain(frame, gains, gain_min=0.8, gain_max=1.25, stream=None):
     return x
 
 
+def adc_bit_shift(frame, nbits=12, exam_nticks=500, threshold_sigma=7.5, threshold_fix=0.8, stream=None):
+    """Microboone::ADCBitShift on every row (a corrected CUDA copy of
+    frame and the per-row shift count, numpy int32)."""
+    import torch
+    x = torch.as_tensor(frame, dtype=torch.float32, device="cuda").contiguous().clone()
+    nch, nt = x.shape
+    ns = torch.empty(nch, dtype=torch.int32, device="cuda")
+    check(lib().wct_nf_adc_bit_shift(C.c_void_p(x.data_ptr()), nt, nch, nt, int(nbits), int(exam_nticks),
+                                     float(threshold_sigma), float(threshold_fix), C.c_void_p(ns.data_ptr()),
+                                     _stream() if stream is None else stream))
+    torch.cuda.synchronize()
+    return x, ns.cpu().numpy()
+
+
 def lf_noisy(frame, threshold=3.5, window=5, nbins=250, cut=14.0, stream=None):
     """Microboone::OneChannelStatus::ID_lf_noisy on every row of frame:
     (flags, ratios) as numpy arrays (bool, float64)."""
