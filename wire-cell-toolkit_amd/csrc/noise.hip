@@ -1182,7 +1182,7 @@ __global__ __launch_bounds__(kNfBlock) void k_st_load(const float* __restrict__ 
     const size_t a4 = ((size_t)4 * n + 15) / 16 * 16;
     float* s = reinterpret_cast<float*>(nf_smem);
     int* hist = reinterpret_cast<int*>(nf_smem + a4);
-    unsigned char* mk = nf_smem + 2 * a4;
+    unsigned char* mk = nf_smem + a4;   // over the histogram, which pct_binned is done with
     const auto src = G(in + (long)c * ld);
     int nz = 0;
     for (int i = threadIdx.x; i < n; i += kNfBlock) {
@@ -1515,7 +1515,7 @@ extern "C" int wct_nf_lf_noisy(const float* rows, int ld, int nch, int nt, doubl
                 window, nbins);
     WCT_REQUIRE(workspace && ws_bytes >= wct_nf_status_workspace_size(nch, nt), WCT_ERR_ARG,
                 "nf lf_noisy: workspace of %zu bytes, %zu needed", ws_bytes, wct_nf_status_workspace_size(nch, nt));
-    const size_t lds = 2 * (((size_t)4 * nt + 15) / 16 * 16) + (size_t)nt;
+    const size_t lds = 2 * (((size_t)4 * nt + 15) / 16 * 16);
     WCT_REQUIRE((long)lds + (long)sizeof(NfShared) <= lds_limit_bytes(), WCT_ERR_UNSUPPORTED,
                 "nf lf_noisy: %d ticks need %zu bytes of LDS per channel", nt, lds);
     hipStream_t s = as_stream(stream);
